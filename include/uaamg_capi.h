@@ -1,0 +1,224 @@
+/*
+ * uaamg_capi.h - C ABI of the MI355X UAAMG-PCG pressure-Poisson solver.
+ *
+ * The reference exposes the solve path as C++ classes in namespace
+ * simd_uaamg that take OpenVDB grids (SIMD_UAAMG/simd_vdb_poisson.h). This
+ * header replaces that surface with plain pointers and sizes: an OpenVDB grid
+ * is passed as its flattened leaves (origins, active masks, leaf buffers), and
+ * vectors are passed DOF-ordered, i.e. in the order LaplacianWithLevel::
+ * gridToVector produces (simd_vdb_poisson.cpp:1584-1600). Each entry point
+ * cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 on success, 1 = FAILED for the
+ *    solve calls (PoissonSolver::FAILED, simd_vdb_poisson.h:151), negative on
+ *    error (see UAAMG_E*). uaamg_last_error() describes the last error of the
+ *    calling thread. No C++ exception crosses this boundary.
+ *  - Host arrays are borrowed for the duration of a call only.
+ *  - A solver handle owns its device memory; use one handle per host thread.
+ */
+#ifndef UAAMG_CAPI_H
+#define UAAMG_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UAAMG_SUCCESS 0
+#define UAAMG_FAILED 1
+#define UAAMG_EINVAL (-1)
+#define UAAMG_ENOMEM (-2)
+#define UAAMG_EHIP (-3)
+#define UAAMG_ENODEV (-4)
+
+/* PoissonSolver::SmootherOption (simd_vdb_poisson.h:143-148) */
+#define UAAMG_SMOOTHER_SCHEDULED_RELAXED_JACOBI 0
+#define UAAMG_SMOOTHER_WEIGHTED_JACOBI 1
+#define UAAMG_SMOOTHER_RED_BLACK_GAUSS_SEIDEL 2
+#define UAAMG_SMOOTHER_SPAI0 3
+
+/* LaplacianApplySIMD::WorkingMode (simd_vdb_poisson.cpp:683-690) */
+#define UAAMG_MODE_LAPLACIAN 0
+#define UAAMG_MODE_RESIDUAL 1
+#define UAAMG_MODE_WEIGHTED_JACOBI 2
+#define UAAMG_MODE_SPAI0 3
+#define UAAMG_MODE_RED_GS 4
+#define UAAMG_MODE_BLACK_GS 5
+
+/* coefficient kinds of a level */
+#define UAAMG_COEF_DIAGONAL 0
+#define UAAMG_COEF_X 1
+#define UAAMG_COEF_Y 2
+#define UAAMG_COEF_Z 3
+#define UAAMG_COEF_SPAI0 4
+
+/*
+ * A flattened OpenVDB FloatGrid (ncomp = 1) or Vec3fGrid (ncomp = 3).
+ * origins[i*3..i*3+2] is the origin of leaf i (a multiple of 8); leaves may come
+ * in any order and are put in OpenVDB Tree4<5,4,3> traversal order internally.
+ * values holds each leaf's buffer: 512 entries (512*3 for Vec3f, AoS) at voxel
+ * offset 64x+8y+z. active[i*8+w] bit b is the active state of offset 64w+b
+ * (NULL = all active). Lookups outside the leaves return background.
+ */
+typedef struct uaamg_grid {
+    int64_t nleaf;
+    const int32_t* origins;
+    const uint64_t* active;
+    const float* values;
+    int32_t ncomp;
+    float background[3];
+} uaamg_grid;
+
+/* PoissonSolver public fields (simd_vdb_poisson.h:165-169) plus device placement */
+typedef struct uaamg_options {
+    float relative_tolerance; /* mRelativeTolerance, default 1e-7 */
+    int32_t max_iterations;   /* mMaxIteration, default 100 */
+    int32_t smoother;         /* mSmoother, UAAMG_SMOOTHER_* */
+    int32_t precision;        /* 0 = FP32 throughout */
+    int32_t device;           /* HIP device ordinal */
+    int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, simd_vdb_poisson.cpp:1815) */
+    void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
+} uaamg_options;
+
+typedef struct uaamg_poisson uaamg_poisson;
+
+/* Fills the reference defaults (PoissonSolver ctor, simd_vdb_poisson.h:153-160). */
+int uaamg_options_default(uaamg_options* opts);
+
+/* Text of the calling thread's last error ("" if none). */
+const char* uaamg_last_error(void);
+
+/* Library version string. */
+const char* uaamg_version(void);
+
+/*
+ * Builds the level-0 Laplacian and the aggregation hierarchy and uploads it.
+ * Replaces LaplacianWithLevel::createPressurePoissonLaplacian (simd_vdb_poisson.h:54-57,
+ * .cpp:95-99) followed by PoissonSolver(LaplacianWithLevel::Ptr) (simd_vdb_poisson.h:153-160).
+ * dx is liquid_phi->voxelSize()[0].
+ */
+int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
+                         const uaamg_options* opts, uaamg_poisson** out);
+
+void uaamg_poisson_destroy(uaamg_poisson* h);
+
+/* Sets mRelativeTolerance / mMaxIteration / mSmoother (simd_vdb_poisson.h:165-169). */
+int uaamg_poisson_set_options(uaamg_poisson* h, const uaamg_options* opts);
+
+/* mNumDof of level 0 (simd_vdb_poisson.h:124). */
+int64_t uaamg_poisson_ndof(const uaamg_poisson* h);
+
+/* mMultigridHierarchy.size() (simd_vdb_poisson.h:170). */
+int uaamg_poisson_nlevels(const uaamg_poisson* h);
+
+/*
+ * Per-level sizes: DOF count, leaves holding DOFs, and leaves of the reference's
+ * DOF tree (which also keeps empty leaves).
+ */
+int uaamg_poisson_level_info(const uaamg_poisson* h, int level, int64_t* ndof, int64_t* nleaf_dof,
+                             int64_t* nleaf_tree);
+
+/*
+ * DOF topology of a level (mDofIndex, simd_vdb_poisson.h:121): origins [nleaf_dof][3], masks [nleaf_dof][8]
+ * and the index of each leaf's first DOF [nleaf_dof]. Any output may be NULL.
+ * Implements the data behind getZeroVectorGrid / the mDofLeafManager visitor.
+ */
+int uaamg_poisson_level_topology(const uaamg_poisson* h, int level, int32_t* origins, uint64_t* masks,
+                                 int64_t* dofbase);
+
+/*
+ * Effective coefficient bricks [nleaf_dof][512] of a level (mDiagonal, mXEntry, mYEntry, mZEntry,
+ * simd_vdb_poisson.h:116-119; SPAI0 of LaplacianApplySIMD::init_SPAI0) as an accessor reads them,
+ * and kept[i] = 0 where the reference trims the leaf to the default (trimDefaultNodes, :1756-1811).
+ */
+int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind, float* bricks, uint8_t* kept);
+
+/*
+ * b = -(1/dx) sum_faces +-(w u + (1-w) u_solid), DOF-ordered (ndof floats).
+ * Replaces LaplacianWithLevel::createPressurePoissonRightHandSide (simd_vdb_poisson.h:59-65, .cpp:101-127).
+ * solid_velocity may be NULL (zero).
+ */
+int uaamg_poisson_build_rhs(const uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                            const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
+                            float* rhs);
+
+/*
+ * PoissonSolver::solveMultigridPCG (simd_vdb_poisson.h:162, .cpp:2155-2226) on host DOF-ordered
+ * vectors: x holds the initial guess on entry and the pressure on exit. Returns UAAMG_SUCCESS or
+ * UAAMG_FAILED; *iterations receives mIterationTaken.
+ */
+int uaamg_poisson_solve_pcg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations);
+
+/* Same with x and b in device memory (DOF-ordered, ndof floats each). */
+int uaamg_poisson_solve_pcg_device(uaamg_poisson* h, float* x_dev, const float* b_dev, int32_t* iterations);
+
+/* PoissonSolver::solvePureMultigrid (simd_vdb_poisson.h:163, .cpp:2228-2267), host vectors. */
+int uaamg_poisson_solve_mg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations);
+
+/*
+ * One operator sweep on a level with host DOF-ordered vectors (LaplacianWithLevel::laplacianApply /
+ * residualApply / weightedJacobiApply / spai0Apply, simd_vdb_poisson.cpp:1347-1385). For
+ * UAAMG_MODE_RED_GS / _BLACK_GS this is one colour pass in place: y = pass(x), weight = SOR weight.
+ * b may be NULL for UAAMG_MODE_LAPLACIAN.
+ */
+int uaamg_poisson_apply(uaamg_poisson* h, int level, int mode, float weight, const float* x, const float* b,
+                        float* y);
+
+/*
+ * First smoothing sweep from a zero guess (setGridToResultAfterFirstJacobi / SPAI0 / RBGS<true>,
+ * simd_vdb_poisson.cpp:1435-1582): kind 0 = Jacobi with weight, 1 = SPAI0, 2 = red-black GS with SOR weight.
+ */
+int uaamg_poisson_first_sweep(uaamg_poisson* h, int level, int kind, float weight, const float* b, float* y);
+
+/* LaplacianWithLevel::restriction from fine_level to fine_level+1 (simd_vdb_poisson.cpp:1624-1684). */
+int uaamg_poisson_restrict(uaamg_poisson* h, int fine_level, const float* fine, float* coarse);
+
+/* LaplacianWithLevel::prolongation<true>: fine += alpha * coarse (simd_vdb_poisson.cpp:1686-1754). */
+int uaamg_poisson_prolong(uaamg_poisson* h, int fine_level, const float* coarse, float* fine, float alpha);
+
+/* One application of the W-cycle preconditioner z = M r (muCyclePreconditioner<2,true>, :1844-1977). */
+int uaamg_poisson_precondition(uaamg_poisson* h, const float* r, float* z);
+
+/* The coarsest-level solve (Eigen CG, 10 iterations from zero; :2114-2152, 1870-1875). */
+int uaamg_poisson_coarsest_solve(uaamg_poisson* h, const float* b, float* x);
+
+/*
+ * Measurement hooks (no reference counterpart; the reference only times whole solves with
+ * CSim::TimerMan). algorithmic_bytes: bytes one sweep of `mode` on `level` must move (each stream
+ * once per DOF; coefficient streams only for leaves that keep stored coefficients).
+ * time_apply: mean milliseconds per launch of `reps` back-to-back sweeps on the handle's stream,
+ * measured with HIP events on that stream.
+ */
+int uaamg_poisson_algorithmic_bytes(const uaamg_poisson* h, int level, int mode, double* bytes);
+int uaamg_poisson_time_apply(uaamg_poisson* h, int level, int mode, int reps, float* ms_per_launch);
+
+/* ---------------------------------------------------------------- scenes */
+/*
+ * Synthetic scenes on [0,1]^3 (liquid SDF, face weights, seeded face velocities) used to drive
+ * the solver without a simulator: config 0 = pool 40% + sphere r=0.15; 1 = pool 30% + 8 seeded
+ * spheres + solid sphere r=0.2 at the centre; 3 = randomised batch problem. n <= 0: preset edge.
+ * These stand in for the caller's grids (FLIP_vdb.cpp:1990-2068 face weights; CG_benchmarks scenes).
+ */
+typedef struct uaamg_scene uaamg_scene;
+int uaamg_scene_create(int config, int32_t n, uint64_t seed, uaamg_scene** out);
+int uaamg_scene_create_custom(int32_t n, float pool, int32_t nspheres, const float* spheres, int32_t has_solid,
+                              const float* solid_sphere, uint64_t velocity_seed, uaamg_scene** out);
+/* Views into the scene's arrays (valid until destroy): phi, face weights, vx, vy, vz. */
+int uaamg_scene_grids(const uaamg_scene* s, uaamg_grid* phi, uaamg_grid* weights, uaamg_grid* vx, uaamg_grid* vy,
+                      uaamg_grid* vz, float* dx, int32_t* n);
+void uaamg_scene_destroy(uaamg_scene* s);
+
+/* fraction_inside of a segment / square (levelset_util.cpp:4-103), exposed for checks. */
+float uaamg_fraction_inside2(float phi_left, float phi_right);
+float uaamg_fraction_inside4(float phi_bl, float phi_br, float phi_tl, float phi_tr);
+
+/* Number of HIP devices visible (0 when none; never fails). */
+int uaamg_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UAAMG_CAPI_H */

@@ -1,0 +1,220 @@
+"""ORACLE - TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper of the CPU restatement of the reference's UAAMG-PCG Poisson
+path (oracle/poisson_oracle.cpp). Only tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py may import this package; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "build" / "liborc.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    L = C.CDLL(str(LIB))
+    vp, i32, i64, f32, f64 = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_double
+    sig = {
+        "orc_set_threads": (None, [i32]),
+        "orc_get_threads": (i32, []),
+        "orc_fraction_inside2": (f32, [f32, f32]),
+        "orc_fraction_inside4": (f32, [f32, f32, f32, f32]),
+        "orc_poisson_create": (vp, [i64, vp, vp, vp, f32, i64, vp, vp, vp, vp, f32, f32, i64]),
+        "orc_poisson_destroy": (None, [vp]),
+        "orc_nlevels": (i32, [vp]),
+        "orc_level_ndof": (i64, [vp, i32]),
+        "orc_level_nleaf_tree": (i64, [vp, i32]),
+        "orc_level_nleaf_dof": (i64, [vp, i32]),
+        "orc_level_topology": (None, [vp, i32, vp, vp, vp]),
+        "orc_level_coefficients": (None, [vp, i32, i32, vp, vp]),
+        "orc_build_rhs": (None, [vp, i64, vp, vp, vp, i64, vp, vp, vp, vp, f32, vp]),
+        "orc_apply": (i32, [vp, i32, i32, f32, vp, vp, vp]),
+        "orc_first_sweep": (i32, [vp, i32, i32, f32, vp, vp]),
+        "orc_restrict": (i32, [vp, i32, vp, vp]),
+        "orc_prolong": (i32, [vp, i32, vp, vp, f32]),
+        "orc_coarsest": (i32, [vp, vp, vp]),
+        "orc_precondition": (i32, [vp, i32, vp, vp]),
+        "orc_solve_pcg": (i32, [vp, i32, f32, i32, vp, vp, C.POINTER(i32), vp, i32]),
+        "orc_solve_mg": (i32, [vp, i32, f32, i32, vp, vp, C.POINTER(i32)]),
+        "orc_solve_pcg64": (i32, [vp, i32, f64, i32, vp, vp, C.POINTER(i32)]),
+        "orc_apply64": (i32, [vp, i32, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OraclePoisson:
+    """Reference-semantics CPU hierarchy + solver for one problem."""
+
+    def __init__(self, phi, weights, dt: float, dx: float, max_coarsest: int = 4000):
+        L = lib()
+        self._keep = []
+
+        def arr(a, dt_):
+            a = np.ascontiguousarray(a, dtype=dt_)
+            self._keep.append(a)
+            return a
+
+        po = arr(phi.origins, np.int32)
+        pa = None if phi.active is None else arr(phi.active, np.uint64)
+        pv = arr(phi.values, np.float32)
+        wo = arr(weights.origins, np.int32)
+        wa = None if weights.active is None else arr(weights.active, np.uint64)
+        wv = arr(weights.values, np.float32)
+        wbg = arr(np.array(weights.background, dtype=np.float32).reshape(-1)[:3], np.float32)
+        phi_bg = float(np.array(phi.background).reshape(-1)[0])
+        self.h = L.orc_poisson_create(po.shape[0], _p(po), _p(pa), _p(pv), phi_bg, wo.shape[0], _p(wo), _p(wa),
+                                      _p(wv), _p(wbg), float(dt), float(dx), int(max_coarsest))
+        self._keep = []
+        self.dx = dx
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_poisson_destroy(self.h)
+            self.h = None
+
+    @property
+    def nlevels(self) -> int:
+        return lib().orc_nlevels(self.h)
+
+    def ndof(self, level: int = 0) -> int:
+        return lib().orc_level_ndof(self.h, level)
+
+    def nleaf_tree(self, level: int) -> int:
+        return lib().orc_level_nleaf_tree(self.h, level)
+
+    def topology(self, level: int):
+        n = lib().orc_level_nleaf_dof(self.h, level)
+        org = np.zeros((n, 3), np.int32)
+        mask = np.zeros((n, 8), np.uint64)
+        base = np.zeros(n, np.int64)
+        lib().orc_level_topology(self.h, level, _p(org), _p(mask), _p(base))
+        return org, mask, base
+
+    def coefficients(self, level: int, kind: int):
+        n = lib().orc_level_nleaf_dof(self.h, level)
+        br = np.zeros((n, 512), np.float32)
+        kept = np.zeros(n, np.uint8)
+        lib().orc_level_coefficients(self.h, level, kind, _p(br), _p(kept))
+        return br, kept
+
+    def build_rhs(self, weights, vx, vy, vz) -> np.ndarray:
+        rhs = np.zeros(self.ndof(0), np.float32)
+        wo = np.ascontiguousarray(weights.origins, np.int32)
+        wv = np.ascontiguousarray(weights.values, np.float32)
+        wbg = np.array(weights.background, dtype=np.float32).reshape(-1)[:3].copy()
+        vo = np.ascontiguousarray(vx.origins, np.int32)
+        xs = [np.ascontiguousarray(v.values, np.float32) for v in (vx, vy, vz)]
+        vbg = float(np.array(vx.background).reshape(-1)[0])
+        lib().orc_build_rhs(self.h, wo.shape[0], _p(wo), _p(wv), _p(wbg), vo.shape[0], _p(vo), *[_p(x) for x in xs],
+                            vbg, _p(rhs))
+        return rhs
+
+    def apply(self, level, mode, x, b=None, weight=0.0):
+        x = np.ascontiguousarray(x, np.float32)
+        b = None if b is None else np.ascontiguousarray(b, np.float32)
+        y = np.zeros(self.ndof(level), np.float32)
+        lib().orc_apply(self.h, level, mode, float(weight), _p(x), _p(b), _p(y))
+        return y
+
+    def first_sweep(self, level, kind, b, weight):
+        b = np.ascontiguousarray(b, np.float32)
+        y = np.zeros(self.ndof(level), np.float32)
+        lib().orc_first_sweep(self.h, level, kind, float(weight), _p(b), _p(y))
+        return y
+
+    def restriction(self, fine_level, fine):
+        fine = np.ascontiguousarray(fine, np.float32)
+        out = np.zeros(self.ndof(fine_level + 1), np.float32)
+        lib().orc_restrict(self.h, fine_level, _p(fine), _p(out))
+        return out
+
+    def prolongation(self, fine_level, coarse, fine, alpha=1.0):
+        coarse = np.ascontiguousarray(coarse, np.float32)
+        out = np.array(fine, np.float32, copy=True)
+        lib().orc_prolong(self.h, fine_level, _p(coarse), _p(out), float(alpha))
+        return out
+
+    def coarsest_solve(self, b):
+        b = np.ascontiguousarray(b, np.float32)
+        x = np.zeros(self.ndof(self.nlevels - 1), np.float32)
+        lib().orc_coarsest(self.h, _p(b), _p(x))
+        return x
+
+    def precondition(self, r, smoother=0):
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.zeros_like(r)
+        lib().orc_precondition(self.h, smoother, _p(r), _p(z))
+        return z
+
+    def solve_pcg(self, b, x0=None, tol=1e-7, maxit=100, smoother=0, history=False):
+        x = np.zeros(self.ndof(0), np.float32) if x0 is None else np.array(x0, np.float32, copy=True)
+        b = np.ascontiguousarray(b, np.float32)
+        it = C.c_int()
+        hist = np.zeros(maxit + 2, np.float32) if history else None
+        st = lib().orc_solve_pcg(self.h, smoother, float(tol), maxit, _p(x), _p(b), C.byref(it), _p(hist),
+                                 0 if hist is None else hist.size)
+        if history:
+            return st, it.value, x, hist[: it.value + 2]
+        return st, it.value, x
+
+    def solve_mg(self, b, x0=None, tol=1e-7, maxit=100, smoother=0):
+        x = np.zeros(self.ndof(0), np.float32) if x0 is None else np.array(x0, np.float32, copy=True)
+        b = np.ascontiguousarray(b, np.float32)
+        it = C.c_int()
+        st = lib().orc_solve_mg(self.h, smoother, float(tol), maxit, _p(x), _p(b), C.byref(it))
+        return st, it.value, x
+
+    def solve_pcg64(self, b, x0=None, tol=1e-12, maxit=200, smoother=0):
+        x = np.zeros(self.ndof(0), np.float64) if x0 is None else np.array(x0, np.float64, copy=True)
+        b = np.ascontiguousarray(b, np.float64)
+        it = C.c_int()
+        st = lib().orc_solve_pcg64(self.h, smoother, float(tol), maxit, _p(x), _p(b), C.byref(it))
+        return st, it.value, x
+
+    def apply64(self, x, level=0):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.ndof(level), np.float64)
+        lib().orc_apply64(self.h, level, _p(x), _p(y))
+        return y
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().orc_get_threads()
+
+
+def fraction_inside2(a, b):
+    return lib().orc_fraction_inside2(a, b)
+
+
+def fraction_inside4(bl, br, tl, tr):
+    return lib().orc_fraction_inside4(bl, br, tl, tr)

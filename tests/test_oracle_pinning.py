@@ -1,0 +1,345 @@
+"""Pins the CPU oracle against independent brute force (no GPU).
+
+Each check is chosen so that a plausible slip in the oracle fails it: a wrong
+face weight index or sign changes the dense level-0 matrix; a dropped 0.5 or a
+wrong child offset breaks A_c = 0.5 R A P (the reference's own explicit-matrix
+identity, CG_benchmarks/main.cpp:1070-1074); a wrong Jacobi/SOR/SPAI0 formula
+fails the textbook sweep on the dense matrix; an unsymmetric smoothing
+schedule breaks <Mx,y> = <x,My>; a wrong tree order permutes DOF numbers.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spl
+
+import oracle as O
+from tests import _brute as B
+
+DT = 0.37
+DX = 0.125
+
+
+@dataclass
+class G:
+    origins: np.ndarray
+    values: np.ndarray
+    active: np.ndarray | None
+    background: tuple
+
+
+def make(seed, nleaf=6, max_coarsest=30, negative=True, weights=(0.0, 0.35, 1.0)):
+    rng = np.random.default_rng(seed)
+    org, phi, act, w = B.random_problem(rng, nleaf=nleaf, negative=negative, weight_levels=weights)
+    phi_bg, w_bg = 0.3, (1.0, 1.0, 1.0)
+    orc = O.OraclePoisson(G(org, phi, act, (phi_bg,)), G(org, w, None, w_bg), DT, DX, max_coarsest=max_coarsest)
+    A, coords = B.assemble_level0(org, phi, act, w, phi_bg, w_bg, DT, DX)
+    return orc, A, coords
+
+
+SEEDS = [1, 2, 3, 4]
+
+
+def level_matrix(orc, level):
+    n = orc.ndof(level)
+    return B.dense_from_apply(lambda e: orc.apply(level, 0, e), n)
+
+
+def test_fraction_inside_closed_forms():
+    f2, f4 = O.fraction_inside2, O.fraction_inside4
+    assert f2(-1, -2) == 1 and f2(1, 2) == 0
+    assert f2(-1, 1) == pytest.approx(0.5) and f2(-1, 3) == pytest.approx(0.25) and f2(3, -1) == pytest.approx(0.25)
+    assert f4(-1, -1, -1, -1) == 1 and f4(1, 1, 1, 1) == 0
+    assert f4(-1, -1, 1, 1) == pytest.approx(0.5)          # bottom edge inside, horizontal cut
+    assert f4(-1, 1, 1, 1) == pytest.approx(0.125)         # one corner: triangle 1/2 * 1/2 * 1/2
+    assert f4(1, -1, -1, -1) == pytest.approx(0.875)
+
+
+def _exact_halfplane_area(a, b, c):
+    """Area of {(u,v) in [0,1]^2 : a + b u + c v < 0} by polygon clipping."""
+    poly = [(0.0, 0.0), (1.0, 0.0), (1.0, 1.0), (0.0, 1.0)]
+    f = lambda p: a + b * p[0] + c * p[1]
+    out = []
+    for i in range(4):
+        p, q = poly[i], poly[(i + 1) % 4]
+        fp, fq = f(p), f(q)
+        if fp < 0:
+            out.append(p)
+        if (fp < 0) != (fq < 0):
+            s = fp / (fp - fq)
+            out.append((p[0] + s * (q[0] - p[0]), p[1] + s * (q[1] - p[1])))
+    if len(out) < 3:
+        return 0.0
+    x = np.array([p[0] for p in out])
+    y = np.array([p[1] for p in out])
+    return 0.5 * abs(np.dot(x, np.roll(y, 1)) - np.dot(y, np.roll(x, 1)))
+
+
+def test_fraction_inside_linear_fields_exact():
+    # for a linear SDF the edge-interpolation formulas give the exact cut area
+    rng = np.random.default_rng(0)
+    for _ in range(400):
+        a, b, c = rng.uniform(-1, 1, 3)
+        bl, br, tl, tr = a, a + b, a + c, a + b + c
+        got = O.fraction_inside4(bl, br, tl, tr)
+        assert got == pytest.approx(_exact_halfplane_area(a, b, c), abs=2e-6)
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_level0_matrix_matches_dense_assembly(seed):
+    orc, A, coords = make(seed)
+    assert orc.ndof(0) == len(coords)
+    M = level_matrix(orc, 0)
+    assert np.allclose(M, A, rtol=2e-6, atol=1e-6 * np.abs(A).max())
+    assert np.allclose(M, M.T, rtol=1e-6, atol=1e-9)
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() > 0
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_dof_numbering_is_tree_order(seed):
+    orc, A, coords = make(seed)
+    org, mask, base = orc.topology(0)
+    bits = np.unpackbits(mask.view(np.uint8), axis=1, bitorder="little")
+    got = []
+    for i in range(org.shape[0]):
+        for off in np.nonzero(bits[i])[0]:
+            lx, ly, lz = B.local(int(off))
+            got.append((int(org[i, 0]) + lx, int(org[i, 1]) + ly, int(org[i, 2]) + lz))
+    assert got == coords
+    counts = bits.sum(1)
+    assert np.array_equal(base, np.concatenate([[0], np.cumsum(counts)[:-1]]))
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_coarse_topology_and_galerkin_identity(seed):
+    orc, A, coords = make(seed)
+    assert orc.nlevels >= 2
+    fine_coords = coords
+    Af = level_matrix(orc, 0)
+    for level in range(1, orc.nlevels):
+        coarse, parent = B.coarsen_coords(fine_coords)
+        assert orc.ndof(level) == len(coarse)
+        org, mask, base = orc.topology(level)
+        bits = np.unpackbits(mask.view(np.uint8), axis=1, bitorder="little")
+        got = []
+        for i in range(org.shape[0]):
+            for off in np.nonzero(bits[i])[0]:
+                lx, ly, lz = B.local(int(off))
+                got.append((int(org[i, 0]) + lx, int(org[i, 1]) + ly, int(org[i, 2]) + lz))
+        assert got == coarse
+        P = B.prolongation_matrix(parent, len(coarse))
+        Ac_expected = 0.5 * (P.T / 8.0) @ Af @ P
+        Ac = level_matrix(orc, level)
+        scale = np.abs(Ac_expected).max()
+        assert np.allclose(Ac, Ac_expected, rtol=3e-5, atol=3e-6 * scale)
+        fine_coords, Af = coarse, Ac
+
+
+def test_interior_coarse_stencil_closed_form():
+    # a solid block of liquid: interior coarse voxels carry 6 t_c / -t_c with t_c = t/4
+    org = np.array([[x, y, z] for x in (0, 8, 16) for y in (0, 8, 16) for z in (0, 8, 16)], np.int32)
+    n = org.shape[0]
+    phi = -np.ones((n, 512), np.float32)
+    w = np.ones((n, 512, 3), np.float32)
+    orc = O.OraclePoisson(G(org, phi, None, (0.5,)), G(org, w, None, (1.0, 1.0, 1.0)), DT, DX, max_coarsest=100)
+    assert orc.nlevels >= 2
+    t1 = DT / (2 * DX) ** 2
+    diag, kept = orc.coefficients(1, 0)
+    xs, _ = orc.coefficients(1, 1)
+    org1, mask1, _ = orc.topology(1)
+    centre = [i for i in range(org1.shape[0]) if tuple(org1[i]) == (0, 0, 0)][0]
+    off = (3 << 6) | (3 << 3) | 3
+    assert diag[centre, off] == pytest.approx(6 * t1, rel=1e-6)
+    assert xs[centre, off] == pytest.approx(-t1, rel=1e-6)
+
+
+@pytest.mark.parametrize("seed", SEEDS[:2])
+def test_smoothers_match_textbook(seed):
+    orc, A, coords = make(seed)
+    rng = np.random.default_rng(seed + 100)
+    for level in range(orc.nlevels):
+        M = level_matrix(orc, level)
+        n = M.shape[0]
+        x = rng.uniform(-1, 1, n).astype(np.float32)
+        b = rng.uniform(-1, 1, n).astype(np.float32)
+        d = np.diag(M)
+        inv = np.where(d != 0, 1 / np.where(d != 0, d, 1), 0)
+        scale = np.abs(M).max() * 4
+        for w in (2.2473, 6 / 7):
+            jac = x + w * inv * (b - M @ x)
+            assert np.allclose(orc.apply(level, 2, x, b, w), jac, rtol=1e-5, atol=1e-5 * scale * np.abs(inv).max())
+        assert np.allclose(orc.apply(level, 1, x, b), b - M @ x, rtol=1e-5, atol=1e-6 * scale)
+        # SPAI0: a_kk / ||a_k||^2 (init_SPAI0)
+        s = d / (M * M).sum(1)
+        assert np.allclose(orc.apply(level, 3, x, b), x + s * (b - M @ x), rtol=1e-4, atol=1e-5)
+        # red-black SOR colour passes on global (x+y+z) parity
+        org, mask, _ = orc.topology(level)
+        bits = np.unpackbits(mask.view(np.uint8), axis=1, bitorder="little")
+        par = []
+        for i in range(org.shape[0]):
+            for off in np.nonzero(bits[i])[0]:
+                lx, ly, lz = B.local(int(off))
+                par.append((int(org[i].sum()) + lx + ly + lz) % 2)
+        red = np.array(par) == 0
+        for omega, mode, sel in ((1.2, 4, red), (1.2, 5, ~red), (1.0, 4, red)):
+            off_sum = M @ x - d * x
+            upd = (1 - omega) * x + omega * inv * (b - off_sum)
+            exp = np.where(sel, upd, x)
+            assert np.allclose(orc.apply(level, mode, x, b, omega), exp, rtol=1e-5, atol=1e-5)
+        # first sweeps from a zero guess
+        assert np.allclose(orc.first_sweep(level, 0, b, 2.2473), 2.2473 * inv * b, rtol=1e-6, atol=1e-9)
+        assert np.allclose(orc.first_sweep(level, 1, b, 0), s * b, rtol=1e-4, atol=1e-9)
+        y_red = np.where(red, 1.2 * inv * b, 0.0)
+        y_black = np.where(~red, -0.2 * y_red + 1.2 * inv * (b - (M @ y_red - d * y_red)), y_red)
+        assert np.allclose(orc.first_sweep(level, 2, b, 1.2), y_black, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("seed", SEEDS[:2])
+def test_transfers_match_dense(seed):
+    orc, A, coords = make(seed)
+    rng = np.random.default_rng(seed + 7)
+    fine_coords = coords
+    for level in range(orc.nlevels - 1):
+        coarse, parent = B.coarsen_coords(fine_coords)
+        P = B.prolongation_matrix(parent, len(coarse))
+        f = rng.uniform(-1, 1, len(fine_coords)).astype(np.float32)
+        c = rng.uniform(-1, 1, len(coarse)).astype(np.float32)
+        assert np.allclose(orc.restriction(level, f), (P.T @ f) / 8, rtol=1e-6, atol=1e-7)
+        for alpha in (1.0, 0.5):
+            assert np.allclose(orc.prolongation(level, c, f, alpha), f + alpha * (P @ c), rtol=1e-6, atol=1e-7)
+        fine_coords = coarse
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_coarsest_matches_textbook_pcg(seed):
+    orc, A, coords = make(seed, max_coarsest=200)
+    L = orc.nlevels - 1
+    Mc = level_matrix(orc, L)
+    b = np.random.default_rng(seed).uniform(-1, 1, Mc.shape[0]).astype(np.float32)
+    got = orc.coarsest_solve(b)
+    exp = B.eigen_cg(Mc, b)
+    assert np.linalg.norm(got - exp) <= 2e-4 * np.linalg.norm(exp)
+
+
+def _dense_wcycle(levels, P, r, smoother_w=(2.2473, 0.8571, 0.5296), level=0):
+    """Dense restatement of muCyclePreconditioner<2,true> with n=3 scheduled Jacobi."""
+    A = levels[level]
+    if level == len(levels) - 1:
+        return B.eigen_cg(A, r)
+    d = np.diag(A)
+    inv = np.where(d != 0, 1 / np.where(d != 0, d, 1), 0)
+    x = smoother_w[0] * inv * r
+    for w in smoother_w[1:]:
+        x = x + w * inv * (r - A @ x)
+    rc = (P[level].T @ (r - A @ x)) / 8
+    xc = _dense_wcycle(levels, P, rc, smoother_w, level + 1)
+    if level + 1 < len(levels) - 1:
+        Ac = levels[level + 1]
+        dc = np.diag(Ac)
+        invc = np.where(dc != 0, 1 / np.where(dc != 0, dc, 1), 0)
+        # second visit: a full cycle started from xc (pre-smoothing not skipped)
+        xc = _cycle_from(levels, P, rc, xc, smoother_w, level + 1)
+    x = x + P[level] @ xc
+    for w in smoother_w[::-1]:
+        x = x + w * inv * (r - A @ x)
+    return x
+
+
+def _cycle_from(levels, P, r, x, smoother_w, level):
+    A = levels[level]
+    if level == len(levels) - 1:
+        return B.eigen_cg(A, r)
+    d = np.diag(A)
+    inv = np.where(d != 0, 1 / np.where(d != 0, d, 1), 0)
+    for w in smoother_w:
+        x = x + w * inv * (r - A @ x)
+    rc = (P[level].T @ (r - A @ x)) / 8
+    xc = _dense_wcycle(levels, P, rc, smoother_w, level + 1)
+    if level + 1 < len(levels) - 1:
+        xc = _cycle_from(levels, P, rc, xc, smoother_w, level + 1)
+    x = x + P[level] @ xc
+    for w in smoother_w[::-1]:
+        x = x + w * inv * (r - A @ x)
+    return x
+
+
+@pytest.mark.parametrize("seed", SEEDS[:3])
+def test_wcycle_matches_dense_restatement(seed):
+    orc, A, coords = make(seed, nleaf=8, max_coarsest=40)
+    levels, P = [], []
+    fine_coords = coords
+    for level in range(orc.nlevels):
+        levels.append(level_matrix(orc, level))
+        if level + 1 < orc.nlevels:
+            coarse, parent = B.coarsen_coords(fine_coords)
+            P.append(B.prolongation_matrix(parent, len(coarse)))
+            fine_coords = coarse
+    r = np.random.default_rng(seed).uniform(-1, 1, levels[0].shape[0]).astype(np.float32)
+    got = orc.precondition(r, 0)
+    exp = _dense_wcycle(levels, P, r.astype(np.float64))
+    assert np.linalg.norm(got - exp) <= 1e-4 * np.linalg.norm(exp)
+
+
+@pytest.mark.parametrize("seed", SEEDS[:2])
+def test_preconditioner_symmetric_when_coarsest_is_exact(seed):
+    # coarsest <= 10 DOF: 10 CG iterations solve it exactly, so M is a fixed
+    # linear operator; the mirrored smoothing schedule makes it symmetric
+    orc, A, coords = make(seed, nleaf=5, max_coarsest=8)
+    assert orc.ndof(orc.nlevels - 1) <= 10
+    n = orc.ndof(0)
+    rng = np.random.default_rng(seed)
+    for sm in (0, 1, 2, 3):
+        x = rng.uniform(-1, 1, n).astype(np.float32)
+        y = rng.uniform(-1, 1, n).astype(np.float32)
+        mx, my = orc.precondition(x, sm), orc.precondition(y, sm)
+        lhs, rhs = float(mx @ y), float(x @ my)
+        assert abs(lhs - rhs) <= 2e-4 * max(abs(lhs), abs(rhs)), f"smoother {sm}"
+        assert float(mx @ x) > 0
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_pcg_converges_to_sparse_direct_solution(seed):
+    orc, A, coords = make(seed, nleaf=8, max_coarsest=60)
+    b = np.random.default_rng(seed).uniform(-1, 1, A.shape[0]).astype(np.float32)
+    xs = np.linalg.solve(A, b.astype(np.float64))
+    for sm in (0, 1, 2, 3):
+        st, it, x = orc.solve_pcg(b, smoother=sm)
+        assert st == 0
+        assert np.linalg.norm(x - xs) <= 1e-5 * np.linalg.norm(xs)
+    # the FP64 twin solves the same float-coefficient matrix in double precision
+    M = level_matrix(orc, 0)
+    xm = np.linalg.solve(M, b.astype(np.float64))
+    st, it, x64 = orc.solve_pcg64(b.astype(np.float64), tol=1e-13)
+    assert np.linalg.norm(x64 - xm) <= 1e-9 * np.linalg.norm(xm)
+
+
+def test_pcg_stops_on_relative_infinity_norm():
+    orc, A, coords = make(9, nleaf=8, max_coarsest=60)
+    b = np.random.default_rng(0).uniform(-1, 1, A.shape[0]).astype(np.float32)
+    st, it, x, hist = orc.solve_pcg(b, tol=1e-3, history=True)
+    assert st == 0
+    assert hist[-1] <= 1e-3 * hist[0] < hist[-2]
+
+
+def test_max_iteration_and_nan_fail():
+    orc, A, coords = make(5, nleaf=8, max_coarsest=60)
+    b = np.random.default_rng(0).uniform(-1, 1, A.shape[0]).astype(np.float32)
+    st, it, _ = orc.solve_pcg(b, maxit=1)
+    assert st == 1 and it == 1
+    bb = b.copy()
+    bb[3] = np.nan
+    st, it, _ = orc.solve_pcg(bb, maxit=6)
+    assert st == 1
+
+
+def test_scene_system_matches_scipy():
+    import uaamg_amd as U
+    s = U.Scene(1, 32, seed=2)
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    b = orc.build_rhs(s.weights, s.vx, s.vy, s.vz)
+    st, it, x = orc.solve_pcg(b)
+    assert st == 0
+    st64, it64, x64 = orc.solve_pcg64(b.astype(np.float64))
+    r = b - orc.apply64(x64)
+    assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
+    assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)

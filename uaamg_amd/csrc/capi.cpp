@@ -1,0 +1,401 @@
+// C ABI over the device solver (include/uaamg_capi.h).
+#include "../../include/uaamg_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "poisson_host.h"
+#include "poisson_solver.h"
+#include "scene.h"
+
+using namespace uaamg;
+
+struct uaamg_poisson {
+    PoissonDevice dev;
+    float* dofbuf[3] = {nullptr, nullptr, nullptr};   // DOF-ordered device vectors for the host API
+    int64_t ndof = 0;
+    ~uaamg_poisson() {
+        for (auto p : dofbuf)
+            if (p) (void)hipFree(p);
+    }
+};
+
+struct uaamg_scene {
+    Scene s;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+bool to_host_grid(const uaamg_grid* g, int ncomp, HostGrid& out, const char* what) {
+    if (!g) {
+        g_err = std::string(what) + ": null grid";
+        return false;
+    }
+    if (g->ncomp != ncomp) {
+        g_err = std::string(what) + ": wrong component count";
+        return false;
+    }
+    if (!make_host_grid(g->nleaf, g->origins, g->active, g->values, g->ncomp, g->background, out)) {
+        g_err = std::string(what) + ": malformed leaves (origin not a multiple of 8, duplicate, or null arrays)";
+        return false;
+    }
+    return true;
+}
+
+int hip_status(int rc, const PoissonDevice& d) {
+    if (rc == -4) g_err = "handle was created without a device (device < 0)";
+    else if (rc < 0) g_err = d.error().empty() ? "device error" : d.error();
+    return rc;
+}
+
+#define UAAMG_GUARD_BEGIN try {
+#define UAAMG_GUARD_END                                              \
+    }                                                                \
+    catch (const std::bad_alloc&) {                                  \
+        return fail(UAAMG_ENOMEM, "host allocation failed");         \
+    }                                                                \
+    catch (const std::exception& e) {                                \
+        return fail(UAAMG_EINVAL, e.what());                         \
+    }                                                                \
+    catch (...) {                                                    \
+        return fail(UAAMG_EINVAL, "unexpected error");               \
+    }
+
+int copy_in(uaamg_poisson* h, int slot, const float* src, int64_t n) {
+    if (!h->dev.on_device()) return fail(UAAMG_ENODEV, "handle was created without a device (device < 0)");
+    if (!src) return 0;
+    hipError_t e = hipMemcpyAsync(h->dofbuf[slot], src, size_t(n) * sizeof(float), hipMemcpyHostToDevice,
+                                  h->dev.stream());
+    return e == hipSuccess ? 0 : fail(UAAMG_EHIP, hipGetErrorString(e));
+}
+int copy_out(uaamg_poisson* h, int slot, float* dst, int64_t n) {
+    hipError_t e = hipMemcpyAsync(dst, h->dofbuf[slot], size_t(n) * sizeof(float), hipMemcpyDeviceToHost,
+                                  h->dev.stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(h->dev.stream());
+    return e == hipSuccess ? 0 : fail(UAAMG_EHIP, hipGetErrorString(e));
+}
+int64_t level_ndof(const uaamg_poisson* h, int level) { return h->dev.host().levels[level].ndof; }
+}  // namespace
+
+extern "C" {
+
+const char* uaamg_last_error(void) { return g_err.c_str(); }
+
+const char* uaamg_version(void) { return "uaamg_amd 0.1 (gfx950)"; }
+
+int uaamg_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int uaamg_options_default(uaamg_options* o) {
+    if (!o) return fail(UAAMG_EINVAL, "null options");
+    std::memset(o, 0, sizeof(*o));
+    o->relative_tolerance = 1e-7f;
+    o->max_iterations = 100;
+    o->smoother = UAAMG_SMOOTHER_SCHEDULED_RELAXED_JACOBI;
+    o->precision = 0;
+    o->device = 0;
+    o->max_coarsest_dof = 4000;
+    o->stream = nullptr;
+    return 0;
+}
+
+int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
+                         const uaamg_options* opts, uaamg_poisson** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out) return fail(UAAMG_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!(dx > 0.f) || !(dt > 0.f)) return fail(UAAMG_EINVAL, "dt and dx must be positive");
+    uaamg_options o;
+    uaamg_options_default(&o);
+    if (opts) o = *opts;
+    if (o.smoother < 0 || o.smoother > 3) return fail(UAAMG_EINVAL, "unknown smoother");
+    if (o.max_iterations < 0) return fail(UAAMG_EINVAL, "max_iterations < 0");
+    if (o.precision != 0) return fail(UAAMG_EINVAL, "unsupported precision");
+    const bool host_only = o.device < 0;   // hierarchy inspection without a GPU; solves fail
+    if (!host_only && uaamg_device_count() <= o.device) return fail(UAAMG_ENODEV, "no HIP device");
+    HostGrid phi, w;
+    if (!to_host_grid(liquid_phi, 1, phi, "liquid_phi")) return UAAMG_EINVAL;
+    if (!to_host_grid(face_weights, 3, w, "face_weights")) return UAAMG_EINVAL;
+    PoissonHierarchyHost hier;
+    build_hierarchy(phi, w, dt, dx, o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000, hier);
+    if (hier.levels[0].ndof == 0) return fail(UAAMG_EINVAL, "no degrees of freedom");
+    std::unique_ptr<uaamg_poisson> h(new uaamg_poisson());
+    SolverOptions so;
+    so.relative_tolerance = o.relative_tolerance;
+    so.max_iterations = o.max_iterations;
+    so.smoother = o.smoother;
+    so.device = o.device;
+    so.max_coarsest_dof = o.max_coarsest_dof;
+    so.stream = static_cast<hipStream_t>(o.stream);
+    if (host_only) {
+        h->dev.init_host_only(std::move(hier), so);
+        h->ndof = h->dev.host().levels[0].ndof;
+        *out = h.release();
+        return 0;
+    }
+    int rc = h->dev.init(std::move(hier), so);
+    if (rc < 0) return hip_status(rc, h->dev);
+    h->ndof = h->dev.host().levels[0].ndof;
+    for (int i = 0; i < 3; ++i) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&h->dofbuf[i]), size_t(h->ndof) * sizeof(float));
+        if (e != hipSuccess) return fail(UAAMG_ENOMEM, hipGetErrorString(e));
+    }
+    *out = h.release();
+    return 0;
+    UAAMG_GUARD_END
+}
+
+void uaamg_poisson_destroy(uaamg_poisson* h) { delete h; }
+
+int uaamg_poisson_set_options(uaamg_poisson* h, const uaamg_options* o) {
+    if (!h || !o) return fail(UAAMG_EINVAL, "null argument");
+    if (o->smoother < 0 || o->smoother > 3) return fail(UAAMG_EINVAL, "unknown smoother");
+    if (o->max_iterations < 0) return fail(UAAMG_EINVAL, "max_iterations < 0");
+    h->dev.options().relative_tolerance = o->relative_tolerance;
+    h->dev.options().max_iterations = o->max_iterations;
+    h->dev.options().smoother = o->smoother;
+    return 0;
+}
+
+int64_t uaamg_poisson_ndof(const uaamg_poisson* h) { return h ? h->ndof : -1; }
+
+int uaamg_poisson_nlevels(const uaamg_poisson* h) { return h ? h->dev.nlevels() : -1; }
+
+int uaamg_poisson_level_info(const uaamg_poisson* h, int level, int64_t* ndof, int64_t* nleaf_dof,
+                             int64_t* nleaf_tree) {
+    if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
+    const auto& l = h->dev.host().levels[level];
+    if (ndof) *ndof = l.ndof;
+    if (nleaf_dof) *nleaf_dof = int64_t(l.nleaf());
+    if (nleaf_tree) *nleaf_tree = int64_t(l.full_origin.size());
+    return 0;
+}
+
+int uaamg_poisson_level_topology(const uaamg_poisson* h, int level, int32_t* origins, uint64_t* masks,
+                                 int64_t* dofbase) {
+    if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
+    const auto& l = h->dev.host().levels[level];
+    for (size_t i = 0; i < l.nleaf(); ++i) {
+        if (origins) {
+            origins[3 * i] = l.origin[i].x;
+            origins[3 * i + 1] = l.origin[i].y;
+            origins[3 * i + 2] = l.origin[i].z;
+        }
+        if (dofbase) dofbase[i] = l.dofbase[i];
+    }
+    if (masks && !l.mask.empty()) std::memcpy(masks, l.mask.data(), l.mask.size() * sizeof(uint64_t));
+    return 0;
+}
+
+int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind, float* bricks, uint8_t* kept) {
+    if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
+    if (kind < 0 || kind >= kNumCoef) return fail(UAAMG_EINVAL, "bad coefficient kind");
+    const auto& l = h->dev.host().levels[level];
+    if (bricks && !l.coef[kind].empty())
+        std::memcpy(bricks, l.coef[kind].data(), l.coef[kind].size() * sizeof(float));
+    if (kept && !l.keep[kind].empty()) std::memcpy(kept, l.keep[kind].data(), l.keep[kind].size());
+    return 0;
+}
+
+int uaamg_poisson_build_rhs(const uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                            const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
+                            float* rhs) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !rhs) return fail(UAAMG_EINVAL, "null argument");
+    HostGrid w, gx, gy, gz, sv;
+    if (!to_host_grid(face_weights, 3, w, "face_weights")) return UAAMG_EINVAL;
+    if (!to_host_grid(vx, 1, gx, "vx") || !to_host_grid(vy, 1, gy, "vy") || !to_host_grid(vz, 1, gz, "vz"))
+        return UAAMG_EINVAL;
+    if (solid_velocity) {
+        if (!to_host_grid(solid_velocity, 3, sv, "solid_velocity")) return UAAMG_EINVAL;
+    } else {
+        sv.ncomp = 3;
+        sv.lookup.build(sv.origin);
+    }
+    build_poisson_rhs(h->dev.host().levels[0], w, gx, gy, gz, sv, rhs);
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_poisson_solve_pcg_device(uaamg_poisson* h, float* x_dev, const float* b_dev, int32_t* iterations) {
+    if (!h || !x_dev || !b_dev) return fail(UAAMG_EINVAL, "null argument");
+    if (!h->dev.on_device()) return fail(UAAMG_ENODEV, "handle was created without a device (device < 0)");
+    int it = 0;
+    int rc = h->dev.solve_pcg_dev(x_dev, b_dev, &it);
+    if (iterations) *iterations = it;
+    return hip_status(rc, h->dev);
+}
+
+int uaamg_poisson_solve_pcg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations) {
+    if (!h || !x || !b) return fail(UAAMG_EINVAL, "null argument");
+    if (int rc = copy_in(h, 0, x, h->ndof)) return rc;
+    if (int rc = copy_in(h, 1, b, h->ndof)) return rc;
+    int it = 0;
+    int rc = h->dev.solve_pcg_dev(h->dofbuf[0], h->dofbuf[1], &it);
+    if (iterations) *iterations = it;
+    if (rc < 0) return hip_status(rc, h->dev);
+    if (int rc2 = copy_out(h, 0, x, h->ndof)) return rc2;
+    return rc;
+}
+
+int uaamg_poisson_solve_mg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations) {
+    if (!h || !x || !b) return fail(UAAMG_EINVAL, "null argument");
+    if (int rc = copy_in(h, 0, x, h->ndof)) return rc;
+    if (int rc = copy_in(h, 1, b, h->ndof)) return rc;
+    int it = 0;
+    int rc = h->dev.solve_mg_dev(h->dofbuf[0], h->dofbuf[1], &it);
+    if (iterations) *iterations = it;
+    if (rc < 0) return hip_status(rc, h->dev);
+    if (int rc2 = copy_out(h, 0, x, h->ndof)) return rc2;
+    return rc;
+}
+
+int uaamg_poisson_apply(uaamg_poisson* h, int level, int mode, float weight, const float* x, const float* b,
+                        float* y) {
+    if (!h || !x || !y || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad argument");
+    if (mode < 0 || mode > 5) return fail(UAAMG_EINVAL, "bad mode");
+    if (mode != UAAMG_MODE_LAPLACIAN && !b) return fail(UAAMG_EINVAL, "mode needs b");
+    const int64_t n = level_ndof(h, level);
+    if (int rc = copy_in(h, 0, x, n)) return rc;
+    if (int rc = copy_in(h, 1, b, n)) return rc;
+    int rc = h->dev.apply_dev(level, mode, weight, h->dofbuf[0], b ? h->dofbuf[1] : nullptr, h->dofbuf[2]);
+    if (rc < 0) return hip_status(rc, h->dev);
+    return copy_out(h, 2, y, n);
+}
+
+int uaamg_poisson_first_sweep(uaamg_poisson* h, int level, int kind, float weight, const float* b, float* y) {
+    if (!h || !b || !y || level < 0 || level >= h->dev.nlevels() || kind < 0 || kind > 2)
+        return fail(UAAMG_EINVAL, "bad argument");
+    const int64_t n = level_ndof(h, level);
+    if (int rc = copy_in(h, 1, b, n)) return rc;
+    int rc = h->dev.first_sweep_dev(level, kind, weight, h->dofbuf[1], h->dofbuf[2]);
+    if (rc < 0) return hip_status(rc, h->dev);
+    return copy_out(h, 2, y, n);
+}
+
+int uaamg_poisson_restrict(uaamg_poisson* h, int fine_level, const float* fine, float* coarse) {
+    if (!h || !fine || !coarse || fine_level < 0 || fine_level + 1 >= h->dev.nlevels())
+        return fail(UAAMG_EINVAL, "bad argument");
+    if (int rc = copy_in(h, 0, fine, level_ndof(h, fine_level))) return rc;
+    int rc = h->dev.restrict_dev(fine_level, h->dofbuf[0], h->dofbuf[2]);
+    if (rc < 0) return hip_status(rc, h->dev);
+    return copy_out(h, 2, coarse, level_ndof(h, fine_level + 1));
+}
+
+int uaamg_poisson_prolong(uaamg_poisson* h, int fine_level, const float* coarse, float* fine, float alpha) {
+    if (!h || !fine || !coarse || fine_level < 0 || fine_level + 1 >= h->dev.nlevels())
+        return fail(UAAMG_EINVAL, "bad argument");
+    if (int rc = copy_in(h, 1, coarse, level_ndof(h, fine_level + 1))) return rc;
+    if (int rc = copy_in(h, 0, fine, level_ndof(h, fine_level))) return rc;
+    int rc = h->dev.prolong_dev(fine_level, h->dofbuf[1], h->dofbuf[0], alpha);
+    if (rc < 0) return hip_status(rc, h->dev);
+    return copy_out(h, 0, fine, level_ndof(h, fine_level));
+}
+
+int uaamg_poisson_precondition(uaamg_poisson* h, const float* r, float* z) {
+    if (!h || !r || !z) return fail(UAAMG_EINVAL, "null argument");
+    if (int rc = copy_in(h, 1, r, h->ndof)) return rc;
+    int rc = h->dev.precondition_dev(h->dofbuf[1], h->dofbuf[2]);
+    if (rc < 0) return hip_status(rc, h->dev);
+    return copy_out(h, 2, z, h->ndof);
+}
+
+int uaamg_poisson_coarsest_solve(uaamg_poisson* h, const float* b, float* x) {
+    if (!h || !b || !x) return fail(UAAMG_EINVAL, "null argument");
+    const int l = h->dev.nlevels() - 1;
+    const int64_t n = level_ndof(h, l);
+    if (int rc = copy_in(h, 1, b, n)) return rc;
+    int rc = h->dev.coarsest_dev(h->dofbuf[1], h->dofbuf[2]);
+    if (rc < 0) return hip_status(rc, h->dev);
+    return copy_out(h, 2, x, n);
+}
+
+int uaamg_poisson_algorithmic_bytes(const uaamg_poisson* h, int level, int mode, double* bytes) {
+    if (!h || !bytes || level < 0 || level >= h->dev.nlevels() || mode < 0 || mode > 5)
+        return fail(UAAMG_EINVAL, "bad argument");
+    *bytes = h->dev.algorithmic_bytes(level, mode);
+    return 0;
+}
+
+int uaamg_poisson_time_apply(uaamg_poisson* h, int level, int mode, int reps, float* ms_per_launch) {
+    if (!h || !ms_per_launch) return fail(UAAMG_EINVAL, "null argument");
+    if (!h->dev.on_device()) return fail(UAAMG_ENODEV, "handle was created without a device (device < 0)");
+    int rc = h->dev.time_apply(level, mode, reps, ms_per_launch);
+    if (rc == -1) return fail(UAAMG_EINVAL, "bad argument");
+    return hip_status(rc, h->dev);
+}
+
+// ------------------------------------------------------------------ scenes
+int uaamg_scene_create(int config, int32_t n, uint64_t seed, uaamg_scene** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out) return fail(UAAMG_EINVAL, "null output");
+    SceneSpec spec;
+    if (!scene_preset(config, n, seed, spec)) return fail(UAAMG_EINVAL, "unknown scene config");
+    std::unique_ptr<uaamg_scene> s(new uaamg_scene());
+    build_scene(spec, s->s);
+    *out = s.release();
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_scene_create_custom(int32_t n, float pool, int32_t nspheres, const float* spheres, int32_t has_solid,
+                              const float* solid_sphere, uint64_t velocity_seed, uaamg_scene** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out || n <= 0 || nspheres < 0 || (nspheres > 0 && !spheres)) return fail(UAAMG_EINVAL, "bad argument");
+    SceneSpec spec;
+    spec.n = n;
+    spec.pool = pool;
+    spec.spheres.assign(spheres, spheres + 4 * nspheres);
+    spec.has_solid = has_solid != 0;
+    if (solid_sphere)
+        for (int i = 0; i < 4; ++i) spec.solid[i] = solid_sphere[i];
+    spec.velocity_seed = velocity_seed;
+    std::unique_ptr<uaamg_scene> s(new uaamg_scene());
+    build_scene(spec, s->s);
+    *out = s.release();
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_scene_grids(const uaamg_scene* sc, uaamg_grid* phi, uaamg_grid* weights, uaamg_grid* vx, uaamg_grid* vy,
+                      uaamg_grid* vz, float* dx, int32_t* n) {
+    if (!sc) return fail(UAAMG_EINVAL, "null scene");
+    const Scene& s = sc->s;
+    auto fill = [&](uaamg_grid* g, const float* values, int ncomp, float bg) {
+        if (!g) return;
+        g->nleaf = s.nleaf();
+        g->origins = s.origins.data();
+        g->active = s.active.data();
+        g->values = values;
+        g->ncomp = ncomp;
+        g->background[0] = g->background[1] = g->background[2] = bg;
+    };
+    fill(phi, s.phi.data(), 1, s.phi_background);
+    fill(weights, s.weights.data(), 3, 0.f);
+    fill(vx, s.vel[0].data(), 1, 0.f);
+    fill(vy, s.vel[1].data(), 1, 0.f);
+    fill(vz, s.vel[2].data(), 1, 0.f);
+    if (dx) *dx = s.dx;
+    if (n) *n = s.spec.n;
+    return 0;
+}
+
+void uaamg_scene_destroy(uaamg_scene* s) { delete s; }
+
+float uaamg_fraction_inside2(float a, float b) { return fraction_inside(a, b); }
+float uaamg_fraction_inside4(float bl, float br, float tl, float tr) { return fraction_inside(bl, br, tl, tr); }
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+// Sparse 8^3 leaf-brick topology on the host.
+//
+// A grid is a set of 8x8x8 leaves, each with an origin that is a multiple of 8,
+// a 512-bit active mask and 512 values per component. Inside a leaf the voxel
+// offset is 64*x + 8*y + z (z fastest), which is the layout of OpenVDB's
+// LeafNode<T,3> buffer (simd_vdb_poisson.cpp:750-751 relies on it).
+//
+// Leaves are kept in the traversal order of OpenVDB's Tree4<T,5,4,3>: root
+// children (4096^3 blocks) sorted lexicographically by their signed origin,
+// then the 32^3 child index of the upper internal node (x-major), then the
+// 16^3 child index of the lower internal node (x-major). LeafManager visits
+// leaves in this order, and the DOF numbering of setDofIndex
+// (simd_vdb_poisson.cpp:637-671) is a prefix scan in this order.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+namespace uaamg {
+
+constexpr int kLeafDim = 8;
+constexpr int kLeafSize = 512;
+
+struct Coord {
+    int32_t x, y, z;
+    Coord operator+(const Coord& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    bool operator==(const Coord& o) const { return x == o.x && y == o.y && z == o.z; }
+    int32_t operator[](int c) const { return c == 0 ? x : (c == 1 ? y : z); }
+    int32_t& operator[](int c) { return c == 0 ? x : (c == 1 ? y : z); }
+};
+
+// floor division by 8 / origin of the leaf containing a voxel
+inline int32_t leaf_floor(int32_t v) { return v & ~int32_t(7); }
+inline Coord leaf_origin_of(const Coord& v) { return {leaf_floor(v.x), leaf_floor(v.y), leaf_floor(v.z)}; }
+inline int voxel_offset(int lx, int ly, int lz) { return (lx << 6) | (ly << 3) | lz; }
+
+// Sort key reproducing Tree4<T,5,4,3> leaf traversal order.
+struct TreeOrderKey {
+    int32_t rx, ry, rz;   // root child origin (multiple of 4096, signed)
+    int32_t upper;        // child index inside the 32^3 internal node
+    int32_t lower;        // child index inside the 16^3 internal node
+    bool operator<(const TreeOrderKey& o) const {
+        if (rx != o.rx) return rx < o.rx;
+        if (ry != o.ry) return ry < o.ry;
+        if (rz != o.rz) return rz < o.rz;
+        if (upper != o.upper) return upper < o.upper;
+        return lower < o.lower;
+    }
+};
+
+inline TreeOrderKey tree_order_key(const Coord& origin) {
+    TreeOrderKey k;
+    k.rx = origin.x & ~int32_t(4095);
+    k.ry = origin.y & ~int32_t(4095);
+    k.rz = origin.z & ~int32_t(4095);
+    k.upper = (((origin.x & 4095) >> 7) << 10) | (((origin.y & 4095) >> 7) << 5) | ((origin.z & 4095) >> 7);
+    k.lower = (((origin.x & 127) >> 3) << 8) | (((origin.y & 127) >> 3) << 4) | ((origin.z & 127) >> 3);
+    return k;
+}
+
+inline bool tree_order_less(const Coord& a, const Coord& b) { return tree_order_key(a) < tree_order_key(b); }
+
+inline uint64_t pack_origin(const Coord& o) {
+    // origins are multiples of 8; 21 bits per axis after the shift covers +-2^23 voxels
+    const uint64_t m = (uint64_t(1) << 21) - 1;
+    return ((uint64_t(uint32_t(o.x >> 3)) & m) << 42) | ((uint64_t(uint32_t(o.y >> 3)) & m) << 21) |
+           (uint64_t(uint32_t(o.z >> 3)) & m);
+}
+
+// Origin -> position lookup for a leaf list.
+class LeafLookup {
+public:
+    void build(const std::vector<Coord>& origins) {
+        map_.clear();
+        map_.reserve(origins.size() * 2 + 1);
+        for (size_t i = 0; i < origins.size(); ++i) map_.emplace(pack_origin(origins[i]), int32_t(i));
+    }
+    int32_t find(const Coord& origin) const {
+        auto it = map_.find(pack_origin(origin));
+        return it == map_.end() ? -1 : it->second;
+    }
+
+private:
+    std::unordered_map<uint64_t, int32_t> map_;
+};
+
+inline bool mask_test(const uint64_t* mask8, int offset) { return (mask8[offset >> 6] >> (offset & 63)) & 1u; }
+inline void mask_set(uint64_t* mask8, int offset) { mask8[offset >> 6] |= uint64_t(1) << (offset & 63); }
+inline int mask_count(const uint64_t* mask8) {
+    int n = 0;
+    for (int w = 0; w < 8; ++w) n += __builtin_popcountll(mask8[w]);
+    return n;
+}
+
+// A host copy of one flattened sparse grid (FloatGrid or Vec3fGrid), leaves in
+// tree order. value() follows ValueAccessor::getValue: the stored value when
+// the leaf exists (active or not), otherwise the background.
+struct HostGrid {
+    int ncomp = 1;
+    float background[3] = {0.f, 0.f, 0.f};
+    std::vector<Coord> origin;
+    std::vector<uint64_t> active;   // [nleaf][8]
+    std::vector<float> values;      // [nleaf][512][ncomp] (Vec3f leaves are AoS)
+    LeafLookup lookup;
+
+    size_t nleaf() const { return origin.size(); }
+    const float* leaf_values(size_t i) const { return values.data() + i * kLeafSize * ncomp; }
+
+    float value(const Coord& v, int comp = 0) const {
+        int32_t li = lookup.find(leaf_origin_of(v));
+        if (li < 0) return background[comp];
+        int off = voxel_offset(v.x & 7, v.y & 7, v.z & 7);
+        return values[(size_t(li) * kLeafSize + off) * ncomp + comp];
+    }
+};
+
+// Build a tree-ordered HostGrid from caller arrays in any leaf order.
+// Returns false on malformed input (origin not a multiple of 8, duplicates).
+bool make_host_grid(int64_t nleaf, const int32_t* origins, const uint64_t* active, const float* values,
+                    int ncomp, const float* background, HostGrid& out);
+
+}  // namespace uaamg

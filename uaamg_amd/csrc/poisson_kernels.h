@@ -1,0 +1,117 @@
+// Device kernels of the UAAMG-PCG Poisson solve on 8^3 leaf bricks (gfx950).
+//
+// Layout in HBM, per level: a level vector is [nleaf][512] values in leaf
+// (tree) order with voxel offset 64x+8y+z; entries that are not DOFs hold 0.
+// Every kernel maps one wavefront to one leaf and one lane to one z-row
+// (x,y) = (lane>>3, lane&7) of 8 contiguous values (32 bytes in FP32), so a
+// wavefront reads and writes its brick as one contiguous 2 KiB block.
+// Coefficient bricks exist only for leaves the reference keeps after
+// trimming; slot[leaf][k] = -1 selects the default stencil constant.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace uaamg {
+
+// slot indices per leaf
+enum { kSlotDiag = 0, kSlotX = 1, kSlotY = 2, kSlotZ = 3, kSlotSpai0 = 4, kSlotStride = 8 };
+
+// smoothing / apply modes of LaplacianApplySIMD (simd_vdb_poisson.cpp:683-690)
+enum ApplyMode : int {
+    kModeLaplacian = 0,
+    kModeResidual = 1,
+    kModeJacobi = 2,
+    kModeSpai0 = 3,
+    kModeRedGS = 4,
+    kModeBlackGS = 5,
+};
+
+struct LevelView {
+    int32_t nleaf;
+    int32_t level;            // 0 selects the level-0 kernel instantiations (separate profile names)
+    float term;
+    const uint64_t* mask;     // [nleaf][8]
+    const int32_t* nbr;       // [nleaf][6]  -x,+x,-y,+y,-z,+z
+    const int32_t* slot;      // [nleaf][kSlotStride]
+    const int64_t* dofbase;   // [nleaf]
+    const int32_t* parent;    // [nleaf] coarse leaf * 8 + octant (fine side of a transfer)
+    const int32_t* child;     // [nleaf][8] fine leaf per octant (coarse side of a transfer)
+    const float* diag;        // [nslot][512]
+    const float* invdiag;     // [nslot][512], same slots as diag
+    const float* cx;
+    const float* cy;
+    const float* cz;
+    const float* spai0;
+    float dflt_diag, dflt_off, dflt_invdiag, dflt_spai0;
+};
+
+// Scalars of the outer PCG, kept in device memory so a whole iteration runs
+// without host round trips except for the convergence test.
+struct PcgScalars {
+    double rho;        // <z, r> of the current direction
+    double sigma;      // <p, A p>
+    double rho_new;
+    double dot_tmp;
+    float alpha;
+    float beta;
+    float nu;          // ||r||_inf (NaN if any entry is not finite)
+    float pad;
+    unsigned int counter[8];   // completion counters of the reduction kernels
+};
+
+// Compact coarsest-level system: per DOF the brick position, 6 neighbour DOFs
+// and the 7 stencil values in the operator's summation order.
+struct CoarsestView {
+    int32_t ndof;
+    const int32_t* pos;       // [ndof] leaf*512 + offset
+    const int32_t* nb;        // [ndof][6]: +x,-x,+y,-y,+z,-z DOF index or -1
+    const float* coef;        // [ndof][8]: diag, c+x, c-x, c+y, c-y, c+z, c-z, invdiag (Eigen's 1/diag or 1)
+    int32_t max_iters;        // 10 (constructCoarsestLevelExactSolver)
+    float tol;                // Eigen default: float epsilon
+};
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlockThreads = 64 * kWavesPerBlock;
+constexpr int kMaxCoarsestDof = 4096;
+
+// ---- launchers (stream-ordered, no host synchronisation) ----
+// y = op(x, b) on every DOF leaf of a level; w = Jacobi / SOR weight.
+hipError_t launch_apply(const LevelView& L, int mode, const float* x, const float* b, float* y, float w,
+                        hipStream_t s);
+// first sweep from a zero guess: 0 = Jacobi (y = w*D^-1 b), 1 = SPAI0, 2 = red SOR part of RBGS
+hipError_t launch_first_sweep(const LevelView& L, int kind, const float* b, float* y, float w, hipStream_t s);
+// coarse = 1/8 * sum of on fine children (restriction, simd_vdb_poisson.cpp:1624-1684)
+hipError_t launch_restrict(const LevelView& coarse, const float* fine_v, float* coarse_v, hipStream_t s);
+// fine += alpha * coarse on on fine voxels (prolongation<true>, :1686-1754)
+hipError_t launch_prolong_add(const LevelView& fine, const float* coarse_v, float* fine_v, float alpha,
+                              hipStream_t s);
+// coarsest 10-iteration Jacobi-PCG from zero, one workgroup
+hipError_t launch_coarsest_cg(const CoarsestView& C, const float* b, float* x, hipStream_t s);
+
+// DOF-ordered vector <-> bricks
+hipError_t launch_scatter_dof(const LevelView& L, const float* dofvec, float* bricks, hipStream_t s);
+hipError_t launch_gather_dof(const LevelView& L, const float* bricks, float* dofvec, hipStream_t s);
+
+// outer PCG vector kernels on level 0 (partials: >= reduction_blocks(L) doubles)
+int reduction_blocks(const LevelView& L);
+// which scalar a dot kernel finalises
+enum DotTarget { kDotRho = 0, kDotRhoNew = 1 };
+hipError_t launch_dot(const LevelView& L, const float* a, const float* b, PcgScalars* sc, double* partials,
+                      int target, hipStream_t s);
+// z = A p and sigma = <p, z>; finalises alpha = rho / sigma
+hipError_t launch_laplacian_dot(const LevelView& L, const float* p, float* z, PcgScalars* sc, double* partials,
+                                hipStream_t s);
+// r -= alpha z ; nu = ||r||_inf
+hipError_t launch_axpy_absmax(const LevelView& L, const float* z, float* r, PcgScalars* sc, double* partials,
+                              hipStream_t s);
+// nu = ||r||_inf only
+hipError_t launch_absmax(const LevelView& L, const float* r, PcgScalars* sc, double* partials, hipStream_t s);
+// x += alpha p ; p = z + beta p
+hipError_t launch_update_xp(const LevelView& L, const float* z, float* x, float* p, const PcgScalars* sc,
+                            hipStream_t s);
+// x += alpha p only (final step)
+hipError_t launch_update_x(const LevelView& L, const float* p, float* x, const PcgScalars* sc, hipStream_t s);
+
+}  // namespace uaamg

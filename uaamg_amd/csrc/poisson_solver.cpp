@@ -1,0 +1,616 @@
+// Device-resident UAAMG-PCG Poisson solver (see poisson_solver.h).
+#include "poisson_solver.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+namespace uaamg {
+
+namespace {
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// weights of the scheduled relaxed Jacobi smoother for n = 3 sweeps
+// (muCyclePreconditioner, simd_vdb_poisson.cpp:1848-1866)
+float scheduled_weight(int n, int i) {
+    if (i >= n) return 6.0f / 7.0f;
+    if (n == 1) return 6.0f / 7.0f;
+    if (n == 2) return i == 0 ? 1.7319f : 0.5695f;
+    static const float w3[3] = {2.2473f, 0.8571f, 0.5296f};
+    return w3[i];
+}
+// muCycleIterative's schedule (:1982-1997): defaults 6/7, n >= 3 uses the 3-entry table
+float scheduled_weight_iterative(int n, int i) {
+    if (i >= n) return 6.0f / 7.0f;
+    if (n == 2) return i == 0 ? 1.7319f : 0.5695f;
+    if (n >= 3) {
+        static const float w3[3] = {2.2473f, 0.8571f, 0.5296f};
+        return i < 3 ? w3[i] : 6.0f / 7.0f;
+    }
+    return 6.0f / 7.0f;
+}
+constexpr int kSweeps = 3;   // n = 3 pre/post sweeps (solveMultigridPCG :2181,2212)
+}  // namespace
+
+DeviceArena::~DeviceArena() {
+    if (base_) (void)hipFree(base_);
+}
+
+hipError_t DeviceArena::reserve(size_t bytes) {
+    if (base_) return hipErrorInvalidValue;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), bytes);
+    if (e != hipSuccess) {
+        base_ = nullptr;
+        return e;
+    }
+    cap_ = bytes;
+    used_ = 0;
+    return hipSuccess;
+}
+
+void* DeviceArena::take(size_t bytes) {
+    size_t b = align_up(bytes);
+    if (used_ + b > cap_) return nullptr;
+    void* p = base_ + used_;
+    used_ += b;
+    return p;
+}
+
+PoissonDevice::~PoissonDevice() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+    if (sc_host_) (void)hipHostFree(sc_host_);
+}
+
+int PoissonDevice::check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (first_error_ == hipSuccess) {
+            first_error_ = e;
+            err_ = std::string(what) + ": " + hipGetErrorString(e);
+        }
+        return -3;
+    }
+    return 0;
+}
+
+int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
+    host_ = std::move(h);
+    opt_ = opt;
+    if (host_.levels.empty()) {
+        err_ = "empty hierarchy";
+        return -1;
+    }
+    if (check(hipSetDevice(opt_.device), "hipSetDevice")) return -3;
+    if (opt_.stream) {
+        stream_ = opt_.stream;
+    } else {
+        if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return -3;
+        own_stream_ = true;
+    }
+    build_spai0(host_);
+    const int L = int(host_.levels.size());
+    sor_w_.assign(L, 1.2f);
+
+    // size the arena
+    size_t bytes = 0;
+    auto add = [&](size_t b) { bytes += align_up(b); };
+    const size_t n0 = host_.levels[0].nleaf();
+    for (int l = 0; l < L; ++l) {
+        const auto& hl = host_.levels[l];
+        const size_t n = hl.nleaf();
+        add(n * 8 * sizeof(uint64_t));
+        add(n * 6 * sizeof(int32_t));
+        add(n * kSlotStride * sizeof(int32_t));
+        add(n * sizeof(int64_t));
+        add(n * sizeof(int32_t));         // parent
+        add(n * 8 * sizeof(int32_t));     // child
+        size_t nslot[kNumCoef] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < kNumCoef; ++k)
+            for (size_t i = 0; i < n; ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
+        for (int k = 0; k < kNumCoef; ++k) add(nslot[k] * kLeafSize * sizeof(float));
+        add(nslot[kCoefDiag] * kLeafSize * sizeof(float));   // invdiag
+        add(3 * n * kLeafSize * sizeof(float));               // X, B, T
+    }
+    add(8 * n0 * kLeafSize * sizeof(float));   // x0 b0 r0 p0 z0 + 3 unit buffers
+    add(3 * std::max<int64_t>(1, host_.levels[0].ndof) * sizeof(float));
+    add(sizeof(PcgScalars));
+    add((size_t(reduction_blocks(LevelView{int32_t(n0)})) + 16) * sizeof(double));
+    const auto& hc = host_.levels.back();
+    add(size_t(hc.ndof) * (sizeof(int32_t) * 7 + sizeof(float) * 8) + 1024);
+    if (check(arena_.reserve(bytes), "hipMalloc")) return -2;
+
+    lv_.resize(L);
+    for (int l = 0; l < L; ++l)
+        if (int rc = upload_level(host_.levels[l], lv_[l], true)) return rc;
+    // parent table of each finer level from the child table of the coarser one
+    for (int l = 0; l + 1 < L; ++l) {
+        const auto& coarse = host_.levels[l + 1];
+        std::vector<int32_t> parent(host_.levels[l].nleaf(), -1);
+        for (size_t c = 0; c < coarse.nleaf(); ++c)
+            for (int b = 0; b < 8; ++b) {
+                int32_t f = coarse.child[c * 8 + b];
+                if (f >= 0) parent[f] = int32_t(c * 8 + b);
+            }
+        if (!parent.empty() && check(hipMemcpyAsync(const_cast<int32_t*>(lv_[l].v.parent), parent.data(),
+                                                    parent.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
+                                     "upload parent"))
+            return -3;
+        // the host vector dies at the end of this iteration
+        if (check(hipStreamSynchronize(stream_), "upload parent")) return -3;
+    }
+    auto vec = [&](size_t nleaf) { return static_cast<float*>(arena_.take(nleaf * kLeafSize * sizeof(float))); };
+    x0_ = vec(n0);
+    b0_ = vec(n0);
+    r0_ = vec(n0);
+    p0_ = vec(n0);
+    z0_ = vec(n0);
+    for (int i = 0; i < 3; ++i) staging_[i] = vec(n0);   // unit-op bricks
+    sc_ = static_cast<PcgScalars*>(arena_.take(sizeof(PcgScalars)));
+    partials_ = static_cast<double*>(arena_.take((size_t(reduction_blocks(lv_[0].v)) + 16) * sizeof(double)));
+    if (!x0_ || !sc_ || !partials_) {
+        err_ = "arena too small";
+        return -2;
+    }
+    if (check(hipMemsetAsync(x0_, 0, 8 * n0 * kLeafSize * sizeof(float), stream_), "memset")) return -3;
+    if (check(hipMemsetAsync(sc_, 0, sizeof(PcgScalars), stream_), "memset")) return -3;
+    if (check(hipHostMalloc(reinterpret_cast<void**>(&sc_host_), sizeof(PcgScalars), hipHostMallocDefault),
+              "hipHostMalloc"))
+        return -3;
+    if (int rc = build_coarsest()) return rc;
+    if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
+    return 0;
+}
+
+void PoissonDevice::init_host_only(PoissonHierarchyHost&& h, const SolverOptions& opt) {
+    host_ = std::move(h);
+    opt_ = opt;
+    build_spai0(host_);
+}
+
+int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spai0) {
+    const size_t n = h.nleaf();
+    LevelView& v = d.v;
+    v.nleaf = int32_t(n);
+    v.level = h.level;
+    v.term = h.term;
+    v.dflt_diag = h.default_coef(kCoefDiag);
+    v.dflt_off = h.default_coef(kCoefX);
+    v.dflt_invdiag = h.default_invdiag();
+    v.dflt_spai0 = h.default_spai0_apply();
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        void* p = arena_.take(std::max<size_t>(bytes, 4));
+        if (p && bytes) (void)check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload");
+        return p;
+    };
+    v.mask = static_cast<const uint64_t*>(up(h.mask.data(), n * 8 * sizeof(uint64_t)));
+    v.nbr = static_cast<const int32_t*>(up(h.nbr.data(), n * 6 * sizeof(int32_t)));
+    std::vector<int32_t> slot(n * kSlotStride, -1);
+    int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < kNumCoef; ++k)
+            if (h.keep[k][i]) slot[i * kSlotStride + k] = count[k]++;
+    v.slot = static_cast<const int32_t*>(up(slot.data(), slot.size() * sizeof(int32_t)));
+    v.dofbase = static_cast<const int64_t*>(up(h.dofbase.data(), n * sizeof(int64_t)));
+    v.parent = static_cast<const int32_t*>(arena_.take(std::max<size_t>(n * sizeof(int32_t), 4)));
+    if (v.parent && check(hipMemsetAsync(const_cast<int32_t*>(v.parent), 0xff, std::max<size_t>(n * sizeof(int32_t), 4),
+                                         stream_),
+                          "memset"))
+        return -3;
+    if (!h.child.empty()) {
+        v.child = static_cast<const int32_t*>(up(h.child.data(), n * 8 * sizeof(int32_t)));
+    } else {
+        v.child = static_cast<const int32_t*>(arena_.take(std::max<size_t>(n * 8 * sizeof(int32_t), 4)));
+        if (v.child && check(hipMemsetAsync(const_cast<int32_t*>(v.child), 0xff,
+                                            std::max<size_t>(n * 8 * sizeof(int32_t), 4), stream_),
+                             "memset"))
+            return -3;
+    }
+    // coefficient bricks of kept leaves only
+    auto pack = [&](const std::vector<float>& src, int kind, const std::vector<uint8_t>& keep) -> const float* {
+        std::vector<float> buf(size_t(count[kind]) * kLeafSize);
+        size_t j = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (keep[i]) {
+                std::memcpy(&buf[j * kLeafSize], &src[i * kLeafSize], kLeafSize * sizeof(float));
+                ++j;
+            }
+        const float* p = static_cast<const float*>(up(buf.data(), buf.size() * sizeof(float)));
+        // the host buffer must outlive the async copy
+        (void)check(hipStreamSynchronize(stream_), "upload sync");
+        return p;
+    };
+    v.diag = pack(h.coef[kCoefDiag], kCoefDiag, h.keep[kCoefDiag]);
+    v.invdiag = pack(h.invdiag, kCoefDiag, h.keep[kCoefDiag]);
+    v.cx = pack(h.coef[kCoefX], kCoefX, h.keep[kCoefX]);
+    v.cy = pack(h.coef[kCoefY], kCoefY, h.keep[kCoefY]);
+    v.cz = pack(h.coef[kCoefZ], kCoefZ, h.keep[kCoefZ]);
+    if (spai0) v.spai0 = pack(h.coef[kCoefSpai0], kCoefSpai0, h.keep[kCoefSpai0]);
+    d.X = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
+    d.B = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
+    d.T = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
+    if (!d.X || !d.B || !d.T) {
+        err_ = "arena too small";
+        return -2;
+    }
+    if (n) {
+        if (check(hipMemsetAsync(d.X, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
+        if (check(hipMemsetAsync(d.B, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
+        if (check(hipMemsetAsync(d.T, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
+    }
+    if (check(hipStreamSynchronize(stream_), "upload sync")) return -3;
+    return first_error_ == hipSuccess ? 0 : -3;
+}
+
+int PoissonDevice::build_coarsest() {
+    const PoissonLevelHost& l = host_.levels.back();
+    const int64_t nd = l.ndof;
+    if (nd > kMaxCoarsestDof) {
+        err_ = "coarsest level exceeds the single-workgroup solver size";
+        return -1;
+    }
+    std::vector<int32_t> pos(nd), nb(nd * 6, -1);
+    std::vector<float> coef(nd * 8, 0.f);
+    // DOF index lookup on the coarsest level
+    auto dof_at = [&](const Coord& g) -> int64_t {
+        int32_t li = l.lookup.find(leaf_origin_of(g));
+        if (li < 0) return -1;
+        const int off = voxel_offset(g.x & 7, g.y & 7, g.z & 7);
+        const uint64_t* m = &l.mask[size_t(li) * 8];
+        if (!mask_test(m, off)) return -1;
+        int64_t k = l.dofbase[li];
+        for (int o = 0; o < off; ++o) k += mask_test(m, o);
+        return k;
+    };
+    auto coef_at = [&](int kind, const Coord& g) -> float {
+        int32_t li = l.lookup.find(leaf_origin_of(g));
+        if (li < 0) return l.default_coef(kind);
+        return l.coef[kind][size_t(li) * kLeafSize + voxel_offset(g.x & 7, g.y & 7, g.z & 7)];
+    };
+    for (size_t i = 0; i < l.nleaf(); ++i) {
+        int64_t k = l.dofbase[i];
+        const Coord o = l.origin[i];
+        for (int off = 0; off < kLeafSize; ++off) {
+            if (!mask_test(&l.mask[i * 8], off)) continue;
+            const Coord g{o.x + (off >> 6), o.y + ((off >> 3) & 7), o.z + (off & 7)};
+            pos[k] = int32_t(i * kLeafSize + off);
+            const float dg = l.coef[kCoefDiag][i * kLeafSize + off];
+            coef[k * 8 + 0] = dg;
+            for (int c = 0; c < 3; ++c) {
+                Coord up = g, dn = g;
+                up[c] += 1;
+                dn[c] -= 1;
+                nb[k * 6 + 2 * c] = int32_t(dof_at(up));
+                nb[k * 6 + 2 * c + 1] = int32_t(dof_at(dn));
+                coef[k * 8 + 1 + 2 * c] = coef_at(kCoefX + c, up);
+                coef[k * 8 + 2 + 2 * c] = coef_at(kCoefX + c, g);
+            }
+            coef[k * 8 + 7] = dg != 0.f ? 1.0f / dg : 1.0f;
+            ++k;
+        }
+    }
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        void* p = arena_.take(std::max<size_t>(bytes, 4));
+        if (p && bytes) (void)check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload");
+        return p;
+    };
+    cv_.ndof = int32_t(nd);
+    cv_.pos = static_cast<const int32_t*>(up(pos.data(), pos.size() * sizeof(int32_t)));
+    cv_.nb = static_cast<const int32_t*>(up(nb.data(), nb.size() * sizeof(int32_t)));
+    cv_.coef = static_cast<const float*>(up(coef.data(), coef.size() * sizeof(float)));
+    cv_.max_iters = 10;
+    cv_.tol = FLT_EPSILON;
+    if (check(hipStreamSynchronize(stream_), "coarsest upload")) return -3;
+    return 0;
+}
+
+void PoissonDevice::smooth(int l, float*& cur, float*& oth, const float* B, int i, bool pre, bool iterative) {
+    const LevelView& v = lv_[l].v;
+    const int n = kSweeps;
+    switch (opt_.smoother) {
+        default:
+        case kScheduledRelaxedJacobi: {
+            const int idx = pre ? i : n - 1 - i;
+            const float w = iterative ? scheduled_weight_iterative(n, idx) : scheduled_weight(n, idx);
+            check(launch_apply(v, kModeJacobi, cur, B, oth, w, stream_), "jacobi");
+            std::swap(cur, oth);
+            break;
+        }
+        case kWeightedJacobi:
+            check(launch_apply(v, kModeJacobi, cur, B, oth, 6.0f / 7.0f, stream_), "jacobi");
+            std::swap(cur, oth);
+            break;
+        case kRedBlackGaussSeidel: {
+            if (iterative) sor_w_[l] = 1.0f;
+            const float w = sor_w_[l];
+            const int first = pre ? kModeRedGS : kModeBlackGS, second = pre ? kModeBlackGS : kModeRedGS;
+            check(launch_apply(v, first, cur, B, cur, w, stream_), "rbgs");
+            check(launch_apply(v, second, cur, B, cur, w, stream_), "rbgs");
+            break;
+        }
+        case kSpai0:
+            check(launch_apply(v, kModeSpai0, cur, B, oth, 0.f, stream_), "spai0");
+            std::swap(cur, oth);
+            break;
+    }
+}
+
+// muCyclePreconditioner<2, skip_first> (simd_vdb_poisson.cpp:1844-1977)
+void PoissonDevice::precond(int l, float* X, const float* B, bool skip_first) {
+    const int L = nlevels();
+    if (l == L - 1) {
+        check(launch_coarsest_cg(cv_, B, X, stream_), "coarsest");
+        return;
+    }
+    const LevelView& v = lv_[l].v;
+    float* cur = X;
+    float* oth = lv_[l].T;
+    int start = 0;
+    if (skip_first) {
+        switch (opt_.smoother) {
+            default:
+            case kScheduledRelaxedJacobi:
+                check(launch_first_sweep(v, 0, B, oth, scheduled_weight(kSweeps, 0), stream_), "first");
+                std::swap(cur, oth);
+                break;
+            case kWeightedJacobi:
+                check(launch_first_sweep(v, 0, B, oth, 6.0f / 7.0f, stream_), "first");
+                std::swap(cur, oth);
+                break;
+            case kRedBlackGaussSeidel:
+                check(launch_first_sweep(v, 2, B, cur, sor_w_[l], stream_), "first");
+                check(launch_apply(v, kModeBlackGS, cur, B, cur, sor_w_[l], stream_), "first black");
+                break;
+            case kSpai0:
+                check(launch_first_sweep(v, 1, B, oth, 0.f, stream_), "first");
+                std::swap(cur, oth);
+                break;
+        }
+        start = 1;
+    }
+    for (int i = start; i < kSweeps; ++i) smooth(l, cur, oth, B, i, true, false);
+    check(launch_apply(v, kModeResidual, cur, B, oth, 0.f, stream_), "residual");
+    DevLevel& c = lv_[l + 1];
+    check(launch_restrict(c.v, oth, c.B, stream_), "restrict");
+    precond(l + 1, c.X, c.B, true);
+    // mu = 2: the second visit of the coarsest level repeats an identical
+    // zero-guess solve, so it is skipped; other levels cycle again
+    if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false);
+    check(launch_prolong_add(v, c.X, cur, 1.0f, stream_), "prolong");
+    for (int i = 0; i < kSweeps; ++i) smooth(l, cur, oth, B, i, false, false);
+}
+
+// muCycleIterative<2> (simd_vdb_poisson.cpp:1979-2110)
+void PoissonDevice::mg_cycle(int l, float* X, const float* B) {
+    const int L = nlevels();
+    float* cur = X;
+    float* oth = lv_[l].T;
+    if (l == L - 1) {
+        for (int i = 0; i < 10 * kSweeps; ++i) {
+            if (opt_.smoother == kScheduledRelaxedJacobi) {
+                check(launch_apply(lv_[l].v, kModeJacobi, cur, B, oth, scheduled_weight_iterative(kSweeps, i),
+                                   stream_),
+                      "jacobi");
+                std::swap(cur, oth);
+            } else {
+                smooth(l, cur, oth, B, 0, true, true);
+            }
+        }
+        return;
+    }
+    const LevelView& v = lv_[l].v;
+    for (int i = 0; i < kSweeps; ++i) smooth(l, cur, oth, B, i, true, true);
+    check(launch_apply(v, kModeResidual, cur, B, oth, 0.f, stream_), "residual");
+    DevLevel& c = lv_[l + 1];
+    check(launch_restrict(c.v, oth, c.B, stream_), "restrict");
+    for (int mu = 0; mu < 2; ++mu) mg_cycle(l + 1, c.X, c.B);
+    check(launch_prolong_add(v, c.X, cur, 0.5f, stream_), "prolong");
+    for (int i = 0; i < kSweeps; ++i) smooth(l, cur, oth, B, i, false, true);
+}
+
+int PoissonDevice::read_nu(float* nu) {
+    if (check(hipMemcpyAsync(&sc_host_->nu, &sc_->nu, sizeof(float), hipMemcpyDeviceToHost, stream_), "read nu"))
+        return -3;
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    *nu = sc_host_->nu;
+    return 0;
+}
+
+// solveMultigridPCG (simd_vdb_poisson.cpp:2155-2226)
+int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations) {
+    if (!on_device()) return -4;
+    if (first_error_ != hipSuccess) return -3;
+    const LevelView& v0 = lv_[0].v;
+    int it = 0;
+    if (iterations) *iterations = 0;
+    check(launch_scatter_dof(v0, x_dof, x0_, stream_), "scatter x");
+    check(launch_scatter_dof(v0, b_dof, b0_, stream_), "scatter b");
+    check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
+    check(launch_absmax(v0, r0_, sc_, partials_, stream_), "absmax");
+    float nu;
+    if (read_nu(&nu)) return -3;
+    const float numax = opt_.relative_tolerance * nu;
+    int status = kFailed;
+    if (nu <= numax) {
+        status = kSuccess;
+    } else {
+        precond(0, p0_, r0_, true);
+        check(launch_dot(v0, p0_, r0_, sc_, partials_, kDotRho, stream_), "dot");
+        float nu_old = nu;
+        for (; it < opt_.max_iterations; ++it) {
+            check(launch_laplacian_dot(v0, p0_, z0_, sc_, partials_, stream_), "A p");
+            check(launch_axpy_absmax(v0, z0_, r0_, sc_, partials_, stream_), "axpy");
+            nu_old = nu;
+            if (read_nu(&nu)) return -3;
+            if (nu <= numax) {
+                check(launch_update_x(v0, p0_, x0_, sc_, stream_), "update x");
+                status = kSuccess;
+                break;
+            }
+            if (nu > nu_old && it > 3) {
+                status = kFailed;
+                break;
+            }
+            precond(0, z0_, r0_, true);
+            check(launch_dot(v0, z0_, r0_, sc_, partials_, kDotRhoNew, stream_), "dot");
+            check(launch_update_xp(v0, z0_, x0_, p0_, sc_, stream_), "update");
+        }
+    }
+    if (iterations) *iterations = it;
+    check(launch_gather_dof(v0, x0_, x_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "solve sync")) return -3;
+    return first_error_ == hipSuccess ? status : -3;
+}
+
+// solvePureMultigrid (simd_vdb_poisson.cpp:2228-2267)
+int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iterations) {
+    if (!on_device()) return -4;
+    if (first_error_ != hipSuccess) return -3;
+    const LevelView& v0 = lv_[0].v;
+    int it = 0;
+    if (iterations) *iterations = 0;
+    check(launch_scatter_dof(v0, x_dof, x0_, stream_), "scatter x");
+    check(launch_scatter_dof(v0, b_dof, b0_, stream_), "scatter b");
+    check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
+    check(launch_absmax(v0, r0_, sc_, partials_, stream_), "absmax");
+    float nu;
+    if (read_nu(&nu)) return -3;
+    const float numax = opt_.relative_tolerance * nu;
+    int status = kFailed;
+    if (nu <= numax) {
+        status = kSuccess;
+    } else {
+        for (; it < opt_.max_iterations; ++it) {
+            mg_cycle(0, x0_, b0_);
+            check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
+            check(launch_absmax(v0, r0_, sc_, partials_, stream_), "absmax");
+            const float nu_old = nu;
+            if (read_nu(&nu)) return -3;
+            if (nu <= numax) {
+                status = kSuccess;
+                break;
+            }
+            if (nu > nu_old && it > 8) {
+                status = kFailed;
+                break;
+            }
+        }
+    }
+    if (iterations) *iterations = it;
+    check(launch_gather_dof(v0, x0_, x_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "solve sync")) return -3;
+    return first_error_ == hipSuccess ? status : -3;
+}
+
+int PoissonDevice::apply_dev(int l, int mode, float w, const float* x_dof, const float* b_dof, float* y_dof) {
+    if (l < 0 || l >= nlevels()) return -1;
+    const LevelView& v = lv_[l].v;
+    float *xb = staging_[0], *bb = staging_[1], *yb = staging_[2];
+    check(launch_scatter_dof(v, x_dof, xb, stream_), "scatter");
+    if (b_dof) check(launch_scatter_dof(v, b_dof, bb, stream_), "scatter");
+    float* out = yb;
+    if (mode == kModeRedGS || mode == kModeBlackGS) out = xb;   // in place
+    check(launch_apply(v, mode, xb, bb, out, w, stream_), "apply");
+    check(launch_gather_dof(v, out, y_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    return 0;
+}
+
+int PoissonDevice::first_sweep_dev(int l, int kind, float w, const float* b_dof, float* y_dof) {
+    if (l < 0 || l >= nlevels() || kind < 0 || kind > 2) return -1;
+    const LevelView& v = lv_[l].v;
+    float *bb = staging_[1], *yb = staging_[2];
+    check(launch_scatter_dof(v, b_dof, bb, stream_), "scatter");
+    check(launch_first_sweep(v, kind, bb, yb, w, stream_), "first");
+    if (kind == 2) check(launch_apply(v, kModeBlackGS, yb, bb, yb, w, stream_), "black");
+    check(launch_gather_dof(v, yb, y_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    return 0;
+}
+
+int PoissonDevice::restrict_dev(int fl, const float* fine_dof, float* coarse_dof) {
+    if (fl < 0 || fl + 1 >= nlevels()) return -1;
+    check(launch_scatter_dof(lv_[fl].v, fine_dof, staging_[0], stream_), "scatter");
+    check(launch_restrict(lv_[fl + 1].v, staging_[0], staging_[1], stream_), "restrict");
+    check(launch_gather_dof(lv_[fl + 1].v, staging_[1], coarse_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    return 0;
+}
+
+int PoissonDevice::prolong_dev(int fl, const float* coarse_dof, float* fine_dof, float alpha) {
+    if (fl < 0 || fl + 1 >= nlevels()) return -1;
+    check(launch_scatter_dof(lv_[fl + 1].v, coarse_dof, staging_[1], stream_), "scatter");
+    check(launch_scatter_dof(lv_[fl].v, fine_dof, staging_[0], stream_), "scatter");
+    check(launch_prolong_add(lv_[fl].v, staging_[1], staging_[0], alpha, stream_), "prolong");
+    check(launch_gather_dof(lv_[fl].v, staging_[0], fine_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    return 0;
+}
+
+int PoissonDevice::precondition_dev(const float* r_dof, float* z_dof) {
+    const LevelView& v0 = lv_[0].v;
+    check(launch_scatter_dof(v0, r_dof, r0_, stream_), "scatter");
+    precond(0, z0_, r0_, true);
+    check(launch_gather_dof(v0, z0_, z_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    return 0;
+}
+
+double PoissonDevice::algorithmic_bytes(int l, int mode) const {
+    if (l < 0 || l >= nlevels()) return 0.0;
+    const PoissonLevelHost& h = host_.levels[l];
+    double total = 0.0;
+    for (size_t i = 0; i < h.nleaf(); ++i) {
+        const double nd = mask_count(&h.mask[i * 8]);
+        const int kd = h.keep[kCoefDiag][i], kx = h.keep[kCoefX][i], ky = h.keep[kCoefY][i], kz = h.keep[kCoefZ][i];
+        const int ks = h.keep[kCoefSpai0].empty() ? 0 : h.keep[kCoefSpai0][i];
+        const double c = 4.0 * (kd + kx + ky + kz), d = 4.0 * kd;
+        double per = 0;
+        switch (mode) {
+            case kModeLaplacian: per = 8 + c; break;
+            case kModeResidual: per = 12 + c; break;
+            case kModeJacobi: per = 12 + c + d; break;
+            case kModeSpai0: per = 12 + c + 4.0 * ks; break;
+            default: per = 12 + 4.0 * (kx + ky + kz) + d; break;   // one GS colour pass
+        }
+        total += nd * per;
+    }
+    return total;
+}
+
+int PoissonDevice::time_apply(int l, int mode, int reps, float* ms) {
+    if (!on_device() || l < 0 || l >= nlevels() || reps <= 0 || mode < 0 || mode > 5) return -1;
+    const LevelView& v = lv_[l].v;
+    const float* X = l == 0 ? x0_ : lv_[l].X;
+    const float* B = l == 0 ? b0_ : lv_[l].B;
+    float* Y = staging_[2];
+    hipEvent_t e0, e1;
+    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -3;
+    check(launch_apply(v, mode, X, B, Y, 0.8571f, stream_), "warm");
+    check(hipEventRecord(e0, stream_), "event");
+    for (int i = 0; i < reps; ++i) check(launch_apply(v, mode, X, B, Y, 0.8571f, stream_), "apply");
+    check(hipEventRecord(e1, stream_), "event");
+    check(hipEventSynchronize(e1), "event sync");
+    float total = 0.f;
+    check(hipEventElapsedTime(&total, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (first_error_ != hipSuccess) return -3;
+    *ms = total / float(reps);
+    return 0;
+}
+
+int PoissonDevice::coarsest_dev(const float* b_dof, float* x_dof) {
+    const int l = nlevels() - 1;
+    const LevelView& v = lv_[l].v;
+    check(launch_scatter_dof(v, b_dof, staging_[1], stream_), "scatter");
+    check(hipMemsetAsync(staging_[2], 0, size_t(v.nleaf) * kLeafSize * sizeof(float), stream_), "memset");
+    check(launch_coarsest_cg(cv_, staging_[1], staging_[2], stream_), "coarsest");
+    check(launch_gather_dof(v, staging_[2], x_dof, stream_), "gather");
+    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
+    return 0;
+}
+
+}  // namespace uaamg

@@ -1,0 +1,120 @@
+// Device-resident UAAMG-PCG Poisson solver: owns the level hierarchy in HBM and
+// drives the W-cycle preconditioner and the outer PCG through the kernels of
+// poisson_kernels.hip. Mirrors PoissonSolver (simd_vdb_poisson.h:139-205).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "poisson_host.h"
+#include "poisson_kernels.h"
+
+namespace uaamg {
+
+enum SmootherOption { kScheduledRelaxedJacobi = 0, kWeightedJacobi = 1, kRedBlackGaussSeidel = 2, kSpai0 = 3 };
+
+struct SolverOptions {
+    float relative_tolerance = 1e-7f;
+    int max_iterations = 100;
+    int smoother = kScheduledRelaxedJacobi;
+    int precision = 0;
+    int device = 0;
+    int64_t max_coarsest_dof = 4000;
+    hipStream_t stream = nullptr;
+};
+
+// One device allocation carved into aligned pieces.
+class DeviceArena {
+public:
+    ~DeviceArena();
+    hipError_t reserve(size_t bytes);
+    void* take(size_t bytes);
+    size_t used() const { return used_; }
+    size_t capacity() const { return cap_; }
+
+private:
+    char* base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+};
+
+class PoissonDevice {
+public:
+    static constexpr int kSuccess = 0;
+    static constexpr int kFailed = 1;
+
+    ~PoissonDevice();
+    // returns 0 or a negative uaamg status
+    int init(PoissonHierarchyHost&& h, const SolverOptions& opt);
+    // keeps the host hierarchy only (inspection without a GPU); solves fail
+    void init_host_only(PoissonHierarchyHost&& h, const SolverOptions& opt);
+    bool on_device() const { return !lv_.empty(); }
+
+    int nlevels() const { return int(host_.levels.size()); }
+    const PoissonHierarchyHost& host() const { return host_; }
+    SolverOptions& options() { return opt_; }
+    hipStream_t stream() const { return stream_; }
+    const std::string& error() const { return err_; }
+
+    // Solves with DOF-ordered vectors in device memory. x is the initial guess
+    // on entry and the solution on exit. Returns kSuccess/kFailed or < 0.
+    int solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations);
+    int solve_mg_dev(float* x_dof, const float* b_dof, int* iterations);
+
+    // unit operations on DOF-ordered device vectors (parity tests)
+    int apply_dev(int level, int mode, float w, const float* x_dof, const float* b_dof, float* y_dof);
+    int first_sweep_dev(int level, int kind, float w, const float* b_dof, float* y_dof);
+    int restrict_dev(int fine_level, const float* fine_dof, float* coarse_dof);
+    int prolong_dev(int fine_level, const float* coarse_dof, float* fine_dof, float alpha);
+    int precondition_dev(const float* r_dof, float* z_dof);
+    int coarsest_dev(const float* b_dof, float* x_dof);
+
+    // scratch DOF-ordered device vectors of level-0 size (host API staging)
+    float* staging(int i) { return staging_[i]; }
+
+    // Bytes one sweep of `mode` on `level` must move at minimum: every stream
+    // read or written once per DOF, coefficient streams only for leaves that
+    // keep stored coefficients (trimmed leaves read constants).
+    double algorithmic_bytes(int level, int mode) const;
+    // Mean duration of `reps` back-to-back launches of one operator sweep on
+    // the solver's stream, timed with HIP events on that stream.
+    int time_apply(int level, int mode, int reps, float* ms_per_launch);
+
+private:
+    struct DevLevel {
+        LevelView v{};
+        float* X = nullptr;
+        float* B = nullptr;
+        float* T = nullptr;
+        float* U = nullptr;   // extra brick vector for unit operations
+        std::vector<float> sor_weight;
+    };
+
+    int upload_level(const PoissonLevelHost& h, DevLevel& d, bool spai0);
+    int build_coarsest();
+    void precond(int level, float* X, const float* B, bool skip_first);
+    void mg_cycle(int level, float* X, const float* B);
+    void smooth(int level, float*& cur, float*& oth, const float* B, int sweep, bool pre, bool iterative);
+    int check(hipError_t e, const char* what);
+    int read_nu(float* nu);
+
+    PoissonHierarchyHost host_;
+    SolverOptions opt_;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    DeviceArena arena_;
+    std::vector<DevLevel> lv_;
+    std::vector<float> sor_w_;
+    // level-0 PCG bricks
+    float *x0_ = nullptr, *b0_ = nullptr, *r0_ = nullptr, *p0_ = nullptr, *z0_ = nullptr;
+    float* staging_[3] = {nullptr, nullptr, nullptr};
+    PcgScalars* sc_ = nullptr;
+    PcgScalars* sc_host_ = nullptr;   // pinned mirror
+    double* partials_ = nullptr;
+    CoarsestView cv_{};
+    hipError_t first_error_ = hipSuccess;
+    std::string err_;
+};
+
+}  // namespace uaamg
