@@ -41,6 +41,9 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="build the workload, then only run the level-0 Jacobi sweep --kernel-reps times and a "
+                         "256 MiB device copy 10 times (short run for hardware-counter passes)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -73,6 +76,17 @@ def main() -> int:
     b = torch.from_numpy(b_host).to(dev)
     x = torch.zeros_like(b)
     iters = None
+    if args.kernel_only:
+        rc, iters = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+        ms = lap.time_apply(0, 2, reps=args.kernel_reps)
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel_only": True, "jacobi_ms": ms, "bytes_per_launch": lap.algorithmic_bytes(0, 2),
+                          "copy_bytes_each_way": src.numel() * 4, "copies": 10, "iters": iters}), flush=True)
+        return 0
     for _ in range(args.warmup):
         x.zero_()
         rc, iters = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
@@ -105,6 +119,8 @@ def main() -> int:
                 "bytes_per_launch": jac_bytes, "ms_per_launch": round(jac_ms, 5)}
 
     info = [lap.level_info(l) for l in range(lap.nlevels)]
+    # one Jacobi sweep per level, back-to-back launches (the coarse levels show launch latency)
+    level_sweep_us = [round(1e3 * lap.time_apply(l, 2, reps=100), 2) for l in range(lap.nlevels)]
     out = {
         "metric": METRIC,
         "value": round(ms_per_solve, 3),
@@ -128,6 +144,7 @@ def main() -> int:
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "pcg_iterations": iters,
+        "level_sweep_us": level_sweep_us,
         "build_s": round(t_build, 3),
         "scene_s": round(t_scene, 3),
         "roofline": roofline,

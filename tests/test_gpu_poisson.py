@@ -169,3 +169,23 @@ def test_nan_rhs_fails(problem):
     bb[len(bb) // 2] = np.nan
     x = lap.getZeroVectorGrid()
     assert solver.solveMultigridPCG(x, bb) == U.FAILED
+
+
+@pytest.mark.parametrize("smoother", [0, 1, 2, 3])
+def test_fused_and_graph_paths_bit_identical(problem, smoother):
+    """The fused W-cycle kernels and graph replay reorder launches, not arithmetic."""
+    s, lap, orc, b = problem
+    variants = {}
+    for flags in (0, 1, 2, 3):
+        l2 = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, flags=flags)
+        solver = U.PoissonSolver(l2)
+        solver.mSmoother = smoother
+        z = solver.precondition(b)
+        x = l2.getZeroVectorGrid()
+        st = solver.solveMultigridPCG(x, b)
+        variants[flags] = (z, x, st, solver.mIterationTaken)
+    z0, x0, st0, it0 = variants[0]
+    for flags, (z, x, st, it) in variants.items():
+        assert np.array_equal(z, z0), f"flags {flags}: W-cycle differs"
+        assert st == st0 and it == it0, f"flags {flags}: {st}/{it} vs {st0}/{it0}"
+        assert np.array_equal(x, x0), f"flags {flags}: solution differs"

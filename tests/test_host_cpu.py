@@ -1,0 +1,122 @@
+"""CPU-side checks of the product library (no GPU needed).
+
+- libuaamg_amd.so loads and exports every function include/uaamg_capi.h declares;
+- the product's host hierarchy build (run through a device=-1 handle, which
+  only inspects) equals the oracle's bit for bit: per-level leaf origins, DOF
+  masks, DOF numbering, the four stencil coefficient grids with their trimmed
+  leaves, SPAI0, and the pressure right-hand side;
+- solves on a device=-1 handle fail loudly (no CPU fallback);
+- scene face weights obey their closed forms.
+"""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+import uaamg_amd as U
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_library_exports_every_declared_symbol():
+    text = (ROOT / "include" / "uaamg_capi.h").read_text()
+    names = set(re.findall(r"\b(uaamg_[a-z0-9_]+)\s*\(", text))
+    assert len(names) > 20
+    lib = U.lib()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+CASES = [(0, 32, 0), (1, 48, 1), (1, 64, 3), (3, 0, 7), (3, 0, 11)]
+
+
+@pytest.mark.parametrize("cfg,n,seed", CASES)
+def test_host_hierarchy_bit_exact_vs_oracle(cfg, n, seed):
+    s = U.Scene(cfg, n, seed=seed)
+    lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, device=-1)
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    assert lap.nlevels == orc.nlevels
+    for l in range(lap.nlevels):
+        po, pm, pb = lap.level_topology(l)
+        oo, om, ob = orc.topology(l)
+        assert np.array_equal(po, oo) and np.array_equal(pm, om) and np.array_equal(pb, ob), f"level {l}"
+        assert lap.level_info(l)["ndof"] == orc.ndof(l)
+        assert lap.level_info(l)["nleaf_tree"] == orc.nleaf_tree(l)
+        for kind in range(5):
+            pc, pk = lap.level_coefficients(l, kind)
+            oc, ok = orc.coefficients(l, kind)
+            assert np.array_equal(pk, ok), f"level {l} kind {kind}: trimmed leaves differ"
+            assert np.array_equal(pc, oc), f"level {l} kind {kind}: coefficients differ"
+    rp = lap.createPressurePoissonRightHandSide(s.weights, s.vx, s.vy, s.vz)
+    ro = orc.build_rhs(s.weights, s.vx, s.vy, s.vz)
+    assert np.array_equal(rp, ro)
+
+
+def test_leaf_order_independent_of_input_order():
+    s = U.Scene(0, 32, seed=0)
+    perm = np.random.default_rng(0).permutation(s.phi.origins.shape[0])
+    phi2 = U.Grid(s.phi.origins[perm], s.phi.values[perm], s.phi.active[perm], s.phi.background)
+    w2 = U.Grid(s.weights.origins[perm], s.weights.values[perm], s.weights.active[perm], s.weights.background)
+    a = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, device=-1)
+    b = U.LaplacianWithLevel(phi2, w2, dt=1.0, dx=s.dx, device=-1)
+    for l in range(a.nlevels):
+        for x, y in zip(a.level_topology(l), b.level_topology(l)):
+            assert np.array_equal(x, y)
+
+
+def test_host_only_handle_refuses_to_solve():
+    s = U.Scene(0, 32, seed=0)
+    lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, device=-1)
+    b = lap.createPressurePoissonRightHandSide(s.weights, s.vx, s.vy, s.vz)
+    solver = U.PoissonSolver(lap)
+    with pytest.raises(U.UaamgError):
+        solver.solveMultigridPCG(lap.getZeroVectorGrid(), b)
+    with pytest.raises(U.UaamgError):
+        lap.laplacianApply(np.zeros(lap.mNumDof, np.float32))
+
+
+def test_bad_inputs_rejected():
+    s = U.Scene(0, 32, seed=0)
+    bad = U.Grid(s.phi.origins + 1, s.phi.values, s.phi.active, s.phi.background)
+    with pytest.raises(U.UaamgError):
+        U.LaplacianWithLevel(bad, s.weights, dt=1.0, dx=s.dx, device=-1)
+    with pytest.raises(U.UaamgError):
+        U.LaplacianWithLevel(s.phi, s.weights, dt=-1.0, dx=s.dx, device=-1)
+    with pytest.raises(U.UaamgError):   # a Vec3f grid where a float grid belongs
+        U.LaplacianWithLevel(s.weights, s.weights, dt=1.0, dx=s.dx, device=-1)
+
+
+def test_fraction_inside_product_equals_oracle():
+    lib = U.lib()
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        v = rng.uniform(-1, 1, 4).astype(np.float32)
+        assert lib.uaamg_fraction_inside4(*v) == O.fraction_inside4(*v)
+        assert lib.uaamg_fraction_inside2(*v[:2]) == O.fraction_inside2(*v[:2])
+
+
+def test_scene_face_weights_closed_forms():
+    s = U.Scene(0, 32, seed=0)   # no solid: interior faces open, wall faces closed
+    n = s.n
+    w = s.weights.values
+    for i in range(s.phi.origins.shape[0]):
+        o = s.phi.origins[i]
+        for off in (0, 7, 63, 64 * 7 + 63, 300):
+            g = (o[0] + (off >> 6), o[1] + ((off >> 3) & 7), o[2] + (off & 7))
+            for c in range(3):
+                inside = all(0 <= g[k] < n for k in range(3))
+                open_face = inside and g[c] - 1 >= 0
+                assert w[i, off, c] == (1.0 if open_face else 0.0)
+    s1 = U.Scene(1, 48, seed=1)   # solid sphere: fractional weights in [0,1], some strictly between
+    w1 = s1.weights.values
+    assert w1.min() >= 0 and w1.max() <= 1
+    assert np.any((w1 > 0) & (w1 < 1))
+
+
+def test_scene_is_deterministic():
+    a = U.Scene(3, 0, seed=5)
+    b = U.Scene(3, 0, seed=5)
+    assert a.n == b.n and 32 <= a.n <= 192
+    assert np.array_equal(a.phi.values, b.phi.values) and np.array_equal(a.vx.values, b.vx.values)

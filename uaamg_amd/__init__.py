@@ -54,6 +54,7 @@ class _Options(C.Structure):
         ("precision", C.c_int32),
         ("device", C.c_int32),
         ("max_coarsest_dof", C.c_int32),
+        ("flags", C.c_int32),
         ("stream", C.c_void_p),
     ]
 
@@ -225,12 +226,13 @@ class LaplacianWithLevel:
     """
 
     def __init__(self, liquid_phi: Grid, face_weights: Grid, dt: float, dx: float, *, device: int = 0,
-                 max_coarsest_dof: int = 4000):
+                 max_coarsest_dof: int = 4000, flags: int = 0):
         L = lib()
         o = _Options()
         L.uaamg_options_default(C.byref(o))
         o.device = device
         o.max_coarsest_dof = max_coarsest_dof
+        o.flags = flags
         h = C.c_void_p()
         gphi, gw = liquid_phi._struct(), face_weights._struct()
         _check(L.uaamg_poisson_create(C.byref(gphi), C.byref(gw), float(dt), float(dx), C.byref(o), C.byref(h)),

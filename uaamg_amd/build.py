@@ -21,6 +21,7 @@ SOURCES = [
     "scene.cpp",
     "poisson_host.cpp",
     "poisson_kernels.hip",
+    "pcg_kernels.hip",
     "poisson_solver.cpp",
     "capi.cpp",
 ]

@@ -139,6 +139,7 @@ int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_we
     so.smoother = o.smoother;
     so.device = o.device;
     so.max_coarsest_dof = o.max_coarsest_dof;
+    so.flags = o.flags;
     so.stream = static_cast<hipStream_t>(o.stream);
     if (host_only) {
         h->dev.init_host_only(std::move(hier), so);

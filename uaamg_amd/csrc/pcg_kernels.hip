@@ -1,0 +1,521 @@
+// Outer-PCG kernels (dot products, ||r||_inf, vector updates, the fused
+// PCG-boundary kernels) and the coarsest-level solve.
+//
+// Reductions are two launches: every block of the first writes one partial
+// (no fences or atomics — on a multi-XCD part an agent-scope release per block
+// writes back the L2 each time), then a one-block finalize kernel sums the
+// partials in a fixed order and updates the scalars. The kernel boundary makes
+// the partials visible once, and the fixed order makes every result
+// reproducible run to run. Dot products accumulate in FP64.
+#include <cfloat>
+#include <cstring>
+
+#include "stencil_device.h"
+
+namespace uaamg {
+namespace {
+
+using namespace dev;
+
+enum FinalizeOp { kFinRho = 0, kFinRhoNew = 1, kFinSigma = 2, kFinNu = 3 };
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Block sum in a fixed order (wave butterflies, then the waves in order);
+// the result is valid in thread 0.
+template <int kThreads>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[kThreads / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kThreads / 64; ++i) t += part[i];
+    return t;
+}
+
+__device__ __forceinline__ void write_partial(double* partials, double v) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+template <int TARGET>
+__global__ __launch_bounds__(kBlockThreads) void dot_kernel(LevelView L, const float* A, const float* Bv,
+                                                            double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    double acc = 0;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        if (rm) {
+            float a[8], b[8];
+            const size_t row = size_t(leaf) * 512 + lane * 8;
+            ld8(A + row, a);
+            ld8(Bv + row, b);
+#pragma unroll
+            for (int z = 0; z < 8; ++z)
+                if ((rm >> z) & 1u) acc += double(a[z]) * double(b[z]);
+        }
+    }
+    write_partial(partials, block_sum<kBlockThreads>(acc));
+}
+
+// z = A p and the partials of <p, z>
+__global__ __launch_bounds__(kBlockThreads) void laplacian_dot_kernel(LevelView L, const float* P, float* Z,
+                                                                      double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    double acc = 0;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        if (rm) {
+            float xc[8], s[8], dg[8], y[8];
+            stencil_offdiag(L, PlainSrc{P}, leaf, lane, xc, s);
+            ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
+#pragma unroll
+            for (int z = 0; z < 8; ++z) {
+                const bool on = (rm >> z) & 1u;
+                y[z] = on ? fmaf(xc[z], dg[z], s[z]) : 0.f;
+                if (on) acc += double(xc[z]) * double(y[z]);
+            }
+            st8(Z + size_t(leaf) * 512 + lane * 8, y);
+        }
+    }
+    write_partial(partials, block_sum<kBlockThreads>(acc));
+}
+
+// Last level-0 smoothing sweep of the W-cycle with the partials of <z, r>:
+// z is the sweep's output and r its right-hand side.
+__global__ __launch_bounds__(kBlockThreads) void jacobi_dot_kernel(LevelView L, const float* X, const float* B,
+                                                                   float* Y, float w, double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    double acc = 0;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        if (rm) {
+            float xc[8], s[8], y[8], b[8];
+            stencil_offdiag(L, PlainSrc{X}, leaf, lane, xc, s);
+            jacobi_row(L, leaf, lane, xc, s, B, w, y);
+            mask_row(rm, y);
+            st8(Y + size_t(leaf) * 512 + lane * 8, y);
+            ld8(B + size_t(leaf) * 512 + lane * 8, b);
+#pragma unroll
+            for (int z = 0; z < 8; ++z)
+                if ((rm >> z) & 1u) acc += double(y[z]) * double(b[z]);
+        }
+    }
+    write_partial(partials, block_sum<kBlockThreads>(acc));
+}
+
+// One PCG step across the iteration boundary: x += alpha p; p' = z + beta p;
+// q = A p' and the partials of <p', q>. p' goes to a second buffer because
+// neighbouring leaves still read p.
+__global__ __launch_bounds__(kBlockThreads) void update_laplacian_dot_kernel(LevelView L, const float* Z,
+                                                                             const float* P, float* Pn, float* X,
+                                                                             float* Q, const PcgScalars* sc,
+                                                                             double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    double acc = 0;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        if (rm) {
+            const float alpha = sc->alpha, beta = sc->beta;
+            const size_t row = size_t(leaf) * 512 + lane * 8;
+            float x[8], p[8];
+            ld8(X + row, x);
+            ld8(P + row, p);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((rm >> k) & 1u) {
+                    const float t = alpha * p[k];
+                    x[k] = x[k] + t;
+                }
+            st8(X + row, x);
+            float pc[8], s[8], dg[8], q[8];
+            stencil_offdiag(L, UpdateSrc{Z, P, beta}, leaf, lane, pc, s);
+            mask_row(rm, pc);
+            st8(Pn + row, pc);
+            ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
+#pragma unroll
+            for (int z = 0; z < 8; ++z) {
+                const bool on = (rm >> z) & 1u;
+                q[z] = on ? fmaf(pc[z], dg[z], s[z]) : 0.f;
+                if (on) acc += double(pc[z]) * double(q[z]);
+            }
+            st8(Q + row, q);
+        }
+    }
+    write_partial(partials, block_sum<kBlockThreads>(acc));
+}
+
+// (optional r -= alpha z, then) partial max|r| with NaN/Inf flagged as -1
+// (grid_abs_max_op, openvdb_grid_math_op.h:4-40)
+template <bool AXPY>
+__global__ __launch_bounds__(kBlockThreads) void absmax_kernel(LevelView L, const float* Zv, float* R,
+                                                               const PcgScalars* sc, double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    float m = 0.f;
+    float bad = 0.f;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        if (rm) {
+            const size_t row = size_t(leaf) * 512 + lane * 8;
+            float r[8];
+            ld8(R + row, r);
+            if (AXPY) {
+                const float na = -sc->alpha;
+                float z[8];
+                ld8(Zv + row, z);
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if ((rm >> k) & 1u) {
+                        const float t = na * z[k];
+                        r[k] = r[k] + t;
+                    }
+                st8(R + row, r);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((rm >> k) & 1u) {
+                    const float a = fabsf(r[k]);
+                    if (!(a <= FLT_MAX)) bad = 1.f;
+                    else m = fmaxf(m, a);
+                }
+        }
+    }
+    m = wave_max(m);
+    bad = wave_max(bad);
+    __shared__ float wm[kWavesPerBlock];
+    if (lane == 0) wm[threadIdx.x >> 6] = bad > 0.f ? -1.f : m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = 0.f;
+        bool b = false;
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            if (wm[i] < 0.f) b = true;
+            else v = fmaxf(v, wm[i]);
+        }
+        partials[blockIdx.x] = b ? -1.0 : double(v);
+    }
+}
+
+constexpr int kFinThreads = 256;
+
+// Fixed-order reduction of n partials and the scalar update of `op`:
+// rho / (rho_new, beta = rho_new / rho, rho) / (sigma, alpha = rho / sigma) /
+// nu, the last also published to the host mailbox as one 64-bit store.
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int n, const double* partials,
+                                                               PcgScalars* sc, HostMailbox* mb) {
+    if (op == kFinNu) {
+        __shared__ float tv[kFinThreads];
+        __shared__ int tb[kFinThreads];
+        float v = 0.f;
+        int b = 0;
+        for (int i = threadIdx.x; i < n; i += kFinThreads) {
+            const double p = partials[i];
+            if (p < 0) b = 1;
+            else v = fmaxf(v, float(p));
+        }
+        tv[threadIdx.x] = v;
+        tb[threadIdx.x] = b;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float mv = 0.f;
+            int any = 0;
+            for (int i = 0; i < kFinThreads; ++i) {
+                mv = fmaxf(mv, tv[i]);
+                any |= tb[i];
+            }
+            const float nu = any ? __builtin_nanf("") : mv;
+            const unsigned int seq = sc->seq + 1;
+            sc->nu = nu;
+            sc->seq = seq;
+            if (mb) {
+                uint32_t bits;
+                memcpy(&bits, &nu, 4);
+                mb->word = (uint64_t(seq) << 32) | bits;
+            }
+        }
+        return;
+    }
+    double t = 0;
+    for (int i = threadIdx.x; i < n; i += kFinThreads) t += partials[i];
+    const double total = block_sum<kFinThreads>(t);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case kFinRho: sc->rho = total; break;
+        case kFinRhoNew: {
+            const float rho_new = float(total), rho = float(sc->rho);
+            sc->beta = rho_new / rho;
+            sc->rho_new = total;
+            sc->rho = total;
+            break;
+        }
+        case kFinSigma:
+            sc->sigma = total;
+            sc->alpha = float(sc->rho) / float(total);
+            break;
+        default: break;
+    }
+}
+
+template <bool UPDATE_P>
+__global__ __launch_bounds__(kBlockThreads) void update_kernel(LevelView L, const float* Zv, float* X, float* P,
+                                                               const PcgScalars* sc) {
+    const int leaf = wave_leaf();
+    if (leaf >= L.nleaf) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t rm = row_mask(L, leaf, lane);
+    if (!rm) return;
+    const float alpha = sc->alpha;
+    const size_t row = size_t(leaf) * 512 + lane * 8;
+    float x[8], p[8];
+    ld8(X + row, x);
+    ld8(P + row, p);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((rm >> k) & 1u) {
+            const float t = alpha * p[k];
+            x[k] = x[k] + t;
+        }
+    st8(X + row, x);
+    if (UPDATE_P) {
+        const float beta = sc->beta;
+        float z[8];
+        ld8(Zv + row, z);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if ((rm >> k) & 1u) {
+                const float t = beta * p[k];
+                p[k] = z[k] + t;
+            }
+        st8(P + row, p);
+    }
+}
+
+// ---------------------------------------------------------- coarsest solve
+// Eigen::ConjugateGradient with DiagonalPreconditioner, zero initial guess,
+// maxIterations = 10, tolerance = float epsilon, restated
+// (constructCoarsestLevelExactSolver, simd_vdb_poisson.cpp:2114-2126). One
+// workgroup of 256 threads; each thread keeps up to 16 DOFs' matrix rows and
+// x, r, p, q in registers, and only p is shared through LDS. Per iteration two
+// block reductions: <p, Ap>, then ||r||^2 and <r, z> together (z is formed
+// right after r is updated; when ||r||^2 passes the test z is simply unused).
+constexpr int kCgThreads = 256;
+constexpr int kCgWaves = kCgThreads / 64;
+constexpr int kCgPer = kMaxCoarsestDof / kCgThreads;
+
+struct CgShared {
+    float sp[kMaxCoarsestDof];
+    float red[2][kCgWaves][2];
+};
+
+// Sums (a, b) over the block with one barrier; red alternates between two
+// buffers so no second barrier is needed before the next call.
+__device__ __forceinline__ float2 cg_sum2(float a, float b, CgShared& sh, int& parity) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh.red[parity][w][0] = a;
+        sh.red[parity][w][1] = b;
+    }
+    __syncthreads();
+    float2 t = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kCgWaves; ++i) {
+        t.x += sh.red[parity][i][0];
+        t.y += sh.red[parity][i][1];
+    }
+    parity ^= 1;
+    return t;
+}
+
+__global__ __launch_bounds__(kCgThreads) void coarsest_cg_kernel(CoarsestView C, const float* B, float* X) {
+    __shared__ CgShared sh;
+    int parity = 0;
+    const int n = C.ndof;
+    const int tid = threadIdx.x;
+    int nb[kCgPer][6];
+    float cf[kCgPer][8], x[kCgPer], r[kCgPer], p[kCgPer], q[kCgPer];
+    float part = 0.f;
+#pragma unroll
+    for (int j = 0; j < kCgPer; ++j) {
+        const int d = tid + j * kCgThreads;
+        x[j] = r[j] = p[j] = q[j] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nb[j][k] = -1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cf[j][k] = 0.f;
+        if (d < n) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nb[j][k] = C.nb[size_t(d) * 6 + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cf[j][k] = C.coef[size_t(d) * 8 + k];
+            r[j] = B[C.pos[d]];
+            part += r[j] * r[j];
+        }
+    }
+    const float rhs_norm2 = cg_sum2(part, 0.f, sh, parity).x;
+    bool done = rhs_norm2 == 0.f;
+    const float threshold = fmaxf(C.tol * C.tol * rhs_norm2, FLT_MIN);
+    if (!done && rhs_norm2 < threshold) done = true;
+    if (!done) {
+        part = 0.f;
+#pragma unroll
+        for (int j = 0; j < kCgPer; ++j) {
+            const int d = tid + j * kCgThreads;
+            if (d < n) {
+                p[j] = cf[j][7] * r[j];
+                sh.sp[d] = p[j];
+                part += r[j] * p[j];
+            }
+        }
+        float abs_new = cg_sum2(part, 0.f, sh, parity).x;   // its barrier also publishes sp
+        for (int it = 0; it < C.max_iters; ++it) {
+            part = 0.f;
+#pragma unroll
+            for (int j = 0; j < kCgPer; ++j) {
+                const int d = tid + j * kCgThreads;
+                if (d < n) {
+                    float v[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v[k] = nb[j][k] >= 0 ? sh.sp[nb[j][k]] : 0.f;
+                    const float f1 = v[0] * cf[j][1], f2 = v[1] * cf[j][2], f3 = v[2] * cf[j][3];
+                    const float f4 = v[3] * cf[j][4], f5 = v[4] * cf[j][5], f6 = v[5] * cf[j][6];
+                    const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+                    q[j] = fmaf(p[j], cf[j][0], s);
+                    part += p[j] * q[j];
+                }
+            }
+            const float alpha = abs_new / cg_sum2(part, 0.f, sh, parity).x;
+            float rr = 0.f, rz = 0.f;
+#pragma unroll
+            for (int j = 0; j < kCgPer; ++j) {
+                const int d = tid + j * kCgThreads;
+                if (d < n) {
+                    x[j] = x[j] + alpha * p[j];
+                    r[j] = r[j] - alpha * q[j];
+                    rr += r[j] * r[j];
+                    q[j] = cf[j][7] * r[j];   // z
+                    rz += r[j] * q[j];
+                }
+            }
+            const float2 sums = cg_sum2(rr, rz, sh, parity);
+            if (sums.x < threshold) break;
+            const float abs_old = abs_new;
+            abs_new = sums.y;
+            const float beta = abs_new / abs_old;
+#pragma unroll
+            for (int j = 0; j < kCgPer; ++j) {
+                const int d = tid + j * kCgThreads;
+                if (d < n) {
+                    p[j] = q[j] + beta * p[j];
+                    sh.sp[d] = p[j];
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kCgPer; ++j) {
+        const int d = tid + j * kCgThreads;
+        if (d < n) X[C.pos[d]] = x[j];
+    }
+}
+
+hipError_t finalize(int op, int n, const double* partials, PcgScalars* sc, HostMailbox* mb, hipStream_t s) {
+    finalize_kernel<<<1, kFinThreads, 0, s>>>(op, n, partials, sc, mb);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int reduction_blocks(const LevelView& L) { return int(grid_for(L.nleaf)) > 0 ? int(grid_for(L.nleaf)) : 1; }
+
+hipError_t launch_coarsest_cg(const CoarsestView& C, const float* b, float* x, hipStream_t s) {
+    if (C.ndof <= 0) return hipSuccess;
+    if (C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
+    coarsest_cg_kernel<<<1, kCgThreads, 0, s>>>(C, b, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot(const LevelView& L, const float* a, const float* b, PcgScalars* sc, double* partials,
+                      int target, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    dot_kernel<0><<<unsigned(g), kBlockThreads, 0, s>>>(L, a, b, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(target == kDotRho ? kFinRho : kFinRhoNew, g, partials, sc, nullptr, s);
+}
+
+hipError_t launch_laplacian_dot(const LevelView& L, const float* p, float* z, PcgScalars* sc, double* partials,
+                                hipStream_t s) {
+    const int g = reduction_blocks(L);
+    laplacian_dot_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, p, z, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(kFinSigma, g, partials, sc, nullptr, s);
+}
+
+hipError_t launch_axpy_absmax(const LevelView& L, const float* z, float* r, PcgScalars* sc, double* partials,
+                              HostMailbox* mb, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    absmax_kernel<true><<<unsigned(g), kBlockThreads, 0, s>>>(L, z, r, sc, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(kFinNu, g, partials, sc, mb, s);
+}
+
+hipError_t launch_absmax(const LevelView& L, const float* r, PcgScalars* sc, double* partials, HostMailbox* mb,
+                         hipStream_t s) {
+    const int g = reduction_blocks(L);
+    absmax_kernel<false><<<unsigned(g), kBlockThreads, 0, s>>>(L, nullptr, const_cast<float*>(r), sc, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(kFinNu, g, partials, sc, mb, s);
+}
+
+hipError_t launch_update_xp(const LevelView& L, const float* z, float* x, float* p, const PcgScalars* sc,
+                            hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    update_kernel<true><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, z, x, p, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_x(const LevelView& L, const float* p, float* x, const PcgScalars* sc, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    update_kernel<false><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, nullptr, x, const_cast<float*>(p), sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_dot(const LevelView& L, const float* x, const float* b, float* y, float w, PcgScalars* sc,
+                             double* partials, int target, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    jacobi_dot_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, x, b, y, w, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(target == kDotRho ? kFinRho : kFinRhoNew, g, partials, sc, nullptr, s);
+}
+
+hipError_t launch_update_laplacian_dot(const LevelView& L, const float* z, const float* p, float* p_new, float* x,
+                                       float* q, PcgScalars* sc, double* partials, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    update_laplacian_dot_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, z, p, p_new, x, q, sc, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(kFinSigma, g, partials, sc, nullptr, s);
+}
+
+}  // namespace uaamg

@@ -57,8 +57,16 @@ struct PcgScalars {
     float alpha;
     float beta;
     float nu;          // ||r||_inf (NaN if any entry is not finite)
-    float pad;
+    unsigned int seq;  // number of completed ||r||_inf reductions
     unsigned int counter[8];   // completion counters of the reduction kernels
+};
+
+// Host-mapped pinned mailbox: each ||r||_inf reduction publishes
+// (seq << 32 | bits of nu) here in one 64-bit store, so the host polls one
+// word without a stream synchronisation and cannot see a torn pair.
+struct alignas(64) HostMailbox {
+    volatile uint64_t word;
+    uint64_t pad[7];
 };
 
 // Compact coarsest-level system: per DOF the brick position, 6 neighbour DOFs
@@ -105,12 +113,29 @@ hipError_t launch_laplacian_dot(const LevelView& L, const float* p, float* z, Pc
                                 hipStream_t s);
 // r -= alpha z ; nu = ||r||_inf
 hipError_t launch_axpy_absmax(const LevelView& L, const float* z, float* r, PcgScalars* sc, double* partials,
-                              hipStream_t s);
+                              HostMailbox* mb, hipStream_t s);
 // nu = ||r||_inf only
-hipError_t launch_absmax(const LevelView& L, const float* r, PcgScalars* sc, double* partials, hipStream_t s);
+hipError_t launch_absmax(const LevelView& L, const float* r, PcgScalars* sc, double* partials, HostMailbox* mb,
+                         hipStream_t s);
 // x += alpha p ; p = z + beta p
 hipError_t launch_update_xp(const LevelView& L, const float* z, float* x, float* p, const PcgScalars* sc,
                             hipStream_t s);
+// ---- fused forms used by the W-cycle and the PCG loop (same arithmetic as the
+// unfused kernels, fewer passes over HBM and fewer launches)
+// first two Jacobi sweeps from a zero guess: y = J_w1(w0 D^-1 b)
+hipError_t launch_jacobi_first2(const LevelView& L, const float* b, float* y, float w0, float w1, hipStream_t s);
+// y = J_w(x + P x_coarse)
+hipError_t launch_prolong_jacobi(const LevelView& L, const float* x, const float* xc, const float* b, float* y,
+                                 float w, hipStream_t s);
+// coarse_b = R (b - A x)
+hipError_t launch_residual_restrict(const LevelView& fine, const float* x, const float* b, float* coarse_b,
+                                    hipStream_t s);
+// y = J_w(x) and <y, b> finalised into rho (kDotRho) or rho_new / beta (kDotRhoNew)
+hipError_t launch_jacobi_dot(const LevelView& L, const float* x, const float* b, float* y, float w, PcgScalars* sc,
+                             double* partials, int target, hipStream_t s);
+// x += alpha p; p_new = z + beta p; q = A p_new; sigma; alpha = rho / sigma
+hipError_t launch_update_laplacian_dot(const LevelView& L, const float* z, const float* p, float* p_new, float* x,
+                                       float* q, PcgScalars* sc, double* partials, hipStream_t s);
 // x += alpha p only (final step)
 hipError_t launch_update_x(const LevelView& L, const float* p, float* x, const PcgScalars* sc, hipStream_t s);
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -61,8 +62,105 @@ void* DeviceArena::take(size_t bytes) {
 
 PoissonDevice::~PoissonDevice() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    destroy_graphs();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
     if (sc_host_) (void)hipHostFree(sc_host_);
+    if (mailbox_) (void)hipHostFree(mailbox_);
+}
+
+void PoissonDevice::destroy_graphs() {
+    for (Graph* g : {&g_init_, &g_first_, &g_iter_[0], &g_iter_[1]}) {
+        if (g->e) (void)hipGraphExecDestroy(g->e);
+        if (g->g) (void)hipGraphDestroy(g->g);
+        g->e = nullptr;
+        g->g = nullptr;
+    }
+    graph_smoother_ = -1;
+}
+
+int PoissonDevice::capture(Graph& G, const std::function<void()>& body) {
+    if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
+    body();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (check(e, "end capture")) return -3;
+    G.g = g;
+    if (check(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0), "instantiate")) return -3;
+    return 0;
+}
+
+// The W-cycle and the PCG iteration have a fixed launch sequence for a given
+// smoother and SOR weights, so each piece is captured once and replayed:
+//   init     r = b - A x, ||r||_inf
+//   first    p0 = M r, rho, q = A p0, sigma, alpha, r -= alpha q, ||r||_inf
+//   iter[k]  z = M r, rho_new, beta, x += alpha p_k, p_k' = z + beta p_k,
+//            q = A p_k', sigma, alpha, r -= alpha q, ||r||_inf
+// (iter[k] reads p from p_[k] and writes p_[k^1]).
+void PoissonDevice::issue_init() {
+    const LevelView& v0 = lv_[0].v;
+    check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
+    check(launch_absmax(v0, r0_, sc_, partials_, mailbox_dev_, stream_), "absmax");
+}
+
+void PoissonDevice::issue_first() {
+    const LevelView& v0 = lv_[0].v;
+    precond(0, p_[0], r0_, true, kDotRho);
+    check(launch_laplacian_dot(v0, p_[0], q0_, sc_, partials_, stream_), "A p");
+    check(launch_axpy_absmax(v0, q0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
+}
+
+void PoissonDevice::issue_iter(int k) {
+    const LevelView& v0 = lv_[0].v;
+    precond(0, z0_, r0_, true, kDotRhoNew);
+    check(launch_update_laplacian_dot(v0, z0_, p_[k], p_[k ^ 1], x0_, q0_, sc_, partials_, stream_), "update A p");
+    check(launch_axpy_absmax(v0, q0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
+}
+
+int PoissonDevice::ensure_graphs() {
+    if (graph_smoother_ == opt_.smoother && graph_sor_ == sor_w_) return 0;
+    destroy_graphs();
+    int rc = capture(g_init_, [&] { issue_init(); });
+    if (!rc) rc = capture(g_first_, [&] { issue_first(); });
+    if (!rc) rc = capture(g_iter_[0], [&] { issue_iter(0); });
+    if (!rc) rc = capture(g_iter_[1], [&] { issue_iter(1); });
+    if (rc) {
+        destroy_graphs();
+        return rc;
+    }
+    graph_smoother_ = opt_.smoother;
+    graph_sor_ = sor_w_;
+    return 0;
+}
+
+// Replays a captured piece, or issues its launches directly when graphs are off.
+int PoissonDevice::run(Graph& g, const std::function<void()>& body) {
+    if (use_graphs_) return check(hipGraphLaunch(g.e, stream_), "graph launch");
+    body();
+    return first_error_ == hipSuccess ? 0 : -3;
+}
+
+// Waits for the next ||r||_inf reduction through the host-mapped mailbox; a
+// stream error or an idle stream without the expected value ends the wait.
+int PoissonDevice::wait_nu(float* nu) {
+    const unsigned int want = ++nu_seq_;
+    uint64_t word = 0;
+    for (unsigned spin = 0;; ++spin) {
+        word = mailbox_->word;
+        if (uint32_t(word >> 32) == want) break;
+        if ((spin & 1023u) == 1023u) {
+            const hipError_t q = hipStreamQuery(stream_);
+            if (q == hipSuccess) {
+                word = mailbox_->word;
+                if (uint32_t(word >> 32) == want) break;
+                err_ = "reduction result missing from the mailbox";
+                return -3;
+            }
+            if (q != hipErrorNotReady) return check(q, "stream query");
+        }
+    }
+    const uint32_t bits = uint32_t(word);
+    std::memcpy(nu, &bits, sizeof(float));
+    return 0;
 }
 
 int PoissonDevice::check(hipError_t e, const char* what) {
@@ -112,9 +210,9 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
             for (size_t i = 0; i < n; ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
         for (int k = 0; k < kNumCoef; ++k) add(nslot[k] * kLeafSize * sizeof(float));
         add(nslot[kCoefDiag] * kLeafSize * sizeof(float));   // invdiag
-        add(3 * n * kLeafSize * sizeof(float));               // X, B, T
+        add(4 * n * kLeafSize * sizeof(float));               // X, B, T, U
     }
-    add(8 * n0 * kLeafSize * sizeof(float));   // x0 b0 r0 p0 z0 + 3 unit buffers
+    add(10 * n0 * kLeafSize * sizeof(float));   // x0 b0 r0 p p' z0 q0 + 3 unit buffers
     add(3 * std::max<int64_t>(1, host_.levels[0].ndof) * sizeof(float));
     add(sizeof(PcgScalars));
     add((size_t(reduction_blocks(LevelView{int32_t(n0)})) + 16) * sizeof(double));
@@ -145,8 +243,10 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
     x0_ = vec(n0);
     b0_ = vec(n0);
     r0_ = vec(n0);
-    p0_ = vec(n0);
+    p_[0] = vec(n0);
+    p_[1] = vec(n0);
     z0_ = vec(n0);
+    q0_ = vec(n0);
     for (int i = 0; i < 3; ++i) staging_[i] = vec(n0);   // unit-op bricks
     sc_ = static_cast<PcgScalars*>(arena_.take(sizeof(PcgScalars)));
     partials_ = static_cast<double*>(arena_.take((size_t(reduction_blocks(lv_[0].v)) + 16) * sizeof(double)));
@@ -154,11 +254,26 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         err_ = "arena too small";
         return -2;
     }
-    if (check(hipMemsetAsync(x0_, 0, 8 * n0 * kLeafSize * sizeof(float), stream_), "memset")) return -3;
+    if (check(hipMemsetAsync(x0_, 0, 10 * n0 * kLeafSize * sizeof(float), stream_), "memset")) return -3;
     if (check(hipMemsetAsync(sc_, 0, sizeof(PcgScalars), stream_), "memset")) return -3;
     if (check(hipHostMalloc(reinterpret_cast<void**>(&sc_host_), sizeof(PcgScalars), hipHostMallocDefault),
               "hipHostMalloc"))
         return -3;
+    if (check(hipHostMalloc(reinterpret_cast<void**>(&mailbox_), sizeof(HostMailbox),
+                            hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc mailbox"))
+        return -3;
+    std::memset(const_cast<HostMailbox*>(mailbox_), 0, sizeof(HostMailbox));
+    if (check(hipHostGetDevicePointer(reinterpret_cast<void**>(&mailbox_dev_), mailbox_, 0), "mailbox alias"))
+        return -3;
+    {
+        const char* env = std::getenv("UAAMG_GRAPHS");
+        use_graphs_ = !(env && env[0] == '0');
+        env = std::getenv("UAAMG_FUSE");
+        fuse_ = !(env && env[0] == '0');
+        if (opt_.flags & 1) fuse_ = false;
+        if (opt_.flags & 2) use_graphs_ = false;
+    }
     if (int rc = build_coarsest()) return rc;
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
     return 0;
@@ -231,7 +346,8 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
     d.X = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
     d.B = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
     d.T = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
-    if (!d.X || !d.B || !d.T) {
+    d.U = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
+    if (!d.X || !d.B || !d.T || !d.U) {
         err_ = "arena too small";
         return -2;
     }
@@ -239,6 +355,7 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
         if (check(hipMemsetAsync(d.X, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
         if (check(hipMemsetAsync(d.B, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
         if (check(hipMemsetAsync(d.T, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
+        if (check(hipMemsetAsync(d.U, 0, n * kLeafSize * sizeof(float), stream_), "memset")) return -3;
     }
     if (check(hipStreamSynchronize(stream_), "upload sync")) return -3;
     return first_error_ == hipSuccess ? 0 : -3;
@@ -337,14 +454,21 @@ void PoissonDevice::smooth(int l, float*& cur, float*& oth, const float* B, int 
     }
 }
 
-// muCyclePreconditioner<2, skip_first> (simd_vdb_poisson.cpp:1844-1977)
-void PoissonDevice::precond(int l, float* X, const float* B, bool skip_first) {
+// muCyclePreconditioner<2, skip_first> (simd_vdb_poisson.cpp:1844-1977).
+// dot_target >= 0 (level 0 only) also forms <X, B> into rho / rho_new.
+void PoissonDevice::precond(int l, float* X, const float* B, bool skip_first, int dot_target) {
     const int L = nlevels();
+    const LevelView& v = lv_[l].v;
     if (l == L - 1) {
         check(launch_coarsest_cg(cv_, B, X, stream_), "coarsest");
+        if (dot_target >= 0) check(launch_dot(v, X, B, sc_, partials_, dot_target, stream_), "dot");
         return;
     }
-    const LevelView& v = lv_[l].v;
+    const bool jacobi = opt_.smoother == kScheduledRelaxedJacobi || opt_.smoother == kWeightedJacobi;
+    if (fuse_ && jacobi) {
+        precond_fused(l, X, B, skip_first, dot_target);
+        return;
+    }
     float* cur = X;
     float* oth = lv_[l].T;
     int start = 0;
@@ -371,15 +495,66 @@ void PoissonDevice::precond(int l, float* X, const float* B, bool skip_first) {
         start = 1;
     }
     for (int i = start; i < kSweeps; ++i) smooth(l, cur, oth, B, i, true, false);
-    check(launch_apply(v, kModeResidual, cur, B, oth, 0.f, stream_), "residual");
     DevLevel& c = lv_[l + 1];
-    check(launch_restrict(c.v, oth, c.B, stream_), "restrict");
-    precond(l + 1, c.X, c.B, true);
+    if (fuse_) {
+        check(launch_residual_restrict(v, cur, B, c.B, stream_), "residual restrict");
+    } else {
+        check(launch_apply(v, kModeResidual, cur, B, oth, 0.f, stream_), "residual");
+        check(launch_restrict(c.v, oth, c.B, stream_), "restrict");
+    }
+    precond(l + 1, c.X, c.B, true, -1);
     // mu = 2: the second visit of the coarsest level repeats an identical
     // zero-guess solve, so it is skipped; other levels cycle again
-    if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false);
+    if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
     check(launch_prolong_add(v, c.X, cur, 1.0f, stream_), "prolong");
     for (int i = 0; i < kSweeps; ++i) smooth(l, cur, oth, B, i, false, false);
+    if (dot_target >= 0) check(launch_dot(v, X, B, sc_, partials_, dot_target, stream_), "dot");
+}
+
+// The Jacobi-family W-cycle visit with fused passes: the first two sweeps from
+// zero in one kernel, residual + restriction in one kernel, prolongation inside
+// the first post-smoothing sweep, and the level-0 dot inside the last sweep.
+// Sweeps rotate through T and U and the last one writes X, so the result lands
+// in the caller's vector as with the reference's even number of pointer swaps.
+void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_first, int dot_target) {
+    const int L = nlevels();
+    const LevelView& v = lv_[l].v;
+    DevLevel& d = lv_[l];
+    const bool weighted = opt_.smoother == kWeightedJacobi;
+    auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
+    auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    float* cur = X;
+    auto next = [&]() { return cur == d.T ? d.U : d.T; };
+    if (skip_first) {
+        float* dst = next();
+        check(launch_jacobi_first2(v, B, dst, wpre(0), wpre(1), stream_), "first2");
+        cur = dst;
+        dst = next();
+        check(launch_apply(v, kModeJacobi, cur, B, dst, wpre(2), stream_), "jacobi");
+        cur = dst;
+    } else {
+        for (int i = 0; i < kSweeps; ++i) {
+            float* dst = next();
+            check(launch_apply(v, kModeJacobi, cur, B, dst, wpre(i), stream_), "jacobi");
+            cur = dst;
+        }
+    }
+    DevLevel& c = lv_[l + 1];
+    check(launch_residual_restrict(v, cur, B, c.B, stream_), "residual restrict");
+    precond(l + 1, c.X, c.B, true, -1);
+    if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
+    {
+        float* dst = next();
+        check(launch_prolong_jacobi(v, cur, c.X, B, dst, wpost(0), stream_), "prolong jacobi");
+        cur = dst;
+        dst = next();
+        check(launch_apply(v, kModeJacobi, cur, B, dst, wpost(1), stream_), "jacobi");
+        cur = dst;
+    }
+    if (dot_target >= 0)
+        check(launch_jacobi_dot(v, cur, B, X, wpost(2), sc_, partials_, dot_target, stream_), "jacobi dot");
+    else
+        check(launch_apply(v, kModeJacobi, cur, B, X, wpost(2), stream_), "jacobi");
 }
 
 // muCycleIterative<2> (simd_vdb_poisson.cpp:1979-2110)
@@ -410,52 +585,44 @@ void PoissonDevice::mg_cycle(int l, float* X, const float* B) {
     for (int i = 0; i < kSweeps; ++i) smooth(l, cur, oth, B, i, false, true);
 }
 
-int PoissonDevice::read_nu(float* nu) {
-    if (check(hipMemcpyAsync(&sc_host_->nu, &sc_->nu, sizeof(float), hipMemcpyDeviceToHost, stream_), "read nu"))
-        return -3;
-    if (check(hipStreamSynchronize(stream_), "sync")) return -3;
-    *nu = sc_host_->nu;
-    return 0;
-}
-
-// solveMultigridPCG (simd_vdb_poisson.cpp:2155-2226)
+// solveMultigridPCG (simd_vdb_poisson.cpp:2155-2226). One captured piece per
+// iteration runs the reference's lines 13-16 of iteration k and lines 6-8 of
+// iteration k+1; the host reads ||r||_inf from the mailbox and applies the
+// convergence and failure tests of lines 9-12 between pieces.
 int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
+    if (use_graphs_ && ensure_graphs()) return -3;
     const LevelView& v0 = lv_[0].v;
     int it = 0;
     if (iterations) *iterations = 0;
     check(launch_scatter_dof(v0, x_dof, x0_, stream_), "scatter x");
     check(launch_scatter_dof(v0, b_dof, b0_, stream_), "scatter b");
-    check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
-    check(launch_absmax(v0, r0_, sc_, partials_, stream_), "absmax");
     float nu;
-    if (read_nu(&nu)) return -3;
+    if (run(g_init_, [&] { issue_init(); })) return -3;
+    if (wait_nu(&nu)) return -3;
     const float numax = opt_.relative_tolerance * nu;
     int status = kFailed;
     if (nu <= numax) {
         status = kSuccess;
-    } else {
-        precond(0, p0_, r0_, true);
-        check(launch_dot(v0, p0_, r0_, sc_, partials_, kDotRho, stream_), "dot");
-        float nu_old = nu;
-        for (; it < opt_.max_iterations; ++it) {
-            check(launch_laplacian_dot(v0, p0_, z0_, sc_, partials_, stream_), "A p");
-            check(launch_axpy_absmax(v0, z0_, r0_, sc_, partials_, stream_), "axpy");
-            nu_old = nu;
-            if (read_nu(&nu)) return -3;
+    } else if (opt_.max_iterations > 0) {
+        if (run(g_first_, [&] { issue_first(); })) return -3;
+        int k = 0;   // p_[k] holds the current direction
+        for (;;) {
+            const float nu_old = nu;
+            if (wait_nu(&nu)) return -3;
             if (nu <= numax) {
-                check(launch_update_x(v0, p0_, x0_, sc_, stream_), "update x");
+                check(launch_update_x(v0, p_[k], x0_, sc_, stream_), "update x");
                 status = kSuccess;
                 break;
             }
-            if (nu > nu_old && it > 3) {
-                status = kFailed;
+            if (nu > nu_old && it > 3) break;
+            if (run(g_iter_[k], [&] { issue_iter(k); })) return -3;
+            k ^= 1;
+            if (++it >= opt_.max_iterations) {
+                if (wait_nu(&nu)) return -3;   // the piece's own reduction
                 break;
             }
-            precond(0, z0_, r0_, true);
-            check(launch_dot(v0, z0_, r0_, sc_, partials_, kDotRhoNew, stream_), "dot");
-            check(launch_update_xp(v0, z0_, x0_, p0_, sc_, stream_), "update");
         }
     }
     if (iterations) *iterations = it;
@@ -464,7 +631,8 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
     return first_error_ == hipSuccess ? status : -3;
 }
 
-// solvePureMultigrid (simd_vdb_poisson.cpp:2228-2267)
+// solvePureMultigrid (simd_vdb_poisson.cpp:2228-2267); issued directly (the
+// fallback path after a FAILED PCG is not latency critical)
 int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
@@ -474,9 +642,9 @@ int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iteration
     check(launch_scatter_dof(v0, x_dof, x0_, stream_), "scatter x");
     check(launch_scatter_dof(v0, b_dof, b0_, stream_), "scatter b");
     check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
-    check(launch_absmax(v0, r0_, sc_, partials_, stream_), "absmax");
+    check(launch_absmax(v0, r0_, sc_, partials_, mailbox_dev_, stream_), "absmax");
     float nu;
-    if (read_nu(&nu)) return -3;
+    if (wait_nu(&nu)) return -3;
     const float numax = opt_.relative_tolerance * nu;
     int status = kFailed;
     if (nu <= numax) {
@@ -485,9 +653,9 @@ int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iteration
         for (; it < opt_.max_iterations; ++it) {
             mg_cycle(0, x0_, b0_);
             check(launch_apply(v0, kModeResidual, x0_, b0_, r0_, 0.f, stream_), "residual");
-            check(launch_absmax(v0, r0_, sc_, partials_, stream_), "absmax");
+            check(launch_absmax(v0, r0_, sc_, partials_, mailbox_dev_, stream_), "absmax");
             const float nu_old = nu;
-            if (read_nu(&nu)) return -3;
+            if (wait_nu(&nu)) return -3;
             if (nu <= numax) {
                 status = kSuccess;
                 break;
@@ -523,6 +691,9 @@ int PoissonDevice::first_sweep_dev(int l, int kind, float w, const float* b_dof,
     const LevelView& v = lv_[l].v;
     float *bb = staging_[1], *yb = staging_[2];
     check(launch_scatter_dof(v, b_dof, bb, stream_), "scatter");
+    // the staging bricks are shared by all levels: rows without DOFs must read
+    // as zero for the black pass that follows the red update
+    check(hipMemsetAsync(yb, 0, size_t(v.nleaf) * kLeafSize * sizeof(float), stream_), "memset");
     check(launch_first_sweep(v, kind, bb, yb, w, stream_), "first");
     if (kind == 2) check(launch_apply(v, kModeBlackGS, yb, bb, yb, w, stream_), "black");
     check(launch_gather_dof(v, yb, y_dof, stream_), "gather");
@@ -552,7 +723,7 @@ int PoissonDevice::prolong_dev(int fl, const float* coarse_dof, float* fine_dof,
 int PoissonDevice::precondition_dev(const float* r_dof, float* z_dof) {
     const LevelView& v0 = lv_[0].v;
     check(launch_scatter_dof(v0, r_dof, r0_, stream_), "scatter");
-    precond(0, z0_, r0_, true);
+    precond(0, z0_, r0_, true, -1);
     check(launch_gather_dof(v0, z0_, z_dof, stream_), "gather");
     if (check(hipStreamSynchronize(stream_), "sync")) return -3;
     return 0;

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,7 @@ struct SolverOptions {
     int precision = 0;
     int device = 0;
     int64_t max_coarsest_dof = 4000;
+    int flags = 0;   // bit 0: unfused kernels, bit 1: no graph capture (debug / A-B checks)
     hipStream_t stream = nullptr;
 };
 
@@ -87,17 +89,29 @@ private:
         float* X = nullptr;
         float* B = nullptr;
         float* T = nullptr;
-        float* U = nullptr;   // extra brick vector for unit operations
-        std::vector<float> sor_weight;
+        float* U = nullptr;   // second smoothing buffer of the fused W-cycle
     };
+
+    struct Graph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+    };
+    int capture(Graph& g, const std::function<void()>& body);
+    void issue_init();
+    void issue_first();
+    void issue_iter(int k);
+    int ensure_graphs();
+    void destroy_graphs();
+    int run(Graph& g, const std::function<void()>& body);
+    int wait_nu(float* nu);
 
     int upload_level(const PoissonLevelHost& h, DevLevel& d, bool spai0);
     int build_coarsest();
-    void precond(int level, float* X, const float* B, bool skip_first);
+    void precond(int level, float* X, const float* B, bool skip_first, int dot_target);
+    void precond_fused(int level, float* X, const float* B, bool skip_first, int dot_target);
     void mg_cycle(int level, float* X, const float* B);
     void smooth(int level, float*& cur, float*& oth, const float* B, int sweep, bool pre, bool iterative);
     int check(hipError_t e, const char* what);
-    int read_nu(float* nu);
 
     PoissonHierarchyHost host_;
     SolverOptions opt_;
@@ -107,12 +121,22 @@ private:
     std::vector<DevLevel> lv_;
     std::vector<float> sor_w_;
     // level-0 PCG bricks
-    float *x0_ = nullptr, *b0_ = nullptr, *r0_ = nullptr, *p0_ = nullptr, *z0_ = nullptr;
+    float *x0_ = nullptr, *b0_ = nullptr, *r0_ = nullptr, *z0_ = nullptr, *q0_ = nullptr;
+    float* p_[2] = {nullptr, nullptr};   // search direction, ping-pong across iterations
     float* staging_[3] = {nullptr, nullptr, nullptr};
     PcgScalars* sc_ = nullptr;
     PcgScalars* sc_host_ = nullptr;   // pinned mirror
     double* partials_ = nullptr;
     CoarsestView cv_{};
+    // captured PCG pieces, valid for graph_smoother_ / graph_sor_
+    Graph g_init_, g_first_, g_iter_[2];
+    int graph_smoother_ = -1;
+    std::vector<float> graph_sor_;
+    bool use_graphs_ = true;
+    bool fuse_ = true;                     // fused W-cycle / PCG kernels
+    HostMailbox* mailbox_ = nullptr;       // pinned host memory
+    HostMailbox* mailbox_dev_ = nullptr;   // its device alias
+    unsigned int nu_seq_ = 0;              // ||r||_inf reductions issued so far
     hipError_t first_error_ = hipSuccess;
     std::string err_;
 };

@@ -1,0 +1,232 @@
+// Device-side building blocks shared by the level kernels and the PCG kernels:
+// row loads, the 7-point stencil over a pluggable row source, one Jacobi row.
+//
+// Arithmetic follows LaplacianApplySIMD::LightWeightApplier
+// (simd_vdb_poisson.cpp:755-1031) term by term: the six neighbour products are
+// rounded separately and summed as ((x+ + x-) + (y+ + y-)) + (z+ + z-), then
+// A*x = fma(x, diag, sum). Sources are compiled with -ffp-contract=off so the
+// only fused multiply-adds are the explicit fmaf calls.
+#pragma once
+
+#include "poisson_kernels.h"
+
+namespace uaamg {
+namespace dev {
+
+__device__ __forceinline__ int wave_leaf() {
+    int leaf = int(blockIdx.x) * kWavesPerBlock + int(threadIdx.x >> 6);
+    return __builtin_amdgcn_readfirstlane(leaf);
+}
+
+__device__ __forceinline__ void ld8(const float* p, float v[8]) {
+    const float4 a = reinterpret_cast<const float4*>(p)[0];
+    const float4 b = reinterpret_cast<const float4*>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+
+__device__ __forceinline__ void st8(float* p, const float v[8]) {
+    reinterpret_cast<float4*>(p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+    reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
+__device__ __forceinline__ void fill8(float v[8], float c) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = c;
+}
+
+__device__ __forceinline__ uint32_t row_mask(const LevelView& L, int leaf, int lane) {
+    const uint64_t w = L.mask[size_t(leaf) * 8 + (lane >> 3)];
+    return uint32_t(w >> ((lane & 7) * 8)) & 0xffu;
+}
+
+// load a row of a brick array, or a constant when the slot is absent
+__device__ __forceinline__ void ld8_slot(const float* base, int slot, int row, float dflt, float v[8]) {
+    if (slot >= 0) ld8(base + size_t(slot) * 512 + row * 8, v);
+    else fill8(v, dflt);
+}
+
+// Sources of the vector a stencil is applied to. row() fills the 8 values of
+// one z-row of a leaf, at() returns one value; both are called only for leaves
+// that exist (an absent neighbour leaf contributes 0).
+struct PlainSrc {   // the stored brick vector
+    const float* X;
+    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const { ld8(X + size_t(leaf) * 512 + r * 8, v); }
+    __device__ __forceinline__ float at(int leaf, int r, int z) const { return X[size_t(leaf) * 512 + r * 8 + z]; }
+};
+
+// x1 = w0 * D^-1 b: the first Jacobi sweep from a zero guess, evaluated where
+// the second sweep needs it (setGridToResultAfterFirstJacobi,
+// simd_vdb_poisson.cpp:1435-1479: (invdiag * rhs) * w). B is 0 on non-DOF
+// voxels, so x1 is too.
+struct FirstJacobiSrc {
+    const float* B;
+    const float* invdiag;
+    const int32_t* slot;
+    float dflt_inv;
+    float w0;
+    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
+        float b[8], inv[8];
+        ld8(B + size_t(leaf) * 512 + r * 8, b);
+        ld8_slot(invdiag, slot[size_t(leaf) * kSlotStride + kSlotDiag], r, dflt_inv, inv);
+#pragma unroll
+        for (int z = 0; z < 8; ++z) v[z] = w0 * (inv[z] * b[z]);
+    }
+    __device__ __forceinline__ float at(int leaf, int r, int z) const {
+        const int sd = slot[size_t(leaf) * kSlotStride + kSlotDiag];
+        const float inv = sd >= 0 ? invdiag[size_t(sd) * 512 + r * 8 + z] : dflt_inv;
+        return w0 * (inv * B[size_t(leaf) * 512 + r * 8 + z]);
+    }
+};
+
+// x + P x_coarse on on-voxels (prolongation<true> with alpha = 1,
+// simd_vdb_poisson.cpp:1686-1754), evaluated where the following sweep needs it.
+struct ProlongSrc {
+    const float* X;
+    const float* Xc;
+    const uint64_t* mask;
+    const int32_t* parent;
+    __device__ __forceinline__ uint32_t bits(int leaf, int r) const {
+        return uint32_t(mask[size_t(leaf) * 8 + (r >> 3)] >> ((r & 7) * 8)) & 0xffu;
+    }
+    __device__ __forceinline__ float4 coarse4(int leaf, int r) const {
+        const int p = parent[leaf];
+        const int cl = p >> 3, oct = p & 7;
+        const int crow = (((r >> 3) >> 1) + 4 * ((oct >> 2) & 1)) * 8 + (((r & 7) >> 1) + 4 * ((oct >> 1) & 1));
+        return *reinterpret_cast<const float4*>(Xc + size_t(cl) * 512 + crow * 8 + 4 * (oct & 1));
+    }
+    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
+        ld8(X + size_t(leaf) * 512 + r * 8, v);
+        const uint32_t m = parent[leaf] >= 0 ? bits(leaf, r) : 0u;   // parent < 0: bounds guard
+        if (m) {
+            const float4 c = coarse4(leaf, r);
+            const float cc[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+            for (int z = 0; z < 8; ++z)
+                if ((m >> z) & 1u) v[z] += cc[z >> 1];
+        }
+    }
+    __device__ __forceinline__ float at(int leaf, int r, int z) const {
+        float v = X[size_t(leaf) * 512 + r * 8 + z];
+        if (parent[leaf] >= 0 && ((bits(leaf, r) >> z) & 1u)) {
+            const float4 c = coarse4(leaf, r);
+            const float cc[4] = {c.x, c.y, c.z, c.w};
+            v += cc[z >> 1];
+        }
+        return v;
+    }
+};
+
+// p' = z + beta p (levelXPlusAlphaY, simd_vdb_poisson.cpp:2305-2320), both 0 off the DOFs
+struct UpdateSrc {
+    const float* Z;
+    const float* P;
+    float beta;
+    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
+        float z[8], p[8];
+        ld8(Z + size_t(leaf) * 512 + r * 8, z);
+        ld8(P + size_t(leaf) * 512 + r * 8, p);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float t = beta * p[k];
+            v[k] = z[k] + t;
+        }
+    }
+    __device__ __forceinline__ float at(int leaf, int r, int z) const {
+        const size_t i = size_t(leaf) * 512 + r * 8 + z;
+        const float t = beta * P[i];
+        return Z[i] + t;
+    }
+};
+
+// Own row of the source and the off-diagonal sum of the 7-point stencil for one row.
+template <class Src>
+__device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
+                                                float s[8]) {
+    const int lx = lane >> 3, ly = lane & 7;
+    const int32_t* nb = L.nbr + size_t(leaf) * 6;
+    const int nmx = nb[0], npx = nb[1], nmy = nb[2], npy = nb[3], nmz = nb[4], npz = nb[5];
+    src.row(leaf, lane, xc);
+
+    float xm[8], xp[8], ym[8], yp[8];
+    {
+        const int sl = lx > 0 ? leaf : nmx, sr = lx > 0 ? lane - 8 : lane + 56;
+        if (sl >= 0) src.row(sl, sr, xm);
+        else fill8(xm, 0.f);
+    }
+    {
+        const int sl = lx < 7 ? leaf : npx, sr = lx < 7 ? lane + 8 : lane - 56;
+        if (sl >= 0) src.row(sl, sr, xp);
+        else fill8(xp, 0.f);
+    }
+    {
+        const int sl = ly > 0 ? leaf : nmy, sr = ly > 0 ? lane - 1 : lane + 7;
+        if (sl >= 0) src.row(sl, sr, ym);
+        else fill8(ym, 0.f);
+    }
+    {
+        const int sl = ly < 7 ? leaf : npy, sr = ly < 7 ? lane + 1 : lane - 7;
+        if (sl >= 0) src.row(sl, sr, yp);
+        else fill8(yp, 0.f);
+    }
+    const float zlo = nmz >= 0 ? src.at(nmz, lane, 7) : 0.f;
+    const float zhi = npz >= 0 ? src.at(npz, lane, 0) : 0.f;
+
+    const int32_t* sl = L.slot + size_t(leaf) * kSlotStride;
+    const int sx = sl[kSlotX], sy = sl[kSlotY], sz = sl[kSlotZ];
+    const int sxp = npx >= 0 ? L.slot[size_t(npx) * kSlotStride + kSlotX] : -1;
+    const int syp = npy >= 0 ? L.slot[size_t(npy) * kSlotStride + kSlotY] : -1;
+    const int szp = npz >= 0 ? L.slot[size_t(npz) * kSlotStride + kSlotZ] : -1;
+    const float d = L.dflt_off;
+
+    float cxm[8], cxp[8], cym[8], cyp[8], czm[8];
+    ld8_slot(L.cx, sx, lane, d, cxm);
+    ld8_slot(L.cx, lx < 7 ? sx : sxp, lx < 7 ? lane + 8 : lane - 56, d, cxp);
+    ld8_slot(L.cy, sy, lane, d, cym);
+    ld8_slot(L.cy, ly < 7 ? sy : syp, ly < 7 ? lane + 1 : lane - 7, d, cyp);
+    ld8_slot(L.cz, sz, lane, d, czm);
+    const float czhi = szp >= 0 ? L.cz[size_t(szp) * 512 + lane * 8] : d;
+
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        const float zpv = z < 7 ? xc[z + 1] : zhi;
+        const float czp = z < 7 ? czm[z + 1] : czhi;
+        const float zmv = z > 0 ? xc[z - 1] : zlo;
+        const float f1 = xp[z] * cxp[z];
+        const float f2 = xm[z] * cxm[z];
+        const float f3 = yp[z] * cyp[z];
+        const float f4 = ym[z] * cym[z];
+        const float f5 = zpv * czp;
+        const float f6 = zmv * czm[z];
+        s[z] = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+    }
+}
+
+
+// Weighted Jacobi on one row: y = fma(w, (b - (fma(x, diag, s))) * D^-1, x).
+__device__ __forceinline__ void jacobi_row(const LevelView& L, int leaf, int lane, const float xc[8], const float s[8],
+                                           const float* B, float w, float y[8]) {
+    const int sd = L.slot[size_t(leaf) * kSlotStride + kSlotDiag];
+    float dg[8], inv[8], b[8];
+    ld8_slot(L.diag, sd, lane, L.dflt_diag, dg);
+    ld8_slot(L.invdiag, sd, lane, L.dflt_invdiag, inv);
+    ld8(B + size_t(leaf) * 512 + lane * 8, b);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        const float ax = fmaf(xc[z], dg[z], s[z]);
+        float r = b[z] - ax;
+        r = r * inv[z];
+        y[z] = fmaf(w, r, xc[z]);
+    }
+}
+
+__device__ __forceinline__ void mask_row(uint32_t rm, float y[8]) {
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+        if (!((rm >> z) & 1u)) y[z] = 0.f;
+}
+
+inline unsigned grid_for(int32_t nleaf) { return unsigned((nleaf + kWavesPerBlock - 1) / kWavesPerBlock); }
+
+}  // namespace dev
+}  // namespace uaamg
