@@ -79,7 +79,8 @@ typedef struct uaamg_options {
     int32_t precision;        /* 0 = FP32 throughout */
     int32_t device;           /* HIP device ordinal */
     int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, simd_vdb_poisson.cpp:1815) */
-    int32_t flags;            /* 0 normally; bit 0 = unfused kernels, bit 1 = no graph replay (A/B checks) */
+    int32_t flags;            /* 0 normally; bit 0 = unfused kernels, bit 1 = no graph replay,
+                                 bit 2 = bottom levels in one single-workgroup launch (A/B checks) */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
 } uaamg_options;
 

@@ -173,10 +173,12 @@ def test_nan_rhs_fails(problem):
 
 @pytest.mark.parametrize("smoother", [0, 1, 2, 3])
 def test_fused_and_graph_paths_bit_identical(problem, smoother):
-    """The fused W-cycle kernels and graph replay reorder launches, not arithmetic."""
+    """Fused kernels, the bottom-levels tail kernel and graph replay reorder
+    launches, not arithmetic: flags 1 = unfused, 2 = no graphs, 4 = bottom
+    levels in the single-workgroup tail kernel."""
     s, lap, orc, b = problem
     variants = {}
-    for flags in (0, 1, 2, 3):
+    for flags in (0, 1, 2, 3, 4, 6):
         l2 = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, flags=flags)
         solver = U.PoissonSolver(l2)
         solver.mSmoother = smoother

@@ -22,6 +22,7 @@ SOURCES = [
     "poisson_host.cpp",
     "poisson_kernels.hip",
     "pcg_kernels.hip",
+    "tail_kernels.hip",
     "poisson_solver.cpp",
     "capi.cpp",
 ]

@@ -1,0 +1,151 @@
+// Coarsest-level solve as a block-wide device routine: Eigen::ConjugateGradient
+// with DiagonalPreconditioner, zero initial guess, maxIterations = 10,
+// tolerance = float epsilon, restated (constructCoarsestLevelExactSolver,
+// simd_vdb_poisson.cpp:2114-2126; Eigen ConjugateGradient.h conjugate_gradient()).
+//
+// Each thread keeps up to kPer DOFs' matrix rows and x, r, p, q in registers;
+// only p is shared through LDS for the neighbour reads. Per iteration two
+// block reductions: <p, Ap>, then ||r||^2 and <r, z> together (z is formed
+// right after r is updated; when ||r||^2 passes the stopping test z is simply
+// unused, as in Eigen's loop).
+#pragma once
+
+#include <cfloat>
+
+#include "poisson_kernels.h"
+
+namespace uaamg {
+namespace dev {
+
+template <int kThreads>
+struct CgShared {
+    float sp[kMaxCoarsestDof];
+    float red[2][kThreads / 64][2];
+};
+
+__device__ __forceinline__ float cg_wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Sums (a, b) over the block with one barrier; red alternates between two
+// buffers, so the next call needs no barrier before it writes.
+template <int kThreads>
+__device__ __forceinline__ float2 cg_sum2(float a, float b, CgShared<kThreads>& sh, int& parity) {
+    a = cg_wave_sum(a);
+    b = cg_wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh.red[parity][w][0] = a;
+        sh.red[parity][w][1] = b;
+    }
+    __syncthreads();
+    float2 t = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < kThreads / 64; ++i) {
+        t.x += sh.red[parity][i][0];
+        t.y += sh.red[parity][i][1];
+    }
+    parity ^= 1;
+    return t;
+}
+
+// Solves the coarsest system for right-hand side B (bricks) into X (bricks,
+// DOF positions only). Every thread of the block must call it.
+template <int kThreads>
+__device__ void coarsest_cg_block(const CoarsestView& C, const float* B, float* X, CgShared<kThreads>& sh) {
+    constexpr int kPer = kMaxCoarsestDof / kThreads;
+    int parity = 0;
+    const int n = C.ndof;
+    const int tid = threadIdx.x;
+    int nb[kPer][6];
+    float cf[kPer][8], x[kPer], r[kPer], p[kPer], q[kPer];
+    float part = 0.f;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int d = tid + j * kThreads;
+        x[j] = r[j] = p[j] = q[j] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nb[j][k] = -1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cf[j][k] = 0.f;
+        if (d < n) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) nb[j][k] = C.nb[size_t(d) * 6 + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cf[j][k] = C.coef[size_t(d) * 8 + k];
+            r[j] = B[C.pos[d]];
+            part += r[j] * r[j];
+        }
+    }
+    const float rhs_norm2 = cg_sum2<kThreads>(part, 0.f, sh, parity).x;
+    bool done = rhs_norm2 == 0.f;
+    const float threshold = fmaxf(C.tol * C.tol * rhs_norm2, FLT_MIN);
+    if (!done && rhs_norm2 < threshold) done = true;
+    if (!done) {
+        part = 0.f;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int d = tid + j * kThreads;
+            if (d < n) {
+                p[j] = cf[j][7] * r[j];
+                sh.sp[d] = p[j];
+                part += r[j] * p[j];
+            }
+        }
+        float abs_new = cg_sum2<kThreads>(part, 0.f, sh, parity).x;   // its barrier also publishes sp
+        for (int it = 0; it < C.max_iters; ++it) {
+            part = 0.f;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int d = tid + j * kThreads;
+                if (d < n) {
+                    float v[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v[k] = nb[j][k] >= 0 ? sh.sp[nb[j][k]] : 0.f;
+                    const float f1 = v[0] * cf[j][1], f2 = v[1] * cf[j][2], f3 = v[2] * cf[j][3];
+                    const float f4 = v[3] * cf[j][4], f5 = v[4] * cf[j][5], f6 = v[5] * cf[j][6];
+                    const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+                    q[j] = fmaf(p[j], cf[j][0], s);
+                    part += p[j] * q[j];
+                }
+            }
+            const float alpha = abs_new / cg_sum2<kThreads>(part, 0.f, sh, parity).x;
+            float rr = 0.f, rz = 0.f;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int d = tid + j * kThreads;
+                if (d < n) {
+                    x[j] = x[j] + alpha * p[j];
+                    r[j] = r[j] - alpha * q[j];
+                    rr += r[j] * r[j];
+                    q[j] = cf[j][7] * r[j];   // z
+                    rz += r[j] * q[j];
+                }
+            }
+            const float2 sums = cg_sum2<kThreads>(rr, rz, sh, parity);
+            if (sums.x < threshold) break;
+            const float abs_old = abs_new;
+            abs_new = sums.y;
+            const float beta = abs_new / abs_old;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int d = tid + j * kThreads;
+                if (d < n) {
+                    p[j] = q[j] + beta * p[j];
+                    sh.sp[d] = p[j];
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int d = tid + j * kThreads;
+        if (d < n) X[C.pos[d]] = x[j];
+    }
+}
+
+}  // namespace dev
+}  // namespace uaamg

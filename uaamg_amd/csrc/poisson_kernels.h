@@ -80,6 +80,23 @@ struct CoarsestView {
     float tol;                // Eigen default: float epsilon
 };
 
+// Bottom of the W-cycle in one launch: nvisits visits of level L-2 (the first
+// from a zero guess when first_skip) with the coarsest solve inside.
+struct TailArgs {
+    LevelView L;          // level L-2
+    const float* B;       // its right-hand side
+    float* X;             // its solution (final sweep output)
+    float* T;             // smoothing buffers
+    float* U;
+    float* Bc;            // coarsest right-hand side and solution
+    float* Xc;
+    CoarsestView C;
+    float wpre[3];        // Jacobi weights of the pre sweeps; post sweeps use wpost
+    float wpost[3];
+    int nvisits;
+    int first_skip;
+};
+
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlockThreads = 64 * kWavesPerBlock;
 constexpr int kMaxCoarsestDof = 4096;
@@ -97,6 +114,8 @@ hipError_t launch_prolong_add(const LevelView& fine, const float* coarse_v, floa
                               hipStream_t s);
 // coarsest 10-iteration Jacobi-PCG from zero, one workgroup
 hipError_t launch_coarsest_cg(const CoarsestView& C, const float* b, float* x, hipStream_t s);
+// the last two levels of a W-cycle visit pair, one workgroup
+hipError_t launch_tail(const TailArgs& a, hipStream_t s);
 
 // DOF-ordered vector <-> bricks
 hipError_t launch_scatter_dof(const LevelView& L, const float* dofvec, float* bricks, hipStream_t s);

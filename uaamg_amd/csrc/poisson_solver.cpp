@@ -273,6 +273,16 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         fuse_ = !(env && env[0] == '0');
         if (opt_.flags & 1) fuse_ = false;
         if (opt_.flags & 2) use_graphs_ = false;
+        // level-to-kernel mapping: on request (flags bit 2 or
+        // UAAMG_TAIL_LEAVES) the bottom pair of levels runs as one
+        // single-workgroup launch when level L-2 is small. Off by default: at
+        // 256^3 (level L-2 = 44 leaves) one workgroup takes 250 us per visit
+        // pair against about 100 us for the per-sweep launches.
+        int max_leaves = (opt_.flags & 4) ? 128 : 0;
+        env = std::getenv("UAAMG_TAIL_LEAVES");
+        if (env) max_leaves = std::atoi(env);
+        tail_level_ = -1;
+        if (max_leaves > 0 && L >= 3 && int64_t(host_.levels[L - 2].nleaf()) <= max_leaves) tail_level_ = L - 2;
     }
     if (int rc = build_coarsest()) return rc;
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
@@ -541,8 +551,28 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
     }
     DevLevel& c = lv_[l + 1];
     check(launch_residual_restrict(v, cur, B, c.B, stream_), "residual restrict");
-    precond(l + 1, c.X, c.B, true, -1);
-    if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
+    if (l + 1 == tail_level_) {
+        // both mu = 2 visits of level L-2 and their coarsest solves, one launch
+        TailArgs ta{};
+        ta.L = c.v;
+        ta.B = c.B;
+        ta.X = c.X;
+        ta.T = c.T;
+        ta.U = c.U;
+        ta.Bc = lv_[L - 1].B;
+        ta.Xc = lv_[L - 1].X;
+        ta.C = cv_;
+        for (int i = 0; i < kSweeps; ++i) {
+            ta.wpre[i] = wpre(i);
+            ta.wpost[i] = wpost(i);
+        }
+        ta.nvisits = 2;
+        ta.first_skip = 1;
+        check(launch_tail(ta, stream_), "tail");
+    } else {
+        precond(l + 1, c.X, c.B, true, -1);
+        if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
+    }
     {
         float* dst = next();
         check(launch_prolong_jacobi(v, cur, c.X, B, dst, wpost(0), stream_), "prolong jacobi");

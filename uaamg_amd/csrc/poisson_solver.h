@@ -23,7 +23,7 @@ struct SolverOptions {
     int precision = 0;
     int device = 0;
     int64_t max_coarsest_dof = 4000;
-    int flags = 0;   // bit 0: unfused kernels, bit 1: no graph capture (debug / A-B checks)
+    int flags = 0;   // bit 0: unfused kernels, bit 1: no graph capture, bit 2: tail kernel (A-B checks)
     hipStream_t stream = nullptr;
 };
 
@@ -134,6 +134,7 @@ private:
     std::vector<float> graph_sor_;
     bool use_graphs_ = true;
     bool fuse_ = true;                     // fused W-cycle / PCG kernels
+    int tail_level_ = -1;                  // level whose visit pairs run in the tail kernel (-1: none)
     HostMailbox* mailbox_ = nullptr;       // pinned host memory
     HostMailbox* mailbox_dev_ = nullptr;   // its device alias
     unsigned int nu_seq_ = 0;              // ||r||_inf reductions issued so far
