@@ -216,6 +216,139 @@ void uaamg_scene_destroy(uaamg_scene* s);
 float uaamg_fraction_inside2(float phi_left, float phi_right);
 float uaamg_fraction_inside4(float phi_bl, float phi_br, float phi_tl, float phi_tr);
 
+/* ------------------------------------------------------------- viscosity */
+/*
+ * Variational viscosity solve (simd_viscosity3d / L_with_level, SIMD_UAAMG/simd_viscosity3d.h:14-321).
+ * Velocities are three face-centred channels: channel c of voxel v sits at world (v - e_c/2) dx
+ * (packed_FloatGrid3, packed3grids.h:5-23). DOF-ordered vectors number channel 0's DOFs first, then
+ * channel 1's, then channel 2's, each in leaf tree order and voxel offset order (set_dof, :504-537).
+ * Grids: liquid SDF and viscosity on the voxel-centre lattice (index v at world v dx); solid SDF on the
+ * vertex lattice (index i at world (i - 1/2) dx), as FLIP_vdb.cpp:2117-2131 sets them up.
+ */
+typedef struct uaamg_viscosity uaamg_viscosity;
+
+typedef struct uaamg_viscosity_options {
+    int32_t max_iterations;   /* m_max_iter, default 100 (simd_viscosity3d.cpp:3629) */
+    int32_t device;           /* HIP device ordinal; -1 = host-only handle (inspection, no solves) */
+    int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, :3638) */
+    int32_t flags;            /* 0 normally; bit 1 = no graph replay (A/B checks); bit 3 = keep level 0 untrimmed */
+    void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
+} uaamg_viscosity_options;
+
+/* coefficient grids of a level (m_diagonal, m_invdiag, m_neg_x/y/z, m_cross_channel, m_SPAI0) */
+#define UAAMG_VGRID_DIAG(c) (0 + (c))
+#define UAAMG_VGRID_INVDIAG(c) (3 + (c))
+#define UAAMG_VGRID_NEG(c, axis) (6 + 3 * (c) + (axis))
+#define UAAMG_VGRID_CROSS(normal, j, k) (15 + 4 * (normal) + 2 * (j) + (k))
+#define UAAMG_VGRID_SPAI0(c) (27 + (c))
+
+/* light_weight_applier working modes (simd_viscosity3d.h:13) plus the two 6-colour sweeps */
+#define UAAMG_VMODE_NORMAL 0
+#define UAAMG_VMODE_RESIDUAL 1
+#define UAAMG_VMODE_JACOBI 2
+#define UAAMG_VMODE_SPAI0 3
+#define UAAMG_VMODE_RED_GS 4
+#define UAAMG_VMODE_BLACK_GS 5
+#define UAAMG_VMODE_XYZ_RBGS 6
+#define UAAMG_VMODE_ZYX_RBGS 7
+
+int uaamg_viscosity_options_default(uaamg_viscosity_options* opts);
+
+/*
+ * Builds level 0 from the viscosity, liquid and solid grids, the coarse levels and the coarsest
+ * explicit solver, and uploads them. Replaces L_with_level::create_viscosity_matrix
+ * (simd_viscosity3d.h:95-98, .cpp:14-20) followed by the simd_viscosity3d constructor's hierarchy
+ * loop (simd_viscosity3d.h:292-294, .cpp:3659-3690). dx is liquid_sdf->voxelSize()[0].
+ */
+int uaamg_viscosity_create(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf,
+                           float dt, float rho, float dx, const uaamg_viscosity_options* opts,
+                           uaamg_viscosity** out);
+void uaamg_viscosity_destroy(uaamg_viscosity* h);
+int uaamg_viscosity_set_options(uaamg_viscosity* h, const uaamg_viscosity_options* opts);
+
+/* m_ndof of level 0 and m_matrix_levels.size() (simd_viscosity3d.h:225, 316). */
+int64_t uaamg_viscosity_ndof(const uaamg_viscosity* h);
+int uaamg_viscosity_nlevels(const uaamg_viscosity* h);
+
+/* Per-level sizes: DOFs in total and per channel, leaves holding a DOF of any channel. */
+int uaamg_viscosity_level_info(const uaamg_viscosity* h, int level, int64_t* ndof, int64_t* nchan3,
+                               int64_t* nleaf_dof);
+
+/*
+ * DOF topology of a level (m_velocity_DOF / m_dof_manager[c], simd_viscosity3d.h:224-226): leaf
+ * origins [nleaf_dof][3], per-channel masks [nleaf_dof][3][8] and per-channel first DOF index
+ * [nleaf_dof][3] (-1 when the channel has no DOF in the leaf). Implements the data behind get_zero_vec
+ * and the warm-start visitor of FLIP_vdb.cpp:3189-3197. Any output may be NULL.
+ */
+int uaamg_viscosity_level_topology(const uaamg_viscosity* h, int level, int32_t* origins, uint64_t* masks,
+                                   int64_t* dofbase);
+
+/*
+ * Coefficient grid `grid` (UAAMG_VGRID_*) of a level as the level operator reads it, on the DOF leaves
+ * ([nleaf_dof][512]), with kept[i] = 0 where the leaf is absent or trimmed to the default
+ * (trim_default_nodes, :1700-1722).
+ */
+int uaamg_viscosity_level_coefficients(uaamg_viscosity* h, int level, int grid, float* bricks, uint8_t* kept);
+
+/*
+ * Explicit matrix of a level in CSR form with ascending columns (get_triplets / build_explicit_matrix,
+ * :2598-2618). Call with ptr == NULL to get *nnz first.
+ */
+int uaamg_viscosity_level_matrix(const uaamg_viscosity* h, int level, int64_t* nnz, int64_t* ptr, int32_t* col,
+                                 float* val);
+
+/*
+ * Right-hand side, DOF-ordered (L_with_level::build_rhs, simd_viscosity3d.h:128, .cpp:1942-2087).
+ * velocity is a Vec3f grid (ncomp 3) of face-centred components; solid_velocity likewise or NULL (zero).
+ */
+int uaamg_viscosity_build_rhs(const uaamg_viscosity* h, const uaamg_grid* velocity, const uaamg_grid* solid_velocity,
+                              float* rhs);
+
+/*
+ * simd_viscosity3d::pcg_solve(in_lhs, tolerance) (simd_viscosity3d.h:296, .cpp:3694-3740) on host
+ * DOF-ordered vectors: x holds the initial guess on entry and the result on exit; *iterations receives
+ * m_iteration. Returns 0 (the reference reports no failure status) or a negative error.
+ */
+int uaamg_viscosity_solve_pcg(uaamg_viscosity* h, float tolerance, float* x, const float* b, int32_t* iterations);
+/* Same with x and b in device memory. */
+int uaamg_viscosity_solve_pcg_device(uaamg_viscosity* h, float tolerance, float* x_dev, const float* b_dev,
+                                     int32_t* iterations);
+
+/*
+ * Level operations on host DOF-ordered vectors: UAAMG_VMODE_* (L_apply, residual_apply, Jacobi_apply,
+ * SPAI0_apply, XYZ_RBGS_apply, ZYX_RBGS_apply; simd_viscosity3d.cpp:2195-2302). RED_GS / BLACK_GS run
+ * that colour pass on channels 0, 1, 2 in turn. b may be NULL for UAAMG_VMODE_NORMAL.
+ */
+int uaamg_viscosity_apply(uaamg_viscosity* h, int level, int mode, const float* x, const float* b, float* y);
+/* restriction from fine_level to fine_level + 1 (:2353-2413). */
+int uaamg_viscosity_restrict(uaamg_viscosity* h, int fine_level, const float* fine, float* coarse);
+/* prolongation<true>: fine += coarse (:2659-2729). */
+int uaamg_viscosity_prolong(uaamg_viscosity* h, int fine_level, const float* coarse, float* fine);
+/* solve_exact on the coarsest level (:2645-2657). */
+int uaamg_viscosity_coarsest_solve(uaamg_viscosity* h, const float* b, float* x);
+/* One W-cycle from zero, z = M r (mucycle(z, r, 2, true, 0, 2), :3828-3884). */
+int uaamg_viscosity_precondition(uaamg_viscosity* h, const float* r, float* z);
+
+/* Measurement hooks for one colour / operator pass over channel c (no reference counterpart). */
+int uaamg_viscosity_algorithmic_bytes(const uaamg_viscosity* h, int level, int mode, int channel, double* bytes);
+int uaamg_viscosity_time_apply(uaamg_viscosity* h, int level, int mode, int channel, int reps,
+                               float* ms_per_launch);
+
+/*
+ * Synthetic viscosity scene (stand-in for FLIP's grids, FLIP_vdb.cpp:3170-3183): the liquid SDF of
+ * scene `config`, the container walls (plus the solid sphere when the scene has one) as solid SDF,
+ * constant viscosity mu, face velocities U[-1,1]; dt and rho as given.
+ */
+typedef struct uaamg_visc_scene uaamg_visc_scene;
+int uaamg_visc_scene_create(int config, int32_t n, uint64_t seed, float mu, float dt, float rho,
+                            uaamg_visc_scene** out);
+int uaamg_visc_scene_grids(const uaamg_visc_scene* s, uaamg_grid* liquid_sdf, uaamg_grid* solid_sdf,
+                           uaamg_grid* viscosity, uaamg_grid* velocity, float* dx, float* dt, float* rho);
+void uaamg_visc_scene_destroy(uaamg_visc_scene* s);
+
+/* volume_fraction of a cube from its 8 corner values (volume_fractions.cpp:100-126), for checks. */
+float uaamg_volume_fraction8(const float* phi8);
+
 /* Number of HIP devices visible (0 when none; never fails). */
 int uaamg_device_count(void);
 

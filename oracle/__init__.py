@@ -1,7 +1,8 @@
 """ORACLE - TEST INFRASTRUCTURE ONLY.
 
-ctypes wrapper of the CPU restatement of the reference's UAAMG-PCG Poisson
-path (oracle/poisson_oracle.cpp). Only tests/, __graft_entry__.smoke() and the
+ctypes wrapper of the CPU restatements of the reference's UAAMG-PCG Poisson
+path (oracle/poisson_oracle.cpp) and variational viscosity path
+(oracle/viscosity_oracle.cpp). Only tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py may import this package; the product never does.
 """
 from __future__ import annotations
@@ -55,6 +56,30 @@ def lib() -> C.CDLL:
         "orc_solve_mg": (i32, [vp, i32, f32, i32, vp, vp, C.POINTER(i32)]),
         "orc_solve_pcg64": (i32, [vp, i32, f64, i32, vp, vp, C.POINTER(i32)]),
         "orc_apply64": (i32, [vp, i32, vp, vp]),
+        "orc_volume_fraction8": (f32, [vp]),
+        "orc_volume_fraction4": (f32, [f32, f32, f32, f32]),
+        "orc_visc_create": (vp, [i64, vp, vp, vp, f32, i64, vp, vp, vp, f32, i64, vp, vp, vp, f32,
+                                 f32, f32, f32, i64, i32]),
+        "orc_visc_destroy": (None, [vp]),
+        "orc_visc_nlevels": (i32, [vp]),
+        "orc_visc_level_ndof": (i64, [vp, i32]),
+        "orc_visc_level_nleaf": (i64, [vp, i32]),
+        "orc_visc_level_default": (f32, [vp, i32, i32]),
+        "orc_visc_level_topology": (None, [vp, i32, vp, vp, vp]),
+        "orc_visc_level_coefficients": (None, [vp, i32, i32, vp, vp]),
+        "orc_visc_level_active": (i64, [vp, i32, i32, vp, vp]),
+        "orc_visc_face_volume": (None, [vp, i32, vp]),
+        "orc_visc_volume_at": (None, [vp, i32, i64, vp, vp]),
+        "orc_visc_dof_at": (None, [vp, i32, i32, i64, vp, vp]),
+        "orc_visc_level_matrix": (i64, [vp, i32, vp, vp, vp]),
+        "orc_visc_build_rhs": (None, [vp, i64, vp, vp, vp, vp, i64, vp, vp, vp, vp, vp]),
+        "orc_visc_apply": (i32, [vp, i32, i32, vp, vp, vp]),
+        "orc_visc_restrict": (i32, [vp, i32, vp, vp]),
+        "orc_visc_prolong": (i32, [vp, i32, vp, vp]),
+        "orc_visc_coarsest": (i32, [vp, vp, vp]),
+        "orc_visc_precondition": (i32, [vp, vp, vp]),
+        "orc_visc_solve": (i32, [vp, f32, i32, vp, vp, vp]),
+        "orc_visc_solve64": (i32, [vp, f64, i32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -218,3 +243,162 @@ def fraction_inside2(a, b):
 
 def fraction_inside4(bl, br, tl, tr):
     return lib().orc_fraction_inside4(bl, br, tl, tr)
+
+
+def volume_fraction8(p) -> float:
+    a = np.ascontiguousarray(p, dtype=np.float32)
+    return float(lib().orc_volume_fraction8(_p(a)))
+
+
+def volume_fraction4(a, b, c, d) -> float:
+    return float(lib().orc_volume_fraction4(a, b, c, d))
+
+
+class OracleViscosity:
+    """Reference-semantics CPU viscosity hierarchy + solver (simd_viscosity3d)."""
+
+    def __init__(self, viscosity, liquid, solid, dt: float, rho: float, dx: float, max_coarsest: int = 4000,
+                 trim: bool = True):
+        L = lib()
+        self._keep = []
+
+        def g(grid):
+            o = np.ascontiguousarray(grid.origins, np.int32)
+            v = np.ascontiguousarray(grid.values, np.float32)
+            a = None if grid.active is None else np.ascontiguousarray(grid.active, np.uint64)
+            self._keep += [o, v, a]
+            bg = grid.background[0] if isinstance(grid.background, (tuple, list)) else grid.background
+            return [o.shape[0], _p(o), _p(a), _p(v), float(bg)]
+
+        self._h = L.orc_visc_create(*g(liquid), *g(solid), *g(viscosity), float(dt), float(rho), float(dx),
+                                    int(max_coarsest), int(trim))
+        self._keep = []
+        self.nlevels = int(L.orc_visc_nlevels(self._h))
+        self.ndof = self.level_ndof(0)
+
+    def level_ndof(self, level: int) -> int:
+        return int(lib().orc_visc_level_ndof(self._h, level))
+
+    def level_nleaf(self, level: int) -> int:
+        return int(lib().orc_visc_level_nleaf(self._h, level))
+
+    def defaults(self, level: int):
+        return float(lib().orc_visc_level_default(self._h, level, 0)), float(lib().orc_visc_level_default(
+            self._h, level, 1))
+
+    def topology(self, level: int):
+        n = self.level_nleaf(level)
+        org = np.zeros((n, 3), np.int32)
+        mask = np.zeros((n, 3, 8), np.uint64)
+        base = np.zeros((n, 3), np.int64)
+        lib().orc_visc_level_topology(self._h, level, _p(org), _p(mask), _p(base))
+        return org, mask, base
+
+    def coefficients(self, level: int, grid: int):
+        n = self.level_nleaf(level)
+        br = np.zeros((n, 512), np.float32)
+        kept = np.zeros(n, np.uint8)
+        lib().orc_visc_level_coefficients(self._h, level, grid, _p(br), _p(kept))
+        return br, kept
+
+    def active(self, level: int, grid: int):
+        n = int(lib().orc_visc_level_active(self._h, level, grid, None, None))
+        xyz = np.zeros((n, 3), np.int32)
+        val = np.zeros(n, np.float32)
+        lib().orc_visc_level_active(self._h, level, grid, _p(xyz), _p(val))
+        return xyz, val
+
+    def face_volume(self, c: int):
+        out = np.zeros(self.ndof, np.float32)
+        lib().orc_visc_face_volume(self._h, c, _p(out))
+        return out
+
+    def volume_at(self, kind: int, xyz):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        out = np.zeros(xyz.shape[0], np.float32)
+        lib().orc_visc_volume_at(self._h, kind, xyz.shape[0], _p(xyz), _p(out))
+        return out
+
+    def dof_at(self, level: int, c: int, xyz):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        out = np.zeros(xyz.shape[0], np.int32)
+        lib().orc_visc_dof_at(self._h, level, c, xyz.shape[0], _p(xyz), _p(out))
+        return out
+
+    def matrix(self, level: int):
+        nnz = int(lib().orc_visc_level_matrix(self._h, level, None, None, None))
+        n = self.level_ndof(level)
+        ptr = np.zeros(n + 1, np.int64)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz, np.float32)
+        lib().orc_visc_level_matrix(self._h, level, _p(ptr), _p(col), _p(val))
+        return ptr, col, val
+
+    def build_rhs(self, velocity, solid_velocity=None):
+        keep = []
+
+        def g(grid, default_n=0):
+            if grid is None:
+                return [0, None, None, None, None]
+            o = np.ascontiguousarray(grid.origins, np.int32)
+            v = np.ascontiguousarray(grid.values, np.float32)
+            a = None if grid.active is None else np.ascontiguousarray(grid.active, np.uint64)
+            bg = np.asarray(grid.background, np.float32).reshape(-1)
+            bg = np.ascontiguousarray(np.resize(bg, 3), np.float32)
+            keep.extend([o, v, a, bg])
+            return [o.shape[0], _p(o), _p(a), _p(v), _p(bg)]
+
+        out = np.zeros(self.ndof, np.float32)
+        lib().orc_visc_build_rhs(self._h, *g(velocity), *g(solid_velocity), _p(out))
+        return out
+
+    def apply(self, level: int, mode: int, x, b=None):
+        x = np.ascontiguousarray(x, np.float32)
+        b = np.zeros_like(x) if b is None else np.ascontiguousarray(b, np.float32)
+        y = np.zeros(self.level_ndof(level), np.float32)
+        lib().orc_visc_apply(self._h, level, mode, _p(x), _p(b), _p(y))
+        return y
+
+    def restriction(self, fine_level: int, fine):
+        fine = np.ascontiguousarray(fine, np.float32)
+        out = np.zeros(self.level_ndof(fine_level + 1), np.float32)
+        lib().orc_visc_restrict(self._h, fine_level, _p(fine), _p(out))
+        return out
+
+    def prolongation(self, fine_level: int, coarse, fine):
+        coarse = np.ascontiguousarray(coarse, np.float32)
+        out = np.array(fine, np.float32, copy=True)
+        lib().orc_visc_prolong(self._h, fine_level, _p(coarse), _p(out))
+        return out
+
+    def coarsest_solve(self, b):
+        b = np.ascontiguousarray(b, np.float32)
+        x = np.zeros(self.level_ndof(self.nlevels - 1), np.float32)
+        lib().orc_visc_coarsest(self._h, _p(b), _p(x))
+        return x
+
+    def precondition(self, r):
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.zeros_like(r)
+        lib().orc_visc_precondition(self._h, _p(r), _p(z))
+        return z
+
+    def solve(self, b, x0=None, tol: float = 1e-6, maxit: int = 100):
+        """pcg_solve; returns (m_iteration, x, ||r||_inf history)."""
+        x = np.zeros(self.ndof, np.float32) if x0 is None else np.array(x0, np.float32, copy=True)
+        b = np.ascontiguousarray(b, np.float32)
+        hist = np.zeros(maxit + 2, np.float32)
+        it = int(lib().orc_visc_solve(self._h, float(tol), int(maxit), _p(x), _p(b), _p(hist)))
+        return it, x, hist[: it + 2]
+
+    def solve64(self, b, x0=None, tol: float = 1e-6, maxit: int = 100):
+        x = np.zeros(self.ndof, np.float64) if x0 is None else np.array(x0, np.float64, copy=True)
+        b = np.ascontiguousarray(b, np.float64)
+        it = int(lib().orc_visc_solve64(self._h, float(tol), int(maxit), _p(x), _p(b)))
+        return it, x
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_visc_destroy(h)
+            self._h = None

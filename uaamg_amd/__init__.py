@@ -21,7 +21,8 @@ import numpy as np
 
 __all__ = [
     "Grid", "Scene", "LaplacianWithLevel", "PoissonSolver", "SmootherOption", "library_path", "lib",
-    "UaamgError", "SUCCESS", "FAILED", "device_count",
+    "UaamgError", "SUCCESS", "FAILED", "device_count", "L_with_level", "simd_viscosity3d", "ViscScene",
+    "packed_FloatGrid3",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -52,6 +53,16 @@ class _Options(C.Structure):
         ("max_iterations", C.c_int32),
         ("smoother", C.c_int32),
         ("precision", C.c_int32),
+        ("device", C.c_int32),
+        ("max_coarsest_dof", C.c_int32),
+        ("flags", C.c_int32),
+        ("stream", C.c_void_p),
+    ]
+
+
+class _ViscOptions(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int32),
         ("device", C.c_int32),
         ("max_coarsest_dof", C.c_int32),
         ("flags", C.c_int32),
@@ -109,6 +120,30 @@ def lib() -> C.CDLL:
         "uaamg_scene_destroy": (None, [vp]),
         "uaamg_fraction_inside2": (f32, [f32, f32]),
         "uaamg_fraction_inside4": (f32, [f32, f32, f32, f32]),
+        "uaamg_viscosity_options_default": (C.c_int, [P(_ViscOptions)]),
+        "uaamg_viscosity_create": (C.c_int, [P(_Grid), P(_Grid), P(_Grid), f32, f32, f32, P(_ViscOptions), P(vp)]),
+        "uaamg_viscosity_destroy": (None, [vp]),
+        "uaamg_viscosity_set_options": (C.c_int, [vp, P(_ViscOptions)]),
+        "uaamg_viscosity_ndof": (i64, [vp]),
+        "uaamg_viscosity_nlevels": (C.c_int, [vp]),
+        "uaamg_viscosity_level_info": (C.c_int, [vp, C.c_int, P(i64), vp, P(i64)]),
+        "uaamg_viscosity_level_topology": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+        "uaamg_viscosity_level_coefficients": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
+        "uaamg_viscosity_level_matrix": (C.c_int, [vp, C.c_int, P(i64), vp, vp, vp]),
+        "uaamg_viscosity_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), vp]),
+        "uaamg_viscosity_solve_pcg": (C.c_int, [vp, f32, vp, vp, P(i32)]),
+        "uaamg_viscosity_solve_pcg_device": (C.c_int, [vp, f32, vp, vp, P(i32)]),
+        "uaamg_viscosity_apply": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),
+        "uaamg_viscosity_restrict": (C.c_int, [vp, C.c_int, vp, vp]),
+        "uaamg_viscosity_prolong": (C.c_int, [vp, C.c_int, vp, vp]),
+        "uaamg_viscosity_coarsest_solve": (C.c_int, [vp, vp, vp]),
+        "uaamg_viscosity_precondition": (C.c_int, [vp, vp, vp]),
+        "uaamg_viscosity_algorithmic_bytes": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_double)]),
+        "uaamg_viscosity_time_apply": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(f32)]),
+        "uaamg_visc_scene_create": (C.c_int, [C.c_int, i32, C.c_uint64, f32, f32, f32, P(vp)]),
+        "uaamg_visc_scene_grids": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(f32), P(f32), P(f32)]),
+        "uaamg_visc_scene_destroy": (None, [vp]),
+        "uaamg_volume_fraction8": (f32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -409,3 +444,6 @@ class PoissonSolver:
         z = np.zeros_like(r)
         _check(lib().uaamg_poisson_precondition(self.mMultigridLevel0._h, _ptr(r), _ptr(z)), "precondition")
         return z
+
+
+from .viscosity import L_with_level, ViscScene, packed_FloatGrid3, simd_viscosity3d  # noqa: E402

@@ -25,6 +25,10 @@ SOURCES = [
     "tail_kernels.hip",
     "poisson_solver.cpp",
     "capi.cpp",
+    "viscosity_host.cpp",
+    "viscosity_kernels.hip",
+    "viscosity_solver.cpp",
+    "capi_viscosity.cpp",
 ]
 
 ARCH = os.environ.get("UAAMG_OFFLOAD_ARCH", "gfx950")

@@ -10,6 +10,7 @@
 
 #include "poisson_host.h"
 #include "poisson_solver.h"
+#include "capi_common.h"
 #include "scene.h"
 
 using namespace uaamg;
@@ -28,7 +29,8 @@ struct uaamg_scene {
     Scene s;
 };
 
-namespace {
+namespace uaamg {
+namespace capi {
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
@@ -51,25 +53,17 @@ bool to_host_grid(const uaamg_grid* g, int ncomp, HostGrid& out, const char* wha
     }
     return true;
 }
+}  // namespace capi
+}  // namespace uaamg
 
+using namespace uaamg::capi;
+
+namespace {
 int hip_status(int rc, const PoissonDevice& d) {
     if (rc == -4) g_err = "handle was created without a device (device < 0)";
     else if (rc < 0) g_err = d.error().empty() ? "device error" : d.error();
     return rc;
 }
-
-#define UAAMG_GUARD_BEGIN try {
-#define UAAMG_GUARD_END                                              \
-    }                                                                \
-    catch (const std::bad_alloc&) {                                  \
-        return fail(UAAMG_ENOMEM, "host allocation failed");         \
-    }                                                                \
-    catch (const std::exception& e) {                                \
-        return fail(UAAMG_EINVAL, e.what());                         \
-    }                                                                \
-    catch (...) {                                                    \
-        return fail(UAAMG_EINVAL, "unexpected error");               \
-    }
 
 int copy_in(uaamg_poisson* h, int slot, const float* src, int64_t n) {
     if (!h->dev.on_device()) return fail(UAAMG_ENODEV, "handle was created without a device (device < 0)");

@@ -340,7 +340,8 @@ hipError_t launch_dot(const LevelView& L, const float* a, const float* b, PcgSca
     const int g = reduction_blocks(L);
     dot_kernel<0><<<unsigned(g), kBlockThreads, 0, s>>>(L, a, b, partials);
     if (hipError_t e = hipGetLastError()) return e;
-    return finalize(target == kDotRho ? kFinRho : kFinRhoNew, g, partials, sc, nullptr, s);
+    const int op = target == kDotRho ? kFinRho : (target == kDotSigma ? kFinSigma : kFinRhoNew);
+    return finalize(op, g, partials, sc, nullptr, s);
 }
 
 hipError_t launch_laplacian_dot(const LevelView& L, const float* p, float* z, PcgScalars* sc, double* partials,

@@ -124,7 +124,7 @@ hipError_t launch_gather_dof(const LevelView& L, const float* bricks, float* dof
 // outer PCG vector kernels on level 0 (partials: >= reduction_blocks(L) doubles)
 int reduction_blocks(const LevelView& L);
 // which scalar a dot kernel finalises
-enum DotTarget { kDotRho = 0, kDotRhoNew = 1 };
+enum DotTarget { kDotRho = 0, kDotRhoNew = 1, kDotSigma = 2 };   // kDotSigma also sets alpha = rho / sigma
 hipError_t launch_dot(const LevelView& L, const float* a, const float* b, PcgScalars* sc, double* partials,
                       int target, hipStream_t s);
 // z = A p and sigma = <p, z>; finalises alpha = rho / sigma

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <unordered_map>
 
 #include "leaf_topology.h"
 #include "poisson_host.h"
@@ -193,6 +194,86 @@ void build_scene(const SceneSpec& spec, Scene& out) {
                                                          uint64_t(uint32_t(gy)), uint64_t(uint32_t(gz))) -
                                       1.f;
                 }
+            }
+        }
+    });
+}
+
+void build_visc_scene(const SceneSpec& spec, float mu, float dt, float rho, ViscScene& out) {
+    out = ViscScene();
+    build_scene(spec, out.liquid);
+    out.mu = mu;
+    out.dt = dt;
+    out.rho = rho;
+    const int n = spec.n;
+    const float dx = out.liquid.dx;
+    const float band = 3.0f * dx;
+    out.solid_background = band;
+    SdfEval sdf{&spec};
+    auto solid_phi = [&](int ix, int iy, int iz) {
+        const int m = std::min(std::min(std::min(ix, n - ix), std::min(iy, n - iy)), std::min(iz, n - iz));
+        float d = dx * float(m);
+        if (spec.has_solid) d = std::min(d, sdf.solid((ix - 0.5f) * dx, (iy - 0.5f) * dx, (iz - 0.5f) * dx));
+        return d;
+    };
+    // vertex leaves covering [-8, n + 8)
+    std::vector<Coord> sorg;
+    for (int lx = -8; lx < n + 8; lx += 8)
+        for (int ly = -8; ly < n + 8; ly += 8)
+            for (int lz = -8; lz < n + 8; lz += 8) sorg.push_back(Coord{lx, ly, lz});
+    std::vector<uint8_t> keep(sorg.size(), 0);
+    std::vector<float> vals(sorg.size() * kLeafSize);
+    std::vector<uint64_t> act(sorg.size() * 8, 0);
+    parallel_for(int64_t(sorg.size()), [&](int64_t b, int64_t e) {
+        for (int64_t li = b; li < e; ++li) {
+            const Coord o = sorg[li];
+            for (int off = 0; off < kLeafSize; ++off) {
+                const float d = solid_phi(o.x + (off >> 6), o.y + ((off >> 3) & 7), o.z + (off & 7));
+                if (std::fabs(d) < band) {
+                    keep[li] = 1;
+                    act[li * 8 + (off >> 6)] |= uint64_t(1) << (off & 63);
+                    vals[li * kLeafSize + off] = d;
+                } else {
+                    vals[li * kLeafSize + off] = d < 0 ? -band : band;
+                }
+            }
+        }
+    });
+    for (size_t li = 0; li < sorg.size(); ++li) {
+        if (!keep[li]) continue;
+        out.solid_origins.insert(out.solid_origins.end(), {sorg[li].x, sorg[li].y, sorg[li].z});
+        out.solid_active.insert(out.solid_active.end(), act.begin() + li * 8, act.begin() + li * 8 + 8);
+        out.solid_phi.insert(out.solid_phi.end(), vals.begin() + li * kLeafSize, vals.begin() + (li + 1) * kLeafSize);
+    }
+    // velocity leaves: liquid leaves and their 26 neighbours
+    std::vector<Coord> vorg;
+    {
+        std::unordered_map<uint64_t, int> m;
+        const Scene& L = out.liquid;
+        for (int64_t i = 0; i < L.nleaf(); ++i)
+            for (int ax = -1; ax <= 1; ++ax)
+                for (int ay = -1; ay <= 1; ++ay)
+                    for (int az = -1; az <= 1; ++az) {
+                        const Coord c{L.origins[3 * i] + 8 * ax, L.origins[3 * i + 1] + 8 * ay, L.origins[3 * i + 2] + 8 * az};
+                        if (m.emplace(pack_origin(c), 1).second) vorg.push_back(c);
+                    }
+    }
+    std::sort(vorg.begin(), vorg.end(), tree_order_less);
+    out.vel_origins.resize(vorg.size() * 3);
+    out.vel.resize(vorg.size() * kLeafSize * 3);
+    parallel_for(int64_t(vorg.size()), [&](int64_t b, int64_t e) {
+        for (int64_t li = b; li < e; ++li) {
+            const Coord o = vorg[li];
+            out.vel_origins[3 * li] = o.x;
+            out.vel_origins[3 * li + 1] = o.y;
+            out.vel_origins[3 * li + 2] = o.z;
+            for (int off = 0; off < kLeafSize; ++off) {
+                const int gx = o.x + (off >> 6), gy = o.y + ((off >> 3) & 7), gz = o.z + (off & 7);
+                for (int c = 0; c < 3; ++c)
+                    out.vel[(size_t(li) * kLeafSize + off) * 3 + c] =
+                        2.f * hash_uniform(spec.velocity_seed, uint64_t(c) + 11, uint64_t(uint32_t(gx)),
+                                           uint64_t(uint32_t(gy)), uint64_t(uint32_t(gz))) -
+                        1.f;
             }
         }
     });

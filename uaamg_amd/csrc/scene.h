@@ -45,6 +45,28 @@ float fraction_inside(float phi_bl, float phi_br, float phi_tl, float phi_tr);
 
 void build_scene(const SceneSpec& spec, Scene& out);
 
+// Inputs of the variational viscosity solve (simd_viscosity3d ctor,
+// simd_viscosity3d.h:285-290) for a scene: the liquid SDF of build_scene; the
+// solid SDF on the vertex lattice (index i at world (i - 1/2) dx, the
+// m_voxel_vertex_transform of FLIP_vdb.cpp:2130-2131) = the container walls
+// of the domain box (positive inside, FLIP_vdb.cpp:2135-2162) united with the
+// solid sphere when the scene has one, kept as leaves where |phi| < 3 dx
+// (background 3 dx); a constant viscosity; face velocities U[-1,1] (channel
+// c of voxel v at world (v - e_c/2) dx) on the liquid leaves and their 26
+// neighbours; zero solid velocity.
+struct ViscScene {
+    Scene liquid;
+    float mu = 1.0f, dt = 1.0f / 24.0f, rho = 1000.0f;
+    float solid_background = 0.f;
+    std::vector<int32_t> solid_origins;
+    std::vector<uint64_t> solid_active;
+    std::vector<float> solid_phi;
+    std::vector<int32_t> vel_origins;
+    std::vector<float> vel;          // [nleaf][512][3]
+};
+
+void build_visc_scene(const SceneSpec& spec, float mu, float dt, float rho, ViscScene& out);
+
 // Preset geometries: 0 = pool 40% + sphere r=0.15 at (0.5,0.7,0.5);
 // 1 = pool 30% + 8 seeded spheres r in [0.05,0.2] + solid sphere r=0.2 at the
 // centre; 3 = one randomised batch problem (edge log-uniform in [32,192],

@@ -1,0 +1,67 @@
+// Device kernels of the variational viscosity solve on 8^3 leaf bricks (gfx950).
+//
+// Layout in HBM, per level: a vector is [leaf][channel][512] floats (the three
+// face-velocity channels of a leaf side by side, voxel offset 64x+8y+z) over
+// the leaves holding a DOF of any channel; entries that are not DOFs hold 0.
+// Coefficient grids are 512-float bricks in one pool, found through
+// slot[leaf][grid] (-1: the level's default constant). One wavefront works
+// on one (leaf, channel) pair, one lane on one z-row (x,y) = (lane>>3, lane&7).
+// Every neighbour value is "the value at a global voxel": the leaf is found in
+// a 3x3x3 leaf-neighbourhood table, a missing leaf reads 0 (vectors) or the
+// default (coefficients), which is how the reference's applier reads absent
+// leaves (simd_viscosity3d.cpp:2991-3000, vdb_SIMD_IO.h:5-191).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace uaamg {
+
+struct ViscLevelView {
+    int32_t nleaf;             // leaves of the table (DOF leaves + coefficient-only leaves)
+    int32_t nleaf_dof;         // leaves holding vector bricks
+    const int32_t* nb27;       // [nleaf][27]
+    const uint64_t* mask;      // [nleaf_dof][3][8]
+    const int32_t* slot;       // [nleaf][32]
+    const float* pool;         // [nslot][512]
+    const int32_t* chan_leaves;   // DOF leaves of channel 0, then 1, then 2
+    int32_t chan_off[4];
+    const int32_t* parent;     // [nleaf_dof] coarse leaf * 8 + octant
+    const int32_t* child;      // [nleaf_dof][8]
+    float dflt[32];
+};
+
+// applier modes (light_weight_applier working_mode, simd_viscosity3d.h:13)
+enum ViscMode : int { kVNormal = 0, kVResidual = 1, kVJacobi = 2, kVSpai0Mode = 3, kVRedGS = 4, kVBlackGS = 5 };
+
+// Eigen-CG restatement of the coarsest level on its explicit rows
+struct ViscCoarsestView {
+    int32_t n;
+    const int64_t* ptr;        // [n+1]
+    const int32_t* col;        // column ascending
+    const float* val;
+    const float* invdiag;      // Eigen DiagonalPreconditioner: 1/diag or 1
+    const int32_t* pos;        // [n] linear index of the DOF in the level's vector
+    int32_t max_iters;
+    float tol;
+};
+
+constexpr int kViscMaxCoarsest = 4096;
+constexpr int kViscMaxRow = 16;    // 1 + 6 + 8 entries per row at most
+
+// y = op(x, b) on channel c of a level (Gauss-Seidel modes in place: y == x)
+hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int channel, const float* x, const float* b, float* y,
+                             hipStream_t s);
+// coarse = restriction of fine (restriction, simd_viscosity3d.cpp:2353-2413), all channels
+hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
+                                float* coarse_v, hipStream_t s);
+// fine += coarse on fine DOFs (prolongation<true>, :2659-2729), all channels
+hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView& coarse, const float* coarse_v,
+                               float* fine_v, hipStream_t s);
+// x = coarsest solve of b (vectors in level layout), one workgroup
+hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float* x, hipStream_t s);
+// zero every brick of a level vector (set_grid_to_zero, :2304-2315)
+hipError_t launch_visc_zero(const ViscLevelView& L, float* v, hipStream_t s);
+
+}  // namespace uaamg

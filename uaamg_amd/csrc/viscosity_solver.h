@@ -1,0 +1,113 @@
+// Device-resident variational viscosity solver: owns the level hierarchy in
+// HBM and drives the W-cycle preconditioner and the outer PCG of
+// simd_viscosity3d (simd_viscosity3d.h:283-321, .cpp:3621-3884).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "poisson_kernels.h"
+#include "poisson_solver.h"
+#include "viscosity_host.h"
+#include "viscosity_kernels.h"
+
+namespace uaamg {
+
+struct ViscOptions {
+    float tolerance = 1e-6f;        // pcg_solve's tolerance argument
+    int max_iterations = 100;       // m_max_iter
+    int device = 0;
+    int64_t max_coarsest_dof = 4000;
+    int flags = 0;                  // bit 1: no graph capture (A-B checks)
+    hipStream_t stream = nullptr;
+};
+
+class ViscDevice {
+public:
+    ~ViscDevice();
+    // returns 0 or a negative uaamg status
+    int init(ViscHierarchyHost&& h, const ViscOptions& opt);
+    void init_host_only(ViscHierarchyHost&& h, const ViscOptions& opt);
+    bool on_device() const { return !lv_.empty(); }
+
+    int nlevels() const { return int(host_.levels.size()); }
+    const ViscHierarchyHost& host() const { return host_; }
+    ViscHierarchyHost& host_mut() { return host_; }
+    ViscOptions& options() { return opt_; }
+    hipStream_t stream() const { return stream_; }
+    const std::string& error() const { return err_; }
+
+    // pcg_solve on DOF-ordered device vectors: x holds the initial guess on
+    // entry and the result on exit; *iterations receives m_iteration.
+    int solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations);
+
+    // unit operations on DOF-ordered device vectors (parity tests). Modes
+    // 0..3 apply to every channel; 4 / 5 run one colour pass on channels 0,
+    // 1, 2 in turn (in place on a copy of x); 6 / 7 are XYZ_RBGS / ZYX_RBGS.
+    int apply_dev(int level, int mode, const float* x_dof, const float* b_dof, float* y_dof);
+    int restrict_dev(int fine_level, const float* fine_dof, float* coarse_dof);
+    int prolong_dev(int fine_level, const float* coarse_dof, float* fine_dof);   // fine += coarse
+    int coarsest_dev(const float* b_dof, float* x_dof);
+    int precondition_dev(const float* r_dof, float* z_dof);
+
+    float* staging(int i) { return staging_[i]; }
+
+    // bytes one pass of `mode` over channel c of `level` must move (each
+    // stream once per DOF, coefficient streams only where stored)
+    double algorithmic_bytes(int level, int mode, int channel) const;
+    // mean ms per launch of `reps` back-to-back passes, HIP events on the stream
+    int time_apply(int level, int mode, int channel, int reps, float* ms_per_launch);
+
+private:
+    struct DevLevel {
+        ViscLevelView v{};
+        LevelView flat{};          // the vector as 3*nleaf_dof bricks for the PCG kernels
+        float *X = nullptr, *B = nullptr, *T = nullptr;
+    };
+    struct Graph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+    };
+
+    int upload_level(const ViscLevelHost& h, DevLevel& d);
+    int upload_spai0(int level);
+    int build_coarsest();
+    void rbgs(int level, float* x, const float* b, bool xyz);
+    void mucycle(int level, float* x, const float* b, bool precon);
+    void apply_all(int level, int mode, const float* x, const float* b, float* y);
+    void issue_init();
+    void issue_first();
+    void issue_iter();
+    int capture(Graph& g, const std::function<void()>& body);
+    int ensure_graphs();
+    void destroy_graphs();
+    int run(Graph& g, const std::function<void()>& body);
+    int wait_nu(float* nu);
+    int check(hipError_t e, const char* what);
+
+    ViscHierarchyHost host_;
+    ViscOptions opt_;
+    hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
+    DeviceArena arena_;
+    std::vector<DevLevel> lv_;
+    float *x0_ = nullptr, *b0_ = nullptr, *r0_ = nullptr, *p0_ = nullptr, *z0_ = nullptr;
+    float* staging_[3] = {nullptr, nullptr, nullptr};
+    PcgScalars* sc_ = nullptr;
+    double* partials_ = nullptr;
+    ViscCoarsestView cv_{};
+    std::vector<float*> spai0_bufs_;
+    Graph g_init_, g_first_, g_iter_;
+    bool graphs_ready_ = false;
+    bool use_graphs_ = true;
+    HostMailbox* mailbox_ = nullptr;
+    HostMailbox* mailbox_dev_ = nullptr;
+    unsigned int nu_seq_ = 0;
+    hipError_t first_error_ = hipSuccess;
+    std::string err_;
+};
+
+}  // namespace uaamg
