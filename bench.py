@@ -41,6 +41,18 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--workload", choices=["poisson", "viscosity", "batch", "serve"], default="poisson",
+                    help="poisson: the headline line (plus viscosity / batch / serving objects unless --no-*); "
+                         "viscosity, batch, serve: only that measurement (profiling runs)")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--no-viscosity", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-size", type=int, default=64)
+    ap.add_argument("--batch-concurrency", type=int, default=4)
+    ap.add_argument("--visc-n", type=int, default=128)
+    ap.add_argument("--visc-steps", type=int, default=10)
     ap.add_argument("--kernel-only", action="store_true",
                     help="build the workload, then only run the level-0 Jacobi sweep --kernel-reps times and a "
                          "256 MiB device copy 10 times (short run for hardware-counter passes)")
@@ -60,6 +72,11 @@ def main() -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 3
     dev = torch.device("cuda:0")
+
+    if args.workload != "poisson":
+        fn = {"viscosity": run_viscosity, "batch": run_batch, "serve": run_serving}[args.workload]
+        print(json.dumps(fn(args, dev)), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     scene = U.Scene(args.config, args.n, seed=args.seed)
@@ -168,8 +185,256 @@ def main() -> int:
         out["rel_l2_vs_cpu_fp32"] = float(np.linalg.norm(x_gpu - x_cpu) / np.linalg.norm(x_cpu))
         out["true_rel_residual_inf"] = float(np.abs(r64).max() / np.abs(b_orc).max())
         out["speedup_vs_cpu"] = round(cpu_s * 1e3 / ms_per_solve, 2)
+    if not args.no_viscosity:
+        out["viscosity"] = run_viscosity(args, dev)
+    if not args.no_batch:
+        out["batch"] = run_batch(args, dev)
+    if not args.no_serve:
+        out["serving"] = run_serving(args, dev)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def run_serving(args, dev) -> dict:
+    """The serving configuration: a pool of problems (80% Poisson with edge
+    log-uniform in [48, 384], 20% viscosity with edge log-uniform in [48, 192]),
+    requests arriving as a Poisson process, each request re-solving one
+    problem warm-started from its previous solution with a right-hand side
+    perturbed by 5% (pre-generated on the device). `--serve-workers` host
+    threads serve the queue (one request per problem at a time). Reports
+    p50 / p99 latency from arrival to completion and completed solves/s."""
+    import queue
+    import threading
+
+    import numpy as np
+    import torch
+
+    import uaamg_amd as U
+
+    rng = np.random.default_rng(2024)
+    n_pool, n_visc, n_rhs = 10, 2, 4
+    t0 = time.perf_counter()
+    pool = []
+    for k in range(n_pool):
+        visc = k < n_visc
+        lo, hi = (48, 192) if visc else (48, 384)
+        n = int(round(lo * np.exp(rng.uniform() * np.log(hi / lo))))
+        n = max(lo, min(hi, 8 * round(n / 8)))
+        if visc:
+            sc = U.ViscScene(3, n, seed=100 + k, mu=1.0)
+            lw = U.L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=0)
+            sol = U.simd_viscosity3d(lw, sc.velocity)
+            b0 = torch.from_numpy(sol.m_rhs.data).to(dev)
+
+            def solve(x, b, sol=sol):
+                sol.solve_pcg_device(1e-6, x.data_ptr(), b.data_ptr())
+            keep = (sc, lw, sol)
+        else:
+            sc = U.Scene(3, n, seed=100 + k)
+            lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=0)
+            ps = U.PoissonSolver(lap)
+            ps._push_options()
+            b0 = torch.from_numpy(lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)).to(dev)
+
+            def solve(x, b, lap=lap):
+                lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+            keep = (sc, lap, ps)
+        g = torch.Generator(device=dev).manual_seed(k)
+        rhs = [b0 * (1 + 0.05 * torch.randn(b0.shape, device=dev, generator=g)) for _ in range(n_rhs)]
+        pool.append({"n": n, "visc": visc, "solve": solve, "x": torch.zeros_like(b0), "rhs": rhs,
+                     "lock": threading.Lock(), "keep": keep})
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    # warm-up: one solve each, and the mean service time that sets the arrival rate
+    svc = []
+    for p in pool:
+        ts = time.perf_counter()
+        p["solve"](p["x"], p["rhs"][0])
+        svc.append(time.perf_counter() - ts)
+    mean_svc = float(np.mean(svc))
+    rate = 1.0 / mean_svc   # one request per mean service time: about 1/workers utilisation per worker
+
+    q: "queue.Queue" = queue.Queue()
+    lat, lock = [], threading.Lock()
+
+    def worker():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            t_arr, k, r = item
+            p = pool[k]
+            with p["lock"]:
+                p["solve"](p["x"], p["rhs"][r])
+            with lock:
+                lat.append(time.perf_counter() - t_arr)
+
+    threads = [threading.Thread(target=worker, daemon=True) for _ in range(args.serve_workers)]
+    for t in threads:
+        t.start()
+    t_start = time.perf_counter()
+    t_next = t_start
+    sent = 0
+    while True:
+        t_next += rng.exponential(1.0 / rate)
+        if t_next - t_start > args.serve_seconds:
+            break
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        q.put((t_next, int(rng.integers(n_pool)), int(rng.integers(n_rhs))))
+        sent += 1
+    for _ in threads:
+        q.put(None)
+    for t in threads:
+        t.join()
+    wall = time.perf_counter() - t_start
+    lat_ms = np.array(lat) * 1e3
+    return {"metric": "p50 / p99 latency (arrival to completion) and solves/s, Poisson-process arrivals",
+            "requests": sent, "completed": len(lat), "solves_per_s": round(len(lat) / wall, 2),
+            "arrival_rate_per_s": round(rate, 2), "workers": args.serve_workers,
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2) if len(lat) else None,
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 2) if len(lat) else None,
+            "mean_service_ms": round(1e3 * mean_svc, 2),
+            "pool": [{"n": p["n"], "kind": "viscosity" if p["visc"] else "poisson"} for p in pool],
+            "build_s": round(t_build, 2)}
+
+
+def run_batch(args, dev) -> dict:
+    """The batch configuration: 64 independent Poisson problems (edge
+    log-uniform in [32, 192], 1-6 spheres over a 10-50% pool, solid sphere
+    with probability 1/2; seeds 0..63), tol 1e-7, FP32, b resident in HBM.
+    Reports solves/s for the problems run one after another and for the
+    batch entry point (one stream per problem, `--batch-concurrency` host
+    threads driving them)."""
+    import numpy as np
+    import torch
+
+    import uaamg_amd as U
+
+    t0 = time.perf_counter()
+    laps, xs, bs, ndof = [], [], [], 0
+    for seed in range(args.batch_size):
+        sc = U.Scene(3, 0, seed=seed)
+        lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=0)
+        s = U.PoissonSolver(lap)
+        s.mRelativeTolerance = 1e-7
+        s._push_options()
+        b = torch.from_numpy(lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)).to(dev)
+        laps.append(lap)
+        bs.append(b)
+        xs.append(torch.zeros_like(b))
+        ndof += lap.mNumDof
+    t_build = time.perf_counter() - t0
+
+    def sequential():
+        its = []
+        for lap, x, b in zip(laps, xs, bs):
+            x.zero_()
+            rc, it = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+            its.append(it)
+        torch.cuda.synchronize()
+        return its
+
+    def batched():
+        for x in xs:
+            x.zero_()
+        torch.cuda.synchronize()
+        st, its = U.solve_batch_device(laps, [x.data_ptr() for x in xs], [b.data_ptr() for b in bs],
+                                       args.batch_concurrency)
+        torch.cuda.synchronize()
+        return its
+
+    sequential()
+    batched()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        its_seq = sequential()
+    t_seq = (time.perf_counter() - t0) / reps
+    x_seq = [x.cpu().numpy() for x in xs]
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        its_bat = batched()
+    t_bat = (time.perf_counter() - t0) / reps
+    same = all(np.array_equal(a, x.cpu().numpy()) for a, x in zip(x_seq, xs)) and its_seq == its_bat
+    n = args.batch_size
+    return {"metric": f"solves/s over {n} independent Poisson problems (edge log-uniform [32,192]), tol 1e-7",
+            "value": round(n / t_bat, 1), "unit": "solves/s", "problems": n, "total_ndof": ndof,
+            "ms_per_batch": round(1e3 * t_bat, 2), "sequential_ms_per_batch": round(1e3 * t_seq, 2),
+            "sequential_solves_per_s": round(n / t_seq, 1), "concurrency": args.batch_concurrency,
+            "mean_iterations": round(float(np.mean(its_bat)), 2), "identical_to_sequential": bool(same),
+            "build_s": round(t_build, 2)}
+
+
+def run_viscosity(args, dev) -> dict:
+    """The variational viscosity solve on the 128^3 pool + sphere scene (mu = 1,
+    dt = 1/24, rho = 1000, tol 1e-6, zero initial guess, zero solid velocity):
+    ms per pcg_solve with b resident in HBM, the dominant kernel's roofline
+    (one red Gauss-Seidel pass over channel 0 of level 0) and the CPU oracle
+    solving the same system."""
+    import numpy as np
+    import torch
+
+    import uaamg_amd as U
+
+    n, tol = args.visc_n, 1e-6
+    t0 = time.perf_counter()
+    sc = U.ViscScene(0, n, seed=0, mu=1.0, dt=1.0 / 24.0, rho=1000.0)
+    t_scene = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lw = U.L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=0)
+    t_build = time.perf_counter() - t0
+    solver = U.simd_viscosity3d(lw, sc.velocity)
+    b = torch.from_numpy(solver.m_rhs.data).to(dev)
+    x = torch.zeros_like(b)
+    for _ in range(2):
+        x.zero_()
+        solver.solve_pcg_device(tol, x.data_ptr(), b.data_ptr())
+    torch.cuda.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.visc_steps):
+        ts = time.perf_counter()
+        x.zero_()
+        it = solver.solve_pcg_device(tol, x.data_ptr(), b.data_ptr())
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    ms = 1e3 * (time.perf_counter() - t_start) / args.visc_steps
+    x_gpu = x.cpu().numpy()
+    red_bytes = lw.algorithmic_bytes(0, 4, 0)
+    red_ms = lw.time_apply(0, 4, 0, reps=args.kernel_reps)
+    ach = red_bytes / (red_ms * 1e-3) / 1e9
+    info = [lw.level_info(l) for l in range(lw.nlevels)]
+    out = {
+        "metric": "ms/viscosity pcg_solve to 1e-6 @128^3 (pool 40% + sphere r=0.15, mu=1, dt=1/24, rho=1000)",
+        "value": round(ms, 3), "unit": "ms/solve", "dtype": "f32",
+        "ndof": lw.m_ndof, "levels": lw.nlevels, "level_ndof": [i["ndof"] for i in info],
+        "iterations": it, "p50_ms": round(float(np.percentile(np.array(lat) * 1e3, 50)), 3),
+        "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 3),
+        "build_s": round(t_build, 3), "scene_s": round(t_scene, 3),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "visc_apply_kernel<4, 0> (level-0 red Gauss-Seidel pass, channel 0)",
+                     "bytes_per_launch": red_bytes, "ms_per_launch": round(red_ms, 5)},
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+        orc = O.OracleViscosity(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx)
+        b_o = orc.build_rhs(sc.velocity, None)
+        t0 = time.perf_counter()
+        it_c, x_cpu, _ = orc.solve(b_o, tol=tol)
+        cpu_s = time.perf_counter() - t0
+        it64, x64 = orc.solve64(b_o.astype(np.float64), tol=1e-12)
+        out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms/solve", "cores": O.get_threads(),
+                               "kind": "port",
+                               "sample": f"one FP32 pcg_solve of the same {n}^3 system (hierarchy build not timed), "
+                                         f"{it_c} iterations, OpenMP over leaves"}
+        out["cpu_iterations"] = it_c
+        out["rel_l2_err_vs_fp64_oracle"] = float(np.linalg.norm(x_gpu - x64) / np.linalg.norm(x64))
+        out["rel_l2_vs_cpu_fp32"] = float(np.linalg.norm(x_gpu - x_cpu) / np.linalg.norm(x_cpu))
+        out["speedup_vs_cpu"] = round(cpu_s * 1e3 / ms, 2)
+    return out
 
 
 if __name__ == "__main__":

@@ -156,6 +156,18 @@ int uaamg_poisson_solve_pcg(uaamg_poisson* h, float* x, const float* b, int32_t*
 /* Same with x and b in device memory (DOF-ordered, ndof floats each). */
 int uaamg_poisson_solve_pcg_device(uaamg_poisson* h, float* x_dev, const float* b_dev, int32_t* iterations);
 
+/*
+ * Many independent solves at once (batch configuration: one handle per problem, each with its own
+ * stream). Up to `concurrency` (<= 16) host threads each drive one handle's solve loop, so small
+ * problems overlap on the GPU. status[i] receives that problem's UAAMG_SUCCESS / UAAMG_FAILED,
+ * iterations[i] its mIterationTaken (may be NULL). No reference counterpart: FLIP solves one
+ * problem per call (FLIP_vdb.cpp:3132-3157); the batch is this solver's entry point for callers that
+ * hold many problems. A handle may appear only once per call.
+ */
+int uaamg_poisson_solve_batch_device(uaamg_poisson* const* handles, int32_t n, float* const* x_dev,
+                                     const float* const* b_dev, int32_t* status, int32_t* iterations,
+                                     int32_t concurrency);
+
 /* PoissonSolver::solvePureMultigrid (simd_vdb_poisson.h:163, .cpp:2228-2267), host vectors. */
 int uaamg_poisson_solve_mg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations);
 
@@ -313,6 +325,11 @@ int uaamg_viscosity_solve_pcg(uaamg_viscosity* h, float tolerance, float* x, con
 /* Same with x and b in device memory. */
 int uaamg_viscosity_solve_pcg_device(uaamg_viscosity* h, float tolerance, float* x_dev, const float* b_dev,
                                      int32_t* iterations);
+
+/* Many independent viscosity solves at once, as uaamg_poisson_solve_batch_device. */
+int uaamg_viscosity_solve_batch_device(uaamg_viscosity* const* handles, int32_t n, float tolerance,
+                                       float* const* x_dev, const float* const* b_dev, int32_t* iterations,
+                                       int32_t concurrency);
 
 /*
  * Level operations on host DOF-ordered vectors: UAAMG_VMODE_* (L_apply, residual_apply, Jacobi_apply,

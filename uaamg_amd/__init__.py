@@ -22,7 +22,7 @@ import numpy as np
 __all__ = [
     "Grid", "Scene", "LaplacianWithLevel", "PoissonSolver", "SmootherOption", "library_path", "lib",
     "UaamgError", "SUCCESS", "FAILED", "device_count", "L_with_level", "simd_viscosity3d", "ViscScene",
-    "packed_FloatGrid3",
+    "packed_FloatGrid3", "solve_batch_device",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -106,6 +106,8 @@ def lib() -> C.CDLL:
         "uaamg_poisson_solve_pcg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_pcg_device": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_mg": (C.c_int, [vp, vp, vp, P(i32)]),
+        "uaamg_poisson_solve_batch_device": (C.c_int, [vp, i32, vp, vp, vp, vp, i32]),
+        "uaamg_viscosity_solve_batch_device": (C.c_int, [vp, i32, f32, vp, vp, vp, i32]),
         "uaamg_poisson_apply": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp, vp]),
         "uaamg_poisson_first_sweep": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp]),
         "uaamg_poisson_restrict": (C.c_int, [vp, C.c_int, vp, vp]),
@@ -242,6 +244,20 @@ class Scene:
         if h:
             lib().uaamg_scene_destroy(h)
             self._h = None
+
+
+def solve_batch_device(laplacians, x_ptrs, b_ptrs, concurrency: int = 4):
+    """Solves many problems at once (one LaplacianWithLevel each, device vectors given as raw
+    pointers); returns (status list, iteration list)."""
+    n = len(laplacians)
+    hs = (C.c_void_p * n)(*[lap._h.value for lap in laplacians])
+    xs = (C.c_void_p * n)(*x_ptrs)
+    bs = (C.c_void_p * n)(*b_ptrs)
+    st = np.zeros(n, np.int32)
+    it = np.zeros(n, np.int32)
+    _check(lib().uaamg_poisson_solve_batch_device(hs, n, xs, bs, _ptr(st), _ptr(it), int(concurrency)),
+           "solve_batch_device")
+    return st.tolist(), it.tolist()
 
 
 class SmootherOption:

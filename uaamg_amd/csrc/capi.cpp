@@ -3,7 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <thread>
+#include <vector>
 #include <memory>
 #include <new>
 #include <string>
@@ -232,6 +236,42 @@ int uaamg_poisson_solve_pcg_device(uaamg_poisson* h, float* x_dev, const float* 
     int rc = h->dev.solve_pcg_dev(x_dev, b_dev, &it);
     if (iterations) *iterations = it;
     return hip_status(rc, h->dev);
+}
+
+int uaamg_poisson_solve_batch_device(uaamg_poisson* const* hs, int32_t n, float* const* x_dev,
+                                     const float* const* b_dev, int32_t* status, int32_t* iterations,
+                                     int32_t concurrency) {
+    UAAMG_GUARD_BEGIN
+    if (n < 0 || (n > 0 && (!hs || !x_dev || !b_dev || !status))) return fail(UAAMG_EINVAL, "null argument");
+    for (int32_t i = 0; i < n; ++i) {
+        if (!hs[i] || !x_dev[i] || !b_dev[i]) return fail(UAAMG_EINVAL, "null handle or vector in the batch");
+        if (!hs[i]->dev.on_device()) return fail(UAAMG_ENODEV, "a batch handle was created without a device");
+        for (int32_t j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) return fail(UAAMG_EINVAL, "a handle appears twice in the batch");
+    }
+    const int nt = std::max(1, std::min<int>(std::min<int>(concurrency, 16), n));
+    std::atomic<int32_t> next{0};
+    std::vector<std::string> errs(size_t(std::max(n, 1)));
+    auto worker = [&]() {
+        for (;;) {
+            const int32_t i = next.fetch_add(1);
+            if (i >= n) return;
+            PoissonDevice& d = hs[i]->dev;
+            int it = 0;
+            int rc = hipSetDevice(d.options().device) == hipSuccess ? d.solve_pcg_dev(x_dev[i], b_dev[i], &it) : -3;
+            status[i] = rc;
+            if (iterations) iterations[i] = it;
+            if (rc < 0) errs[size_t(i)] = d.error().empty() ? "device error" : d.error();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    for (int32_t i = 0; i < n; ++i)
+        if (status[i] < 0) return fail(UAAMG_EHIP, "batch problem " + std::to_string(i) + ": " + errs[size_t(i)]);
+    return 0;
+    UAAMG_GUARD_END
 }
 
 int uaamg_poisson_solve_pcg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations) {

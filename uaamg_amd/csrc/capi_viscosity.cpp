@@ -1,7 +1,11 @@
 // C ABI of the variational viscosity solve (include/uaamg_capi.h, viscosity section).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <memory>
 #include <vector>
 
@@ -196,6 +200,43 @@ int uaamg_viscosity_solve_pcg_device(uaamg_viscosity* h, float tolerance, float*
     const int rc = h->dev.solve_pcg_dev(x_dev, b_dev, &it);
     if (iterations) *iterations = it;
     return rc < 0 ? dev_status(rc, h->dev) : 0;
+}
+
+int uaamg_viscosity_solve_batch_device(uaamg_viscosity* const* hs, int32_t n, float tolerance, float* const* x_dev,
+                                       const float* const* b_dev, int32_t* iterations, int32_t concurrency) {
+    UAAMG_GUARD_BEGIN
+    if (n < 0 || (n > 0 && (!hs || !x_dev || !b_dev))) return fail(UAAMG_EINVAL, "null argument");
+    for (int32_t i = 0; i < n; ++i) {
+        if (!hs[i] || !x_dev[i] || !b_dev[i]) return fail(UAAMG_EINVAL, "null handle or vector in the batch");
+        if (!hs[i]->dev.on_device()) return fail(UAAMG_ENODEV, "a batch handle was created without a device");
+        for (int32_t j = 0; j < i; ++j)
+            if (hs[j] == hs[i]) return fail(UAAMG_EINVAL, "a handle appears twice in the batch");
+    }
+    const int nt = std::max(1, std::min<int>(std::min<int>(concurrency, 16), n));
+    std::atomic<int32_t> next{0};
+    std::vector<int> rcs(size_t(std::max(n, 1)), 0);
+    std::vector<std::string> errs(size_t(std::max(n, 1)));
+    auto worker = [&]() {
+        for (;;) {
+            const int32_t i = next.fetch_add(1);
+            if (i >= n) return;
+            ViscDevice& d = hs[i]->dev;
+            d.options().tolerance = tolerance;
+            int it = 0;
+            int rc = hipSetDevice(d.options().device) == hipSuccess ? d.solve_pcg_dev(x_dev[i], b_dev[i], &it) : -3;
+            rcs[size_t(i)] = rc;
+            if (iterations) iterations[i] = it;
+            if (rc < 0) errs[size_t(i)] = d.error().empty() ? "device error" : d.error();
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+    for (int32_t i = 0; i < n; ++i)
+        if (rcs[size_t(i)] < 0) return fail(UAAMG_EHIP, "batch problem " + std::to_string(i) + ": " + errs[size_t(i)]);
+    return 0;
+    UAAMG_GUARD_END
 }
 
 int uaamg_viscosity_solve_pcg(uaamg_viscosity* h, float tolerance, float* x, const float* b, int32_t* iterations) {
