@@ -173,12 +173,13 @@ def test_nan_rhs_fails(problem):
 
 @pytest.mark.parametrize("smoother", [0, 1, 2, 3])
 def test_fused_and_graph_paths_bit_identical(problem, smoother):
-    """Fused kernels, the bottom-levels tail kernel and graph replay reorder
-    launches, not arithmetic: flags 1 = unfused, 2 = no graphs, 4 = bottom
-    levels in the single-workgroup tail kernel."""
+    """Fused kernels, the block-visit kernels, the bottom-levels tail kernel and
+    graph replay reorder launches, not arithmetic: flags 1 = unfused, 2 = no
+    graphs, 4 = bottom levels in the single-workgroup tail kernel, 16 = no
+    block-visit kernels on the coarse levels."""
     s, lap, orc, b = problem
     variants = {}
-    for flags in (0, 1, 2, 3, 4, 6):
+    for flags in (0, 1, 2, 3, 4, 6, 16, 18, 20):
         l2 = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, flags=flags)
         solver = U.PoissonSolver(l2)
         solver.mSmoother = smoother

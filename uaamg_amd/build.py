@@ -23,6 +23,7 @@ SOURCES = [
     "poisson_kernels.hip",
     "pcg_kernels.hip",
     "tail_kernels.hip",
+    "block_kernels.hip",
     "poisson_solver.cpp",
     "capi.cpp",
     "viscosity_host.cpp",

@@ -1,0 +1,234 @@
+// One W-cycle level visit in two launches (coarse levels).
+//
+// On the small levels of the hierarchy every smoothing sweep is its own
+// latency-bound launch. A block-visit kernel gives each leaf one workgroup
+// that stages the leaf plus a halo of H voxels (from up to 26 neighbour
+// leaves) in LDS and runs consecutive sweeps on a shrinking region: sweep k
+// is valid on [k, S-k)^3 of the S = 8 + 2H region, so after H-1 sweeps plus
+// the residual (or H sweeps) the leaf's own 8^3 values are exact. The halo
+// voxels are recomputed redundantly by each neighbouring workgroup; every
+// value is produced by the same arithmetic as the per-sweep kernels
+// (stencil order ((x+ + x-) + (y+ + y-)) + (z+ + z-), fma(x, diag, s),
+// fma(w, (b - Ax) * D^-1, x), non-DOF voxels 0), so results are bit-identical.
+//
+//   kPreZero  (H = 3): x1 = w0 D^-1 b, two sweeps, residual, restriction
+//             (first visit, muCyclePreconditioner from a zero guess)
+//   kPreAgain (H = 4): three sweeps from the level's x, residual, restriction
+//   kPost     (H = 3): x + P x_coarse, three sweeps
+#include "stencil_device.h"
+
+namespace uaamg {
+namespace {
+
+using namespace dev;
+
+enum BlockMode { kPreZero = 0, kPreAgain = 1, kPost = 2 };
+constexpr int kBlkThreads = 512;
+
+template <int H>
+struct Region {
+    static constexpr int S = 8 + 2 * H;
+    static constexpr int N = S * S * S;
+    static constexpr size_t lds_bytes = size_t(N) * (8 * sizeof(float) + 1);
+};
+
+template <int H>
+__device__ __forceinline__ int ridx(int x, int y, int z) {
+    return (x * Region<H>::S + y) * Region<H>::S + z;
+}
+
+// One Jacobi sweep y = J_w(x) on [lo, S - lo)^3 of the region.
+template <int H>
+__device__ __forceinline__ void sweep(int lo, const float* x, float* y, const float* b, const float* dg,
+                                      const float* inv, const float* cx, const float* cy, const float* cz,
+                                      const uint8_t* m, float w) {
+    constexpr int S = Region<H>::S;
+    const int e = S - 2 * lo;
+    const int cnt = e * e * e;
+    for (int q = threadIdx.x; q < cnt; q += kBlkThreads) {
+        const int vx = lo + q / (e * e), vy = lo + (q / e) % e, vz = lo + q % e;
+        const int i = ridx<H>(vx, vy, vz);
+        float r = 0.f;
+        if (m[i]) {
+            const int ix = S * S, iy = S;
+            const float f1 = x[i + ix] * cx[i + ix];
+            const float f2 = x[i - ix] * cx[i];
+            const float f3 = x[i + iy] * cy[i + iy];
+            const float f4 = x[i - iy] * cy[i];
+            const float f5 = x[i + 1] * cz[i + 1];
+            const float f6 = x[i - 1] * cz[i];
+            const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+            const float ax = fmaf(x[i], dg[i], s);
+            float t = b[i] - ax;
+            t = t * inv[i];
+            r = fmaf(w, t, x[i]);
+        }
+        y[i] = r;
+    }
+}
+
+template <int H, int MODE>
+__global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, const float* Xin, const float* Xc,
+                                                                  const float* B, float* Xout, float* Cv, float w0,
+                                                                  float w1, float w2) {
+    constexpr int S = Region<H>::S, N = Region<H>::N;
+    extern __shared__ float lds[];
+    float* b = lds;
+    float* xa = b + N;
+    float* xb = xa + N;
+    float* dg = xb + N;
+    float* inv = dg + N;
+    float* cx = inv + N;
+    float* cy = cx + N;
+    float* cz = cy + N;
+    uint8_t* m = reinterpret_cast<uint8_t*>(cz + N);
+    __shared__ int nl[27], sd[27], sx[27], sy[27], sz[27], par[27];
+
+    const int leaf = blockIdx.x;
+    const int tid = threadIdx.x;
+    if (tid < 27) {
+        const int n = L.nb27[size_t(leaf) * 27 + tid];
+        nl[tid] = n;
+        sd[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotDiag] : -1;
+        sx[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotX] : -1;
+        sy[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotY] : -1;
+        sz[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotZ] : -1;
+        par[tid] = (MODE == kPost && n >= 0) ? L.parent[n] : -1;
+    }
+    __syncthreads();
+
+    // stage the region: values of absent leaves are 0 (vectors, mask) or the
+    // default stencil constants (coefficients), as the per-sweep kernels read them
+    for (int q = tid; q < N; q += kBlkThreads) {
+        const int gx = q / (S * S) - H, gy = (q / S) % S - H, gz = q % S - H;
+        const int ax = gx < 0 ? 0 : (gx > 7 ? 2 : 1), ay = gy < 0 ? 0 : (gy > 7 ? 2 : 1), az = gz < 0 ? 0 : (gz > 7 ? 2 : 1);
+        const int k = ax * 9 + ay * 3 + az;
+        const int n = nl[k];
+        const int lx = gx & 7, ly = gy & 7, lz = gz & 7;
+        const int off = (lx << 6) | (ly << 3) | lz;
+        float vb = 0.f, vx = 0.f;
+        uint8_t vm = 0;
+        const int s0 = sd[k], s1 = sx[k], s2 = sy[k], s3 = sz[k];
+        const float vdg = s0 >= 0 ? L.diag[size_t(s0) * 512 + off] : L.dflt_diag;
+        const float vinv = s0 >= 0 ? L.invdiag[size_t(s0) * 512 + off] : L.dflt_invdiag;
+        if (n >= 0) {
+            vm = uint8_t((L.mask[size_t(n) * 8 + lx] >> ((ly << 3) | lz)) & 1u);
+            vb = B[size_t(n) * 512 + off];
+            if (MODE == kPreZero) {
+                vx = w0 * (vinv * vb);
+            } else {
+                vx = Xin[size_t(n) * 512 + off];
+                if (MODE == kPost && vm && par[k] >= 0) {
+                    const int p = par[k], cl = p >> 3, oct = p & 7;
+                    const int crow = ((lx >> 1) + 4 * ((oct >> 2) & 1)) * 8 + ((ly >> 1) + 4 * ((oct >> 1) & 1));
+                    vx += Xc[size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1)];
+                }
+            }
+        }
+        b[q] = vb;
+        xa[q] = vx;
+        m[q] = vm;
+        dg[q] = vdg;
+        inv[q] = vinv;
+        cx[q] = s1 >= 0 ? L.cx[size_t(s1) * 512 + off] : L.dflt_off;
+        cy[q] = s2 >= 0 ? L.cy[size_t(s2) * 512 + off] : L.dflt_off;
+        cz[q] = s3 >= 0 ? L.cz[size_t(s3) * 512 + off] : L.dflt_off;
+    }
+    __syncthreads();
+
+    const float* fin;   // region array holding the final sweep
+    if (MODE == kPreZero) {
+        sweep<H>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w1);
+        __syncthreads();
+        sweep<H>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w2);
+        __syncthreads();
+        fin = xa;
+    } else {
+        sweep<H>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w0);
+        __syncthreads();
+        sweep<H>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w1);
+        __syncthreads();
+        sweep<H>(3, xa, xb, b, dg, inv, cx, cy, cz, m, w2);
+        __syncthreads();
+        fin = xb;
+    }
+
+    // the leaf's own values
+    for (int q = tid; q < 512; q += kBlkThreads) {
+        const int i = ridx<H>(H + (q >> 6), H + ((q >> 3) & 7), H + (q & 7));
+        Xout[size_t(leaf) * 512 + q] = fin[i];
+    }
+    if (MODE == kPost) return;
+
+    // residual on the own brick, then restriction into the parent's octant
+    float* rs = (fin == xa) ? xb : xa;
+    for (int q = tid; q < 512; q += kBlkThreads) {
+        const int i = ridx<H>(H + (q >> 6), H + ((q >> 3) & 7), H + (q & 7));
+        float r = 0.f;
+        if (m[i]) {
+            const int ix = S * S, iy = S;
+            const float* x = fin;
+            const float f1 = x[i + ix] * cx[i + ix];
+            const float f2 = x[i - ix] * cx[i];
+            const float f3 = x[i + iy] * cy[i + iy];
+            const float f4 = x[i - iy] * cy[i];
+            const float f5 = x[i + 1] * cz[i + 1];
+            const float f6 = x[i - 1] * cz[i];
+            const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+            r = b[i] - fmaf(x[i], dg[i], s);
+        }
+        rs[q] = r;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int ccx = tid >> 4, ccy = (tid >> 2) & 3, ccz = tid & 3;
+        float sum = 0.f;
+        for (int ii = 0; ii < 2; ++ii)
+            for (int jj = 0; jj < 2; ++jj)
+                for (int kk = 0; kk < 2; ++kk) sum += rs[((2 * ccx + ii) * 8 + (2 * ccy + jj)) * 8 + 2 * ccz + kk];
+        const int p = L.parent[leaf];
+        if (p < 0) return;
+        const int cl = p >> 3, oct = p & 7;
+        const int off = ((ccx + 4 * ((oct >> 2) & 1)) * 8 + (ccy + 4 * ((oct >> 1) & 1))) * 8 + ccz + 4 * (oct & 1);
+        Cv[size_t(cl) * 512 + off] = sum * 0.125f;
+    }
+}
+
+template <int H, int MODE>
+hipError_t launch_mode(const LevelView& L, const float* xin, const float* xc, const float* b, float* xout, float* cv,
+                       float w0, float w1, float w2, hipStream_t s) {
+    block_visit_kernel<H, MODE><<<unsigned(L.nleaf), kBlkThreads, Region<H>::lds_bytes, s>>>(L, xin, xc, b, xout, cv,
+                                                                                             w0, w1, w2);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t block_kernels_init() {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPreZero>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<4, kPreAgain>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<4>::lds_bytes));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPost>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    return e;
+}
+
+hipError_t launch_block_pre(const LevelView& L, bool from_zero, const float* x, const float* b, float* x_out,
+                            float* coarse_b, float w0, float w1, float w2, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    if (!L.nb27) return hipErrorInvalidValue;
+    if (from_zero) return launch_mode<3, kPreZero>(L, nullptr, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
+    return launch_mode<4, kPreAgain>(L, x, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
+}
+
+hipError_t launch_block_post(const LevelView& L, const float* x, const float* xc, const float* b, float* x_out,
+                             float w0, float w1, float w2, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    if (!L.nb27) return hipErrorInvalidValue;
+    return launch_mode<3, kPost>(L, x, xc, b, x_out, nullptr, w0, w1, w2, s);
+}
+
+}  // namespace uaamg

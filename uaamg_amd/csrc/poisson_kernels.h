@@ -45,6 +45,7 @@ struct LevelView {
     const float* cz;
     const float* spai0;
     float dflt_diag, dflt_off, dflt_invdiag, dflt_spai0;
+    const int32_t* nb27;      // [nleaf][27] leaf at offset (ax,ay,az) in {-1,0,1}^3 (block-visit kernels)
 };
 
 // Scalars of the outer PCG, kept in device memory so a whole iteration runs
@@ -157,5 +158,15 @@ hipError_t launch_update_laplacian_dot(const LevelView& L, const float* z, const
                                        float* q, PcgScalars* sc, double* partials, hipStream_t s);
 // x += alpha p only (final step)
 hipError_t launch_update_x(const LevelView& L, const float* p, float* x, const PcgScalars* sc, hipStream_t s);
+
+// ---- block-visit kernels (one workgroup per leaf, halo staged in LDS; needs L.nb27)
+hipError_t block_kernels_init();
+// pre-smoothing (3 weighted Jacobi sweeps w0, w1, w2; from a zero guess when
+// from_zero, else from x) into x_out, and coarse_b = R (b - A x_out)
+hipError_t launch_block_pre(const LevelView& L, bool from_zero, const float* x, const float* b, float* x_out,
+                            float* coarse_b, float w0, float w1, float w2, hipStream_t s);
+// x_out = three sweeps (w0, w1, w2) from x + P x_coarse
+hipError_t launch_block_post(const LevelView& L, const float* x, const float* xc, const float* b, float* x_out,
+                             float w0, float w1, float w2, hipStream_t s);
 
 }  // namespace uaamg

@@ -205,6 +205,7 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         add(n * sizeof(int64_t));
         add(n * sizeof(int32_t));         // parent
         add(n * 8 * sizeof(int32_t));     // child
+        add(n * 27 * sizeof(int32_t));    // 3x3x3 leaf neighbourhood
         size_t nslot[kNumCoef] = {0, 0, 0, 0, 0};
         for (int k = 0; k < kNumCoef; ++k)
             for (size_t i = 0; i < n; ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
@@ -283,6 +284,12 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         if (env) max_leaves = std::atoi(env);
         tail_level_ = -1;
         if (max_leaves > 0 && L >= 3 && int64_t(host_.levels[L - 2].nleaf()) <= max_leaves) tail_level_ = L - 2;
+        // coarse levels (1 .. L-2) of at most this many leaves run each visit
+        // as two block-visit launches (flags bit 4 or UAAMG_BLOCK_LEAVES=0: off)
+        block_max_leaves_ = (opt_.flags & 16) ? 0 : 512;
+        env = std::getenv("UAAMG_BLOCK_LEAVES");
+        if (env) block_max_leaves_ = std::atoi(env);
+        if (block_max_leaves_ > 0 && check(block_kernels_init(), "block kernel attributes")) return -3;
     }
     if (int rc = build_coarsest()) return rc;
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
@@ -312,6 +319,15 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
     };
     v.mask = static_cast<const uint64_t*>(up(h.mask.data(), n * 8 * sizeof(uint64_t)));
     v.nbr = static_cast<const int32_t*>(up(h.nbr.data(), n * 6 * sizeof(int32_t)));
+    {
+        std::vector<int32_t> nb27(n * 27, -1);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 27; ++k)
+                nb27[i * 27 + k] = h.lookup.find({h.origin[i].x + 8 * (k / 9 - 1), h.origin[i].y + 8 * ((k / 3) % 3 - 1),
+                                                  h.origin[i].z + 8 * (k % 3 - 1)});
+        v.nb27 = static_cast<const int32_t*>(up(nb27.data(), nb27.size() * sizeof(int32_t)));
+        (void)check(hipStreamSynchronize(stream_), "upload sync");   // nb27 dies at scope end
+    }
     std::vector<int32_t> slot(n * kSlotStride, -1);
     int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < n; ++i)
@@ -533,6 +549,16 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
     const bool weighted = opt_.smoother == kWeightedJacobi;
     auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
     auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    if (l >= 1 && l + 1 != tail_level_ && int64_t(v.nleaf) <= block_max_leaves_) {
+        // two launches per visit: pre-smoothing + residual + restriction, then
+        // prolongation + post-smoothing (the coarser visits run in between)
+        DevLevel& c = lv_[l + 1];
+        check(launch_block_pre(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+        precond(l + 1, c.X, c.B, true, -1);
+        if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
+        check(launch_block_post(v, d.T, c.X, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
+        return;
+    }
     float* cur = X;
     auto next = [&]() { return cur == d.T ? d.U : d.T; };
     if (skip_first) {

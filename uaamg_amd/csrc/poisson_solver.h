@@ -135,6 +135,7 @@ private:
     bool use_graphs_ = true;
     bool fuse_ = true;                     // fused W-cycle / PCG kernels
     int tail_level_ = -1;                  // level whose visit pairs run in the tail kernel (-1: none)
+    int block_max_leaves_ = 0;             // coarse levels up to this size use the block-visit kernels
     HostMailbox* mailbox_ = nullptr;       // pinned host memory
     HostMailbox* mailbox_dev_ = nullptr;   // its device alias
     unsigned int nu_seq_ = 0;              // ||r||_inf reductions issued so far
