@@ -94,7 +94,6 @@ def main() -> int:
     x = torch.zeros_like(b)
     iters = None
     if args.kernel_only:
-        rc, iters = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
         ms = lap.time_apply(0, 2, reps=args.kernel_reps)
         src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
@@ -386,6 +385,15 @@ def run_viscosity(args, dev) -> dict:
     lw = U.L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=0)
     t_build = time.perf_counter() - t0
     solver = U.simd_viscosity3d(lw, sc.velocity)
+    if args.kernel_only:
+        ms = lw.time_apply(0, 4, 0, reps=args.kernel_reps)
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        return {"kernel_only": True, "red_pass_ms": ms, "bytes_per_launch": lw.algorithmic_bytes(0, 4, 0),
+                "copy_bytes_each_way": src.numel() * 4, "copies": 10}
     b = torch.from_numpy(solver.m_rhs.data).to(dev)
     x = torch.zeros_like(b)
     for _ in range(2):

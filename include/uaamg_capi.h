@@ -243,7 +243,8 @@ typedef struct uaamg_viscosity_options {
     int32_t max_iterations;   /* m_max_iter, default 100 (simd_viscosity3d.cpp:3629) */
     int32_t device;           /* HIP device ordinal; -1 = host-only handle (inspection, no solves) */
     int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, :3638) */
-    int32_t flags;            /* 0 normally; bit 1 = no graph replay (A/B checks); bit 3 = keep level 0 untrimmed */
+    int32_t flags;            /* 0 normally; bit 1 = no graph replay (A/B checks); bit 3 = keep level 0 untrimmed;
+                                 bit 5 = small coarse levels as single-workgroup half-visits */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
 } uaamg_viscosity_options;
 

@@ -125,14 +125,17 @@ def test_warm_start_and_max_iterations(problem):
 
 
 def test_graph_replay_bit_identical(problem):
-    """Graph replay reorders nothing: flags 2 issues every launch directly."""
+    """Graph replay and the single-workgroup half-visits reorder launches, not
+    arithmetic: flags 2 issues every launch directly, 32 runs small coarse
+    levels in one workgroup."""
     s, lw, orc, b = problem
     out = []
-    for flags in (0, 2):
+    for flags in (0, 2, 32, 34):
         l2 = U.L_with_level(s.viscosity, s.liquid, s.solid, s.dt, s.rho, s.dx, flags=flags)
         solver = U.simd_viscosity3d(l2, s.velocity)
         x = l2.get_zero_vec()
         solver.pcg_solve(x, 1e-6)
         out.append((x.data.copy(), solver.m_iteration))
-    assert out[0][1] == out[1][1]
-    assert np.array_equal(out[0][0], out[1][0])
+    for x, it in out[1:]:
+        assert it == out[0][1]
+        assert np.array_equal(x, out[0][0])

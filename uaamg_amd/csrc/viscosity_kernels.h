@@ -64,4 +64,11 @@ hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float
 // zero every brick of a level vector (set_grid_to_zero, :2304-2315)
 hipError_t launch_visc_zero(const ViscLevelView& L, float* v, hipStream_t s);
 
+// One workgroup per half-visit of a small level (same passes, one launch):
+// pre = (zero x when precon,) 2 XYZ sweeps, residual into t, restriction into
+// coarse_b; post = x += P x_coarse, 2 ZYX sweeps.
+hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, float* x, const float* b,
+                                 float* t, float* coarse_b, bool precon, hipStream_t s);
+hipError_t launch_visc_small_post(const ViscLevelView& L, float* x, const float* b, const float* xc, hipStream_t s);
+
 }  // namespace uaamg

@@ -92,13 +92,11 @@ struct Unit {
     static constexpr int x = A == 0, y = A == 1, z = A == 2;
 };
 
+// One row (lane) of the applier on channel C of `leaf`: Y = op(X, B).
 template <int MODE, int C>
-__global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, const float* Xv, const float* Bv,
-                                                               float* Yv) {
-    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
-    if (w >= L.chan_off[C + 1] - L.chan_off[C]) return;
-    const int leaf = L.chan_leaves[L.chan_off[C] + w];
-    const int lane = threadIdx.x & 63, lx = lane >> 3, ly = lane & 7;
+__device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int lane, const float* Xv, const float* Bv,
+                                         float* Yv) {
+    const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(L.mask[(size_t(leaf) * 3 + C) * 8 + lx] >> (ly * 8)) & 0xffu;
     if (!rm) return;   // rows without a DOF are not written
     constexpr int d0 = C, d1 = (C + 1) % 3, d2 = (C + 2) % 3;
@@ -248,13 +246,18 @@ __global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, 
     st8v(Yv + row, y);
 }
 
-// restriction: coarse (leaf, channel) waves; 1/8 of the on fine children
-__global__ __launch_bounds__(kVThreads) void visc_restrict_kernel(ViscLevelView Cl, ViscLevelView F, const float* fine,
-                                                                  float* coarse) {
+template <int MODE, int C>
+__global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, const float* Xv, const float* Bv,
+                                                               float* Yv) {
     const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
-    if (w >= Cl.nleaf_dof * 3) return;
-    const int leaf = w / 3, c = w - 3 * (w / 3);
-    const int lane = threadIdx.x & 63, lx = lane >> 3, ly = lane & 7;
+    if (w >= L.chan_off[C + 1] - L.chan_off[C]) return;
+    visc_row<MODE, C>(L, L.chan_leaves[L.chan_off[C] + w], threadIdx.x & 63, Xv, Bv, Yv);
+}
+
+// restriction of one (coarse leaf, channel) row: 1/8 of the on fine children
+__device__ __forceinline__ void visc_restrict_row(const ViscLevelView& Cl, const ViscLevelView& F, int leaf, int c,
+                                                  int lane, const float* fine, float* coarse) {
+    const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(Cl.mask[(size_t(leaf) * 3 + c) * 8 + lx] >> (ly * 8)) & 0xffu;
     if (!rm) return;
     float out[8];
@@ -283,12 +286,10 @@ __global__ __launch_bounds__(kVThreads) void visc_restrict_kernel(ViscLevelView 
     st8v(coarse + (size_t(leaf) * 3 + c) * 512 + size_t(lx * 8 + ly) * 8, out);
 }
 
-// prolongation<true>: fine (leaf, channel) waves add the parent coarse value
-__global__ __launch_bounds__(kVThreads) void visc_prolong_kernel(ViscLevelView F, const float* coarse, float* fine) {
-    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
-    if (w >= F.nleaf_dof * 3) return;
-    const int leaf = w / 3, c = w - 3 * (w / 3);
-    const int lane = threadIdx.x & 63, lx = lane >> 3, ly = lane & 7;
+// prolongation<true> of one (fine leaf, channel) row
+__device__ __forceinline__ void visc_prolong_row(const ViscLevelView& F, int leaf, int c, int lane, const float* coarse,
+                                                 float* fine) {
+    const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(F.mask[(size_t(leaf) * 3 + c) * 8 + lx] >> (ly * 8)) & 0xffu;
     if (!rm) return;
     const int p = F.parent[leaf];
@@ -303,6 +304,85 @@ __global__ __launch_bounds__(kVThreads) void visc_prolong_kernel(ViscLevelView F
     for (int z = 0; z < 8; ++z)
         if ((rm >> z) & 1u) v[z] += cv[z >> 1];
     st8v(row, v);
+}
+
+// ---------------------------------------------------- small-level visits
+// On the coarse levels a W-cycle visit is ~29 dependent launches (12 colour
+// passes, residual, restriction, prolongation, 12 colour passes). When every
+// channel of a level has at most a few dozen leaves, one workgroup runs the
+// whole half-visit with a workgroup barrier between passes: the same row
+// routine, the same pass order, the same values.
+constexpr int kSmallThreads = 512;
+constexpr int kSmallWaves = kSmallThreads / 64;
+
+template <int MODE, int C>
+__device__ __forceinline__ void small_pass(const ViscLevelView& L, const float* X, const float* B, float* Y) {
+    const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = L.chan_off[C + 1] - L.chan_off[C];
+    for (int w = w0; w < n; w += kSmallWaves) visc_row<MODE, C>(L, L.chan_leaves[L.chan_off[C] + w], lane, X, B, Y);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void small_rbgs(const ViscLevelView& L, float* X, const float* B, bool xyz) {
+    if (xyz) {
+        small_pass<kVRedGS, 0>(L, X, B, X);
+        small_pass<kVBlackGS, 0>(L, X, B, X);
+        small_pass<kVRedGS, 1>(L, X, B, X);
+        small_pass<kVBlackGS, 1>(L, X, B, X);
+        small_pass<kVRedGS, 2>(L, X, B, X);
+        small_pass<kVBlackGS, 2>(L, X, B, X);
+    } else {
+        small_pass<kVBlackGS, 2>(L, X, B, X);
+        small_pass<kVRedGS, 2>(L, X, B, X);
+        small_pass<kVBlackGS, 1>(L, X, B, X);
+        small_pass<kVRedGS, 1>(L, X, B, X);
+        small_pass<kVBlackGS, 0>(L, X, B, X);
+        small_pass<kVRedGS, 0>(L, X, B, X);
+    }
+}
+
+// (zero X when precon,) 2 XYZ sweeps, residual into T, restriction into Cb
+__global__ __launch_bounds__(kSmallThreads) void visc_small_pre_kernel(ViscLevelView L, ViscLevelView Cl, float* X,
+                                                                       const float* B, float* T, float* Cb,
+                                                                       int precon) {
+    if (precon) {
+        const size_t n4 = size_t(L.nleaf_dof) * 3 * 512 / 4;
+        float4* x4 = reinterpret_cast<float4*>(X);
+        for (size_t i = threadIdx.x; i < n4; i += kSmallThreads) x4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+    }
+    small_rbgs(L, X, B, true);
+    small_rbgs(L, X, B, true);
+    small_pass<kVResidual, 0>(L, X, B, T);
+    small_pass<kVResidual, 1>(L, X, B, T);
+    small_pass<kVResidual, 2>(L, X, B, T);
+    const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int w = w0; w < Cl.nleaf_dof * 3; w += kSmallWaves) visc_restrict_row(Cl, L, w / 3, w % 3, lane, T, Cb);
+}
+
+// X += P Xc, then 2 ZYX sweeps
+__global__ __launch_bounds__(kSmallThreads) void visc_small_post_kernel(ViscLevelView L, float* X, const float* B,
+                                                                        const float* Xc) {
+    const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int w = w0; w < L.nleaf_dof * 3; w += kSmallWaves) visc_prolong_row(L, w / 3, w % 3, lane, Xc, X);
+    __syncthreads();
+    small_rbgs(L, X, B, false);
+    small_rbgs(L, X, B, false);
+}
+
+// restriction: coarse (leaf, channel) waves
+__global__ __launch_bounds__(kVThreads) void visc_restrict_kernel(ViscLevelView Cl, ViscLevelView F, const float* fine,
+                                                                  float* coarse) {
+    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
+    if (w >= Cl.nleaf_dof * 3) return;
+    visc_restrict_row(Cl, F, w / 3, w - 3 * (w / 3), threadIdx.x & 63, fine, coarse);
+}
+
+// prolongation<true>: fine (leaf, channel) waves add the parent coarse value
+__global__ __launch_bounds__(kVThreads) void visc_prolong_kernel(ViscLevelView F, const float* coarse, float* fine) {
+    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
+    if (w >= F.nleaf_dof * 3) return;
+    visc_prolong_row(F, w / 3, w - 3 * (w / 3), threadIdx.x & 63, coarse, fine);
 }
 
 __global__ __launch_bounds__(kVThreads) void visc_zero_kernel(float4* v, size_t n4) {
@@ -475,6 +555,17 @@ hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float
     if (C.n <= 0) return hipSuccess;
     if (C.n > kViscMaxCoarsest) return hipErrorInvalidValue;
     visc_coarsest_kernel<<<1, kCgT, 0, s>>>(C, b, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, float* x, const float* b,
+                                 float* t, float* coarse_b, bool precon, hipStream_t s) {
+    visc_small_pre_kernel<<<1, kSmallThreads, 0, s>>>(L, coarse, x, b, t, coarse_b, precon ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_visc_small_post(const ViscLevelView& L, float* x, const float* b, const float* xc, hipStream_t s) {
+    visc_small_post_kernel<<<1, kSmallThreads, 0, s>>>(L, x, b, xc);
     return hipGetLastError();
 }
 

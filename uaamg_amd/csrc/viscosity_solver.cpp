@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 
@@ -133,6 +134,11 @@ int ViscDevice::init(ViscHierarchyHost&& h, const ViscOptions& opt) {
         own_stream_ = true;
     }
     use_graphs_ = !(opt_.flags & 2);
+    // on request (flags bit 5 or UAAMG_VSMALL_LEAVES), coarse levels whose
+    // widest channel has at most this many leaves run each half-visit in one
+    // single-workgroup launch
+    small_max_leaves_ = (opt_.flags & 32) ? 16 : 0;
+    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) small_max_leaves_ = std::atoi(env);
     const int L = int(host_.levels.size());
     size_t bytes = 1 << 20;
     for (const auto& l : host_.levels) {
@@ -231,6 +237,16 @@ void ViscDevice::mucycle(int level, float* x, const float* b, bool precon) {
         return;
     }
     const ViscLevelView& v = lv_[level].v;
+    const ViscLevelHost& hl = host_.levels[size_t(level)];
+    const size_t widest = std::max(hl.chan_leaves[0].size(), std::max(hl.chan_leaves[1].size(), hl.chan_leaves[2].size()));
+    if (level >= 1 && int64_t(widest) <= small_max_leaves_) {
+        DevLevel& c = lv_[level + 1];
+        check(launch_visc_small_pre(v, c.v, x, b, lv_[level].T, c.B, precon, stream_), "small pre");
+        mucycle(level + 1, c.X, c.B, true);
+        if (level + 1 < L - 1) mucycle(level + 1, c.X, c.B, false);
+        check(launch_visc_small_post(v, x, b, c.X, stream_), "small post");
+        return;
+    }
     if (precon) check(launch_visc_zero(v, x, stream_), "zero");
     for (int i = 0; i < 2; ++i) rbgs(level, x, b, true);
     apply_all(level, kVResidual, x, b, lv_[level].T);

@@ -103,6 +103,7 @@ private:
     Graph g_init_, g_first_, g_iter_;
     bool graphs_ready_ = false;
     bool use_graphs_ = true;
+    int64_t small_max_leaves_ = 0;   // coarse levels up to this width use the single-workgroup half-visits
     HostMailbox* mailbox_ = nullptr;
     HostMailbox* mailbox_dev_ = nullptr;
     unsigned int nu_seq_ = 0;
