@@ -39,6 +39,13 @@ extern "C" {
 #define UAAMG_SMOOTHER_RED_BLACK_GAUSS_SEIDEL 2
 #define UAAMG_SMOOTHER_SPAI0 3
 
+/* uaamg_options.precision */
+#define UAAMG_PRECISION_FP32 0       /* FP32 throughout (the reference's arithmetic) */
+#define UAAMG_PRECISION_BF16_CYCLE 1 /* W-cycle level vectors stored in BF16, computed in FP32, for the
+                                        Jacobi-family smoothers (RBGS / SPAI0 cycles stay FP32); outer
+                                        PCG in FP32 with FP64 dot products and an FP64 residual
+                                        replacement after convergence. No reference counterpart. */
+
 /* LaplacianApplySIMD::WorkingMode (simd_vdb_poisson.cpp:683-690) */
 #define UAAMG_MODE_LAPLACIAN 0
 #define UAAMG_MODE_RESIDUAL 1
@@ -76,11 +83,13 @@ typedef struct uaamg_options {
     float relative_tolerance; /* mRelativeTolerance, default 1e-7 */
     int32_t max_iterations;   /* mMaxIteration, default 100 */
     int32_t smoother;         /* mSmoother, UAAMG_SMOOTHER_* */
-    int32_t precision;        /* 0 = FP32 throughout */
+    int32_t precision;        /* UAAMG_PRECISION_*, default FP32 */
     int32_t device;           /* HIP device ordinal */
     int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, simd_vdb_poisson.cpp:1815) */
     int32_t flags;            /* 0 normally; bit 0 = unfused kernels, bit 1 = no graph replay,
-                                 bit 2 = bottom levels in one single-workgroup launch (A/B checks) */
+                                 bit 2 = bottom levels in one single-workgroup launch, bit 4 = no
+                                 block-visit kernels (A/B checks); bit 6 = FP64 residual replacement
+                                 at FP32 precision */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
 } uaamg_options;
 
@@ -155,6 +164,13 @@ int uaamg_poisson_solve_pcg(uaamg_poisson* h, float* x, const float* b, int32_t*
 
 /* Same with x and b in device memory (DOF-ordered, ndof floats each). */
 int uaamg_poisson_solve_pcg_device(uaamg_poisson* h, float* x_dev, const float* b_dev, int32_t* iterations);
+
+/*
+ * After a solve with FP64 residual replacement (UAAMG_PRECISION_BF16_CYCLE, or flags bit 6):
+ * ||b - A x||_inf of the returned x evaluated in FP64, and the number of PCG restarts it took.
+ * Both are -1 when the last solve ran without replacement. No reference counterpart.
+ */
+int uaamg_poisson_last_residual(const uaamg_poisson* h, float* true_residual_inf, int32_t* replacements);
 
 /*
  * Many independent solves at once (batch configuration: one handle per problem, each with its own

@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
         "uaamg_poisson_solve_pcg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_pcg_device": (C.c_int, [vp, vp, vp, P(i32)]),
+        "uaamg_poisson_last_residual": (C.c_int, [vp, P(f32), P(i32)]),
         "uaamg_poisson_solve_mg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_batch_device": (C.c_int, [vp, i32, vp, vp, vp, vp, i32]),
         "uaamg_viscosity_solve_batch_device": (C.c_int, [vp, i32, f32, vp, vp, vp, i32]),
@@ -260,6 +261,10 @@ def solve_batch_device(laplacians, x_ptrs, b_ptrs, concurrency: int = 4):
     return st.tolist(), it.tolist()
 
 
+PRECISION_FP32 = 0         # FP32 throughout (the reference's arithmetic)
+PRECISION_BF16_CYCLE = 1   # BF16-stored W-cycle vectors, FP32 arithmetic, FP64 residual replacement
+
+
 class SmootherOption:
     """PoissonSolver::SmootherOption (simd_vdb_poisson.h:143-148)."""
 
@@ -277,13 +282,14 @@ class LaplacianWithLevel:
     """
 
     def __init__(self, liquid_phi: Grid, face_weights: Grid, dt: float, dx: float, *, device: int = 0,
-                 max_coarsest_dof: int = 4000, flags: int = 0):
+                 max_coarsest_dof: int = 4000, flags: int = 0, precision: int = 0):
         L = lib()
         o = _Options()
         L.uaamg_options_default(C.byref(o))
         o.device = device
         o.max_coarsest_dof = max_coarsest_dof
         o.flags = flags
+        o.precision = precision
         h = C.c_void_p()
         gphi, gw = liquid_phi._struct(), face_weights._struct()
         _check(L.uaamg_poisson_create(C.byref(gphi), C.byref(gw), float(dt), float(dx), C.byref(o), C.byref(h)),
@@ -395,6 +401,12 @@ class LaplacianWithLevel:
         rc = _check(lib().uaamg_poisson_solve_pcg_device(self._h, C.c_void_p(x_ptr), C.c_void_p(b_ptr),
                                                          C.byref(it)), "solve_pcg_device")
         return rc, int(it.value)
+
+    def last_residual(self) -> tuple[float, int]:
+        """(FP64 ||b - A x||_inf of the last solve, residual replacements made); -1s without replacement."""
+        r, k = C.c_float(), C.c_int32()
+        _check(lib().uaamg_poisson_last_residual(self._h, C.byref(r), C.byref(k)), "last_residual")
+        return float(r.value), int(k.value)
 
     def coarsest_solve(self, b: np.ndarray) -> np.ndarray:
         b = np.ascontiguousarray(b, np.float32)

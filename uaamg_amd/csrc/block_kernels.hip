@@ -67,10 +67,9 @@ __device__ __forceinline__ void sweep(int lo, const float* x, float* y, const fl
     }
 }
 
-template <int H, int MODE>
-__global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, const float* Xin, const float* Xc,
-                                                                  const float* B, float* Xout, float* Cv, float w0,
-                                                                  float w1, float w2) {
+template <int H, int MODE, class V>
+__global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, const V* Xin, const V* Xc, const V* B,
+                                                                  V* Xout, V* Cv, float w0, float w1, float w2) {
     constexpr int S = Region<H>::S, N = Region<H>::N;
     extern __shared__ float lds[];
     float* b = lds;
@@ -113,15 +112,15 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
         const float vinv = s0 >= 0 ? L.invdiag[size_t(s0) * 512 + off] : L.dflt_invdiag;
         if (n >= 0) {
             vm = uint8_t((L.mask[size_t(n) * 8 + lx] >> ((ly << 3) | lz)) & 1u);
-            vb = B[size_t(n) * 512 + off];
+            vb = float(B[size_t(n) * 512 + off]);
             if (MODE == kPreZero) {
                 vx = w0 * (vinv * vb);
             } else {
-                vx = Xin[size_t(n) * 512 + off];
+                vx = float(Xin[size_t(n) * 512 + off]);
                 if (MODE == kPost && vm && par[k] >= 0) {
                     const int p = par[k], cl = p >> 3, oct = p & 7;
                     const int crow = ((lx >> 1) + 4 * ((oct >> 2) & 1)) * 8 + ((ly >> 1) + 4 * ((oct >> 1) & 1));
-                    vx += Xc[size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1)];
+                    vx += float(Xc[size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1)]);
                 }
             }
         }
@@ -156,7 +155,7 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
     // the leaf's own values
     for (int q = tid; q < 512; q += kBlkThreads) {
         const int i = ridx<H>(H + (q >> 6), H + ((q >> 3) & 7), H + (q & 7));
-        Xout[size_t(leaf) * 512 + q] = fin[i];
+        Xout[size_t(leaf) * 512 + q] = V(fin[i]);
     }
     if (MODE == kPost) return;
 
@@ -190,45 +189,74 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
         if (p < 0) return;
         const int cl = p >> 3, oct = p & 7;
         const int off = ((ccx + 4 * ((oct >> 2) & 1)) * 8 + (ccy + 4 * ((oct >> 1) & 1))) * 8 + ccz + 4 * (oct & 1);
-        Cv[size_t(cl) * 512 + off] = sum * 0.125f;
+        Cv[size_t(cl) * 512 + off] = V(sum * 0.125f);
     }
 }
 
-template <int H, int MODE>
-hipError_t launch_mode(const LevelView& L, const float* xin, const float* xc, const float* b, float* xout, float* cv,
-                       float w0, float w1, float w2, hipStream_t s) {
-    block_visit_kernel<H, MODE><<<unsigned(L.nleaf), kBlkThreads, Region<H>::lds_bytes, s>>>(L, xin, xc, b, xout, cv,
-                                                                                             w0, w1, w2);
+template <int H, int MODE, class V>
+hipError_t launch_mode(const LevelView& L, const V* xin, const V* xc, const V* b, V* xout, V* cv, float w0, float w1,
+                       float w2, hipStream_t s) {
+    block_visit_kernel<H, MODE, V><<<unsigned(L.nleaf), kBlkThreads, Region<H>::lds_bytes, s>>>(L, xin, xc, b, xout,
+                                                                                                cv, w0, w1, w2);
     return hipGetLastError();
+}
+
+template <class V>
+hipError_t set_lds_limits() {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPreZero, V>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<4, kPreAgain, V>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<4>::lds_bytes));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPost, V>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    return e;
+}
+
+template <class V>
+hipError_t block_pre_t(const LevelView& L, bool from_zero, const V* x, const V* b, V* x_out, V* coarse_b, float w0,
+                       float w1, float w2, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    if (!L.nb27) return hipErrorInvalidValue;
+    if (from_zero) return launch_mode<3, kPreZero, V>(L, nullptr, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
+    return launch_mode<4, kPreAgain, V>(L, x, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
+}
+
+template <class V>
+hipError_t block_post_t(const LevelView& L, const V* x, const V* xc, const V* b, V* x_out, float w0, float w1, float w2,
+                        hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    if (!L.nb27) return hipErrorInvalidValue;
+    return launch_mode<3, kPost, V>(L, x, xc, b, x_out, nullptr, w0, w1, w2, s);
 }
 
 }  // namespace
 
 hipError_t block_kernels_init() {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPreZero>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<4, kPreAgain>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<4>::lds_bytes));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPost>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    hipError_t e = set_lds_limits<float>();
+    if (e == hipSuccess) e = set_lds_limits<bf16>();
     return e;
 }
 
 hipError_t launch_block_pre(const LevelView& L, bool from_zero, const float* x, const float* b, float* x_out,
                             float* coarse_b, float w0, float w1, float w2, hipStream_t s) {
-    if (L.nleaf <= 0) return hipSuccess;
-    if (!L.nb27) return hipErrorInvalidValue;
-    if (from_zero) return launch_mode<3, kPreZero>(L, nullptr, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
-    return launch_mode<4, kPreAgain>(L, x, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
+    return block_pre_t<float>(L, from_zero, x, b, x_out, coarse_b, w0, w1, w2, s);
 }
 
 hipError_t launch_block_post(const LevelView& L, const float* x, const float* xc, const float* b, float* x_out,
                              float w0, float w1, float w2, hipStream_t s) {
-    if (L.nleaf <= 0) return hipSuccess;
-    if (!L.nb27) return hipErrorInvalidValue;
-    return launch_mode<3, kPost>(L, x, xc, b, x_out, nullptr, w0, w1, w2, s);
+    return block_post_t<float>(L, x, xc, b, x_out, w0, w1, w2, s);
+}
+
+hipError_t launch_block_pre(const LevelView& L, bool from_zero, const bf16* x, const bf16* b, bf16* x_out,
+                            bf16* coarse_b, float w0, float w1, float w2, hipStream_t s) {
+    return block_pre_t<bf16>(L, from_zero, x, b, x_out, coarse_b, w0, w1, w2, s);
+}
+
+hipError_t launch_block_post(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* x_out,
+                             float w0, float w1, float w2, hipStream_t s) {
+    return block_post_t<bf16>(L, x, xc, b, x_out, w0, w1, w2, s);
 }
 
 }  // namespace uaamg

@@ -121,7 +121,8 @@ int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_we
     if (opts) o = *opts;
     if (o.smoother < 0 || o.smoother > 3) return fail(UAAMG_EINVAL, "unknown smoother");
     if (o.max_iterations < 0) return fail(UAAMG_EINVAL, "max_iterations < 0");
-    if (o.precision != 0) return fail(UAAMG_EINVAL, "unsupported precision");
+    if (o.precision != UAAMG_PRECISION_FP32 && o.precision != UAAMG_PRECISION_BF16_CYCLE)
+        return fail(UAAMG_EINVAL, "unsupported precision");
     const bool host_only = o.device < 0;   // hierarchy inspection without a GPU; solves fail
     if (!host_only && uaamg_device_count() <= o.device) return fail(UAAMG_ENODEV, "no HIP device");
     HostGrid phi, w;
@@ -135,6 +136,7 @@ int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_we
     so.relative_tolerance = o.relative_tolerance;
     so.max_iterations = o.max_iterations;
     so.smoother = o.smoother;
+    so.precision = o.precision;
     so.device = o.device;
     so.max_coarsest_dof = o.max_coarsest_dof;
     so.flags = o.flags;
@@ -166,10 +168,18 @@ int uaamg_poisson_set_options(uaamg_poisson* h, const uaamg_options* o) {
     h->dev.options().relative_tolerance = o->relative_tolerance;
     h->dev.options().max_iterations = o->max_iterations;
     h->dev.options().smoother = o->smoother;
+    // precision and device placement are fixed when the handle is created
     return 0;
 }
 
 int64_t uaamg_poisson_ndof(const uaamg_poisson* h) { return h ? h->ndof : -1; }
+
+int uaamg_poisson_last_residual(const uaamg_poisson* h, float* true_residual_inf, int32_t* replacements) {
+    if (!h) return fail(UAAMG_EINVAL, "null handle");
+    if (true_residual_inf) *true_residual_inf = h->dev.last_true_residual();
+    if (replacements) *replacements = h->dev.last_replacements();
+    return 0;
+}
 
 int uaamg_poisson_nlevels(const uaamg_poisson* h) { return h ? h->dev.nlevels() : -1; }
 

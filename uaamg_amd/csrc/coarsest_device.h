@@ -53,8 +53,8 @@ __device__ __forceinline__ float2 cg_sum2(float a, float b, CgShared<kThreads>& 
 
 // Solves the coarsest system for right-hand side B (bricks) into X (bricks,
 // DOF positions only). Every thread of the block must call it.
-template <int kThreads>
-__device__ void coarsest_cg_block(const CoarsestView& C, const float* B, float* X, CgShared<kThreads>& sh) {
+template <int kThreads, class V = float>
+__device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgShared<kThreads>& sh) {
     constexpr int kPer = kMaxCoarsestDof / kThreads;
     int parity = 0;
     const int n = C.ndof;
@@ -75,7 +75,7 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const float* B, float* 
             for (int k = 0; k < 6; ++k) nb[j][k] = C.nb[size_t(d) * 6 + k];
 #pragma unroll
             for (int k = 0; k < 8; ++k) cf[j][k] = C.coef[size_t(d) * 8 + k];
-            r[j] = B[C.pos[d]];
+            r[j] = float(B[C.pos[d]]);
             part += r[j] * r[j];
         }
     }
@@ -143,7 +143,7 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const float* B, float* 
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int d = tid + j * kThreads;
-        if (d < n) X[C.pos[d]] = x[j];
+        if (d < n) X[C.pos[d]] = V(x[j]);
     }
 }
 

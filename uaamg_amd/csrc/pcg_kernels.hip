@@ -25,11 +25,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
@@ -101,8 +96,9 @@ __global__ __launch_bounds__(kBlockThreads) void laplacian_dot_kernel(LevelView 
 
 // Last level-0 smoothing sweep of the W-cycle with the partials of <z, r>:
 // z is the sweep's output and r its right-hand side.
-__global__ __launch_bounds__(kBlockThreads) void jacobi_dot_kernel(LevelView L, const float* X, const float* B,
-                                                                   float* Y, float w, double* partials) {
+template <class VX = float>
+__global__ __launch_bounds__(kBlockThreads) void jacobi_dot_kernel(LevelView L, const VX* X, const float* B, float* Y,
+                                                                   float w, double* partials) {
     const int leaf = wave_leaf();
     const int lane = threadIdx.x & 63;
     double acc = 0;
@@ -110,7 +106,7 @@ __global__ __launch_bounds__(kBlockThreads) void jacobi_dot_kernel(LevelView L, 
         const uint32_t rm = row_mask(L, leaf, lane);
         if (rm) {
             float xc[8], s[8], y[8], b[8];
-            stencil_offdiag(L, PlainSrc{X}, leaf, lane, xc, s);
+            stencil_offdiag(L, PlainSrcT<VX>{X}, leaf, lane, xc, s);
             jacobi_row(L, leaf, lane, xc, s, B, w, y);
             mask_row(rm, y);
             st8(Y + size_t(leaf) * 512 + lane * 8, y);
@@ -165,6 +161,31 @@ __global__ __launch_bounds__(kBlockThreads) void update_laplacian_dot_kernel(Lev
     write_partial(partials, block_sum<kBlockThreads>(acc));
 }
 
+// running max|r| of one lane; a NaN or Inf sets `bad`
+__device__ __forceinline__ void absmax_acc(float r, float& m, float& bad) {
+    const float a = fabsf(r);
+    if (!(a <= FLT_MAX)) bad = 1.f;
+    else m = fmaxf(m, a);
+}
+
+// the block's partial max|r|, -1 when any entry was not finite
+__device__ __forceinline__ void write_absmax_partial(float m, float bad, double* partials) {
+    m = wave_max(m);
+    bad = wave_max(bad);
+    __shared__ float wm[kWavesPerBlock];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = bad > 0.f ? -1.f : m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = 0.f;
+        bool b = false;
+        for (int i = 0; i < kWavesPerBlock; ++i) {
+            if (wm[i] < 0.f) b = true;
+            else v = fmaxf(v, wm[i]);
+        }
+        partials[blockIdx.x] = b ? -1.0 : double(v);
+    }
+}
+
 // (optional r -= alpha z, then) partial max|r| with NaN/Inf flagged as -1
 // (grid_abs_max_op, openvdb_grid_math_op.h:4-40)
 template <bool AXPY>
@@ -194,27 +215,10 @@ __global__ __launch_bounds__(kBlockThreads) void absmax_kernel(LevelView L, cons
             }
 #pragma unroll
             for (int k = 0; k < 8; ++k)
-                if ((rm >> k) & 1u) {
-                    const float a = fabsf(r[k]);
-                    if (!(a <= FLT_MAX)) bad = 1.f;
-                    else m = fmaxf(m, a);
-                }
+                if ((rm >> k) & 1u) absmax_acc(r[k], m, bad);
         }
     }
-    m = wave_max(m);
-    bad = wave_max(bad);
-    __shared__ float wm[kWavesPerBlock];
-    if (lane == 0) wm[threadIdx.x >> 6] = bad > 0.f ? -1.f : m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float v = 0.f;
-        bool b = false;
-        for (int i = 0; i < kWavesPerBlock; ++i) {
-            if (wm[i] < 0.f) b = true;
-            else v = fmaxf(v, wm[i]);
-        }
-        partials[blockIdx.x] = b ? -1.0 : double(v);
-    }
+    write_absmax_partial(m, bad, partials);
 }
 
 constexpr int kFinThreads = 256;
@@ -314,9 +318,43 @@ __global__ __launch_bounds__(kBlockThreads) void update_kernel(LevelView L, cons
 // ---------------------------------------------------------- coarsest solve
 constexpr int kCgThreads = 512;
 
-__global__ __launch_bounds__(kCgThreads) void coarsest_cg_kernel(CoarsestView C, const float* B, float* X) {
+template <class V>
+__global__ __launch_bounds__(kCgThreads) void coarsest_cg_kernel(CoarsestView C, const V* B, V* X) {
     __shared__ CgShared<kCgThreads> sh;
-    coarsest_cg_block<kCgThreads>(C, B, X, sh);
+    coarsest_cg_block<kCgThreads, V>(C, B, X, sh);
+}
+
+// r = b - A x with FP64 products and sums (fma(x, diag, s) of the operator
+// becomes x * diag + s in FP64), rounded once to FP32, and the partial
+// maxima of |r| as absmax_kernel forms them.
+__global__ __launch_bounds__(kBlockThreads) void residual64_kernel(LevelView L, const float* X, const float* B,
+                                                                   float* R, double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    float m = 0.f, bad = 0.f;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        float r[8];
+        fill8(r, 0.f);
+        if (rm) {
+            float xc[8], dg[8], b[8];
+            double s[8];
+            stencil_offdiag64(L, PlainSrc{X}, leaf, lane, xc, s);
+            ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
+            ld8(B + size_t(leaf) * 512 + lane * 8, b);
+#pragma unroll
+            for (int z = 0; z < 8; ++z) {
+                const double ax = double(xc[z]) * double(dg[z]) + s[z];
+                r[z] = float(double(b[z]) - ax);
+            }
+            mask_row(rm, r);
+        }
+        st8(R + size_t(leaf) * 512 + lane * 8, r);
+#pragma unroll
+        for (int z = 0; z < 8; ++z)
+            if ((rm >> z) & 1u) absmax_acc(r[z], m, bad);
+    }
+    write_absmax_partial(m, bad, partials);
 }
 
 hipError_t finalize(int op, int n, const double* partials, PcgScalars* sc, HostMailbox* mb, hipStream_t s) {
@@ -331,8 +369,23 @@ int reduction_blocks(const LevelView& L) { return int(grid_for(L.nleaf)) > 0 ? i
 hipError_t launch_coarsest_cg(const CoarsestView& C, const float* b, float* x, hipStream_t s) {
     if (C.ndof <= 0) return hipSuccess;
     if (C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
-    coarsest_cg_kernel<<<1, kCgThreads, 0, s>>>(C, b, x);
+    coarsest_cg_kernel<float><<<1, kCgThreads, 0, s>>>(C, b, x);
     return hipGetLastError();
+}
+
+hipError_t launch_coarsest_cg(const CoarsestView& C, const bf16* b, bf16* x, hipStream_t s) {
+    if (C.ndof <= 0) return hipSuccess;
+    if (C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
+    coarsest_cg_kernel<bf16><<<1, kCgThreads, 0, s>>>(C, b, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual64_absmax(const LevelView& L, const float* x, const float* b, float* r, PcgScalars* sc,
+                                    double* partials, HostMailbox* mb, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    residual64_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, x, b, r, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(kFinNu, g, partials, sc, mb, s);
 }
 
 hipError_t launch_dot(const LevelView& L, const float* a, const float* b, PcgScalars* sc, double* partials,
@@ -384,7 +437,15 @@ hipError_t launch_update_x(const LevelView& L, const float* p, float* x, const P
 hipError_t launch_jacobi_dot(const LevelView& L, const float* x, const float* b, float* y, float w, PcgScalars* sc,
                              double* partials, int target, hipStream_t s) {
     const int g = reduction_blocks(L);
-    jacobi_dot_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, x, b, y, w, partials);
+    jacobi_dot_kernel<float><<<unsigned(g), kBlockThreads, 0, s>>>(L, x, b, y, w, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(target == kDotRho ? kFinRho : kFinRhoNew, g, partials, sc, nullptr, s);
+}
+
+hipError_t launch_jacobi_dot(const LevelView& L, const bf16* x, const float* b, float* y, float w, PcgScalars* sc,
+                             double* partials, int target, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    jacobi_dot_kernel<bf16><<<unsigned(g), kBlockThreads, 0, s>>>(L, x, b, y, w, partials);
     if (hipError_t e = hipGetLastError()) return e;
     return finalize(target == kDotRho ? kFinRho : kFinRhoNew, g, partials, sc, nullptr, s);
 }

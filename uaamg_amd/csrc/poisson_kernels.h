@@ -15,6 +15,9 @@
 
 namespace uaamg {
 
+// storage type of the W-cycle vectors in the mixed-precision form (arithmetic stays FP32)
+using bf16 = __bf16;
+
 // slot indices per leaf
 enum { kSlotDiag = 0, kSlotX = 1, kSlotY = 2, kSlotZ = 3, kSlotSpai0 = 4, kSlotStride = 8 };
 
@@ -168,5 +171,37 @@ hipError_t launch_block_pre(const LevelView& L, bool from_zero, const float* x, 
 // x_out = three sweeps (w0, w1, w2) from x + P x_coarse
 hipError_t launch_block_post(const LevelView& L, const float* x, const float* xc, const float* b, float* x_out,
                              float w0, float w1, float w2, hipStream_t s);
+
+// ---- mixed-precision W-cycle: the same passes with the level vectors stored
+// in BF16 and every operation computed in FP32. Level 0's right-hand side is
+// the outer PCG's FP32 residual and its last sweep writes the FP32 z; every
+// other vector of the cycle is BF16.
+hipError_t launch_jacobi_first2(const LevelView& L, const float* b, bf16* y, float w0, float w1, hipStream_t s);
+hipError_t launch_jacobi_first2(const LevelView& L, const bf16* b, bf16* y, float w0, float w1, hipStream_t s);
+// y = J_w(x) (weighted Jacobi sweep)
+hipError_t launch_jacobi(const LevelView& L, const bf16* x, const float* b, bf16* y, float w, hipStream_t s);
+hipError_t launch_jacobi(const LevelView& L, const bf16* x, const float* b, float* y, float w, hipStream_t s);
+hipError_t launch_jacobi(const LevelView& L, const float* x, const float* b, bf16* y, float w, hipStream_t s);
+hipError_t launch_jacobi(const LevelView& L, const bf16* x, const bf16* b, bf16* y, float w, hipStream_t s);
+hipError_t launch_residual_restrict(const LevelView& fine, const bf16* x, const float* b, bf16* coarse_b,
+                                    hipStream_t s);
+hipError_t launch_residual_restrict(const LevelView& fine, const bf16* x, const bf16* b, bf16* coarse_b,
+                                    hipStream_t s);
+hipError_t launch_prolong_jacobi(const LevelView& L, const bf16* x, const bf16* xc, const float* b, bf16* y, float w,
+                                 hipStream_t s);
+hipError_t launch_prolong_jacobi(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* y, float w,
+                                 hipStream_t s);
+hipError_t launch_jacobi_dot(const LevelView& L, const bf16* x, const float* b, float* y, float w, PcgScalars* sc,
+                             double* partials, int target, hipStream_t s);
+hipError_t launch_coarsest_cg(const CoarsestView& C, const bf16* b, bf16* x, hipStream_t s);
+hipError_t launch_block_pre(const LevelView& L, bool from_zero, const bf16* x, const bf16* b, bf16* x_out,
+                            bf16* coarse_b, float w0, float w1, float w2, hipStream_t s);
+hipError_t launch_block_post(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* x_out,
+                             float w0, float w1, float w2, hipStream_t s);
+
+// ---- FP64 residual replacement: r = b - A x with every product and sum in
+// FP64, stored as FP32, then nu = ||r||_inf through the mailbox
+hipError_t launch_residual64_absmax(const LevelView& L, const float* x, const float* b, float* r, PcgScalars* sc,
+                                    double* partials, HostMailbox* mb, hipStream_t s);
 
 }  // namespace uaamg

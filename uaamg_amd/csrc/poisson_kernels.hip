@@ -14,16 +14,16 @@ namespace {
 
 using namespace dev;
 
-template <int MODE, bool kLevel0>
-__global__ __launch_bounds__(kBlockThreads) void apply_kernel(LevelView L, const float* X, const float* B, float* Y,
-                                                              float w, float one_minus_w) {
+template <int MODE, bool kLevel0, class VX = float, class VB = float, class VY = float>
+__global__ __launch_bounds__(kBlockThreads) void apply_kernel(LevelView L, const VX* X, const VB* B, VY* Y, float w,
+                                                              float one_minus_w) {
     const int leaf = wave_leaf();
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(L, leaf, lane);
     if (rm == 0) return;
     float xc[8], s[8], y[8];
-    stencil_offdiag(L, PlainSrc{X}, leaf, lane, xc, s);
+    stencil_offdiag(L, PlainSrcT<VX>{X}, leaf, lane, xc, s);
     const int sd = L.slot[size_t(leaf) * kSlotStride + kSlotDiag];
     const size_t row = size_t(leaf) * 512 + lane * 8;
     if (MODE == kModeRedGS || MODE == kModeBlackGS) {
@@ -180,15 +180,15 @@ __global__ __launch_bounds__(kBlockThreads) void prolong_add_kernel(LevelView Fl
 
 // First two scheduled/weighted Jacobi sweeps from a zero guess in one pass:
 // the first sweep's result is rebuilt where the second sweep reads it.
-template <bool kLevel0>
-__global__ __launch_bounds__(kBlockThreads) void jacobi_first2_kernel(LevelView L, const float* B, float* Y, float w0,
+template <bool kLevel0, class VB = float, class VY = float>
+__global__ __launch_bounds__(kBlockThreads) void jacobi_first2_kernel(LevelView L, const VB* B, VY* Y, float w0,
                                                                       float w1) {
     const int leaf = wave_leaf();
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(L, leaf, lane);
     if (rm == 0) return;
-    const FirstJacobiSrc src{B, L.invdiag, L.slot, L.dflt_invdiag, w0};
+    const FirstJacobiSrcT<VB> src{B, L.invdiag, L.slot, L.dflt_invdiag, w0};
     float xc[8], s[8], y[8];
     stencil_offdiag(L, src, leaf, lane, xc, s);
     jacobi_row(L, leaf, lane, xc, s, B, w1, y);
@@ -197,15 +197,15 @@ __global__ __launch_bounds__(kBlockThreads) void jacobi_first2_kernel(LevelView 
 }
 
 // Prolongation (x += P x_coarse) fused into the first post-smoothing sweep.
-template <bool kLevel0>
-__global__ __launch_bounds__(kBlockThreads) void prolong_jacobi_kernel(LevelView L, const float* X, const float* Xc,
-                                                                       const float* B, float* Y, float w) {
+template <bool kLevel0, class V = float, class VB = float>
+__global__ __launch_bounds__(kBlockThreads) void prolong_jacobi_kernel(LevelView L, const V* X, const V* Xc,
+                                                                       const VB* B, V* Y, float w) {
     const int leaf = wave_leaf();
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(L, leaf, lane);
     if (rm == 0) return;
-    const ProlongSrc src{X, Xc, L.mask, L.parent};
+    const ProlongSrcT<V> src{X, Xc, L.mask, L.parent};
     float xc[8], s[8], y[8];
     stencil_offdiag(L, src, leaf, lane, xc, s);
     jacobi_row(L, leaf, lane, xc, s, B, w, y);
@@ -217,9 +217,9 @@ __global__ __launch_bounds__(kBlockThreads) void prolong_jacobi_kernel(LevelView
 // coarse voxel's eight children lie in one fine leaf, so the leaf writes the
 // 4^3 octant of its parent coarse leaf (restriction, :1624-1684, sums the
 // children in x,y,z nested order and scales by 1/8).
-template <bool kLevel0>
-__global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelView F, const float* X, const float* B,
-                                                                          float* Cv) {
+template <bool kLevel0, class V = float, class VB = float>
+__global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelView F, const V* X, const VB* B,
+                                                                          V* Cv) {
     __shared__ float rs[kWavesPerBlock][512];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelV
         const uint32_t rm = row_mask(F, leaf, lane);
         if (rm) {
             float xc[8], s[8], dg[8], b[8];
-            stencil_offdiag(F, PlainSrc{X}, leaf, lane, xc, s);
+            stencil_offdiag(F, PlainSrcT<V>{X}, leaf, lane, xc, s);
             ld8_slot(F.diag, F.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, F.dflt_diag, dg);
             ld8(B + size_t(leaf) * 512 + lane * 8, b);
 #pragma unroll
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelV
     if (p < 0) return;   // every DOF leaf has a parent; kept as a bounds guard
     const int cl = p >> 3, oct = p & 7;
     const int off = ((cx + 4 * ((oct >> 2) & 1)) * 8 + (cy + 4 * ((oct >> 1) & 1))) * 8 + cz + 4 * (oct & 1);
-    Cv[size_t(cl) * 512 + off] = sum * 0.125f;
+    st1(Cv + size_t(cl) * 512 + off, sum * 0.125f);
 }
 
 __device__ __forceinline__ int64_t row_dof_base(const LevelView& L, int leaf, int lane) {
@@ -381,6 +381,70 @@ hipError_t launch_residual_restrict(const LevelView& fine, const float* x, const
     if (fine.level == 0)
         residual_restrict_kernel<true><<<grid_for(fine.nleaf), kBlockThreads, 0, s>>>(fine, x, b, coarse_b);
     else residual_restrict_kernel<false><<<grid_for(fine.nleaf), kBlockThreads, 0, s>>>(fine, x, b, coarse_b);
+    return hipGetLastError();
+}
+
+// ---- mixed-precision W-cycle launchers (level 0 reads the FP32 PCG residual)
+hipError_t launch_jacobi_first2(const LevelView& L, const float* b, bf16* y, float w0, float w1, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    jacobi_first2_kernel<true, float, bf16><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, b, y, w0, w1);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_first2(const LevelView& L, const bf16* b, bf16* y, float w0, float w1, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    jacobi_first2_kernel<false, bf16, bf16><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, b, y, w0, w1);
+    return hipGetLastError();
+}
+
+template <bool kLevel0, class VX, class VB, class VY>
+static hipError_t jacobi_t(const LevelView& L, const VX* x, const VB* b, VY* y, float w, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    apply_kernel<kModeJacobi, kLevel0, VX, VB, VY>
+        <<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, x, b, y, w, 1.0f - w);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi(const LevelView& L, const bf16* x, const float* b, bf16* y, float w, hipStream_t s) {
+    return jacobi_t<true>(L, x, b, y, w, s);
+}
+hipError_t launch_jacobi(const LevelView& L, const bf16* x, const float* b, float* y, float w, hipStream_t s) {
+    return jacobi_t<true>(L, x, b, y, w, s);
+}
+hipError_t launch_jacobi(const LevelView& L, const float* x, const float* b, bf16* y, float w, hipStream_t s) {
+    return jacobi_t<true>(L, x, b, y, w, s);
+}
+hipError_t launch_jacobi(const LevelView& L, const bf16* x, const bf16* b, bf16* y, float w, hipStream_t s) {
+    return jacobi_t<false>(L, x, b, y, w, s);
+}
+
+hipError_t launch_residual_restrict(const LevelView& fine, const bf16* x, const float* b, bf16* coarse_b,
+                                    hipStream_t s) {
+    if (fine.nleaf <= 0) return hipSuccess;
+    residual_restrict_kernel<true, bf16, float>
+        <<<grid_for(fine.nleaf), kBlockThreads, 0, s>>>(fine, x, b, coarse_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_restrict(const LevelView& fine, const bf16* x, const bf16* b, bf16* coarse_b,
+                                    hipStream_t s) {
+    if (fine.nleaf <= 0) return hipSuccess;
+    residual_restrict_kernel<false, bf16, bf16>
+        <<<grid_for(fine.nleaf), kBlockThreads, 0, s>>>(fine, x, b, coarse_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong_jacobi(const LevelView& L, const bf16* x, const bf16* xc, const float* b, bf16* y, float w,
+                                 hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    prolong_jacobi_kernel<true, bf16, float><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, x, xc, b, y, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong_jacobi(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* y, float w,
+                                 hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    prolong_jacobi_kernel<false, bf16, bf16><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, x, xc, b, y, w);
     return hipGetLastError();
 }
 

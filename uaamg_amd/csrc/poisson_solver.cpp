@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 
 namespace uaamg {
@@ -212,6 +213,7 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         for (int k = 0; k < kNumCoef; ++k) add(nslot[k] * kLeafSize * sizeof(float));
         add(nslot[kCoefDiag] * kLeafSize * sizeof(float));   // invdiag
         add(4 * n * kLeafSize * sizeof(float));               // X, B, T, U
+        if (opt_.precision == 1) add(4 * n * kLeafSize * sizeof(bf16));
     }
     add(10 * n0 * kLeafSize * sizeof(float));   // x0 b0 r0 p p' z0 q0 + 3 unit buffers
     add(3 * std::max<int64_t>(1, host_.levels[0].ndof) * sizeof(float));
@@ -290,6 +292,28 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         env = std::getenv("UAAMG_BLOCK_LEAVES");
         if (env) block_max_leaves_ = std::atoi(env);
         if (block_max_leaves_ > 0 && check(block_kernels_init(), "block kernel attributes")) return -3;
+    }
+    if (opt_.precision != 0 && opt_.precision != 1) {
+        err_ = "unsupported precision";
+        return -1;
+    }
+    mixed_ = opt_.precision == 1;
+    refine_ = mixed_ || (opt_.flags & 64);
+    if (mixed_) {
+        tail_level_ = -1;
+        for (int l = 0; l < L; ++l) {
+            const size_t n = host_.levels[l].nleaf() * kLeafSize;
+            bf16** bufs[4] = {&lv_[l].Xh, &lv_[l].Bh, &lv_[l].Th, &lv_[l].Uh};
+            for (bf16** b : bufs) {
+                *b = static_cast<bf16*>(arena_.take(n * sizeof(bf16)));
+                if (!*b && n) {
+                    err_ = "arena too small";
+                    return -2;
+                }
+                // rows without DOFs are never written by the sweeps
+                if (n && check(hipMemsetAsync(*b, 0, n * sizeof(bf16), stream_), "memset")) return -3;
+            }
+        }
     }
     if (int rc = build_coarsest()) return rc;
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
@@ -491,6 +515,10 @@ void PoissonDevice::precond(int l, float* X, const float* B, bool skip_first, in
         return;
     }
     const bool jacobi = opt_.smoother == kScheduledRelaxedJacobi || opt_.smoother == kWeightedJacobi;
+    if (mixed_ && fuse_ && jacobi) {
+        precond_mixed<float, float>(l, X, B, skip_first, dot_target);
+        return;
+    }
     if (fuse_ && jacobi) {
         precond_fused(l, X, B, skip_first, dot_target);
         return;
@@ -613,6 +641,77 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
         check(launch_apply(v, kModeJacobi, cur, B, X, wpost(2), stream_), "jacobi");
 }
 
+// The fused Jacobi-family W-cycle with its level vectors stored in BF16: the
+// same passes as precond_fused, every value computed in FP32 and rounded to
+// BF16 where it is stored. Level 0 reads the FP32 residual B and its last
+// sweep writes the FP32 X; below level 0 (TX = TB = bf16) every vector of the
+// cycle is BF16.
+template <class TX, class TB>
+void PoissonDevice::precond_mixed(int l, TX* X, const TB* B, bool skip_first, int dot_target) {
+    const int L = nlevels();
+    const LevelView& v = lv_[l].v;
+    DevLevel& d = lv_[l];
+    if (l == L - 1) {
+        check(launch_coarsest_cg(cv_, B, X, stream_), "coarsest");
+        if constexpr (std::is_same_v<TX, float>) {
+            if (dot_target >= 0) check(launch_dot(v, X, B, sc_, partials_, dot_target, stream_), "dot");
+        }
+        return;
+    }
+    const bool weighted = opt_.smoother == kWeightedJacobi;
+    auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
+    auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    DevLevel& c = lv_[l + 1];
+    auto coarse = [&]() {
+        precond_mixed<bf16, bf16>(l + 1, c.Xh, c.Bh, true, -1);
+        if (l + 1 != L - 1) precond_mixed<bf16, bf16>(l + 1, c.Xh, c.Bh, false, -1);
+    };
+    if constexpr (std::is_same_v<TX, bf16>) {
+        if (l >= 1 && int64_t(v.nleaf) <= block_max_leaves_) {
+            check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+            coarse();
+            check(launch_block_post(v, d.Th, c.Xh, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
+            return;
+        }
+    }
+    bf16* cur = nullptr;
+    auto next = [&]() { return cur == d.Th ? d.Uh : d.Th; };
+    if (skip_first) {
+        bf16* dst = next();
+        check(launch_jacobi_first2(v, B, dst, wpre(0), wpre(1), stream_), "first2");
+        cur = dst;
+        dst = next();
+        check(launch_jacobi(v, cur, B, dst, wpre(2), stream_), "jacobi");
+        cur = dst;
+    } else {
+        bf16* dst = next();
+        check(launch_jacobi(v, X, B, dst, wpre(0), stream_), "jacobi");
+        cur = dst;
+        for (int i = 1; i < kSweeps; ++i) {
+            dst = next();
+            check(launch_jacobi(v, cur, B, dst, wpre(i), stream_), "jacobi");
+            cur = dst;
+        }
+    }
+    check(launch_residual_restrict(v, cur, B, c.Bh, stream_), "residual restrict");
+    coarse();
+    {
+        bf16* dst = next();
+        check(launch_prolong_jacobi(v, cur, c.Xh, B, dst, wpost(0), stream_), "prolong jacobi");
+        cur = dst;
+        dst = next();
+        check(launch_jacobi(v, cur, B, dst, wpost(1), stream_), "jacobi");
+        cur = dst;
+    }
+    if constexpr (std::is_same_v<TX, float>) {
+        if (dot_target >= 0) {
+            check(launch_jacobi_dot(v, cur, B, X, wpost(2), sc_, partials_, dot_target, stream_), "jacobi dot");
+            return;
+        }
+    }
+    check(launch_jacobi(v, cur, B, X, wpost(2), stream_), "jacobi");
+}
+
 // muCycleIterative<2> (simd_vdb_poisson.cpp:1979-2110)
 void PoissonDevice::mg_cycle(int l, float* X, const float* B) {
     const int L = nlevels();
@@ -659,32 +758,67 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
     if (wait_nu(&nu)) return -3;
     const float numax = opt_.relative_tolerance * nu;
     int status = kFailed;
+    true_nu_ = -1.f;
+    replacements_ = -1;
     if (nu <= numax) {
         status = kSuccess;
     } else if (opt_.max_iterations > 0) {
-        if (run(g_first_, [&] { issue_first(); })) return -3;
-        int k = 0;   // p_[k] holds the current direction
+        status = pcg_iterate(nu, numax, it);
+        if (status < 0) return status;
+    }
+    if (refine_ && status == kSuccess) {
+        // FP64 residual replacement: the recursive residual met the tolerance;
+        // recompute r = b - A x in FP64 and, while the true residual misses
+        // it, restart the PCG recurrence from x on that residual (the
+        // iteration count continues). Restarts stop when one no longer halves
+        // the true residual (the FP32 iterate's attainable floor); the status
+        // then stays the recursive criterion's, and last_true_residual() says
+        // how far the true residual got.
+        replacements_ = 0;
+        float prev = INFINITY;
         for (;;) {
-            const float nu_old = nu;
+            check(launch_residual64_absmax(v0, x0_, b0_, r0_, sc_, partials_, mailbox_dev_, stream_), "residual64");
             if (wait_nu(&nu)) return -3;
-            if (nu <= numax) {
-                check(launch_update_x(v0, p_[k], x0_, sc_, stream_), "update x");
-                status = kSuccess;
-                break;
-            }
-            if (nu > nu_old && it > 3) break;
-            if (run(g_iter_[k], [&] { issue_iter(k); })) return -3;
-            k ^= 1;
-            if (++it >= opt_.max_iterations) {
-                if (wait_nu(&nu)) return -3;   // the piece's own reduction
-                break;
-            }
+            true_nu_ = nu;
+            if (nu <= numax || !(nu < 0.5f * prev)) break;
+            if (replacements_ >= kMaxReplacements || it >= opt_.max_iterations) break;
+            prev = nu;
+            ++replacements_;
+            status = pcg_iterate(nu, numax, it);
+            if (status < 0) return status;
+            if (status != kSuccess) break;
         }
     }
     if (iterations) *iterations = it;
     check(launch_gather_dof(v0, x0_, x_dof, stream_), "gather");
     if (check(hipStreamSynchronize(stream_), "solve sync")) return -3;
     return first_error_ == hipSuccess ? status : -3;
+}
+
+// Lines 6-16 of solveMultigridPCG from the residual in r0_ (nu = its
+// ||r||_inf): p = M r, then the iteration pieces until ||r||_inf <= numax
+// (x updated, kSuccess), the residual grows after this recurrence's third
+// iteration or `it` reaches max_iterations (kFailed), or an error (< 0).
+int PoissonDevice::pcg_iterate(float& nu, float numax, int& it) {
+    const LevelView& v0 = lv_[0].v;
+    const int it0 = it;   // the growth test counts this recurrence's own iterations
+    if (run(g_first_, [&] { issue_first(); })) return -3;
+    int k = 0;   // p_[k] holds the current direction
+    for (;;) {
+        const float nu_old = nu;
+        if (wait_nu(&nu)) return -3;
+        if (nu <= numax) {
+            check(launch_update_x(v0, p_[k], x0_, sc_, stream_), "update x");
+            return kSuccess;
+        }
+        if (nu > nu_old && it - it0 > 3) return kFailed;
+        if (run(g_iter_[k], [&] { issue_iter(k); })) return -3;
+        k ^= 1;
+        if (++it >= opt_.max_iterations) {
+            if (wait_nu(&nu)) return -3;   // the piece's own reduction
+            return kFailed;
+        }
+    }
 }
 
 // solvePureMultigrid (simd_vdb_poisson.cpp:2228-2267); issued directly (the

@@ -20,10 +20,11 @@ struct SolverOptions {
     float relative_tolerance = 1e-7f;
     int max_iterations = 100;
     int smoother = kScheduledRelaxedJacobi;
-    int precision = 0;
+    int precision = 0;   // 0: FP32 throughout; 1: BF16-stored W-cycle vectors + FP64 residual replacement
     int device = 0;
     int64_t max_coarsest_dof = 4000;
-    int flags = 0;   // bit 0: unfused kernels, bit 1: no graph capture, bit 2: tail kernel (A-B checks)
+    int flags = 0;   // bit 0: unfused kernels, bit 1: no graph capture, bit 2: tail kernel (A-B checks),
+                     // bit 4: no block-visit kernels, bit 6: FP64 residual replacement at FP32
     hipStream_t stream = nullptr;
 };
 
@@ -45,6 +46,7 @@ class PoissonDevice {
 public:
     static constexpr int kSuccess = 0;
     static constexpr int kFailed = 1;
+    static constexpr int kMaxReplacements = 4;
 
     ~PoissonDevice();
     // returns 0 or a negative uaamg status
@@ -83,6 +85,11 @@ public:
     // the solver's stream, timed with HIP events on that stream.
     int time_apply(int level, int mode, int reps, float* ms_per_launch);
 
+    // ||b - A x||_inf of the last solve's result (FP64 residual) when residual
+    // replacement is on, and how many replacements it made; -1 otherwise
+    float last_true_residual() const { return true_nu_; }
+    int last_replacements() const { return replacements_; }
+
 private:
     struct DevLevel {
         LevelView v{};
@@ -90,6 +97,11 @@ private:
         float* B = nullptr;
         float* T = nullptr;
         float* U = nullptr;   // second smoothing buffer of the fused W-cycle
+        // the same vectors in BF16 (mixed-precision form only)
+        bf16* Xh = nullptr;
+        bf16* Bh = nullptr;
+        bf16* Th = nullptr;
+        bf16* Uh = nullptr;
     };
 
     struct Graph {
@@ -109,6 +121,9 @@ private:
     int build_coarsest();
     void precond(int level, float* X, const float* B, bool skip_first, int dot_target);
     void precond_fused(int level, float* X, const float* B, bool skip_first, int dot_target);
+    template <class TX, class TB>
+    void precond_mixed(int level, TX* X, const TB* B, bool skip_first, int dot_target);
+    int pcg_iterate(float& nu, float numax, int& it);
     void mg_cycle(int level, float* X, const float* B);
     void smooth(int level, float*& cur, float*& oth, const float* B, int sweep, bool pre, bool iterative);
     int check(hipError_t e, const char* what);
@@ -136,6 +151,10 @@ private:
     bool fuse_ = true;                     // fused W-cycle / PCG kernels
     int tail_level_ = -1;                  // level whose visit pairs run in the tail kernel (-1: none)
     int block_max_leaves_ = 0;             // coarse levels up to this size use the block-visit kernels
+    bool mixed_ = false;                   // BF16-stored W-cycle vectors (Jacobi-family smoothers)
+    bool refine_ = false;                  // FP64 residual replacement after convergence
+    float true_nu_ = -1.f;
+    int replacements_ = -1;
     HostMailbox* mailbox_ = nullptr;       // pinned host memory
     HostMailbox* mailbox_dev_ = nullptr;   // its device alias
     unsigned int nu_seq_ = 0;              // ||r||_inf reductions issued so far

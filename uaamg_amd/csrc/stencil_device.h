@@ -30,6 +30,41 @@ __device__ __forceinline__ void st8(float* p, const float v[8]) {
     reinterpret_cast<float4*>(p)[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
 
+// BF16 storage (W-cycle vectors of the mixed-precision form): a z-row is 16
+// bytes; widening is exact, narrowing rounds to nearest even
+// (v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ void ld8(const bf16* p, float v[8]) {
+    const uint4 q = *reinterpret_cast<const uint4*>(p);
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[2 * i] = __uint_as_float(w[i] << 16);
+        v[2 * i + 1] = __uint_as_float(w[i] & 0xffff0000u);
+    }
+}
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b) {
+    const bf16 ha = bf16(a), hb = bf16(b);
+    return uint32_t(__builtin_bit_cast(uint16_t, ha)) | (uint32_t(__builtin_bit_cast(uint16_t, hb)) << 16);
+}
+
+__device__ __forceinline__ void st8(bf16* p, const float v[8]) {
+    *reinterpret_cast<uint4*>(p) =
+        make_uint4(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]), pack_bf16(v[4], v[5]), pack_bf16(v[6], v[7]));
+}
+
+__device__ __forceinline__ float ld1(const float* p) { return *p; }
+__device__ __forceinline__ float ld1(const bf16* p) { return float(*p); }
+__device__ __forceinline__ void st1(float* p, float v) { *p = v; }
+__device__ __forceinline__ void st1(bf16* p, float v) { *p = bf16(v); }
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float4 ld4(const bf16* p) {
+    const uint2 q = *reinterpret_cast<const uint2*>(p);
+    return make_float4(__uint_as_float(q.x << 16), __uint_as_float(q.x & 0xffff0000u), __uint_as_float(q.y << 16),
+                       __uint_as_float(q.y & 0xffff0000u));
+}
+
 __device__ __forceinline__ void fill8(float v[8], float c) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) v[i] = c;
@@ -49,18 +84,21 @@ __device__ __forceinline__ void ld8_slot(const float* base, int slot, int row, f
 // Sources of the vector a stencil is applied to. row() fills the 8 values of
 // one z-row of a leaf, at() returns one value; both are called only for leaves
 // that exist (an absent neighbour leaf contributes 0).
-struct PlainSrc {   // the stored brick vector
-    const float* X;
+template <class V>
+struct PlainSrcT {   // the stored brick vector
+    const V* X;
     __device__ __forceinline__ void row(int leaf, int r, float v[8]) const { ld8(X + size_t(leaf) * 512 + r * 8, v); }
-    __device__ __forceinline__ float at(int leaf, int r, int z) const { return X[size_t(leaf) * 512 + r * 8 + z]; }
+    __device__ __forceinline__ float at(int leaf, int r, int z) const { return ld1(X + size_t(leaf) * 512 + r * 8 + z); }
 };
+using PlainSrc = PlainSrcT<float>;
 
 // x1 = w0 * D^-1 b: the first Jacobi sweep from a zero guess, evaluated where
 // the second sweep needs it (setGridToResultAfterFirstJacobi,
 // simd_vdb_poisson.cpp:1435-1479: (invdiag * rhs) * w). B is 0 on non-DOF
 // voxels, so x1 is too.
-struct FirstJacobiSrc {
-    const float* B;
+template <class VB>
+struct FirstJacobiSrcT {
+    const VB* B;
     const float* invdiag;
     const int32_t* slot;
     float dflt_inv;
@@ -75,15 +113,17 @@ struct FirstJacobiSrc {
     __device__ __forceinline__ float at(int leaf, int r, int z) const {
         const int sd = slot[size_t(leaf) * kSlotStride + kSlotDiag];
         const float inv = sd >= 0 ? invdiag[size_t(sd) * 512 + r * 8 + z] : dflt_inv;
-        return w0 * (inv * B[size_t(leaf) * 512 + r * 8 + z]);
+        return w0 * (inv * ld1(B + size_t(leaf) * 512 + r * 8 + z));
     }
 };
+using FirstJacobiSrc = FirstJacobiSrcT<float>;
 
 // x + P x_coarse on on-voxels (prolongation<true> with alpha = 1,
 // simd_vdb_poisson.cpp:1686-1754), evaluated where the following sweep needs it.
-struct ProlongSrc {
-    const float* X;
-    const float* Xc;
+template <class V>
+struct ProlongSrcT {
+    const V* X;
+    const V* Xc;
     const uint64_t* mask;
     const int32_t* parent;
     __device__ __forceinline__ uint32_t bits(int leaf, int r) const {
@@ -93,7 +133,7 @@ struct ProlongSrc {
         const int p = parent[leaf];
         const int cl = p >> 3, oct = p & 7;
         const int crow = (((r >> 3) >> 1) + 4 * ((oct >> 2) & 1)) * 8 + (((r & 7) >> 1) + 4 * ((oct >> 1) & 1));
-        return *reinterpret_cast<const float4*>(Xc + size_t(cl) * 512 + crow * 8 + 4 * (oct & 1));
+        return ld4(Xc + size_t(cl) * 512 + crow * 8 + 4 * (oct & 1));
     }
     __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
         ld8(X + size_t(leaf) * 512 + r * 8, v);
@@ -107,7 +147,7 @@ struct ProlongSrc {
         }
     }
     __device__ __forceinline__ float at(int leaf, int r, int z) const {
-        float v = X[size_t(leaf) * 512 + r * 8 + z];
+        float v = ld1(X + size_t(leaf) * 512 + r * 8 + z);
         if (parent[leaf] >= 0 && ((bits(leaf, r) >> z) & 1u)) {
             const float4 c = coarse4(leaf, r);
             const float cc[4] = {c.x, c.y, c.z, c.w};
@@ -116,6 +156,7 @@ struct ProlongSrc {
         return v;
     }
 };
+using ProlongSrc = ProlongSrcT<float>;
 
 // p' = z + beta p (levelXPlusAlphaY, simd_vdb_poisson.cpp:2305-2320), both 0 off the DOFs
 struct UpdateSrc {
@@ -139,16 +180,28 @@ struct UpdateSrc {
     }
 };
 
-// Own row of the source and the off-diagonal sum of the 7-point stencil for one row.
+// The operands of the 7-point stencil for one row: neighbour values and the
+// coefficients of the faces they couple through.
+struct StencilRow {
+    float xm[8], xp[8], ym[8], yp[8];
+    float zlo, zhi;
+    float cxm[8], cxp[8], cym[8], cyp[8], czm[8];
+    float czhi;
+};
+
+// Own row of the source and the off-diagonal operands of one row.
 template <class Src>
-__device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
-                                                float s[8]) {
+__device__ __forceinline__ void stencil_gather(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
+                                               StencilRow& o) {
     const int lx = lane >> 3, ly = lane & 7;
     const int32_t* nb = L.nbr + size_t(leaf) * 6;
     const int nmx = nb[0], npx = nb[1], nmy = nb[2], npy = nb[3], nmz = nb[4], npz = nb[5];
     src.row(leaf, lane, xc);
 
-    float xm[8], xp[8], ym[8], yp[8];
+    float* xm = o.xm;
+    float* xp = o.xp;
+    float* ym = o.ym;
+    float* yp = o.yp;
     {
         const int sl = lx > 0 ? leaf : nmx, sr = lx > 0 ? lane - 8 : lane + 56;
         if (sl >= 0) src.row(sl, sr, xm);
@@ -169,8 +222,8 @@ __device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& s
         if (sl >= 0) src.row(sl, sr, yp);
         else fill8(yp, 0.f);
     }
-    const float zlo = nmz >= 0 ? src.at(nmz, lane, 7) : 0.f;
-    const float zhi = npz >= 0 ? src.at(npz, lane, 0) : 0.f;
+    o.zlo = nmz >= 0 ? src.at(nmz, lane, 7) : 0.f;
+    o.zhi = npz >= 0 ? src.at(npz, lane, 0) : 0.f;
 
     const int32_t* sl = L.slot + size_t(leaf) * kSlotStride;
     const int sx = sl[kSlotX], sy = sl[kSlotY], sz = sl[kSlotZ];
@@ -179,33 +232,62 @@ __device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& s
     const int szp = npz >= 0 ? L.slot[size_t(npz) * kSlotStride + kSlotZ] : -1;
     const float d = L.dflt_off;
 
-    float cxm[8], cxp[8], cym[8], cyp[8], czm[8];
-    ld8_slot(L.cx, sx, lane, d, cxm);
-    ld8_slot(L.cx, lx < 7 ? sx : sxp, lx < 7 ? lane + 8 : lane - 56, d, cxp);
-    ld8_slot(L.cy, sy, lane, d, cym);
-    ld8_slot(L.cy, ly < 7 ? sy : syp, ly < 7 ? lane + 1 : lane - 7, d, cyp);
-    ld8_slot(L.cz, sz, lane, d, czm);
-    const float czhi = szp >= 0 ? L.cz[size_t(szp) * 512 + lane * 8] : d;
+    ld8_slot(L.cx, sx, lane, d, o.cxm);
+    ld8_slot(L.cx, lx < 7 ? sx : sxp, lx < 7 ? lane + 8 : lane - 56, d, o.cxp);
+    ld8_slot(L.cy, sy, lane, d, o.cym);
+    ld8_slot(L.cy, ly < 7 ? sy : syp, ly < 7 ? lane + 1 : lane - 7, d, o.cyp);
+    ld8_slot(L.cz, sz, lane, d, o.czm);
+    o.czhi = szp >= 0 ? L.cz[size_t(szp) * 512 + lane * 8] : d;
+}
 
+// Own row of the source and the off-diagonal sum of the 7-point stencil for one row.
+template <class Src>
+__device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
+                                                float s[8]) {
+    StencilRow o;
+    stencil_gather(L, src, leaf, lane, xc, o);
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        const float zpv = z < 7 ? xc[z + 1] : zhi;
-        const float czp = z < 7 ? czm[z + 1] : czhi;
-        const float zmv = z > 0 ? xc[z - 1] : zlo;
-        const float f1 = xp[z] * cxp[z];
-        const float f2 = xm[z] * cxm[z];
-        const float f3 = yp[z] * cyp[z];
-        const float f4 = ym[z] * cym[z];
+        const float zpv = z < 7 ? xc[z + 1] : o.zhi;
+        const float czp = z < 7 ? o.czm[z + 1] : o.czhi;
+        const float zmv = z > 0 ? xc[z - 1] : o.zlo;
+        const float f1 = o.xp[z] * o.cxp[z];
+        const float f2 = o.xm[z] * o.cxm[z];
+        const float f3 = o.yp[z] * o.cyp[z];
+        const float f4 = o.ym[z] * o.cym[z];
         const float f5 = zpv * czp;
-        const float f6 = zmv * czm[z];
+        const float f6 = zmv * o.czm[z];
+        s[z] = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+    }
+}
+
+// The same off-diagonal sum with every product and sum in FP64 (exact
+// products of FP32 operands), for the true residual of the refinement step.
+template <class Src>
+__device__ __forceinline__ void stencil_offdiag64(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
+                                                  double s[8]) {
+    StencilRow o;
+    stencil_gather(L, src, leaf, lane, xc, o);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        const double zpv = z < 7 ? xc[z + 1] : o.zhi;
+        const double czp = z < 7 ? o.czm[z + 1] : o.czhi;
+        const double zmv = z > 0 ? xc[z - 1] : o.zlo;
+        const double f1 = double(o.xp[z]) * double(o.cxp[z]);
+        const double f2 = double(o.xm[z]) * double(o.cxm[z]);
+        const double f3 = double(o.yp[z]) * double(o.cyp[z]);
+        const double f4 = double(o.ym[z]) * double(o.cym[z]);
+        const double f5 = zpv * czp;
+        const double f6 = zmv * double(o.czm[z]);
         s[z] = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
     }
 }
 
 
 // Weighted Jacobi on one row: y = fma(w, (b - (fma(x, diag, s))) * D^-1, x).
+template <class VB>
 __device__ __forceinline__ void jacobi_row(const LevelView& L, int leaf, int lane, const float xc[8], const float s[8],
-                                           const float* B, float w, float y[8]) {
+                                           const VB* B, float w, float y[8]) {
     const int sd = L.slot[size_t(leaf) * kSlotStride + kSlotDiag];
     float dg[8], inv[8], b[8];
     ld8_slot(L.diag, sd, lane, L.dflt_diag, dg);
