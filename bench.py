@@ -47,6 +47,8 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-precision", type=int, default=1, help="Poisson precision in the serving pool")
+    ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="server memory budget (0: 90%% of free)")
     ap.add_argument("--no-viscosity", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=64)
@@ -85,11 +87,15 @@ def main() -> int:
     lap = U.LaplacianWithLevel(scene.phi, scene.weights, dt=1.0, dx=scene.dx, device=0)
     t_build = time.perf_counter() - t0
     b_host = lap.createPressurePoissonRightHandSide(scene.weights, scene.vx, scene.vy, scene.vz)
-    solver = U.PoissonSolver(lap)
-    solver.mRelativeTolerance = args.tol
-    solver.mSmoother = args.smoother
-    solver._push_options()
 
+    def make_solver(l):
+        sol = U.PoissonSolver(l)
+        sol.mRelativeTolerance = args.tol
+        sol.mSmoother = args.smoother
+        sol._push_options()
+        return sol
+
+    solver = make_solver(lap)
     b = torch.from_numpy(b_host).to(dev)
     x = torch.zeros_like(b)
     iters = None
@@ -103,27 +109,42 @@ def main() -> int:
         print(json.dumps({"kernel_only": True, "jacobi_ms": ms, "bytes_per_launch": lap.algorithmic_bytes(0, 2),
                           "copy_bytes_each_way": src.numel() * 4, "copies": 10, "iters": iters}), flush=True)
         return 0
-    for _ in range(args.warmup):
-        x.zero_()
-        rc, iters = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
-        if rc != U.SUCCESS:
-            raise RuntimeError(f"warmup solve returned {rc}")
-    torch.cuda.synchronize()
-    lat = []
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        x.zero_()
-        rc, iters = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+    def timed_solves(l, x):
+        """warm-up then timed solves from a zero guess; (ms/solve, latencies ms, iterations)"""
+        it = None
+        for _ in range(args.warmup):
+            x.zero_()
+            rc, it = l.solve_pcg_device(x.data_ptr(), b.data_ptr())
+            if rc != U.SUCCESS:
+                raise RuntimeError(f"warmup solve returned {rc}")
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - ts)
-        if rc != U.SUCCESS:
-            raise RuntimeError(f"solve returned {rc}")
-    torch.cuda.synchronize()
-    t_total = time.perf_counter() - t_start
-    ms_per_solve = 1e3 * t_total / args.steps
-    lat_ms = np.array(lat) * 1e3
-    x_gpu = x.cpu().numpy()
+        lat = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            ts = time.perf_counter()
+            x.zero_()
+            rc, it = l.solve_pcg_device(x.data_ptr(), b.data_ptr())
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - ts)
+            if rc != U.SUCCESS:
+                raise RuntimeError(f"solve returned {rc}")
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t_start) / args.steps, np.array(lat) * 1e3, it
+
+    # the plain FP32 form (the reference's arithmetic)
+    ms32, lat32, it32 = timed_solves(lap, x)
+    x32 = x.cpu().numpy()
+    # the mixed-precision form: BF16-stored W-cycle vectors, FP64 residual replacement
+    t0 = time.perf_counter()
+    lap_mx = U.LaplacianWithLevel(scene.phi, scene.weights, dt=1.0, dx=scene.dx, device=0,
+                                  precision=U.PRECISION_BF16_CYCLE)
+    t_build_mx = time.perf_counter() - t0
+    make_solver(lap_mx)
+    x_mx = torch.zeros_like(b)
+    ms_mx, lat_mx, it_mx = timed_solves(lap_mx, x_mx)
+    true_nu_mx, reps_mx = lap_mx.last_residual()
+    x_gpu = x_mx.cpu().numpy()
+    ms_per_solve, lat_ms, iters = ms_mx, lat_mx, it_mx
 
     # dominant kernel: level-0 smoother sweep (scheduled Jacobi)
     jac_bytes = lap.algorithmic_bytes(0, 2)
@@ -149,10 +170,12 @@ def main() -> int:
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "f32",
+        "storage": "bf16 W-cycle level vectors; f32 PCG vectors, f64 dot products and residual replacement",
         "data": "synthetic",
         "config": {
             "workload": f"poisson {args.n}^3 config {args.config} (30% pool + 8 seeded spheres + solid sphere "
-                        f"r=0.2), tol {args.tol:g}, FP32 plain, smoother {args.smoother}",
+                        f"r=0.2), tol {args.tol:g}, mixed precision (BF16-stored W-cycle), smoother "
+                        f"{args.smoother}",
             "n": args.n, "ndof": lap.mNumDof, "levels": lap.nlevels,
             "level_ndof": [i["ndof"] for i in info], "level_leaves": [i["nleaf_dof"] for i in info],
         },
@@ -160,8 +183,13 @@ def main() -> int:
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "pcg_iterations": iters,
+        "residual_replacements": reps_mx,
+        "fp32_plain": {"ms_per_solve": round(ms32, 3), "p50_ms": round(float(np.percentile(lat32, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat32, 99)), 3), "pcg_iterations": it32},
+        "speedup_vs_fp32_plain": round(ms32 / ms_mx, 3),
         "level_sweep_us": level_sweep_us,
         "build_s": round(t_build, 3),
+        "build_s_mixed": round(t_build_mx, 3),
         "scene_s": round(t_scene, 3),
         "roofline": roofline,
     }
@@ -175,6 +203,7 @@ def main() -> int:
         cpu_s = time.perf_counter() - t0
         st64, it64, x64 = orc.solve_pcg64(b_orc.astype(np.float64), tol=1e-12, smoother=args.smoother)
         r64 = b_orc.astype(np.float64) - orc.apply64(x_gpu.astype(np.float64))
+        r64_32 = b_orc.astype(np.float64) - orc.apply64(x32.astype(np.float64))
         out["cpu_baseline"] = {"value": round(cpu_s * 1e3, 2), "unit": "ms/solve", "cores": O.get_threads(),
                                "kind": "port",
                                "sample": f"one FP32 solve of the same {args.n}^3 system (hierarchy build not timed), "
@@ -183,6 +212,9 @@ def main() -> int:
         out["rel_l2_err_vs_fp64_oracle"] = float(np.linalg.norm(x_gpu - x64) / np.linalg.norm(x64))
         out["rel_l2_vs_cpu_fp32"] = float(np.linalg.norm(x_gpu - x_cpu) / np.linalg.norm(x_cpu))
         out["true_rel_residual_inf"] = float(np.abs(r64).max() / np.abs(b_orc).max())
+        out["fp32_plain"]["rel_l2_err_vs_fp64_oracle"] = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
+        out["fp32_plain"]["true_rel_residual_inf"] = float(np.abs(r64_32).max() / np.abs(b_orc).max())
+        out["fp32_plain"]["rel_l2_vs_cpu_fp32"] = float(np.linalg.norm(x32 - x_cpu) / np.linalg.norm(x_cpu))
         out["speedup_vs_cpu"] = round(cpu_s * 1e3 / ms_per_solve, 2)
     if not args.no_viscosity:
         out["viscosity"] = run_viscosity(args, dev)
@@ -195,23 +227,27 @@ def main() -> int:
 
 
 def run_serving(args, dev) -> dict:
-    """The serving configuration: a pool of problems (80% Poisson with edge
-    log-uniform in [48, 384], 20% viscosity with edge log-uniform in [48, 192]),
-    requests arriving as a Poisson process, each request re-solving one
-    problem warm-started from its previous solution with a right-hand side
-    perturbed by 5% (pre-generated on the device). `--serve-workers` host
-    threads serve the queue (one request per problem at a time). Reports
-    p50 / p99 latency from arrival to completion and completed solves/s."""
-    import queue
+    """The serving configuration through the C++ solve server: a pool of
+    problems (80% Poisson with edge log-uniform in [48, 384], 20% viscosity
+    with edge log-uniform in [48, 192]) registered once, requests arriving as a
+    Poisson process, each re-solving one problem warm-started from its resident
+    previous solution with a right-hand side perturbed by 5% (pre-generated on
+    the device). The server admits requests as its `--serve-workers` slots and
+    its memory budget allow; problems own their streams, so admitted solves
+    overlap on the device. Latency is the server's own submission-to-completion
+    time; p50 / p99 and completed solves/s over the run."""
     import threading
 
     import numpy as np
     import torch
 
     import uaamg_amd as U
+    from uaamg_amd.server import SolveServer
 
     rng = np.random.default_rng(2024)
     n_pool, n_visc, n_rhs = 10, 2, 4
+    budget = int(args.serve_budget_gb * (1 << 30)) if args.serve_budget_gb > 0 else 0
+    srv = SolveServer(device=0, slots=args.serve_workers, memory_budget=budget)
     t0 = time.perf_counter()
     pool = []
     for k in range(n_pool):
@@ -221,56 +257,53 @@ def run_serving(args, dev) -> dict:
         n = max(lo, min(hi, 8 * round(n / 8)))
         if visc:
             sc = U.ViscScene(3, n, seed=100 + k, mu=1.0)
-            lw = U.L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=0)
-            sol = U.simd_viscosity3d(lw, sc.velocity)
-            b0 = torch.from_numpy(sol.m_rhs.data).to(dev)
-
-            def solve(x, b, sol=sol):
-                sol.solve_pcg_device(1e-6, x.data_ptr(), b.data_ptr())
-            keep = (sc, lw, sol)
+            lw = U.L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=-1)
+            b0 = torch.from_numpy(lw.build_rhs(sc.velocity).data).to(dev)
+            pid = srv.add_viscosity(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx)
+            tol = 1e-6
         else:
             sc = U.Scene(3, n, seed=100 + k)
-            lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=0)
-            ps = U.PoissonSolver(lap)
-            ps._push_options()
+            lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=-1)
             b0 = torch.from_numpy(lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)).to(dev)
-
-            def solve(x, b, lap=lap):
-                lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
-            keep = (sc, lap, ps)
+            pid = srv.add_poisson(sc.phi, sc.weights, 1.0, sc.dx, precision=args.serve_precision)
+            tol = args.tol
         g = torch.Generator(device=dev).manual_seed(k)
         rhs = [b0 * (1 + 0.05 * torch.randn(b0.shape, device=dev, generator=g)) for _ in range(n_rhs)]
-        pool.append({"n": n, "visc": visc, "solve": solve, "x": torch.zeros_like(b0), "rhs": rhs,
-                     "lock": threading.Lock(), "keep": keep})
+        pool.append({"n": n, "visc": visc, "pid": pid, "rhs": rhs, "tol": tol, "ndof": int(b0.numel())})
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
-    # warm-up: one solve each, and the mean service time that sets the arrival rate
+    # warm-up: one solve each (residency, graph capture) and the mean service time
     svc = []
     for p in pool:
-        ts = time.perf_counter()
-        p["solve"](p["x"], p["rhs"][0])
-        svc.append(time.perf_counter() - ts)
+        st, it, qms, tms = srv.wait(srv.submit(p["pid"], p["rhs"][0].data_ptr(), tol=p["tol"]))
+        if st < 0:
+            raise RuntimeError(f"serving warm-up solve returned {st}")
+        svc.append(tms * 1e-3)
     mean_svc = float(np.mean(svc))
-    rate = 1.0 / mean_svc   # one request per mean service time: about 1/workers utilisation per worker
+    rate = 1.0 / mean_svc   # one request per mean service time
+    base = srv.stats()
 
-    q: "queue.Queue" = queue.Queue()
-    lat, lock = [], threading.Lock()
+    tickets, lock, done = [], threading.Lock(), threading.Event()
+    results = []
 
-    def worker():
+    def collector():
+        k = 0
         while True:
-            item = q.get()
-            if item is None:
-                return
-            t_arr, k, r = item
-            p = pool[k]
-            with p["lock"]:
-                p["solve"](p["x"], p["rhs"][r])
             with lock:
-                lat.append(time.perf_counter() - t_arr)
+                t = tickets[k] if k < len(tickets) else None
+            if t is None:
+                if done.is_set():
+                    with lock:
+                        if k >= len(tickets):
+                            return
+                    continue
+                time.sleep(1e-4)
+                continue
+            results.append(srv.wait(t))
+            k += 1
 
-    threads = [threading.Thread(target=worker, daemon=True) for _ in range(args.serve_workers)]
-    for t in threads:
-        t.start()
+    col = threading.Thread(target=collector, daemon=True)
+    col.start()
     t_start = time.perf_counter()
     t_next = t_start
     sent = 0
@@ -281,21 +314,32 @@ def run_serving(args, dev) -> dict:
         now = time.perf_counter()
         if t_next > now:
             time.sleep(t_next - now)
-        q.put((t_next, int(rng.integers(n_pool)), int(rng.integers(n_rhs))))
+        p = pool[int(rng.integers(n_pool))]
+        t = srv.submit(p["pid"], p["rhs"][int(rng.integers(n_rhs))].data_ptr(), tol=p["tol"])
+        with lock:
+            tickets.append(t)
         sent += 1
-    for _ in threads:
-        q.put(None)
-    for t in threads:
-        t.join()
+    done.set()
+    col.join()
     wall = time.perf_counter() - t_start
-    lat_ms = np.array(lat) * 1e3
-    return {"metric": "p50 / p99 latency (arrival to completion) and solves/s, Poisson-process arrivals",
-            "requests": sent, "completed": len(lat), "solves_per_s": round(len(lat) / wall, 2),
-            "arrival_rate_per_s": round(rate, 2), "workers": args.serve_workers,
-            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2) if len(lat) else None,
-            "p99_ms": round(float(np.percentile(lat_ms, 99)), 2) if len(lat) else None,
+    st = srv.stats()
+    srv.close()
+    lat_ms = np.array([r[3] for r in results])
+    fails = sum(1 for r in results if r[0] != 0)
+    return {"metric": "p50 / p99 latency (submission to completion) and solves/s, Poisson-process arrivals",
+            "engine": "C++ solve server (uaamg_server_*), continuous admission",
+            "requests": sent, "completed": len(results), "failed": fails,
+            "solves_per_s": round(len(results) / wall, 2),
+            "arrival_rate_per_s": round(rate, 2), "slots": args.serve_workers,
+            "p50_ms": round(float(np.percentile(lat_ms, 50)), 2) if len(results) else None,
+            "p99_ms": round(float(np.percentile(lat_ms, 99)), 2) if len(results) else None,
+            "mean_queue_ms": round(float(np.mean([r[2] for r in results])), 3) if len(results) else None,
             "mean_service_ms": round(1e3 * mean_svc, 2),
-            "pool": [{"n": p["n"], "kind": "viscosity" if p["visc"] else "poisson"} for p in pool],
+            "peak_inflight": st["peak_inflight"], "evictions": st["evictions"] - base["evictions"],
+            "resident_bytes": st["resident_bytes"], "memory_budget_bytes": budget or "90% of free",
+            "poisson_precision": "bf16 W-cycle" if args.serve_precision == 1 else "fp32",
+            "pool": [{"n": p["n"], "kind": "viscosity" if p["visc"] else "poisson", "ndof": p["ndof"]}
+                     for p in pool],
             "build_s": round(t_build, 2)}
 
 

@@ -386,6 +386,59 @@ float uaamg_volume_fraction8(const float* phi8);
 /* Number of HIP devices visible (0 when none; never fails). */
 int uaamg_device_count(void);
 
+/* ================================================================ solve server
+ * A request loop over many resident problems on one GPU (no reference counterpart: FLIP_vdb
+ * builds a solver per time step and calls it synchronously, FLIP_vdb.cpp:3126-3208). Problems
+ * are registered once from the same inputs as uaamg_poisson_create / uaamg_viscosity_create;
+ * the server keeps their host hierarchies and makes a problem resident in device memory
+ * (hierarchy, PCG vectors, warm-start solution) when one of its requests is admitted. Each of
+ * `slots` worker threads takes the oldest queued request whose problem is idle and whose
+ * footprint (uaamg_server_problem_info's device_bytes, summed over the levels from their
+ * active-leaf counts) fits the memory budget, evicting idle resident problems least recently
+ * used first; an evicted problem's solution is kept on the host. Problems own their streams,
+ * so admitted solves overlap on the device; the requests of one problem run in submission order.
+ */
+typedef struct uaamg_server uaamg_server;
+
+typedef struct uaamg_server_options {
+    int32_t device;              /* HIP device ordinal */
+    int32_t slots;               /* worker threads = solves in flight at most, 1..16 (default 4) */
+    int64_t memory_budget_bytes; /* device bytes for resident problems; 0 = 90% of free memory */
+} uaamg_server_options;
+
+typedef struct uaamg_server_counters {
+    int64_t completed, failed, evictions, acquisitions, resident_bytes, peak_inflight, queued;
+} uaamg_server_counters;
+
+int uaamg_server_options_default(uaamg_server_options* opts);
+int uaamg_server_create(const uaamg_server_options* opts, uaamg_server** out);
+/* Finishes every queued request, then frees every problem. */
+void uaamg_server_destroy(uaamg_server* s);
+
+/* Registers a problem (the hierarchy is built on the host here); *problem receives its id. The
+ * options' stream and device fields are replaced by the problem's own stream on the server's
+ * device. */
+int uaamg_server_add_poisson(uaamg_server* s, const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt,
+                             float dx, const uaamg_options* opts, int64_t* problem);
+int uaamg_server_add_viscosity(uaamg_server* s, const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf,
+                               const uaamg_grid* solid_sdf, float dt, float rho, float dx,
+                               const uaamg_viscosity_options* opts, int64_t* problem);
+int uaamg_server_problem_info(uaamg_server* s, int64_t problem, int64_t* ndof, int64_t* device_bytes,
+                              int32_t* resident);
+
+/* Queues one solve to `tolerance` (relative, as the problem's solver defines it). b_dev: the
+ * right-hand side (device, DOF order); x_out_dev: where the solution is copied (device, NULL
+ * for none); x0_dev: the initial guess (device; NULL = the problem's previous solution, zero
+ * before its first solve). The caller keeps the three buffers alive until uaamg_server_wait
+ * returns for the ticket. UAAMG_ENOMEM when the problem can never fit the budget. */
+int uaamg_server_submit(uaamg_server* s, int64_t problem, const float* b_dev, float* x_out_dev, const float* x0_dev,
+                        float tolerance, int64_t* ticket);
+/* Blocks until the ticket's solve completes. *status receives the solver's status (UAAMG_SUCCESS,
+ * UAAMG_FAILED or a negative error), the times are in ms from submission (queueing, and total). */
+int uaamg_server_wait(uaamg_server* s, int64_t ticket, int32_t* status, int32_t* iterations, double* queue_ms,
+                      double* total_ms);
+int uaamg_server_stats(uaamg_server* s, uaamg_server_counters* out);
+
 #ifdef __cplusplus
 }
 #endif

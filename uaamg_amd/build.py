@@ -30,6 +30,8 @@ SOURCES = [
     "viscosity_kernels.hip",
     "viscosity_solver.cpp",
     "capi_viscosity.cpp",
+    "server.cpp",
+    "capi_server.cpp",
 ]
 
 ARCH = os.environ.get("UAAMG_OFFLOAD_ARCH", "gfx950")

@@ -110,11 +110,12 @@ int uaamg_options_default(uaamg_options* o) {
     return 0;
 }
 
-int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
-                         const uaamg_options* opts, uaamg_poisson** out) {
-    UAAMG_GUARD_BEGIN
-    if (!out) return fail(UAAMG_EINVAL, "null output handle");
-    *out = nullptr;
+}  // extern "C"
+
+namespace uaamg {
+namespace capi {
+int prepare_poisson(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
+                    const uaamg_options* opts, PoissonHierarchyHost& hier, SolverOptions& so) {
     if (!(dx > 0.f) || !(dt > 0.f)) return fail(UAAMG_EINVAL, "dt and dx must be positive");
     uaamg_options o;
     uaamg_options_default(&o);
@@ -128,11 +129,9 @@ int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_we
     HostGrid phi, w;
     if (!to_host_grid(liquid_phi, 1, phi, "liquid_phi")) return UAAMG_EINVAL;
     if (!to_host_grid(face_weights, 3, w, "face_weights")) return UAAMG_EINVAL;
-    PoissonHierarchyHost hier;
     build_hierarchy(phi, w, dt, dx, o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000, hier);
     if (hier.levels[0].ndof == 0) return fail(UAAMG_EINVAL, "no degrees of freedom");
-    std::unique_ptr<uaamg_poisson> h(new uaamg_poisson());
-    SolverOptions so;
+    so = SolverOptions{};
     so.relative_tolerance = o.relative_tolerance;
     so.max_iterations = o.max_iterations;
     so.smoother = o.smoother;
@@ -141,6 +140,23 @@ int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_we
     so.max_coarsest_dof = o.max_coarsest_dof;
     so.flags = o.flags;
     so.stream = static_cast<hipStream_t>(o.stream);
+    return 0;
+}
+}  // namespace capi
+}  // namespace uaamg
+
+extern "C" {
+
+int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
+                         const uaamg_options* opts, uaamg_poisson** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out) return fail(UAAMG_EINVAL, "null output handle");
+    *out = nullptr;
+    PoissonHierarchyHost hier;
+    SolverOptions so;
+    if (int rc = capi::prepare_poisson(liquid_phi, face_weights, dt, dx, opts, hier, so)) return rc;
+    const bool host_only = so.device < 0;
+    std::unique_ptr<uaamg_poisson> h(new uaamg_poisson());
     if (host_only) {
         h->dev.init_host_only(std::move(hier), so);
         h->ndof = h->dev.host().levels[0].ndof;

@@ -7,6 +7,8 @@
 
 #include "../../include/uaamg_capi.h"
 #include "leaf_topology.h"
+#include "poisson_solver.h"
+#include "viscosity_solver.h"
 
 namespace uaamg {
 namespace capi {
@@ -14,6 +16,13 @@ namespace capi {
 int fail(int code, const std::string& msg);
 // validated tree-ordered copy of a caller grid with ncomp components
 bool to_host_grid(const uaamg_grid* g, int ncomp, HostGrid& out, const char* what);
+// validated options and the host hierarchy of a problem (shared by the
+// create entry points and the server); 0 or a uaamg status with the error set
+int prepare_poisson(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
+                    const uaamg_options* opts, PoissonHierarchyHost& hier, SolverOptions& so);
+int prepare_viscosity(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf,
+                      float dt, float rho, float dx, const uaamg_viscosity_options* opts, ViscHierarchyHost& hh,
+                      ViscOptions& vo);
 }  // namespace capi
 }  // namespace uaamg
 

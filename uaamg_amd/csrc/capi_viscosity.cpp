@@ -66,39 +66,57 @@ int uaamg_viscosity_options_default(uaamg_viscosity_options* o) {
     return 0;
 }
 
-int uaamg_viscosity_create(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf,
-                           float dt, float rho, float dx, const uaamg_viscosity_options* opts,
-                           uaamg_viscosity** out) {
-    UAAMG_GUARD_BEGIN
-    if (!out) return fail(UAAMG_EINVAL, "null output handle");
-    *out = nullptr;
+}  // extern "C"
+
+namespace uaamg {
+namespace capi {
+int prepare_viscosity(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf,
+                      float dt, float rho, float dx, const uaamg_viscosity_options* opts, ViscHierarchyHost& hh,
+                      ViscOptions& vo) {
     if (!(dx > 0.f) || !(dt > 0.f) || !(rho > 0.f)) return fail(UAAMG_EINVAL, "dt, rho and dx must be positive");
     uaamg_viscosity_options o;
     uaamg_viscosity_options_default(&o);
     if (opts) o = *opts;
     if (o.max_coarsest_dof < 1 || o.max_coarsest_dof > 4096)
         return fail(UAAMG_EINVAL, "max_coarsest_dof must lie in [1, 4096]");
+    if (o.device >= 0) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device)
+            return fail(UAAMG_ENODEV, "no HIP device with that ordinal");
+    }
     HostGrid mu, liq, sol;
     if (!to_host_grid(viscosity, 1, mu, "viscosity")) return UAAMG_EINVAL;
     if (!to_host_grid(liquid_sdf, 1, liq, "liquid_sdf")) return UAAMG_EINVAL;
     if (!to_host_grid(solid_sdf, 1, sol, "solid_sdf")) return UAAMG_EINVAL;
-    ViscHierarchyHost hh;
     if (!build_visc_hierarchy(liq, sol, mu, dt, rho, dx, o.max_coarsest_dof, !(o.flags & 8), hh))
         return fail(UAAMG_EINVAL, "hierarchy construction failed");
-    std::unique_ptr<uaamg_viscosity> h(new uaamg_viscosity());
-    ViscOptions vo;
+    vo = ViscOptions{};
     vo.max_iterations = o.max_iterations;
     vo.device = o.device;
     vo.max_coarsest_dof = o.max_coarsest_dof;
     vo.flags = o.flags;
     vo.stream = static_cast<hipStream_t>(o.stream);
+    return 0;
+}
+}  // namespace capi
+}  // namespace uaamg
+
+extern "C" {
+
+int uaamg_viscosity_create(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf,
+                           float dt, float rho, float dx, const uaamg_viscosity_options* opts,
+                           uaamg_viscosity** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out) return fail(UAAMG_EINVAL, "null output handle");
+    *out = nullptr;
+    ViscHierarchyHost hh;
+    ViscOptions vo;
+    if (int rc = capi::prepare_viscosity(viscosity, liquid_sdf, solid_sdf, dt, rho, dx, opts, hh, vo)) return rc;
+    std::unique_ptr<uaamg_viscosity> h(new uaamg_viscosity());
     h->ndof = hh.levels[0].ndof;
-    if (o.device < 0) {
+    if (vo.device < 0) {
         h->dev.init_host_only(std::move(hh), vo);
     } else {
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device)
-            return fail(UAAMG_ENODEV, "no HIP device with that ordinal");
         int rc = h->dev.init(std::move(hh), vo);
         if (rc) return dev_status(rc, h->dev);
         for (auto& p : h->dofbuf)

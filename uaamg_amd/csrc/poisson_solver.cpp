@@ -37,8 +37,12 @@ float scheduled_weight_iterative(int n, int i) {
 constexpr int kSweeps = 3;   // n = 3 pre/post sweeps (solveMultigridPCG :2181,2212)
 }  // namespace
 
-DeviceArena::~DeviceArena() {
+DeviceArena::~DeviceArena() { release(); }
+
+void DeviceArena::release() {
     if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    cap_ = used_ = 0;
 }
 
 hipError_t DeviceArena::reserve(size_t bytes) {
@@ -61,12 +65,28 @@ void* DeviceArena::take(size_t bytes) {
     return p;
 }
 
-PoissonDevice::~PoissonDevice() {
+PoissonDevice::~PoissonDevice() { release_device(); }
+
+void PoissonDevice::release_device() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     destroy_graphs();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    own_stream_ = false;
     if (sc_host_) (void)hipHostFree(sc_host_);
     if (mailbox_) (void)hipHostFree(mailbox_);
+    sc_host_ = nullptr;
+    mailbox_ = nullptr;
+    mailbox_dev_ = nullptr;
+    nu_seq_ = 0;
+    lv_.clear();
+    x0_ = b0_ = r0_ = z0_ = q0_ = nullptr;
+    p_[0] = p_[1] = nullptr;
+    for (auto& s : staging_) s = nullptr;
+    sc_ = nullptr;
+    partials_ = nullptr;
+    cv_ = CoarsestView{};
+    arena_.release();
 }
 
 void PoissonDevice::destroy_graphs() {
@@ -182,18 +202,14 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         err_ = "empty hierarchy";
         return -1;
     }
-    if (check(hipSetDevice(opt_.device), "hipSetDevice")) return -3;
-    if (opt_.stream) {
-        stream_ = opt_.stream;
-    } else {
-        if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return -3;
-        own_stream_ = true;
-    }
     build_spai0(host_);
-    const int L = int(host_.levels.size());
-    sor_w_.assign(L, 1.2f);
+    return acquire_device();
+}
 
-    // size the arena
+// Device memory of the hierarchy, the PCG vectors and the scratch: one arena.
+size_t PoissonDevice::device_bytes() const {
+    const int L = int(host_.levels.size());
+    if (L == 0) return 0;
     size_t bytes = 0;
     auto add = [&](size_t b) { bytes += align_up(b); };
     const size_t n0 = host_.levels[0].nleaf();
@@ -221,7 +237,28 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
     add((size_t(reduction_blocks(LevelView{int32_t(n0)})) + 16) * sizeof(double));
     const auto& hc = host_.levels.back();
     add(size_t(hc.ndof) * (sizeof(int32_t) * 7 + sizeof(float) * 8) + 1024);
-    if (check(arena_.reserve(bytes), "hipMalloc")) return -2;
+    return bytes;
+}
+
+int PoissonDevice::acquire_device() {
+    if (on_device()) return 0;
+    if (host_.levels.empty()) {
+        err_ = "empty hierarchy";
+        return -1;
+    }
+    first_error_ = hipSuccess;
+    err_.clear();
+    if (check(hipSetDevice(opt_.device), "hipSetDevice")) return -3;
+    if (opt_.stream) {
+        stream_ = opt_.stream;
+    } else {
+        if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return -3;
+        own_stream_ = true;
+    }
+    const int L = int(host_.levels.size());
+    sor_w_.assign(L, 1.2f);
+    const size_t n0 = host_.levels[0].nleaf();
+    if (check(arena_.reserve(device_bytes()), "hipMalloc")) return -2;
 
     lv_.resize(L);
     for (int l = 0; l < L; ++l)

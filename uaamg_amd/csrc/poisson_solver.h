@@ -33,6 +33,7 @@ class DeviceArena {
 public:
     ~DeviceArena();
     hipError_t reserve(size_t bytes);
+    void release();
     void* take(size_t bytes);
     size_t used() const { return used_; }
     size_t capacity() const { return cap_; }
@@ -54,6 +55,12 @@ public:
     // keeps the host hierarchy only (inspection without a GPU); solves fail
     void init_host_only(PoissonHierarchyHost&& h, const SolverOptions& opt);
     bool on_device() const { return !lv_.empty(); }
+    // device residency: the bytes acquire_device() reserves (one arena, sized
+    // from the levels' leaf counts), uploading the kept host hierarchy, and
+    // freeing every device resource while keeping the host hierarchy
+    size_t device_bytes() const;
+    int acquire_device();
+    void release_device();
 
     int nlevels() const { return int(host_.levels.size()); }
     const PoissonHierarchyHost& host() const { return host_; }

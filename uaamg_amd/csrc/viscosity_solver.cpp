@@ -14,12 +14,27 @@ constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 }  // namespace
 
-ViscDevice::~ViscDevice() {
+ViscDevice::~ViscDevice() { release_device(); }
+
+void ViscDevice::release_device() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     destroy_graphs();
     for (float* p : spai0_bufs_) (void)hipFree(p);
+    spai0_bufs_.clear();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    own_stream_ = false;
     if (mailbox_) (void)hipHostFree(mailbox_);
+    mailbox_ = nullptr;
+    mailbox_dev_ = nullptr;
+    nu_seq_ = 0;
+    lv_.clear();
+    x0_ = b0_ = r0_ = p0_ = z0_ = nullptr;
+    for (auto& s : staging_) s = nullptr;
+    sc_ = nullptr;
+    partials_ = nullptr;
+    cv_ = ViscCoarsestView{};
+    arena_.release();
 }
 
 int ViscDevice::check(hipError_t e, const char* what) {
@@ -127,19 +142,12 @@ int ViscDevice::init(ViscHierarchyHost&& h, const ViscOptions& opt) {
         err_ = "empty hierarchy";
         return -1;
     }
-    if (check(hipSetDevice(opt_.device), "hipSetDevice")) return -3;
-    if (opt_.stream) stream_ = opt_.stream;
-    else {
-        if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return -3;
-        own_stream_ = true;
-    }
-    use_graphs_ = !(opt_.flags & 2);
-    // on request (flags bit 5 or UAAMG_VSMALL_LEAVES), coarse levels whose
-    // widest channel has at most this many leaves run each half-visit in one
-    // single-workgroup launch
-    small_max_leaves_ = (opt_.flags & 32) ? 16 : 0;
-    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) small_max_leaves_ = std::atoi(env);
-    const int L = int(host_.levels.size());
+    return acquire_device();
+}
+
+// Device memory of the hierarchy, the PCG vectors and the coarsest system: one arena.
+size_t ViscDevice::device_bytes() const {
+    if (host_.levels.empty()) return 0;
     size_t bytes = 1 << 20;
     for (const auto& l : host_.levels) {
         bytes += align_up(l.nb27.size() * 4) + align_up(l.mask.size() * 8) + align_up(l.slot.size() * 4) +
@@ -154,7 +162,34 @@ int ViscDevice::init(ViscHierarchyHost&& h, const ViscOptions& opt) {
     bytes += align_up(host_.cptr.size() * 8) + align_up(host_.ccol.size() * 4) + 3 * align_up(host_.cval.size() * 4) +
              8 * kAlign;
     bytes += align_up(sizeof(PcgScalars)) + align_up((l0.nleaf_dof * 3 / kWavesPerBlock + 2) * 8);
-    if (hipError_t e = arena_.reserve(bytes)) {
+    return bytes;
+}
+
+int ViscDevice::acquire_device() {
+    if (on_device()) return 0;
+    if (host_.levels.empty()) {
+        err_ = "empty hierarchy";
+        return -1;
+    }
+    first_error_ = hipSuccess;
+    err_.clear();
+    if (check(hipSetDevice(opt_.device), "hipSetDevice")) return -3;
+    if (opt_.stream) stream_ = opt_.stream;
+    else {
+        if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return -3;
+        own_stream_ = true;
+    }
+    use_graphs_ = !(opt_.flags & 2);
+    // on request (flags bit 5 or UAAMG_VSMALL_LEAVES), coarse levels whose
+    // widest channel has at most this many leaves run each half-visit in one
+    // single-workgroup launch
+    small_max_leaves_ = (opt_.flags & 32) ? 16 : 0;
+    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) small_max_leaves_ = std::atoi(env);
+    const int L = int(host_.levels.size());
+    const ViscLevelHost& l0 = host_.levels[0];
+    const size_t vec0 = l0.nleaf_dof * 3 * 512 * 4;
+    const size_t ndof0 = size_t(l0.ndof);
+    if (hipError_t e = arena_.reserve(device_bytes())) {
         err_ = std::string("device allocation: ") + hipGetErrorString(e);
         return e == hipErrorOutOfMemory ? -2 : -3;
     }

@@ -32,6 +32,11 @@ public:
     int init(ViscHierarchyHost&& h, const ViscOptions& opt);
     void init_host_only(ViscHierarchyHost&& h, const ViscOptions& opt);
     bool on_device() const { return !lv_.empty(); }
+    // device residency (see PoissonDevice): arena size from the levels' leaf
+    // counts, upload of the kept host hierarchy, release of every device resource
+    size_t device_bytes() const;
+    int acquire_device();
+    void release_device();
 
     int nlevels() const { return int(host_.levels.size()); }
     const ViscHierarchyHost& host() const { return host_; }
