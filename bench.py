@@ -346,17 +346,19 @@ def run_serving(args, dev) -> dict:
 def run_batch(args, dev) -> dict:
     """The batch configuration: 64 independent Poisson problems (edge
     log-uniform in [32, 192], 1-6 spheres over a 10-50% pool, solid sphere
-    with probability 1/2; seeds 0..63), tol 1e-7, FP32, b resident in HBM.
-    Reports solves/s for the problems run one after another and for the
-    batch entry point (one stream per problem, `--batch-concurrency` host
-    threads driving them)."""
+    with probability 1/2; seeds 0..63), tol 1e-7, b resident in HBM. Reports
+    solves/s for: the problems one after another (one handle each), the
+    stream-per-problem entry point (`--batch-concurrency` host threads), and
+    the merged batch (one leaf table per level, one launch per W-cycle stage
+    for all problems) in FP32 and with the BF16 W-cycle. The merged FP32
+    results must equal the sequential ones bit for bit."""
     import numpy as np
     import torch
 
     import uaamg_amd as U
 
     t0 = time.perf_counter()
-    laps, xs, bs, ndof = [], [], [], 0
+    scenes, laps, xs, bs, ndof = [], [], [], [], 0
     for seed in range(args.batch_size):
         sc = U.Scene(3, 0, seed=seed)
         lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=0)
@@ -364,11 +366,16 @@ def run_batch(args, dev) -> dict:
         s.mRelativeTolerance = 1e-7
         s._push_options()
         b = torch.from_numpy(lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)).to(dev)
+        scenes.append(sc)
         laps.append(lap)
         bs.append(b)
         xs.append(torch.zeros_like(b))
         ndof += lap.mNumDof
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    merged = U.PoissonBatch([(sc.phi, sc.weights, 1.0, sc.dx) for sc in scenes], precision=0)
+    t_build_merged = time.perf_counter() - t0
+    merged_mx = U.PoissonBatch([(sc.phi, sc.weights, 1.0, sc.dx) for sc in scenes], precision=1)
 
     def sequential():
         its = []
@@ -379,7 +386,7 @@ def run_batch(args, dev) -> dict:
         torch.cuda.synchronize()
         return its
 
-    def batched():
+    def streams():
         for x in xs:
             x.zero_()
         torch.cuda.synchronize()
@@ -388,26 +395,51 @@ def run_batch(args, dev) -> dict:
         torch.cuda.synchronize()
         return its
 
-    sequential()
-    batched()
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        its_seq = sequential()
-    t_seq = (time.perf_counter() - t0) / reps
+    def merged_solve(bt):
+        def f():
+            for x in xs:
+                x.zero_()
+            torch.cuda.synchronize()
+            st, its = bt.solve_device([x.data_ptr() for x in xs], [b.data_ptr() for b in bs])
+            torch.cuda.synchronize()
+            if any(v != U.SUCCESS for v in st):
+                raise RuntimeError(f"merged batch statuses {st}")
+            return its
+        return f
+
+    def timed(fn, reps=3):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            its = fn()
+        return (time.perf_counter() - t) / reps, its
+
+    t_seq, its_seq = timed(sequential)
     x_seq = [x.cpu().numpy() for x in xs]
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        its_bat = batched()
-    t_bat = (time.perf_counter() - t0) / reps
-    same = all(np.array_equal(a, x.cpu().numpy()) for a, x in zip(x_seq, xs)) and its_seq == its_bat
+    t_str, its_str = timed(streams)
+    same_str = all(np.array_equal(a, x.cpu().numpy()) for a, x in zip(x_seq, xs)) and its_seq == its_str
+    t_mrg, its_mrg = timed(merged_solve(merged))
+    same_mrg = all(np.array_equal(a, x.cpu().numpy()) for a, x in zip(x_seq, xs)) and its_seq == its_mrg
+    t_mx, its_mx = timed(merged_solve(merged_mx))
+    x_mx = [x.cpu().numpy() for x in xs]
+    dev_mx = max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)) for a, b in zip(x_mx, x_seq))
     n = args.batch_size
+    info = merged.info()
     return {"metric": f"solves/s over {n} independent Poisson problems (edge log-uniform [32,192]), tol 1e-7",
-            "value": round(n / t_bat, 1), "unit": "solves/s", "problems": n, "total_ndof": ndof,
-            "ms_per_batch": round(1e3 * t_bat, 2), "sequential_ms_per_batch": round(1e3 * t_seq, 2),
-            "sequential_solves_per_s": round(n / t_seq, 1), "concurrency": args.batch_concurrency,
-            "mean_iterations": round(float(np.mean(its_bat)), 2), "identical_to_sequential": bool(same),
-            "build_s": round(t_build, 2)}
+            "value": round(n / t_mx, 1), "unit": "solves/s", "form": "merged batch, BF16 W-cycle",
+            "problems": n, "total_ndof": ndof,
+            "ms_per_batch": round(1e3 * t_mx, 2),
+            "merged_fp32_ms_per_batch": round(1e3 * t_mrg, 2), "merged_fp32_solves_per_s": round(n / t_mrg, 1),
+            "streams_ms_per_batch": round(1e3 * t_str, 2), "streams_solves_per_s": round(n / t_str, 1),
+            "sequential_ms_per_batch": round(1e3 * t_seq, 2), "sequential_solves_per_s": round(n / t_seq, 1),
+            "concurrency": args.batch_concurrency, "merged_levels": info["nlevels"],
+            "merged_leaves0": info["leaves0"], "merged_device_bytes": info["device_bytes"],
+            "mean_iterations": round(float(np.mean(its_mrg)), 2),
+            "mean_iterations_bf16": round(float(np.mean(its_mx)), 2),
+            "merged_fp32_identical_to_sequential": bool(same_mrg),
+            "streams_identical_to_sequential": bool(same_str),
+            "bf16_max_rel_l2_vs_fp32": dev_mx,
+            "build_s": round(t_build, 2), "merge_build_s": round(t_build_merged, 2)}
 
 
 def run_viscosity(args, dev) -> dict:

@@ -184,6 +184,35 @@ int uaamg_poisson_solve_batch_device(uaamg_poisson* const* handles, int32_t n, f
                                      const float* const* b_dev, int32_t* status, int32_t* iterations,
                                      int32_t concurrency);
 
+/*
+ * The merged batch (batch configuration, one launch per W-cycle stage): n independent problems whose
+ * level-l leaves are merged into one leaf table per level (deepest hierarchies first, per-problem
+ * coefficient defaults, level-0 ranges aligned to the reduction block), solved together: every
+ * smoothing sweep, transfer, coarsest group and PCG vector operation is one launch for the whole
+ * batch, one hipGraph covers one PCG iteration of all problems, and each problem keeps its own PCG
+ * scalars and convergence state on the device. Each problem's iterates equal a solve of it alone
+ * (same kernels, same reduction partition and order). opts applies to every problem (Jacobi-family
+ * smoothers; precision FP32 or BF16 W-cycle; flags bit 1 = no graph replay, bit 4 = no block-visit
+ * kernels). No reference counterpart (FLIP solves one problem per call, FLIP_vdb.cpp:3132-3157).
+ */
+typedef struct uaamg_poisson_batch uaamg_poisson_batch;
+int uaamg_poisson_batch_create(int32_t n, const uaamg_grid* const* liquid_phi, const uaamg_grid* const* face_weights,
+                               const float* dt, const float* dx, const uaamg_options* opts,
+                               uaamg_poisson_batch** out);
+void uaamg_poisson_batch_destroy(uaamg_poisson_batch* h);
+/* mRelativeTolerance / mMaxIteration for the next solves (smoother and precision are fixed). */
+int uaamg_poisson_batch_set_options(uaamg_poisson_batch* h, const uaamg_options* opts);
+/* problems, merged levels, merged level-0 leaves (with alignment padding), device bytes */
+int uaamg_poisson_batch_info(const uaamg_poisson_batch* h, int32_t* nproblems, int32_t* nlevels, int64_t* leaves0,
+                             int64_t* device_bytes);
+/* mNumDof of problem i (-1 for a bad index) */
+int64_t uaamg_poisson_batch_ndof(const uaamg_poisson_batch* h, int32_t i);
+/* Solves every problem: x_dev[i] / b_dev[i] are DOF-ordered device vectors of problem i (x: initial
+ * guess in, solution out); status[i] = UAAMG_SUCCESS / UAAMG_FAILED, iterations[i] its
+ * mIterationTaken (either may be NULL). Returns 0 or a negative error. */
+int uaamg_poisson_batch_solve_device(uaamg_poisson_batch* h, float* const* x_dev, const float* const* b_dev,
+                                     int32_t* status, int32_t* iterations);
+
 /* PoissonSolver::solvePureMultigrid (simd_vdb_poisson.h:163, .cpp:2228-2267), host vectors. */
 int uaamg_poisson_solve_mg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations);
 

@@ -1,0 +1,124 @@
+"""GPU parity of the merged batch (uaamg_poisson_batch_*): every problem's
+result, status and iteration count are bit-identical to solving it alone with
+its own handle, for a batch of uneven problems (one-, two- and three-level
+hierarchies, given out of depth order), in FP32 and in the BF16 W-cycle form,
+with and without graph replay; per-problem failure (iteration limit, a NaN
+right-hand side) and a zero right-hand side stay confined to their problem.
+"""
+import numpy as np
+import pytest
+
+import uaamg_amd as U
+
+pytestmark = pytest.mark.gpu
+
+# (config, n, seed): hierarchies of 2, 1, 3, 2, 3, 1, 2 levels
+PROBLEMS = [(3, 24, 1), (3, 16, 0), (1, 48, 1), (3, 40, 3), (3, 64, 7), (3, 16, 3), (0, 32, 0)]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def batch_inputs(hip_available):
+    import torch
+    dev = torch.device("cuda:0")
+    scenes, rhs = [], []
+    for cfg, n, seed in PROBLEMS:
+        _say(f"[setup] batch problem cfg={cfg} n={n} seed={seed}")
+        s = U.Scene(cfg, n, seed=seed)
+        lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, device=-1)
+        scenes.append(s)
+        rhs.append(torch.from_numpy(lap.createPressurePoissonRightHandSide(s.weights, s.vx, s.vy, s.vz)).to(dev))
+    return scenes, rhs
+
+
+def _alone(scenes, rhs, precision=0, flags=0, max_iterations=100):
+    import torch
+    out = []
+    for s, b in zip(scenes, rhs):
+        lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, precision=precision, flags=flags)
+        solver = U.PoissonSolver(lap)
+        solver.mMaxIteration = max_iterations
+        solver._push_options()
+        x = torch.zeros_like(b)
+        rc, it = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+        torch.cuda.synchronize()
+        out.append((rc, it, x.cpu().numpy()))
+    return out
+
+
+def _batched(scenes, rhs, precision=0, flags=0, max_iterations=100, x0=None):
+    import torch
+    bt = U.PoissonBatch([(s.phi, s.weights, 1.0, s.dx) for s in scenes], precision=precision, flags=flags,
+                        max_iterations=max_iterations)
+    xs = [torch.zeros_like(b) if x0 is None else x0[i].clone() for i, b in enumerate(rhs)]
+    st, it = bt.solve_device([x.data_ptr() for x in xs], [b.data_ptr() for b in rhs])
+    torch.cuda.synchronize()
+    return bt, [(st[i], it[i], xs[i].cpu().numpy()) for i in range(len(rhs))]
+
+
+@pytest.mark.parametrize("precision,flags", [(0, 0), (0, 2), (1, 16), (1, 18)])
+def test_batch_equals_each_problem_alone(batch_inputs, precision, flags):
+    """flags 2: no graph replay; 16: per-sweep kernels on the coarse levels (the
+    BF16 form's block-visit kernels round differently from its per-sweep ones,
+    so the BF16 comparison pins one kernel mapping on both sides)."""
+    scenes, rhs = batch_inputs
+    alone = _alone(scenes, rhs, precision, flags)
+    bt, batched = _batched(scenes, rhs, precision, flags)
+    info = bt.info()
+    _say(f"  precision {precision} flags {flags}: merged levels {info['nlevels']}, leaves0 {info['leaves0']}, "
+         f"iterations {[r[1] for r in batched]}")
+    assert info["nproblems"] == len(scenes) and info["nlevels"] == 3
+    for i, ((s1, i1, x1), (s2, i2, x2)) in enumerate(zip(alone, batched)):
+        assert bt.ndof(i) == x1.size
+        assert s1 == s2 == U.SUCCESS, f"problem {i}"
+        assert i1 == i2, f"problem {i}: {i1} vs {i2}"
+        assert np.array_equal(x1, x2), f"problem {i}"
+
+
+def test_batch_iteration_limit(batch_inputs):
+    scenes, rhs = batch_inputs
+    alone = _alone(scenes, rhs, max_iterations=2)
+    bt, batched = _batched(scenes, rhs, max_iterations=2)
+    for (s1, i1, x1), (s2, i2, x2) in zip(alone, batched):
+        assert s1 == s2 and i1 == i2 and np.array_equal(x1, x2)
+
+
+def test_batch_confines_failure_and_trivial_problems(batch_inputs):
+    """problem 3 gets a NaN in its right-hand side (it fails at the iteration
+    limit), problem 6 a zero right-hand side (converged before iterating); the
+    others are unaffected."""
+    import torch
+    scenes, rhs = batch_inputs
+    bad = [b.clone() for b in rhs]
+    bad[3][bad[3].numel() // 2] = float("nan")
+    bad[6].zero_()
+    bt, batched = _batched(scenes, bad, max_iterations=8)
+    clean = _alone(scenes, rhs, max_iterations=8)
+    assert batched[3][0] == U.FAILED
+    assert batched[6][0] == U.SUCCESS and batched[6][1] == 0 and not batched[6][2].any()
+    for i in (0, 1, 2, 4, 5):
+        assert batched[i][0] == clean[i][0] and batched[i][1] == clean[i][1]
+        assert np.array_equal(batched[i][2], clean[i][2]), f"problem {i}"
+    torch.cuda.synchronize()
+
+
+def test_batch_warm_start_and_repeat(batch_inputs):
+    """A second solve from the first's solutions (the serving warm start) and a
+    repeat solve on the same handle give the per-problem results again."""
+    import torch
+    scenes, rhs = batch_inputs
+    bt, first = _batched(scenes, rhs)
+    x0 = [torch.from_numpy(r[2]).to(rhs[0].device) for r in first]
+    xs = [x.clone() for x in x0]
+    st, it = bt.solve_device([x.data_ptr() for x in xs], [b.data_ptr() for b in rhs])
+    torch.cuda.synchronize()
+    for i, (s, b) in enumerate(zip(scenes, rhs)):
+        lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx)
+        x = x0[i].clone()
+        rc, it1 = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+        torch.cuda.synchronize()
+        assert st[i] == rc and it[i] == it1
+        assert torch.equal(xs[i], x), f"problem {i}"

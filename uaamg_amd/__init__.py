@@ -108,6 +108,12 @@ def lib() -> C.CDLL:
         "uaamg_poisson_last_residual": (C.c_int, [vp, P(f32), P(i32)]),
         "uaamg_poisson_solve_mg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_batch_device": (C.c_int, [vp, i32, vp, vp, vp, vp, i32]),
+        "uaamg_poisson_batch_create": (C.c_int, [i32, vp, vp, vp, vp, P(_Options), P(vp)]),
+        "uaamg_poisson_batch_destroy": (None, [vp]),
+        "uaamg_poisson_batch_set_options": (C.c_int, [vp, P(_Options)]),
+        "uaamg_poisson_batch_info": (C.c_int, [vp, P(i32), P(i32), P(i64), P(i64)]),
+        "uaamg_poisson_batch_ndof": (i64, [vp, i32]),
+        "uaamg_poisson_batch_solve_device": (C.c_int, [vp, vp, vp, vp, vp]),
         "uaamg_viscosity_solve_batch_device": (C.c_int, [vp, i32, f32, vp, vp, vp, i32]),
         "uaamg_poisson_apply": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp, vp]),
         "uaamg_poisson_first_sweep": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp]),
@@ -259,6 +265,62 @@ def solve_batch_device(laplacians, x_ptrs, b_ptrs, concurrency: int = 4):
     _check(lib().uaamg_poisson_solve_batch_device(hs, n, xs, bs, _ptr(st), _ptr(it), int(concurrency)),
            "solve_batch_device")
     return st.tolist(), it.tolist()
+
+
+class PoissonBatch:
+    """Many Poisson problems merged into one leaf table per level and solved
+    together (uaamg_poisson_batch_*): one launch per W-cycle stage for the
+    whole batch, per-problem PCG scalars and convergence on the device.
+    `problems` is a list of (liquid_phi, face_weights, dt, dx)."""
+
+    def __init__(self, problems, *, precision: int = 0, flags: int = 0, smoother: int = 0,
+                 tolerance: float = 1e-7, max_iterations: int = 100, device: int = 0):
+        L = lib()
+        n = len(problems)
+        o = _Options()
+        L.uaamg_options_default(C.byref(o))
+        o.precision, o.flags, o.smoother, o.device = precision, flags, smoother, device
+        o.relative_tolerance, o.max_iterations = float(tolerance), int(max_iterations)
+        self._grids = [(p[0]._struct(), p[1]._struct()) for p in problems]
+        phis = (C.c_void_p * n)(*[C.addressof(g[0]) for g in self._grids])
+        ws = (C.c_void_p * n)(*[C.addressof(g[1]) for g in self._grids])
+        dts = np.array([p[2] for p in problems], np.float32)
+        dxs = np.array([p[3] for p in problems], np.float32)
+        h = C.c_void_p()
+        _check(L.uaamg_poisson_batch_create(n, phis, ws, _ptr(dts), _ptr(dxs), C.byref(o), C.byref(h)),
+               "uaamg_poisson_batch_create")
+        self._h = h
+        self.n = n
+
+    def set_options(self, tolerance: float, max_iterations: int = 100):
+        o = _Options()
+        lib().uaamg_options_default(C.byref(o))
+        o.relative_tolerance, o.max_iterations = float(tolerance), int(max_iterations)
+        _check(lib().uaamg_poisson_batch_set_options(self._h, C.byref(o)), "batch set_options")
+
+    def info(self) -> dict:
+        n, nl = C.c_int32(), C.c_int32()
+        l0, nb = C.c_int64(), C.c_int64()
+        _check(lib().uaamg_poisson_batch_info(self._h, C.byref(n), C.byref(nl), C.byref(l0), C.byref(nb)), "info")
+        return {"nproblems": n.value, "nlevels": nl.value, "leaves0": l0.value, "device_bytes": nb.value}
+
+    def ndof(self, i: int) -> int:
+        return int(lib().uaamg_poisson_batch_ndof(self._h, i))
+
+    def solve_device(self, x_ptrs, b_ptrs):
+        """x_ptrs / b_ptrs: device addresses of each problem's DOF vectors; returns (status, iterations)."""
+        xs = (C.c_void_p * self.n)(*x_ptrs)
+        bs = (C.c_void_p * self.n)(*b_ptrs)
+        st = np.zeros(self.n, np.int32)
+        it = np.zeros(self.n, np.int32)
+        _check(lib().uaamg_poisson_batch_solve_device(self._h, xs, bs, _ptr(st), _ptr(it)), "batch solve")
+        return st.tolist(), it.tolist()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().uaamg_poisson_batch_destroy(h)
+            self._h = None
 
 
 PRECISION_FP32 = 0         # FP32 throughout (the reference's arithmetic)

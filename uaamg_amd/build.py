@@ -30,6 +30,7 @@ SOURCES = [
     "viscosity_kernels.hip",
     "viscosity_solver.cpp",
     "capi_viscosity.cpp",
+    "batch_solver.cpp",
     "server.cpp",
     "capi_server.cpp",
 ]

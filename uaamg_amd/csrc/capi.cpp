@@ -12,6 +12,7 @@
 #include <new>
 #include <string>
 
+#include "batch_solver.h"
 #include "poisson_host.h"
 #include "poisson_solver.h"
 #include "capi_common.h"
@@ -458,5 +459,83 @@ void uaamg_scene_destroy(uaamg_scene* s) { delete s; }
 
 float uaamg_fraction_inside2(float a, float b) { return fraction_inside(a, b); }
 float uaamg_fraction_inside4(float bl, float br, float tl, float tr) { return fraction_inside(bl, br, tl, tr); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------ merged batch
+
+struct uaamg_poisson_batch {
+    PoissonBatch b;
+};
+
+extern "C" {
+
+int uaamg_poisson_batch_create(int32_t n, const uaamg_grid* const* liquid_phi, const uaamg_grid* const* face_weights,
+                               const float* dt, const float* dx, const uaamg_options* opts,
+                               uaamg_poisson_batch** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out) return fail(UAAMG_EINVAL, "null output handle");
+    *out = nullptr;
+    if (n <= 0 || !liquid_phi || !face_weights || !dt || !dx) return fail(UAAMG_EINVAL, "empty or null batch");
+    std::vector<PoissonHierarchyHost> probs(static_cast<size_t>(n));
+    SolverOptions so;
+    for (int32_t i = 0; i < n; ++i) {
+        SolverOptions oi;
+        if (int rc = capi::prepare_poisson(liquid_phi[i], face_weights[i], dt[i], dx[i], opts, probs[size_t(i)], oi))
+            return rc;
+        if (i == 0) so = oi;
+    }
+    if (so.device < 0) return fail(UAAMG_ENODEV, "the merged batch needs a device");
+    std::unique_ptr<uaamg_poisson_batch> h(new uaamg_poisson_batch());
+    int rc = h->b.init(std::move(probs), so);
+    if (rc < 0) {
+        const int code = rc == -1 ? UAAMG_EINVAL : rc == -2 ? UAAMG_ENOMEM : rc == -4 ? UAAMG_ENODEV : UAAMG_EHIP;
+        return fail(code, h->b.error().empty() ? "batch initialisation failed" : h->b.error());
+    }
+    *out = h.release();
+    return 0;
+    UAAMG_GUARD_END
+}
+
+void uaamg_poisson_batch_destroy(uaamg_poisson_batch* h) { delete h; }
+
+int uaamg_poisson_batch_set_options(uaamg_poisson_batch* h, const uaamg_options* o) {
+    if (!h || !o) return fail(UAAMG_EINVAL, "null argument");
+    if (o->max_iterations < 0) return fail(UAAMG_EINVAL, "max_iterations < 0");
+    h->b.options().relative_tolerance = o->relative_tolerance;
+    h->b.options().max_iterations = o->max_iterations;
+    return 0;
+}
+
+int uaamg_poisson_batch_info(const uaamg_poisson_batch* h, int32_t* nproblems, int32_t* nlevels, int64_t* leaves0,
+                             int64_t* device_bytes) {
+    if (!h) return fail(UAAMG_EINVAL, "null handle");
+    if (nproblems) *nproblems = h->b.nproblems();
+    if (nlevels) *nlevels = h->b.nlevels();
+    if (leaves0) *leaves0 = h->b.level_leaves(0);
+    if (device_bytes) *device_bytes = int64_t(h->b.device_bytes());
+    return 0;
+}
+
+int64_t uaamg_poisson_batch_ndof(const uaamg_poisson_batch* h, int32_t i) { return h ? h->b.ndof(i) : -1; }
+
+int uaamg_poisson_batch_solve_device(uaamg_poisson_batch* h, float* const* x_dev, const float* const* b_dev,
+                                     int32_t* status, int32_t* iterations) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !x_dev || !b_dev) return fail(UAAMG_EINVAL, "null argument");
+    const int n = h->b.nproblems();
+    std::vector<int> st(static_cast<size_t>(n)), it(static_cast<size_t>(n));
+    int rc = h->b.solve(x_dev, b_dev, st.data(), it.data());
+    if (rc < 0) {
+        const int code = rc == -1 ? UAAMG_EINVAL : rc == -4 ? UAAMG_ENODEV : UAAMG_EHIP;
+        return fail(code, h->b.error().empty() ? "batch solve failed" : h->b.error());
+    }
+    for (int i = 0; i < n; ++i) {
+        if (status) status[i] = st[size_t(i)];
+        if (iterations) iterations[i] = it[size_t(i)];
+    }
+    return 0;
+    UAAMG_GUARD_END
+}
 
 }  // extern "C"

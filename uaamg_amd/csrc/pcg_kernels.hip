@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <cstring>
 
+#include "batch_kernels.h"
 #include "coarsest_device.h"
 #include "stencil_device.h"
 
@@ -357,6 +358,213 @@ __global__ __launch_bounds__(kBlockThreads) void residual64_kernel(LevelView L, 
     write_absmax_partial(m, bad, partials);
 }
 
+// ---------------------------------------------------------- batched solve
+// The same arithmetic as the single-problem kernels, with each leaf's scalars
+// taken from its problem and the problem's state deciding what is updated.
+__device__ __forceinline__ int leaf_problem(const int32_t* leafprob, int leaf) {
+    return __builtin_amdgcn_readfirstlane(leafprob[leaf]);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void bupdate_laplacian_dot_kernel(LevelView L, const int32_t* leafprob,
+                                                                              const BatchPcg* st, const float* Z,
+                                                                              const float* P, float* Pn, float* X,
+                                                                              float* Q, double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    double acc = 0;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        const BatchPcg& sp = st[leaf_problem(leafprob, leaf)];
+        const int state = sp.state;
+        if (rm && (state == kBActive || state == kBConverging)) {
+            const float alpha = sp.alpha, beta = sp.beta;
+            const size_t row = size_t(leaf) * 512 + lane * 8;
+            float x[8], p[8];
+            ld8(X + row, x);
+            ld8(P + row, p);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((rm >> k) & 1u) {
+                    const float t = alpha * p[k];
+                    x[k] = x[k] + t;
+                }
+            st8(X + row, x);
+            if (state == kBActive) {
+                float pc[8], s[8], dg[8], q[8];
+                stencil_offdiag(L, UpdateSrc{Z, P, beta}, leaf, lane, pc, s);
+                mask_row(rm, pc);
+                st8(Pn + row, pc);
+                ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
+#pragma unroll
+                for (int z = 0; z < 8; ++z) {
+                    const bool on = (rm >> z) & 1u;
+                    q[z] = on ? fmaf(pc[z], dg[z], s[z]) : 0.f;
+                    if (on) acc += double(pc[z]) * double(q[z]);
+                }
+                st8(Q + row, q);
+            }
+        }
+    }
+    write_partial(partials, block_sum<kBlockThreads>(acc));
+}
+
+__global__ __launch_bounds__(kBlockThreads) void baxpy_absmax_kernel(LevelView L, const int32_t* leafprob,
+                                                                     const BatchPcg* st, const float* Zv, float* R,
+                                                                     double* partials) {
+    const int leaf = wave_leaf();
+    const int lane = threadIdx.x & 63;
+    float m = 0.f, bad = 0.f;
+    if (leaf < L.nleaf) {
+        const uint32_t rm = row_mask(L, leaf, lane);
+        const BatchPcg& sp = st[leaf_problem(leafprob, leaf)];
+        if (rm && sp.state == kBActive) {
+            const size_t row = size_t(leaf) * 512 + lane * 8;
+            float r[8], z[8];
+            ld8(R + row, r);
+            const float na = -sp.alpha;
+            ld8(Zv + row, z);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((rm >> k) & 1u) {
+                    const float t = na * z[k];
+                    r[k] = r[k] + t;
+                }
+            st8(R + row, r);
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if ((rm >> k) & 1u) absmax_acc(r[k], m, bad);
+        }
+    }
+    write_absmax_partial(m, bad, partials);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void bupdate_x_kernel(LevelView L, const int32_t* leafprob,
+                                                                  const BatchPcg* st, const float* P, float* X) {
+    const int leaf = wave_leaf();
+    if (leaf >= L.nleaf) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t rm = row_mask(L, leaf, lane);
+    const BatchPcg& sp = st[leaf_problem(leafprob, leaf)];
+    if (!rm || sp.state != kBConverging) return;
+    const float alpha = sp.alpha;
+    const size_t row = size_t(leaf) * 512 + lane * 8;
+    float x[8], p[8];
+    ld8(X + row, x);
+    ld8(P + row, p);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((rm >> k) & 1u) {
+            const float t = alpha * p[k];
+            x[k] = x[k] + t;
+        }
+    st8(X + row, x);
+}
+
+// One workgroup per problem. The sums and the scalar updates repeat
+// finalize_kernel's on the problem's own block range; the state rules repeat
+// the host loop of solve_pcg_dev / pcg_iterate.
+__global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const int32_t* blk_begin,
+                                                                const double* partials, BatchPcg* st) {
+    const int prob = blockIdx.x;
+    BatchPcg& sp = st[prob];
+    const int b0 = blk_begin[prob], n = blk_begin[prob + 1] - b0;
+    const double* part = partials + b0;
+    if (op == kBFinNu0 || op == kBFinNuFirst || op == kBFinNuIter) {
+        __shared__ float tv[kFinThreads];
+        __shared__ int tb[kFinThreads];
+        float v = 0.f;
+        int b = 0;
+        for (int i = threadIdx.x; i < n; i += kFinThreads) {
+            const double p = part[i];
+            if (p < 0) b = 1;
+            else v = fmaxf(v, float(p));
+        }
+        tv[threadIdx.x] = v;
+        tb[threadIdx.x] = b;
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        float mv = 0.f;
+        int any = 0;
+        for (int i = 0; i < kFinThreads; ++i) {
+            mv = fmaxf(mv, tv[i]);
+            any |= tb[i];
+        }
+        const float nu = any ? __builtin_nanf("") : mv;
+        if (op == kBFinNu0) {
+            sp.nu = sp.nu_prev = nu;
+            sp.numax = sp.tol * nu;
+            sp.iters = 0;
+            sp.state = nu <= sp.numax ? kBSuccess : (sp.max_iters > 0 ? kBActive : kBFailed);
+            return;
+        }
+        if (sp.state != kBActive) return;
+        sp.nu = nu;
+        if (op == kBFinNuIter) {
+            sp.iters += 1;
+            if (sp.iters >= sp.max_iters) {
+                sp.state = kBFailed;
+                return;
+            }
+        }
+        if (nu <= sp.numax) sp.state = kBConverging;
+        else if (nu > sp.nu_prev && sp.iters > 3) sp.state = kBFailed;
+        sp.nu_prev = nu;
+        return;
+    }
+    if (op == kBFinSigma && sp.state == kBConverging) {   // its last x update ran in this piece
+        if (threadIdx.x == 0) sp.state = kBSuccess;
+        return;
+    }
+    if (sp.state != kBActive) return;
+    double t = 0;
+    for (int i = threadIdx.x; i < n; i += kFinThreads) t += part[i];
+    const double total = block_sum<kFinThreads>(t);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case kBFinRho: sp.rho = total; break;
+        case kBFinRhoNew: {
+            const float rho_new = float(total), rho = float(sp.rho);
+            sp.beta = rho_new / rho;
+            sp.rho_new = total;
+            sp.rho = total;
+            break;
+        }
+        case kBFinSigma:
+            sp.sigma = total;
+            sp.alpha = float(sp.rho) / float(total);
+            break;
+        default: break;
+    }
+}
+
+// converging -> success after the final x update
+__global__ __launch_bounds__(kFinThreads) void bcommit_kernel(int nprob, BatchPcg* st) {
+    for (int p = threadIdx.x; p < nprob; p += kFinThreads)
+        if (st[p].state == kBConverging) st[p].state = kBSuccess;
+}
+
+__global__ __launch_bounds__(kFinThreads) void bstatus_kernel(int nprob, const BatchPcg* st, unsigned int* seq,
+                                                              HostMailbox* mb) {
+    __shared__ int cnt[kFinThreads];
+    int c = 0;
+    for (int p = threadIdx.x; p < nprob; p += kFinThreads) c += st[p].state == kBActive ? 1 : 0;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    int total = 0;
+    for (int i = 0; i < kFinThreads; ++i) total += cnt[i];
+    const unsigned int q = *seq + 1;
+    *seq = q;
+    mb->word = (uint64_t(q) << 32) | uint32_t(total);
+}
+
+template <class V>
+__global__ __launch_bounds__(kCgThreads) void bcoarsest_kernel(const CoarsestView* views, const V* B, V* X) {
+    __shared__ CgShared<kCgThreads> sh;
+    const CoarsestView C = views[blockIdx.x];
+    coarsest_cg_block<kCgThreads, V>(C, B, X, sh);
+}
+
 hipError_t finalize(int op, int n, const double* partials, PcgScalars* sc, HostMailbox* mb, hipStream_t s) {
     finalize_kernel<<<1, kFinThreads, 0, s>>>(op, n, partials, sc, mb);
     return hipGetLastError();
@@ -456,6 +664,93 @@ hipError_t launch_update_laplacian_dot(const LevelView& L, const float* z, const
     update_laplacian_dot_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, z, p, p_new, x, q, sc, partials);
     if (hipError_t e = hipGetLastError()) return e;
     return finalize(kFinSigma, g, partials, sc, nullptr, s);
+}
+
+// ---- batched solve launchers (batch_kernels.h)
+hipError_t launch_absmax_partials(const LevelView& L, const float* r, double* partials, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    absmax_kernel<false><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, nullptr, const_cast<float*>(r), nullptr,
+                                                                     partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot_partials(const LevelView& L, const float* a, const float* b, double* partials, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    dot_kernel<0><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, a, b, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_laplacian_dot_partials(const LevelView& L, const float* p, float* q, double* partials,
+                                         hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    laplacian_dot_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, p, q, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_dot_partials(const LevelView& L, const float* x, const float* b, float* y, float w,
+                                      double* partials, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    jacobi_dot_kernel<float><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, x, b, y, w, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_dot_partials(const LevelView& L, const bf16* x, const float* b, float* y, float w,
+                                      double* partials, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    jacobi_dot_kernel<bf16><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, x, b, y, w, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_bupdate_laplacian_dot(const LevelView& L, const int32_t* leafprob, const BatchPcg* st,
+                                        const float* z, const float* p, float* p_new, float* x, float* q,
+                                        double* partials, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    bupdate_laplacian_dot_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, z, p, p_new, x, q,
+                                                                            partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_baxpy_absmax(const LevelView& L, const int32_t* leafprob, const BatchPcg* st, const float* q,
+                               float* r, double* partials, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    baxpy_absmax_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, q, r, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_bupdate_x(const LevelView& L, const int32_t* leafprob, BatchPcg* st, const float* p, float* x,
+                            hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    bupdate_x_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, p, x);
+    if (hipError_t e = hipGetLastError()) return e;
+    return hipSuccess;
+}
+
+hipError_t launch_bfinalize(int op, int nprob, const int32_t* blk_begin, const double* partials, BatchPcg* st,
+                            hipStream_t s) {
+    if (nprob <= 0) return hipSuccess;
+    if (op < 0) {   // commit: converging -> success
+        bcommit_kernel<<<1, kFinThreads, 0, s>>>(nprob, st);
+        return hipGetLastError();
+    }
+    bfinalize_kernel<<<unsigned(nprob), kFinThreads, 0, s>>>(op, blk_begin, partials, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_bstatus(int nprob, const BatchPcg* st, unsigned int* seq, HostMailbox* mb, hipStream_t s) {
+    bstatus_kernel<<<1, kFinThreads, 0, s>>>(nprob, st, seq, mb);
+    return hipGetLastError();
+}
+
+hipError_t launch_bcoarsest(const CoarsestView* views, int n, const float* b, float* x, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    bcoarsest_kernel<float><<<unsigned(n), kCgThreads, 0, s>>>(views, b, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_bcoarsest(const CoarsestView* views, int n, const bf16* b, bf16* x, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    bcoarsest_kernel<bf16><<<unsigned(n), kCgThreads, 0, s>>>(views, b, x);
+    return hipGetLastError();
 }
 
 }  // namespace uaamg

@@ -14,6 +14,7 @@ namespace uaamg {
 namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+}  // namespace
 
 // weights of the scheduled relaxed Jacobi smoother for n = 3 sweeps
 // (muCyclePreconditioner, simd_vdb_poisson.cpp:1848-1866)
@@ -34,8 +35,6 @@ float scheduled_weight_iterative(int n, int i) {
     }
     return 6.0f / 7.0f;
 }
-constexpr int kSweeps = 3;   // n = 3 pre/post sweeps (solveMultigridPCG :2181,2212)
-}  // namespace
 
 DeviceArena::~DeviceArena() { release(); }
 
@@ -448,15 +447,12 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
     return first_error_ == hipSuccess ? 0 : -3;
 }
 
-int PoissonDevice::build_coarsest() {
-    const PoissonLevelHost& l = host_.levels.back();
+void coarsest_system(const PoissonLevelHost& l, int64_t leaf_offset, std::vector<int32_t>& pos,
+                     std::vector<int32_t>& nb, std::vector<float>& coef) {
     const int64_t nd = l.ndof;
-    if (nd > kMaxCoarsestDof) {
-        err_ = "coarsest level exceeds the single-workgroup solver size";
-        return -1;
-    }
-    std::vector<int32_t> pos(nd), nb(nd * 6, -1);
-    std::vector<float> coef(nd * 8, 0.f);
+    pos.assign(size_t(nd), 0);
+    nb.assign(size_t(nd) * 6, -1);
+    coef.assign(size_t(nd) * 8, 0.f);
     // DOF index lookup on the coarsest level
     auto dof_at = [&](const Coord& g) -> int64_t {
         int32_t li = l.lookup.find(leaf_origin_of(g));
@@ -479,7 +475,7 @@ int PoissonDevice::build_coarsest() {
         for (int off = 0; off < kLeafSize; ++off) {
             if (!mask_test(&l.mask[i * 8], off)) continue;
             const Coord g{o.x + (off >> 6), o.y + ((off >> 3) & 7), o.z + (off & 7)};
-            pos[k] = int32_t(i * kLeafSize + off);
+            pos[k] = int32_t((int64_t(i) + leaf_offset) * kLeafSize + off);
             const float dg = l.coef[kCoefDiag][i * kLeafSize + off];
             coef[k * 8 + 0] = dg;
             for (int c = 0; c < 3; ++c) {
@@ -495,6 +491,18 @@ int PoissonDevice::build_coarsest() {
             ++k;
         }
     }
+}
+
+int PoissonDevice::build_coarsest() {
+    const PoissonLevelHost& l = host_.levels.back();
+    const int64_t nd = l.ndof;
+    if (nd > kMaxCoarsestDof) {
+        err_ = "coarsest level exceeds the single-workgroup solver size";
+        return -1;
+    }
+    std::vector<int32_t> pos, nb;
+    std::vector<float> coef;
+    coarsest_system(l, 0, pos, nb, coef);
     auto up = [&](const void* src, size_t bytes) -> void* {
         void* p = arena_.take(std::max<size_t>(bytes, 4));
         if (p && bytes) (void)check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload");

@@ -16,6 +16,17 @@ namespace uaamg {
 
 enum SmootherOption { kScheduledRelaxedJacobi = 0, kWeightedJacobi = 1, kRedBlackGaussSeidel = 2, kSpai0 = 3 };
 
+constexpr int kSweeps = 3;   // n = 3 pre/post sweeps (solveMultigridPCG, simd_vdb_poisson.cpp:2181,2212)
+// Jacobi weight of sweep i of the preconditioner's / the pure multigrid cycle's schedule
+float scheduled_weight(int n, int i);
+float scheduled_weight_iterative(int n, int i);
+
+// The coarsest level as explicit rows for the single-workgroup CG: per DOF the
+// brick position (leaf_offset + leaf) * 512 + offset, the six neighbour DOFs
+// and the stencil values (diag, c+x, c-x, c+y, c-y, c+z, c-z, Eigen's invdiag).
+void coarsest_system(const PoissonLevelHost& l, int64_t leaf_offset, std::vector<int32_t>& pos,
+                     std::vector<int32_t>& nb, std::vector<float>& coef);
+
 struct SolverOptions {
     float relative_tolerance = 1e-7f;
     int max_iterations = 100;
