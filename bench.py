@@ -188,6 +188,7 @@ def main() -> int:
                        "p99_ms": round(float(np.percentile(lat32, 99)), 3), "pcg_iterations": it32},
         "speedup_vs_fp32_plain": round(ms32 / ms_mx, 3),
         "level_sweep_us": level_sweep_us,
+        "level_kernel_map": {"fp32": lap.kernel_map(), "bf16": lap_mx.kernel_map()},
         "build_s": round(t_build, 3),
         "build_s_mixed": round(t_build_mx, 3),
         "scene_s": round(t_scene, 3),

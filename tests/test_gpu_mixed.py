@@ -55,7 +55,7 @@ def _solve(s, b, precision, flags=0, tol=TOL, x0=None):
 @pytest.mark.parametrize("flags", [0, 16])
 def test_mixed_solution_and_true_residual(problem, flags):
     """flags 16 runs the coarse levels through the per-sweep kernels instead of
-    the block-visit kernels (both store BF16)."""
+    the block-visit kernels (both store BF16, rounding at the same points)."""
     s, orc, b, x64 = problem
     lap, solver, st, x = _solve(s, b, BF16, flags)
     _, solver32, st32, x32 = _solve(s, b, U.PRECISION_FP32)
@@ -87,14 +87,16 @@ def test_mixed_w_cycle_within_bf16_rounding(problem):
     assert err <= 3e-2
 
 
-def test_mixed_graph_replay_bit_identical(problem):
+def test_mixed_graph_replay_and_kernel_mapping_bit_identical(problem):
+    """flags 2: no graph replay; 16: per-sweep kernels on every coarse level."""
     s, orc, b, x64 = problem
     out = []
-    for flags in (0, 2):
+    for flags in (0, 2, 16, 18):
         lap, solver, st, x = _solve(s, b, BF16, flags)
         out.append((st, solver.mIterationTaken, x))
-    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
-    assert np.array_equal(out[0][2], out[1][2])
+    for st, it, x in out[1:]:
+        assert st == out[0][0] and it == out[0][1]
+        assert np.array_equal(x, out[0][2])
 
 
 def test_fp32_with_residual_replacement(problem):

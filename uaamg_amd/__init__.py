@@ -106,6 +106,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_solve_pcg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_pcg_device": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_last_residual": (C.c_int, [vp, P(f32), P(i32)]),
+        "uaamg_poisson_kernel_map": (C.c_int, [vp, C.c_int, P(i32), P(f32), P(f32)]),
         "uaamg_poisson_solve_mg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_batch_device": (C.c_int, [vp, i32, vp, vp, vp, vp, i32]),
         "uaamg_poisson_batch_create": (C.c_int, [i32, vp, vp, vp, vp, P(_Options), P(vp)]),
@@ -463,6 +464,16 @@ class LaplacianWithLevel:
         rc = _check(lib().uaamg_poisson_solve_pcg_device(self._h, C.c_void_p(x_ptr), C.c_void_p(b_ptr),
                                                          C.byref(it)), "solve_pcg_device")
         return rc, int(it.value)
+
+    def kernel_map(self) -> list[dict]:
+        """Per level: whether visits run as block-visit launches, and the timings behind the choice."""
+        out = []
+        for l in range(self.nlevels):
+            b, mb, ms = C.c_int32(), C.c_float(), C.c_float()
+            _check(lib().uaamg_poisson_kernel_map(self._h, l, C.byref(b), C.byref(mb), C.byref(ms)), "kernel_map")
+            out.append({"level": l, "block": bool(b.value), "ms_block": round(mb.value, 4),
+                        "ms_sweep": round(ms.value, 4)})
+        return out
 
     def last_residual(self) -> tuple[float, int]:
         """(FP64 ||b - A x||_inf of the last solve, residual replacements made); -1s without replacement."""

@@ -9,12 +9,16 @@
 // voxels are recomputed redundantly by each neighbouring workgroup; every
 // value is produced by the same arithmetic as the per-sweep kernels
 // (stencil order ((x+ + x-) + (y+ + y-)) + (z+ + z-), fma(x, diag, s),
-// fma(w, (b - Ax) * D^-1, x), non-DOF voxels 0), so results are bit-identical.
+// fma(w, (b - Ax) * D^-1, x), non-DOF voxels 0), and with BF16 storage each
+// sweep's output is rounded where the per-sweep kernels store it, so results
+// are bit-identical to theirs in either precision.
 //
 //   kPreZero  (H = 3): x1 = w0 D^-1 b, two sweeps, residual, restriction
 //             (first visit, muCyclePreconditioner from a zero guess)
 //   kPreAgain (H = 4): three sweeps from the level's x, residual, restriction
 //   kPost     (H = 3): x + P x_coarse, three sweeps
+#include <type_traits>
+
 #include "stencil_device.h"
 
 namespace uaamg {
@@ -37,8 +41,17 @@ __device__ __forceinline__ int ridx(int x, int y, int z) {
     return (x * Region<H>::S + y) * Region<H>::S + z;
 }
 
-// One Jacobi sweep y = J_w(x) on [lo, S - lo)^3 of the region.
-template <int H>
+// value as the vector storage type holds it (the per-sweep kernels store
+// every sweep's output, so the block form rounds where they would)
+template <class V>
+__device__ __forceinline__ float stored(float v) {
+    if constexpr (std::is_same_v<V, float>) return v;
+    else return float(V(v));
+}
+
+// One Jacobi sweep y = J_w(x) on [lo, S - lo)^3 of the region, each output
+// rounded to the storage type V.
+template <int H, class V>
 __device__ __forceinline__ void sweep(int lo, const float* x, float* y, const float* b, const float* dg,
                                       const float* inv, const float* cx, const float* cy, const float* cz,
                                       const uint8_t* m, float w) {
@@ -63,7 +76,7 @@ __device__ __forceinline__ void sweep(int lo, const float* x, float* y, const fl
             t = t * inv[i];
             r = fmaf(w, t, x[i]);
         }
-        y[i] = r;
+        y[i] = stored<V>(r);
     }
 }
 
@@ -137,17 +150,17 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
 
     const float* fin;   // region array holding the final sweep
     if (MODE == kPreZero) {
-        sweep<H>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w1);
+        sweep<H, V>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w1);
         __syncthreads();
-        sweep<H>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w2);
+        sweep<H, V>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w2);
         __syncthreads();
         fin = xa;
     } else {
-        sweep<H>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w0);
+        sweep<H, V>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w0);
         __syncthreads();
-        sweep<H>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w1);
+        sweep<H, V>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w1);
         __syncthreads();
-        sweep<H>(3, xa, xb, b, dg, inv, cx, cy, cz, m, w2);
+        sweep<H, V>(3, xa, xb, b, dg, inv, cx, cy, cz, m, w2);
         __syncthreads();
         fin = xb;
     }

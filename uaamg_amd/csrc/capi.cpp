@@ -191,6 +191,15 @@ int uaamg_poisson_set_options(uaamg_poisson* h, const uaamg_options* o) {
 
 int64_t uaamg_poisson_ndof(const uaamg_poisson* h) { return h ? h->ndof : -1; }
 
+int uaamg_poisson_kernel_map(const uaamg_poisson* h, int level, int32_t* block, float* ms_block, float* ms_sweep) {
+    if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
+    const auto& m = h->dev.block_levels();
+    if (block) *block = level < int(m.size()) ? m[size_t(level)] : 0;
+    if (ms_block) *ms_block = h->dev.tune_ms(level, 0);
+    if (ms_sweep) *ms_sweep = h->dev.tune_ms(level, 1);
+    return 0;
+}
+
 int uaamg_poisson_last_residual(const uaamg_poisson* h, float* true_residual_inf, int32_t* replacements) {
     if (!h) return fail(UAAMG_EINVAL, "null handle");
     if (true_residual_inf) *true_residual_inf = h->dev.last_true_residual();

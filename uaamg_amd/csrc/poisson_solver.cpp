@@ -257,6 +257,7 @@ int PoissonDevice::acquire_device() {
     const int L = int(host_.levels.size());
     sor_w_.assign(L, 1.2f);
     const size_t n0 = host_.levels[0].nleaf();
+    bool tune_pending = false;
     if (check(arena_.reserve(device_bytes()), "hipMalloc")) return -2;
 
     lv_.resize(L);
@@ -322,12 +323,21 @@ int PoissonDevice::acquire_device() {
         if (env) max_leaves = std::atoi(env);
         tail_level_ = -1;
         if (max_leaves > 0 && L >= 3 && int64_t(host_.levels[L - 2].nleaf()) <= max_leaves) tail_level_ = L - 2;
-        // coarse levels (1 .. L-2) of at most this many leaves run each visit
-        // as two block-visit launches (flags bit 4 or UAAMG_BLOCK_LEAVES=0: off)
-        block_max_leaves_ = (opt_.flags & 16) ? 0 : 512;
+        // level-to-kernel mapping of the coarse levels (1 .. L-2): each visit
+        // as two block-visit launches or as the per-sweep kernels. By default
+        // the faster of the two on this device, timed below for every level of
+        // at most kTuneMaxLeaves leaves; UAAMG_BLOCK_LEAVES=n fixes it to
+        // "levels of at most n leaves" instead; flags bit 4 switches the
+        // block-visit kernels off.
+        block_max_leaves_ = (opt_.flags & 16) ? 0 : kTuneMaxLeaves;
         env = std::getenv("UAAMG_BLOCK_LEAVES");
+        const bool fixed = env != nullptr;
         if (env) block_max_leaves_ = std::atoi(env);
+        use_block_.assign(size_t(L), 0);
+        for (int l = 1; l + 1 < L; ++l)
+            use_block_[size_t(l)] = int64_t(host_.levels[l].nleaf()) <= block_max_leaves_ ? 1 : 0;
         if (block_max_leaves_ > 0 && check(block_kernels_init(), "block kernel attributes")) return -3;
+        tune_pending = block_max_leaves_ > 0 && !fixed;
     }
     if (opt_.precision != 0 && opt_.precision != 1) {
         err_ = "unsupported precision";
@@ -352,8 +362,91 @@ int PoissonDevice::acquire_device() {
         }
     }
     if (int rc = build_coarsest()) return rc;
+    if (tune_pending) tune_levels();
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
-    return 0;
+    return first_error_ == hipSuccess ? 0 : -3;
+}
+
+// One first visit and one repeated visit of level l's own work (pre-smoothing,
+// residual, restriction, prolongation, post-smoothing; the coarser levels'
+// work between them is left out), in the solver's precision, on the level's
+// scratch vectors.
+void PoissonDevice::visit_pair(int l, bool block) {
+    const LevelView& v = lv_[l].v;
+    DevLevel& d = lv_[l];
+    DevLevel& c = lv_[l + 1];
+    const bool weighted = opt_.smoother == kWeightedJacobi;
+    auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
+    auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    for (int first = 1; first >= 0; --first) {
+        if (block) {
+            if (mixed_) {
+                check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                check(launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, wpost(0), wpost(1), wpost(2), stream_), "tune");
+            } else {
+                check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                check(launch_block_post(v, d.T, c.X, d.B, d.X, wpost(0), wpost(1), wpost(2), stream_), "tune");
+            }
+        } else if (mixed_) {
+            if (first) {
+                check(launch_jacobi_first2(v, d.Bh, d.Th, wpre(0), wpre(1), stream_), "tune");
+                check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(2), stream_), "tune");
+            } else {
+                check(launch_jacobi(v, d.Xh, d.Bh, d.Th, wpre(0), stream_), "tune");
+                check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(1), stream_), "tune");
+                check(launch_jacobi(v, d.Uh, d.Bh, d.Th, wpre(2), stream_), "tune");
+            }
+            check(launch_residual_restrict(v, d.Uh, d.Bh, c.Bh, stream_), "tune");
+            check(launch_prolong_jacobi(v, d.Uh, c.Xh, d.Bh, d.Th, wpost(0), stream_), "tune");
+            check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpost(1), stream_), "tune");
+            check(launch_jacobi(v, d.Uh, d.Bh, d.Xh, wpost(2), stream_), "tune");
+        } else {
+            if (first) {
+                check(launch_jacobi_first2(v, d.B, d.T, wpre(0), wpre(1), stream_), "tune");
+                check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpre(2), stream_), "tune");
+            } else {
+                check(launch_apply(v, kModeJacobi, d.X, d.B, d.T, wpre(0), stream_), "tune");
+                check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpre(1), stream_), "tune");
+                check(launch_apply(v, kModeJacobi, d.U, d.B, d.T, wpre(2), stream_), "tune");
+            }
+            check(launch_residual_restrict(v, d.U, d.B, c.B, stream_), "tune");
+            check(launch_prolong_jacobi(v, d.U, c.X, d.B, d.T, wpost(0), stream_), "tune");
+            check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpost(1), stream_), "tune");
+            check(launch_apply(v, kModeJacobi, d.U, d.B, d.X, wpost(2), stream_), "tune");
+        }
+    }
+}
+
+// Times both kernel mappings of every candidate coarse level with HIP events
+// on the solver's stream and keeps the faster. The scratch vectors hold zeros
+// before and after (zero in, zero out), so no solve state is touched.
+void PoissonDevice::tune_levels() {
+    const int L = nlevels();
+    tune_ms_[0].assign(size_t(L), -1.f);
+    tune_ms_[1].assign(size_t(L), -1.f);
+    const bool jacobi = opt_.smoother == kScheduledRelaxedJacobi || opt_.smoother == kWeightedJacobi;
+    if (!fuse_ || !jacobi) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return;
+    constexpr int kReps = 8;
+    for (int l = 1; l + 1 < L; ++l) {
+        if (!use_block_[size_t(l)] || l + 1 == tail_level_) continue;
+        float ms[2] = {0.f, 0.f};
+        for (int variant = 0; variant < 2; ++variant) {
+            visit_pair(l, variant == 0);   // warm-up
+            check(hipEventRecord(e0, stream_), "event");
+            for (int r = 0; r < kReps; ++r) visit_pair(l, variant == 0);
+            check(hipEventRecord(e1, stream_), "event");
+            check(hipEventSynchronize(e1), "event");
+            check(hipEventElapsedTime(&ms[variant], e0, e1), "event");
+            ms[variant] /= kReps;
+        }
+        tune_ms_[0][size_t(l)] = ms[0];
+        tune_ms_[1][size_t(l)] = ms[1];
+        use_block_[size_t(l)] = ms[0] < ms[1] ? 1 : 0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 void PoissonDevice::init_host_only(PoissonHierarchyHost&& h, const SolverOptions& opt) {
@@ -622,7 +715,7 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
     const bool weighted = opt_.smoother == kWeightedJacobi;
     auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
     auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
-    if (l >= 1 && l + 1 != tail_level_ && int64_t(v.nleaf) <= block_max_leaves_) {
+    if (l >= 1 && l + 1 != tail_level_ && use_block_[size_t(l)]) {
         // two launches per visit: pre-smoothing + residual + restriction, then
         // prolongation + post-smoothing (the coarser visits run in between)
         DevLevel& c = lv_[l + 1];
@@ -712,7 +805,7 @@ void PoissonDevice::precond_mixed(int l, TX* X, const TB* B, bool skip_first, in
         if (l + 1 != L - 1) precond_mixed<bf16, bf16>(l + 1, c.Xh, c.Bh, false, -1);
     };
     if constexpr (std::is_same_v<TX, bf16>) {
-        if (l >= 1 && int64_t(v.nleaf) <= block_max_leaves_) {
+        if (l >= 1 && use_block_[size_t(l)]) {
             check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "block pre");
             coarse();
             check(launch_block_post(v, d.Th, c.Xh, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");

@@ -59,6 +59,7 @@ public:
     static constexpr int kSuccess = 0;
     static constexpr int kFailed = 1;
     static constexpr int kMaxReplacements = 4;
+    static constexpr int kTuneMaxLeaves = 4096;   // coarse levels above this size keep the per-sweep kernels
 
     ~PoissonDevice();
     // returns 0 or a negative uaamg status
@@ -105,6 +106,14 @@ public:
 
     // ||b - A x||_inf of the last solve's result (FP64 residual) when residual
     // replacement is on, and how many replacements it made; -1 otherwise
+    // Level-to-kernel mapping chosen at acquire time: per level, whether a
+    // W-cycle visit runs as two block-visit launches, and the two timings the
+    // choice came from (ms per visit pair; -1 where the level was not timed).
+    const std::vector<uint8_t>& block_levels() const { return use_block_; }
+    float tune_ms(int level, int variant) const {
+        return level < int(tune_ms_[variant].size()) ? tune_ms_[variant][size_t(level)] : -1.f;
+    }
+
     float last_true_residual() const { return true_nu_; }
     int last_replacements() const { return replacements_; }
 
@@ -142,6 +151,8 @@ private:
     template <class TX, class TB>
     void precond_mixed(int level, TX* X, const TB* B, bool skip_first, int dot_target);
     int pcg_iterate(float& nu, float numax, int& it);
+    void tune_levels();
+    void visit_pair(int level, bool block);
     void mg_cycle(int level, float* X, const float* B);
     void smooth(int level, float*& cur, float*& oth, const float* B, int sweep, bool pre, bool iterative);
     int check(hipError_t e, const char* what);
@@ -168,7 +179,9 @@ private:
     bool use_graphs_ = true;
     bool fuse_ = true;                     // fused W-cycle / PCG kernels
     int tail_level_ = -1;                  // level whose visit pairs run in the tail kernel (-1: none)
-    int block_max_leaves_ = 0;             // coarse levels up to this size use the block-visit kernels
+    int block_max_leaves_ = 0;             // threshold mode: coarse levels up to this size use the block-visit kernels
+    std::vector<uint8_t> use_block_;       // per level: visits run as two block-visit launches
+    std::vector<float> tune_ms_[2];        // per level: measured ms of a visit pair, [0] block-visit, [1] per-sweep
     bool mixed_ = false;                   // BF16-stored W-cycle vectors (Jacobi-family smoothers)
     bool refine_ = false;                  // FP64 residual replacement after convergence
     float true_nu_ = -1.f;
