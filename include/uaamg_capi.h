@@ -212,6 +212,8 @@ int64_t uaamg_poisson_batch_ndof(const uaamg_poisson_batch* h, int32_t i);
  * mIterationTaken (either may be NULL). Returns 0 or a negative error. */
 int uaamg_poisson_batch_solve_device(uaamg_poisson_batch* h, float* const* x_dev, const float* const* b_dev,
                                      int32_t* status, int32_t* iterations);
+/* Per problem after the last solve, as uaamg_poisson_last_residual (arrays of nproblems entries). */
+int uaamg_poisson_batch_last_residual(const uaamg_poisson_batch* h, float* true_residual_inf, int32_t* replacements);
 
 /* PoissonSolver::solvePureMultigrid (simd_vdb_poisson.h:163, .cpp:2228-2267), host vectors. */
 int uaamg_poisson_solve_mg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations);

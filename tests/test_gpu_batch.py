@@ -45,7 +45,7 @@ def _alone(scenes, rhs, precision=0, flags=0, max_iterations=100):
         x = torch.zeros_like(b)
         rc, it = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
         torch.cuda.synchronize()
-        out.append((rc, it, x.cpu().numpy()))
+        out.append((rc, it, x.cpu().numpy(), lap.last_residual()))
     return out
 
 
@@ -71,18 +71,21 @@ def test_batch_equals_each_problem_alone(batch_inputs, precision, flags):
     _say(f"  precision {precision} flags {flags}: merged levels {info['nlevels']}, leaves0 {info['leaves0']}, "
          f"iterations {[r[1] for r in batched]}")
     assert info["nproblems"] == len(scenes) and info["nlevels"] == 3
-    for i, ((s1, i1, x1), (s2, i2, x2)) in enumerate(zip(alone, batched)):
+    res = bt.last_residual()
+    for i, ((s1, i1, x1, r1), (s2, i2, x2)) in enumerate(zip(alone, batched)):
         assert bt.ndof(i) == x1.size
         assert s1 == s2 == U.SUCCESS, f"problem {i}"
         assert i1 == i2, f"problem {i}: {i1} vs {i2}"
         assert np.array_equal(x1, x2), f"problem {i}"
+        # the residual replacement (BF16 form) restarts the same problems the same number of times
+        assert r1[1] == res[i][1] and (r1[0] == res[i][0] or precision == 0), f"problem {i}: {r1} vs {res[i]}"
 
 
 def test_batch_iteration_limit(batch_inputs):
     scenes, rhs = batch_inputs
     alone = _alone(scenes, rhs, max_iterations=2)
     bt, batched = _batched(scenes, rhs, max_iterations=2)
-    for (s1, i1, x1), (s2, i2, x2) in zip(alone, batched):
+    for (s1, i1, x1, _), (s2, i2, x2) in zip(alone, batched):
         assert s1 == s2 and i1 == i2 and np.array_equal(x1, x2)
 
 

@@ -115,6 +115,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_batch_info": (C.c_int, [vp, P(i32), P(i32), P(i64), P(i64)]),
         "uaamg_poisson_batch_ndof": (i64, [vp, i32]),
         "uaamg_poisson_batch_solve_device": (C.c_int, [vp, vp, vp, vp, vp]),
+        "uaamg_poisson_batch_last_residual": (C.c_int, [vp, vp, vp]),
         "uaamg_viscosity_solve_batch_device": (C.c_int, [vp, i32, f32, vp, vp, vp, i32]),
         "uaamg_poisson_apply": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp, vp]),
         "uaamg_poisson_first_sweep": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp]),
@@ -316,6 +317,13 @@ class PoissonBatch:
         it = np.zeros(self.n, np.int32)
         _check(lib().uaamg_poisson_batch_solve_device(self._h, xs, bs, _ptr(st), _ptr(it)), "batch solve")
         return st.tolist(), it.tolist()
+
+    def last_residual(self):
+        """Per problem: (FP64 true residual, restarts) of the last solve; -1s without replacement."""
+        r = np.zeros(self.n, np.float32)
+        k = np.zeros(self.n, np.int32)
+        _check(lib().uaamg_poisson_batch_last_residual(self._h, _ptr(r), _ptr(k)), "batch last_residual")
+        return list(zip(r.tolist(), k.tolist()))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -24,20 +24,36 @@ struct BatchPcg {
     double rho, sigma, rho_new;
     float alpha, beta;
     float nu, nu_prev, numax;
-    int32_t state;       // BatchState
-    int32_t iters;       // PCG iterations taken (mIterationTaken)
+    int32_t state;         // BatchState
+    int32_t iters;         // PCG iterations taken (mIterationTaken)
     int32_t max_iters;
     float tol;
-    int32_t pad;
+    int32_t it0;           // iterations at the start of the current recurrence (growth rule)
+    float true_nu;         // FP64 true residual of the last replacement check (-1 without)
+    float prev_true;
+    int32_t replacements;  // PCG restarts from the true residual (-1 without replacement)
+    int32_t refine;        // 1 while the replacement step may still restart this problem
 };
 
 enum BatchState : int32_t { kBActive = 0, kBConverging = 1, kBSuccess = 2, kBFailed = 3 };
 
 // finalize operations of the batch
-enum BatchFinOp : int { kBFinNu0 = 0, kBFinRho = 1, kBFinRhoNew = 2, kBFinSigma = 3, kBFinNuFirst = 4, kBFinNuIter = 5 };
+enum BatchFinOp : int {
+    kBFinNu0 = 0,
+    kBFinRho = 1,
+    kBFinRhoNew = 2,
+    kBFinSigma = 3,
+    kBFinNuFirst = 4,
+    kBFinNuIter = 5,
+    kBFinReplace = 6,   // FP64 true residual: restart problems whose true residual misses the tolerance
+};
+constexpr int kBMaxReplacements = 4;
 
 // partials of the level-0 reductions, one per block (no finalize)
 hipError_t launch_absmax_partials(const LevelView& L, const float* r, double* partials, hipStream_t s);
+// r = b - A x in FP64 (stored FP32) and its partial max|r|
+hipError_t launch_residual64_partials(const LevelView& L, const float* x, const float* b, float* r, double* partials,
+                                      hipStream_t s);
 hipError_t launch_dot_partials(const LevelView& L, const float* a, const float* b, double* partials, hipStream_t s);
 hipError_t launch_laplacian_dot_partials(const LevelView& L, const float* p, float* q, double* partials,
                                          hipStream_t s);

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -582,11 +583,16 @@ int PoissonBatch::solve(float* const* x, const float* const* b, int* status, int
         }
         graphs_ready_ = true;
     }
+    const bool refine = mixed_ || (opt_.flags & 64);
     for (int k = 0; k < K; ++k) {
         BatchPcg& s = st_host_[size_t(k)];
         s = BatchPcg{};
         s.tol = opt_.relative_tolerance;
         s.max_iters = opt_.max_iterations;
+        s.true_nu = -1.f;
+        s.prev_true = INFINITY;
+        s.replacements = refine ? 0 : -1;
+        s.refine = refine ? 1 : 0;
     }
     check(hipMemcpyAsync(st_, st_host_.data(), size_t(K) * sizeof(BatchPcg), hipMemcpyHostToDevice, stream_),
           "upload state");
@@ -599,23 +605,34 @@ int PoissonBatch::solve(float* const* x, const float* const* b, int* status, int
     int active = 0;
     if (run(g_init_, [&] { issue_init(); })) return -3;
     if (wait_active(&active)) return -3;
-    int k = 0;
-    if (active > 0) {
-        if (run(g_first_, [&] { issue_first(); })) return -3;
-        // every piece is followed by its status wait (the sequence numbers stay in step)
-        for (int pieces = 0;; ++pieces) {
-            if (wait_active(&active)) return -3;
-            if (active == 0) break;
-            if (pieces > opt_.max_iterations) {   // the per-problem rules end every problem by then
-                err_ = "batched PCG did not terminate";
-                return -3;
+    // recurrences: the first from the initial residual, then (residual
+    // replacement) restarts from the FP64 true residual of the problems whose
+    // true residual misses the tolerance
+    for (int round = 0; round <= kBMaxReplacements; ++round) {
+        int k = 0;
+        if (active > 0) {
+            if (run(g_first_, [&] { issue_first(); })) return -3;
+            // every piece is followed by its status wait (the sequence numbers stay in step)
+            for (int pieces = 0;; ++pieces) {
+                if (wait_active(&active)) return -3;
+                if (active == 0) break;
+                if (pieces > opt_.max_iterations) {   // the per-problem rules end every problem by then
+                    err_ = "batched PCG did not terminate";
+                    return -3;
+                }
+                if (run(g_iter_[k], [&] { issue_iter(k); })) return -3;
+                k ^= 1;
             }
-            if (run(g_iter_[k], [&] { issue_iter(k); })) return -3;
-            k ^= 1;
+            // problems that converged in the last piece: x += alpha p, then commit
+            check(launch_bupdate_x(lv_[0].v, leafprob_, st_, p_[k], x0_, stream_), "update x");
+            check(launch_bfinalize(-1, K, blk_begin_, partials_, st_, stream_), "commit");
         }
-        // problems that converged in the last piece: x += alpha p, then commit
-        check(launch_bupdate_x(lv_[0].v, leafprob_, st_, p_[k], x0_, stream_), "update x");
-        check(launch_bfinalize(-1, K, blk_begin_, partials_, st_, stream_), "commit");
+        if (!refine) break;
+        check(launch_residual64_partials(lv_[0].v, x0_, b0_, r0_, partials_, stream_), "residual64");
+        check(launch_bfinalize(kBFinReplace, K, blk_begin_, partials_, st_, stream_), "replace");
+        check(launch_bstatus(K, st_, seq_dev_, mailbox_dev_, stream_), "status");
+        if (wait_active(&active)) return -3;
+        if (active == 0) break;
     }
     for (int kk = 0; kk < K; ++kk) {
         const int i = order_[size_t(kk)];
@@ -631,6 +648,8 @@ int PoissonBatch::solve(float* const* x, const float* const* b, int* status, int
         if (status) status[i] = s.state == kBSuccess ? 0 : 1;
         if (iterations) iterations[i] = s.iters;
     }
+    last_.resize(size_t(K));
+    for (int kk = 0; kk < K; ++kk) last_[size_t(order_[size_t(kk)])] = st_host_[size_t(kk)];
     return first_error_ == hipSuccess ? 0 : -3;
 }
 

@@ -54,6 +54,10 @@ public:
     // guess in, solution out). status[i] = 0 SUCCESS / 1 FAILED, iterations[i]
     // = its mIterationTaken. Returns 0 or a negative uaamg status.
     int solve(float* const* x, const float* const* b, int* status, int* iterations);
+    // after a solve with residual replacement (BF16 W-cycle, or flags bit 6):
+    // input problem i's FP64 true residual and restart count (-1 without)
+    float last_true_residual(int i) const { return i < int(last_.size()) ? last_[size_t(i)].true_nu : -1.f; }
+    int last_replacements(int i) const { return i < int(last_.size()) ? last_[size_t(i)].replacements : -1; }
 
 private:
     struct Level {
@@ -99,6 +103,7 @@ private:
     CoarsestView* views_ = nullptr;       // device array, merged order
     BatchPcg* st_ = nullptr;
     std::vector<BatchPcg> st_host_;
+    std::vector<BatchPcg> last_;         // input order, after the last solve
     unsigned int* seq_dev_ = nullptr;
     double* partials_ = nullptr;
     float *x0_ = nullptr, *b0_ = nullptr, *r0_ = nullptr, *z0_ = nullptr, *q0_ = nullptr;

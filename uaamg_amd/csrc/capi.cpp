@@ -528,6 +528,15 @@ int uaamg_poisson_batch_info(const uaamg_poisson_batch* h, int32_t* nproblems, i
 
 int64_t uaamg_poisson_batch_ndof(const uaamg_poisson_batch* h, int32_t i) { return h ? h->b.ndof(i) : -1; }
 
+int uaamg_poisson_batch_last_residual(const uaamg_poisson_batch* h, float* true_residual_inf, int32_t* replacements) {
+    if (!h) return fail(UAAMG_EINVAL, "null handle");
+    for (int i = 0; i < h->b.nproblems(); ++i) {
+        if (true_residual_inf) true_residual_inf[i] = h->b.last_true_residual(i);
+        if (replacements) replacements[i] = h->b.last_replacements(i);
+    }
+    return 0;
+}
+
 int uaamg_poisson_batch_solve_device(uaamg_poisson_batch* h, float* const* x_dev, const float* const* b_dev,
                                      int32_t* status, int32_t* iterations) {
     UAAMG_GUARD_BEGIN

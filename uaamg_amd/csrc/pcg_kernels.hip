@@ -469,7 +469,7 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
     BatchPcg& sp = st[prob];
     const int b0 = blk_begin[prob], n = blk_begin[prob + 1] - b0;
     const double* part = partials + b0;
-    if (op == kBFinNu0 || op == kBFinNuFirst || op == kBFinNuIter) {
+    if (op == kBFinNu0 || op == kBFinNuFirst || op == kBFinNuIter || op == kBFinReplace) {
         __shared__ float tv[kFinThreads];
         __shared__ int tb[kFinThreads];
         float v = 0.f;
@@ -494,7 +494,23 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
             sp.nu = sp.nu_prev = nu;
             sp.numax = sp.tol * nu;
             sp.iters = 0;
+            sp.it0 = 0;
             sp.state = nu <= sp.numax ? kBSuccess : (sp.max_iters > 0 ? kBActive : kBFailed);
+            return;
+        }
+        if (op == kBFinReplace) {   // the restart rule of solve_pcg_dev's residual replacement
+            if (sp.state != kBSuccess || !sp.refine) return;
+            sp.true_nu = nu;
+            if (nu <= sp.numax || !(nu < 0.5f * sp.prev_true) || sp.replacements >= kBMaxReplacements ||
+                sp.iters >= sp.max_iters) {
+                sp.refine = 0;
+                return;
+            }
+            sp.prev_true = nu;
+            sp.replacements += 1;
+            sp.it0 = sp.iters;
+            sp.nu = sp.nu_prev = nu;
+            sp.state = kBActive;
             return;
         }
         if (sp.state != kBActive) return;
@@ -507,7 +523,7 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
             }
         }
         if (nu <= sp.numax) sp.state = kBConverging;
-        else if (nu > sp.nu_prev && sp.iters > 3) sp.state = kBFailed;
+        else if (nu > sp.nu_prev && sp.iters - sp.it0 > 3) sp.state = kBFailed;
         sp.nu_prev = nu;
         return;
     }
@@ -671,6 +687,13 @@ hipError_t launch_absmax_partials(const LevelView& L, const float* r, double* pa
     if (L.nleaf <= 0) return hipSuccess;
     absmax_kernel<false><<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, nullptr, const_cast<float*>(r), nullptr,
                                                                      partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual64_partials(const LevelView& L, const float* x, const float* b, float* r, double* partials,
+                                      hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    residual64_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, x, b, r, partials);
     return hipGetLastError();
 }
 
