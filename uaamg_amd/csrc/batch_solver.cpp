@@ -65,7 +65,7 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
     }
     mixed_ = opt_.precision == 1;
     use_graphs_ = !(opt_.flags & 2);
-    block_max_leaves_ = (opt_.flags & 16) ? 0 : 512;
+    block_max_leaves_ = (opt_.flags & 16) ? 0 : PoissonDevice::kTuneMaxLeaves;
     if (const char* env = std::getenv("UAAMG_BLOCK_LEAVES")) block_max_leaves_ = std::atoi(env);
     for (const auto& p : probs) {
         if (p.levels.empty() || p.levels[0].ndof == 0) {
@@ -349,7 +349,84 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
         return -3;
     if (block_max_leaves_ > 0 && check(block_kernels_init(), "block kernel attributes")) return -3;
     st_host_.assign(size_t(K), BatchPcg{});
+    // level-to-kernel mapping: the faster of block-visit and per-sweep launches
+    // for each merged coarse level (PoissonDevice::tune_levels), unless
+    // UAAMG_BLOCK_LEAVES fixes a leaf-count threshold
+    use_block_.assign(size_t(L), 0);
+    for (int l = 1; l < L; ++l)
+        use_block_[size_t(l)] = lv_[size_t(l)].sv.nleaf > 0 && int64_t(lv_[size_t(l)].sv.nleaf) <= block_max_leaves_;
+    if (block_max_leaves_ > 0 && !std::getenv("UAAMG_BLOCK_LEAVES")) tune_levels();
     return check(hipStreamSynchronize(stream_), "init sync");
+}
+
+void PoissonBatch::visit_pair(int l, bool block) {
+    Level& d = lv_[size_t(l)];
+    Level& c = lv_[size_t(l + 1)];
+    const LevelView& v = d.sv;
+    const bool weighted = opt_.smoother == kWeightedJacobi;
+    auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
+    auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    for (int first = 1; first >= 0; --first) {
+        if (block) {
+            if (mixed_) {
+                check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                check(launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, wpost(0), wpost(1), wpost(2), stream_), "tune");
+            } else {
+                check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                check(launch_block_post(v, d.T, c.X, d.B, d.X, wpost(0), wpost(1), wpost(2), stream_), "tune");
+            }
+        } else if (mixed_) {
+            if (first) {
+                check(launch_jacobi_first2(v, d.Bh, d.Th, wpre(0), wpre(1), stream_), "tune");
+                check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(2), stream_), "tune");
+            } else {
+                check(launch_jacobi(v, d.Xh, d.Bh, d.Th, wpre(0), stream_), "tune");
+                check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(1), stream_), "tune");
+                check(launch_jacobi(v, d.Uh, d.Bh, d.Th, wpre(2), stream_), "tune");
+            }
+            check(launch_residual_restrict(v, d.Uh, d.Bh, c.Bh, stream_), "tune");
+            check(launch_prolong_jacobi(v, d.Uh, c.Xh, d.Bh, d.Th, wpost(0), stream_), "tune");
+            check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpost(1), stream_), "tune");
+            check(launch_jacobi(v, d.Uh, d.Bh, d.Xh, wpost(2), stream_), "tune");
+        } else {
+            if (first) {
+                check(launch_jacobi_first2(v, d.B, d.T, wpre(0), wpre(1), stream_), "tune");
+                check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpre(2), stream_), "tune");
+            } else {
+                check(launch_apply(v, kModeJacobi, d.X, d.B, d.T, wpre(0), stream_), "tune");
+                check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpre(1), stream_), "tune");
+                check(launch_apply(v, kModeJacobi, d.U, d.B, d.T, wpre(2), stream_), "tune");
+            }
+            check(launch_residual_restrict(v, d.U, d.B, c.B, stream_), "tune");
+            check(launch_prolong_jacobi(v, d.U, c.X, d.B, d.T, wpost(0), stream_), "tune");
+            check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpost(1), stream_), "tune");
+            check(launch_apply(v, kModeJacobi, d.U, d.B, d.X, wpost(2), stream_), "tune");
+        }
+    }
+}
+
+// Times both mappings of every candidate level on the (all-zero) scratch
+// vectors with HIP events and keeps the faster; zero in, zero out.
+void PoissonBatch::tune_levels() {
+    const int L = nlevels();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return;
+    constexpr int kReps = 8;
+    for (int l = 1; l + 1 < L; ++l) {
+        if (!use_block_[size_t(l)]) continue;
+        float ms[2] = {0.f, 0.f};
+        for (int variant = 0; variant < 2; ++variant) {
+            visit_pair(l, variant == 0);
+            check(hipEventRecord(e0, stream_), "event");
+            for (int r = 0; r < kReps; ++r) visit_pair(l, variant == 0);
+            check(hipEventRecord(e1, stream_), "event");
+            check(hipEventSynchronize(e1), "event");
+            check(hipEventElapsedTime(&ms[variant], e0, e1), "event");
+        }
+        use_block_[size_t(l)] = ms[0] < ms[1] ? 1 : 0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 // ------------------------------------------------------------ W-cycle
@@ -368,7 +445,7 @@ void PoissonBatch::precond(int l, float* X, const float* B, bool first, int dot_
         auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
         auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
         Level& c = lv_[size_t(l + 1)];
-        if (l >= 1 && int64_t(v.nleaf) <= block_max_leaves_) {
+        if (l >= 1 && use_block_[size_t(l)]) {
             check(launch_block_pre(v, first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
             precond(l + 1, c.X, c.B, true, -1);
             if (c.sv.nleaf > 0) precond(l + 1, c.X, c.B, false, -1);
@@ -426,7 +503,7 @@ void PoissonBatch::precond_mixed(int l, TX* X, const TB* B, bool first, int dot_
         };
         bool done = false;
         if constexpr (std::is_same_v<TX, bf16>) {
-            if (l >= 1 && int64_t(v.nleaf) <= block_max_leaves_) {
+            if (l >= 1 && use_block_[size_t(l)]) {
                 check(launch_block_pre(v, first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "block pre");
                 coarse();
                 check(launch_block_post(v, d.Th, c.Xh, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");

@@ -84,6 +84,8 @@ private:
     int run(Graph& g, const std::function<void()>& body);
     int wait_active(int* active);
     void destroy_graphs();
+    void tune_levels();
+    void visit_pair(int level, bool block);
 
     SolverOptions opt_;
     hipStream_t stream_ = nullptr;
@@ -112,7 +114,8 @@ private:
     bool graphs_ready_ = false;
     bool use_graphs_ = true;
     bool mixed_ = false;
-    int64_t block_max_leaves_ = 512;
+    int64_t block_max_leaves_ = 0;
+    std::vector<uint8_t> use_block_;      // per merged level: visits as two block-visit launches
     HostMailbox* mailbox_ = nullptr;
     HostMailbox* mailbox_dev_ = nullptr;
     unsigned int seq_ = 0;
