@@ -29,6 +29,20 @@ METRIC = "ms/Poisson solve to 1e-7 rel. residual @256³ vs plain FP32; solves/s 
 HBM_PEAK_GBS = 8000.0
 
 
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/hbm_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled
+    by the copy-kernel calibration), or None when it has not been measured."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    entry = table.get("kernels", {}).get(kernel)
+    return None if entry is None else entry.get("bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,13 +115,17 @@ def main() -> int:
     iters = None
     if args.kernel_only:
         ms = lap.time_apply(0, 2, reps=args.kernel_reps)
+        lap_k = U.LaplacianWithLevel(scene.phi, scene.weights, dt=1.0, dx=scene.dx, device=0,
+                                     precision=U.PRECISION_BF16_CYCLE)
+        ms_bf16 = lap_k.time_apply(0, 6, reps=args.kernel_reps)
         src = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         for _ in range(10):
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"kernel_only": True, "jacobi_ms": ms, "bytes_per_launch": lap.algorithmic_bytes(0, 2),
-                          "copy_bytes_each_way": src.numel() * 4, "copies": 10, "iters": iters}), flush=True)
+                          "jacobi_bf16_ms": ms_bf16, "bf16_bytes_per_launch": lap_k.algorithmic_bytes(0, 6),
+                          "copy_bytes_each_way": src.numel() * 4, "copies": 10}), flush=True)
         return 0
     def timed_solves(l, x):
         """warm-up then timed solves from a zero guess; (ms/solve, latencies ms, iterations)"""
@@ -146,14 +164,19 @@ def main() -> int:
     x_gpu = x_mx.cpu().numpy()
     ms_per_solve, lat_ms, iters = ms_mx, lat_mx, it_mx
 
-    # dominant kernel: level-0 smoother sweep (scheduled Jacobi)
-    jac_bytes = lap.algorithmic_bytes(0, 2)
-    jac_ms = lap.time_apply(0, 2, reps=args.kernel_reps)
-    achieved = jac_bytes / (jac_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "apply_kernel<2, true> (level-0 weighted Jacobi sweep)",
-                "bytes_per_launch": jac_bytes, "ms_per_launch": round(jac_ms, 5)}
+    # dominant operator kernel of the measured (mixed) form: the level-0
+    # weighted Jacobi sweep with BF16 x / y and the FP32 residual as b; the
+    # FP32 form's sweep beside it
+    def roof(l, mode, name):
+        nbytes = l.algorithmic_bytes(0, mode)
+        ms = l.time_apply(0, mode, reps=args.kernel_reps)
+        ach = nbytes / (ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
+                "kernel": name, "bytes_per_launch": nbytes, "ms_per_launch": round(ms, 5)}
+
+    roofline = roof(lap_mx, 6, "apply_kernel<2, true, bf16, float, bf16> (level-0 Jacobi sweep, BF16 W-cycle)")
+    roofline_fp32 = roof(lap, 2, "apply_kernel<2, true> (level-0 weighted Jacobi sweep, FP32)")
 
     info = [lap.level_info(l) for l in range(lap.nlevels)]
     # one Jacobi sweep per level, back-to-back launches (the coarse levels show launch latency)
@@ -193,6 +216,7 @@ def main() -> int:
         "build_s_mixed": round(t_build_mx, 3),
         "scene_s": round(t_scene, 3),
         "roofline": roofline,
+        "roofline_fp32": roofline_fp32,
     }
 
     if not args.no_cpu_baseline:

@@ -53,6 +53,7 @@ extern "C" {
 #define UAAMG_MODE_SPAI0 3
 #define UAAMG_MODE_RED_GS 4
 #define UAAMG_MODE_BLACK_GS 5
+#define UAAMG_MODE_JACOBI_BF16 6 /* algorithmic_bytes / time_apply only: the BF16 W-cycle's Jacobi sweep */
 
 /* coefficient kinds of a level */
 #define UAAMG_COEF_DIAGONAL 0

@@ -395,7 +395,7 @@ int uaamg_poisson_coarsest_solve(uaamg_poisson* h, const float* b, float* x) {
 }
 
 int uaamg_poisson_algorithmic_bytes(const uaamg_poisson* h, int level, int mode, double* bytes) {
-    if (!h || !bytes || level < 0 || level >= h->dev.nlevels() || mode < 0 || mode > 5)
+    if (!h || !bytes || level < 0 || level >= h->dev.nlevels() || mode < 0 || mode > UAAMG_MODE_JACOBI_BF16)
         return fail(UAAMG_EINVAL, "bad argument");
     *bytes = h->dev.algorithmic_bytes(level, mode);
     return 0;

@@ -29,6 +29,7 @@ enum ApplyMode : int {
     kModeSpai0 = 3,
     kModeRedGS = 4,
     kModeBlackGS = 5,
+    kModeJacobiBf16 = 6,   // measurement only: the weighted Jacobi sweep of the BF16-stored W-cycle
 };
 
 struct LevelView {

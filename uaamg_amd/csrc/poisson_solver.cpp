@@ -1072,6 +1072,7 @@ double PoissonDevice::algorithmic_bytes(int l, int mode) const {
             case kModeResidual: per = 12 + c; break;
             case kModeJacobi: per = 12 + c + d; break;
             case kModeSpai0: per = 12 + c + 4.0 * ks; break;
+            case kModeJacobiBf16: per = (l == 0 ? 8 : 6) + c + d; break;   // x, y at 2 B; b at 4 B on level 0
             default: per = 12 + 4.0 * (kx + ky + kz) + d; break;   // one GS colour pass
         }
         total += nd * per;
@@ -1080,16 +1081,24 @@ double PoissonDevice::algorithmic_bytes(int l, int mode) const {
 }
 
 int PoissonDevice::time_apply(int l, int mode, int reps, float* ms) {
-    if (!on_device() || l < 0 || l >= nlevels() || reps <= 0 || mode < 0 || mode > 5) return -1;
+    if (!on_device() || l < 0 || l >= nlevels() || reps <= 0 || mode < 0 || mode > kModeJacobiBf16) return -1;
+    if (mode == kModeJacobiBf16 && !mixed_) return -1;
     const LevelView& v = lv_[l].v;
     const float* X = l == 0 ? x0_ : lv_[l].X;
     const float* B = l == 0 ? b0_ : lv_[l].B;
     float* Y = staging_[2];
+    DevLevel& d = lv_[l];
+    // the BF16 sweep as the mixed W-cycle runs it: level 0 reads the FP32 residual
+    auto launch = [&]() {
+        if (mode != kModeJacobiBf16) return launch_apply(v, mode, X, B, Y, 0.8571f, stream_);
+        if (l == 0) return launch_jacobi(v, d.Th, b0_, d.Uh, 0.8571f, stream_);
+        return launch_jacobi(v, d.Th, d.Bh, d.Uh, 0.8571f, stream_);
+    };
     hipEvent_t e0, e1;
     if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -3;
-    check(launch_apply(v, mode, X, B, Y, 0.8571f, stream_), "warm");
+    check(launch(), "warm");
     check(hipEventRecord(e0, stream_), "event");
-    for (int i = 0; i < reps; ++i) check(launch_apply(v, mode, X, B, Y, 0.8571f, stream_), "apply");
+    for (int i = 0; i < reps; ++i) check(launch(), "apply");
     check(hipEventRecord(e1, stream_), "event");
     check(hipEventSynchronize(e1), "event sync");
     float total = 0.f;
