@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
     uint8_t* m = reinterpret_cast<uint8_t*>(cz + N);
     __shared__ int nl[27], sd[27], sx[27], sy[27], sz[27], par[27];
 
-    const int leaf = blockIdx.x;
+    const int leaf = logical_block();
     const int tid = threadIdx.x;
     if (tid < 27) {
         const int n = L.nb27[size_t(leaf) * 27 + tid];

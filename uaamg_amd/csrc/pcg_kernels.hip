@@ -46,8 +46,10 @@ __device__ __forceinline__ double block_sum(double v) {
     return t;
 }
 
+// partial of the block's leaf range (indexed by the logical block, so the
+// finalize sums in leaf order whatever the dispatch order)
 __device__ __forceinline__ void write_partial(double* partials, double v) {
-    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    if (threadIdx.x == 0) partials[logical_block()] = v;
 }
 
 template <int TARGET>
@@ -183,7 +185,7 @@ __device__ __forceinline__ void write_absmax_partial(float m, float bad, double*
             if (wm[i] < 0.f) b = true;
             else v = fmaxf(v, wm[i]);
         }
-        partials[blockIdx.x] = b ? -1.0 : double(v);
+        partials[logical_block()] = b ? -1.0 : double(v);
     }
 }
 

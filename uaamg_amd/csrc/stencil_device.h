@@ -9,12 +9,13 @@
 #pragma once
 
 #include "poisson_kernels.h"
+#include "xcd_map.h"
 
 namespace uaamg {
 namespace dev {
 
 __device__ __forceinline__ int wave_leaf() {
-    int leaf = int(blockIdx.x) * kWavesPerBlock + int(threadIdx.x >> 6);
+    int leaf = logical_block() * kWavesPerBlock + int(threadIdx.x >> 6);
     return __builtin_amdgcn_readfirstlane(leaf);
 }
 

@@ -9,6 +9,7 @@
 
 #include "viscosity_host.h"
 #include "viscosity_kernels.h"
+#include "xcd_map.h"
 
 namespace uaamg {
 namespace {
@@ -249,7 +250,7 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
 template <int MODE, int C>
 __global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, const float* Xv, const float* Bv,
                                                                float* Yv) {
-    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
+    const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= L.chan_off[C + 1] - L.chan_off[C]) return;
     visc_row<MODE, C>(L, L.chan_leaves[L.chan_off[C] + w], threadIdx.x & 63, Xv, Bv, Yv);
 }
@@ -373,14 +374,14 @@ __global__ __launch_bounds__(kSmallThreads) void visc_small_post_kernel(ViscLeve
 // restriction: coarse (leaf, channel) waves
 __global__ __launch_bounds__(kVThreads) void visc_restrict_kernel(ViscLevelView Cl, ViscLevelView F, const float* fine,
                                                                   float* coarse) {
-    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
+    const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= Cl.nleaf_dof * 3) return;
     visc_restrict_row(Cl, F, w / 3, w - 3 * (w / 3), threadIdx.x & 63, fine, coarse);
 }
 
 // prolongation<true>: fine (leaf, channel) waves add the parent coarse value
 __global__ __launch_bounds__(kVThreads) void visc_prolong_kernel(ViscLevelView F, const float* coarse, float* fine) {
-    const int w = blockIdx.x * kVWaves + (threadIdx.x >> 6);
+    const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= F.nleaf_dof * 3) return;
     visc_prolong_row(F, w / 3, w - 3 * (w / 3), threadIdx.x & 63, coarse, fine);
 }
