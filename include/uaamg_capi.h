@@ -148,6 +148,15 @@ int uaamg_poisson_level_topology(const uaamg_poisson* h, int level, int32_t* ori
 int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind, float* bricks, uint8_t* kept);
 
 /*
+ * LaplacianWithLevel::getTriplets (simd_vdb_poisson.cpp:129-206) of a level: *count receives the
+ * number of triplets; when rows / cols / vals are non-NULL they receive them (row DOF, column DOF,
+ * value) in the reference's order - DOFs ascending, per DOF the diagonal, then per axis the lower
+ * and the upper neighbour where that neighbour is a DOF.
+ */
+int uaamg_poisson_level_triplets(const uaamg_poisson* h, int level, int64_t* count, int32_t* rows, int32_t* cols,
+                                 float* vals);
+
+/*
  * b = -(1/dx) sum_faces +-(w u + (1-w) u_solid), DOF-ordered (ndof floats).
  * Replaces LaplacianWithLevel::createPressurePoissonRightHandSide (simd_vdb_poisson.h:59-65, .cpp:101-127).
  * solid_velocity may be NULL (zero).

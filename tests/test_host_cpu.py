@@ -120,3 +120,28 @@ def test_scene_is_deterministic():
     b = U.Scene(3, 0, seed=5)
     assert a.n == b.n and 32 <= a.n <= 192
     assert np.array_equal(a.phi.values, b.phi.values) and np.array_equal(a.vx.values, b.vx.values)
+
+
+@pytest.mark.parametrize("cfg,n,seed", [(1, 48, 1), (3, 0, 7)])
+def test_triplets_are_the_level_operator(cfg, n, seed):
+    """getTriplets of every level, as a sparse matrix in FP64, applies like the
+    oracle's FP64 operator on level 0 and like its FP32 operator on every level
+    (to FP32 rounding); it is symmetric, and each row lists its diagonal first."""
+    import scipy.sparse as sp
+    s = U.Scene(cfg, n, seed=seed)
+    lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, device=-1)
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    rng = np.random.default_rng(3)
+    for l in range(lap.nlevels):
+        r, c, v = lap.getTriplets(l)
+        nd = lap.level_ndof(l)
+        first = np.r_[True, r[1:] != r[:-1]]
+        assert np.array_equal(r[first], np.arange(nd)) and np.array_equal(c[first], np.arange(nd))
+        A = sp.csr_matrix((v.astype(np.float64), (r, c)), shape=(nd, nd))
+        assert abs(A - A.T).max() == 0.0, f"level {l} not symmetric"
+        x = rng.uniform(-1, 1, nd).astype(np.float32)
+        y = A @ x.astype(np.float64)
+        y32 = orc.apply(l, 0, x)
+        assert np.allclose(y, y32, rtol=0, atol=1e-5 * np.abs(y).max()), f"level {l}"
+        if l == 0:
+            assert np.allclose(y, orc.apply64(x.astype(np.float64)), rtol=1e-12, atol=1e-12 * np.abs(y).max())

@@ -107,6 +107,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_solve_pcg_device": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_last_residual": (C.c_int, [vp, P(f32), P(i32)]),
         "uaamg_poisson_kernel_map": (C.c_int, [vp, C.c_int, P(i32), P(f32), P(f32)]),
+        "uaamg_poisson_level_triplets": (C.c_int, [vp, C.c_int, P(i64), vp, vp, vp]),
         "uaamg_poisson_solve_mg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_batch_device": (C.c_int, [vp, i32, vp, vp, vp, vp, i32]),
         "uaamg_poisson_batch_create": (C.c_int, [i32, vp, vp, vp, vp, P(_Options), P(vp)]),
@@ -472,6 +473,16 @@ class LaplacianWithLevel:
         rc = _check(lib().uaamg_poisson_solve_pcg_device(self._h, C.c_void_p(x_ptr), C.c_void_p(b_ptr),
                                                          C.byref(it)), "solve_pcg_device")
         return rc, int(it.value)
+
+    def getTriplets(self, level: int = 0):
+        """(rows, cols, vals) of the level's explicit matrix in getTriplets order."""
+        n = C.c_int64()
+        _check(lib().uaamg_poisson_level_triplets(self._h, level, C.byref(n), None, None, None), "triplets")
+        r = np.zeros(n.value, np.int32)
+        c = np.zeros(n.value, np.int32)
+        v = np.zeros(n.value, np.float32)
+        _check(lib().uaamg_poisson_level_triplets(self._h, level, C.byref(n), _ptr(r), _ptr(c), _ptr(v)), "triplets")
+        return r, c, v
 
     def kernel_map(self) -> list[dict]:
         """Per level: whether visits run as block-visit launches, and the timings behind the choice."""

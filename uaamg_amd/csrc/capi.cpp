@@ -191,6 +191,21 @@ int uaamg_poisson_set_options(uaamg_poisson* h, const uaamg_options* o) {
 
 int64_t uaamg_poisson_ndof(const uaamg_poisson* h) { return h ? h->ndof : -1; }
 
+int uaamg_poisson_level_triplets(const uaamg_poisson* h, int level, int64_t* count, int32_t* rows, int32_t* cols,
+                                 float* vals) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !count || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad argument");
+    std::vector<int32_t> r, c;
+    std::vector<float> v;
+    level_triplets(h->dev.host().levels[size_t(level)], r, c, v);
+    *count = int64_t(r.size());
+    if (rows) std::memcpy(rows, r.data(), r.size() * sizeof(int32_t));
+    if (cols) std::memcpy(cols, c.data(), c.size() * sizeof(int32_t));
+    if (vals) std::memcpy(vals, v.data(), v.size() * sizeof(float));
+    return 0;
+    UAAMG_GUARD_END
+}
+
 int uaamg_poisson_kernel_map(const uaamg_poisson* h, int level, int32_t* block, float* ms_block, float* ms_sweep) {
     if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
     const auto& m = h->dev.block_levels();

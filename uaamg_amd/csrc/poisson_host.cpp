@@ -447,6 +447,53 @@ void build_spai0(PoissonHierarchyHost& h) {
     h.spai0_ready = true;
 }
 
+void level_triplets(const PoissonLevelHost& l, std::vector<int32_t>& rows, std::vector<int32_t>& cols,
+                    std::vector<float>& vals) {
+    rows.clear();
+    cols.clear();
+    vals.clear();
+    auto dof_at = [&](const Coord& g) -> int64_t {
+        const int32_t li = l.lookup.find(leaf_origin_of(g));
+        if (li < 0) return -1;
+        const int off = voxel_offset(g.x & 7, g.y & 7, g.z & 7);
+        const uint64_t* m = &l.mask[size_t(li) * 8];
+        if (!mask_test(m, off)) return -1;
+        int64_t k = l.dofbase[size_t(li)];
+        for (int o = 0; o < off; ++o) k += mask_test(m, o);
+        return k;
+    };
+    // an accessor read: the leaf's effective value, the default where no leaf exists
+    auto coef_at = [&](int kind, const Coord& g) -> float {
+        const int32_t li = l.lookup.find(leaf_origin_of(g));
+        if (li < 0) return l.default_coef(kind);
+        return l.coef[kind][size_t(li) * kLeafSize + size_t(voxel_offset(g.x & 7, g.y & 7, g.z & 7))];
+    };
+    auto put = [&](int64_t r, int64_t c, float v) {
+        rows.push_back(int32_t(r));
+        cols.push_back(int32_t(c));
+        vals.push_back(v);
+    };
+    for (size_t i = 0; i < l.nleaf(); ++i) {
+        int64_t k = l.dofbase[i];
+        const Coord o = l.origin[i];
+        for (int off = 0; off < kLeafSize; ++off) {
+            if (!mask_test(&l.mask[i * 8], off)) continue;
+            const Coord g{o.x + (off >> 6), o.y + ((off >> 3) & 7), o.z + (off & 7)};
+            put(k, k, l.coef[kCoefDiag][i * kLeafSize + size_t(off)]);
+            for (int c = 0; c < 3; ++c) {
+                Coord dn = g, up = g;
+                dn[c] -= 1;
+                up[c] += 1;
+                const int64_t kd = dof_at(dn);
+                if (kd >= 0) put(k, kd, coef_at(kCoefX + c, g));
+                const int64_t ku = dof_at(up);
+                if (ku >= 0) put(k, ku, coef_at(kCoefX + c, up));
+            }
+            ++k;
+        }
+    }
+}
+
 void build_poisson_rhs(const PoissonLevelHost& l0, const HostGrid& weights, const HostGrid& vx, const HostGrid& vy,
                        const HostGrid& vz, const HostGrid& solid_vel, float* out_rhs) {
     const HostGrid* vel[3] = {&vx, &vy, &vz};

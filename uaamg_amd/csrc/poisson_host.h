@@ -79,6 +79,14 @@ void build_spai0(PoissonHierarchyHost& h);
 void build_poisson_rhs(const PoissonLevelHost& l0, const HostGrid& weights, const HostGrid& vx, const HostGrid& vy,
                        const HostGrid& vz, const HostGrid& solid_vel, float* out_rhs);
 
+// The explicit matrix of a level as getTriplets emits it
+// (LaplacianTripletsReducer, simd_vdb_poisson.cpp:129-196): DOFs in order, and
+// per DOF (row, row, diag), then per axis x, y, z the lower neighbour DOF with
+// the coefficient of the DOF's own face and the upper neighbour DOF with the
+// coefficient of the neighbour's face, each only where that neighbour is a DOF.
+void level_triplets(const PoissonLevelHost& l, std::vector<int32_t>& rows, std::vector<int32_t>& cols,
+                    std::vector<float>& vals);
+
 // Run fn(begin, end) over [0, n) on the host worker threads.
 template <class Fn>
 void parallel_for(int64_t n, Fn&& fn);
