@@ -522,8 +522,11 @@ def run_viscosity(args, dev) -> dict:
         "iterations": it, "p50_ms": round(float(np.percentile(np.array(lat) * 1e3, 50)), 3),
         "p99_ms": round(float(np.percentile(np.array(lat) * 1e3, 99)), 3),
         "build_s": round(t_build, 3), "scene_s": round(t_scene, 3),
+        "level_kernel_map": lw.kernel_map(),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles("visc_apply_kernel<4, 0> (level-0 red Gauss-Seidel pass, "
+                                                      "channel 0)"),
                      "kernel": "visc_apply_kernel<4, 0> (level-0 red Gauss-Seidel pass, channel 0)",
                      "bytes_per_launch": red_bytes, "ms_per_launch": round(red_ms, 5)},
     }

@@ -410,6 +410,10 @@ int uaamg_viscosity_precondition(uaamg_viscosity* h, const float* r, float* z);
 
 /* Measurement hooks for one colour / operator pass over channel c (no reference counterpart). */
 int uaamg_viscosity_algorithmic_bytes(const uaamg_viscosity* h, int level, int mode, int channel, double* bytes);
+/* Level-to-kernel mapping chosen when the handle was created: *small = 1 when the level's half-visits
+ * run as one single-workgroup launch each, and the ms per half-visit pair of both mappings as timed
+ * then (-1 when not timed). No reference counterpart. */
+int uaamg_viscosity_kernel_map(const uaamg_viscosity* h, int level, int32_t* small, float* ms_small, float* ms_pass);
 int uaamg_viscosity_time_apply(uaamg_viscosity* h, int level, int mode, int channel, int reps,
                                float* ms_per_launch);
 

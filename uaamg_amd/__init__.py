@@ -152,6 +152,7 @@ def lib() -> C.CDLL:
         "uaamg_viscosity_precondition": (C.c_int, [vp, vp, vp]),
         "uaamg_viscosity_algorithmic_bytes": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_double)]),
         "uaamg_viscosity_time_apply": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(f32)]),
+        "uaamg_viscosity_kernel_map": (C.c_int, [vp, C.c_int, P(i32), P(f32), P(f32)]),
         "uaamg_visc_scene_create": (C.c_int, [C.c_int, i32, C.c_uint64, f32, f32, f32, P(vp)]),
         "uaamg_visc_scene_grids": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(f32), P(f32), P(f32)]),
         "uaamg_visc_scene_destroy": (None, [vp]),

@@ -130,6 +130,15 @@ int uaamg_viscosity_create(const uaamg_grid* viscosity, const uaamg_grid* liquid
 
 void uaamg_viscosity_destroy(uaamg_viscosity* h) { delete h; }
 
+int uaamg_viscosity_kernel_map(const uaamg_viscosity* h, int level, int32_t* small, float* ms_small, float* ms_pass) {
+    if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
+    const auto& m = h->dev.small_levels();
+    if (small) *small = level < int(m.size()) ? m[size_t(level)] : 0;
+    if (ms_small) *ms_small = h->dev.tune_ms(level, 0);
+    if (ms_pass) *ms_pass = h->dev.tune_ms(level, 1);
+    return 0;
+}
+
 int uaamg_viscosity_set_options(uaamg_viscosity* h, const uaamg_viscosity_options* o) {
     if (!h || !o) return fail(UAAMG_EINVAL, "null argument");
     if (o->max_iterations < 0) return fail(UAAMG_EINVAL, "max_iterations must be >= 0");

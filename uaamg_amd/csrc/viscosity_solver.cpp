@@ -180,12 +180,25 @@ int ViscDevice::acquire_device() {
         own_stream_ = true;
     }
     use_graphs_ = !(opt_.flags & 2);
-    // on request (flags bit 5 or UAAMG_VSMALL_LEAVES), coarse levels whose
-    // widest channel has at most this many leaves run each half-visit in one
-    // single-workgroup launch
-    small_max_leaves_ = (opt_.flags & 32) ? 16 : 0;
-    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) small_max_leaves_ = std::atoi(env);
+    // level-to-kernel mapping of the coarse levels: each half-visit as one
+    // single-workgroup launch or as the per-pass launches. By default the
+    // faster of the two, timed below for levels whose widest channel has at
+    // most kSmallTuneLeaves leaves; flags bit 5 fixes "widest channel <= 16",
+    // UAAMG_VSMALL_LEAVES=n "<= n"; flags bit 7 keeps the per-pass launches.
+    bool tune = !(opt_.flags & (32 | 128));
+    small_max_leaves_ = (opt_.flags & 128) ? 0 : ((opt_.flags & 32) ? 16 : kSmallTuneLeaves);
+    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) {
+        small_max_leaves_ = std::atoi(env);
+        tune = false;
+    }
     const int L = int(host_.levels.size());
+    use_small_.assign(size_t(L), 0);
+    for (int l = 1; l + 1 < L; ++l) {
+        const ViscLevelHost& hl = host_.levels[size_t(l)];
+        const size_t widest =
+            std::max(hl.chan_leaves[0].size(), std::max(hl.chan_leaves[1].size(), hl.chan_leaves[2].size()));
+        use_small_[size_t(l)] = int64_t(widest) <= small_max_leaves_ ? 1 : 0;
+    }
     const ViscLevelHost& l0 = host_.levels[0];
     const size_t vec0 = l0.nleaf_dof * 3 * 512 * 4;
     const size_t ndof0 = size_t(l0.ndof);
@@ -218,8 +231,55 @@ int ViscDevice::acquire_device() {
     mailbox_->word = 0;
     if (check(hipHostGetDevicePointer(reinterpret_cast<void**>(&mailbox_dev_), mailbox_, 0), "mailbox map")) return -3;
     if (int rc = build_coarsest()) return rc;
+    if (tune) tune_levels();
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
-    return 0;
+    return first_error_ == hipSuccess ? 0 : -3;
+}
+
+// One half-visit pair of level l's own work (the coarser levels' work between
+// them left out) in either mapping, on the level's zero scratch vectors (zero
+// in, zero out).
+void ViscDevice::half_visit_pair(int l, bool small) {
+    DevLevel& d = lv_[size_t(l)];
+    DevLevel& c = lv_[size_t(l + 1)];
+    if (small) {
+        check(launch_visc_small_pre(d.v, c.v, d.X, d.B, d.T, c.B, true, stream_), "tune");
+        check(launch_visc_small_post(d.v, d.X, d.B, c.X, stream_), "tune");
+        return;
+    }
+    check(launch_visc_zero(d.v, d.X, stream_), "tune");
+    for (int i = 0; i < 2; ++i) rbgs(l, d.X, d.B, true);
+    apply_all(l, kVResidual, d.X, d.B, d.T);
+    check(launch_visc_restrict(c.v, d.v, d.T, c.B, stream_), "tune");
+    check(launch_visc_prolong(d.v, c.v, c.X, d.X, stream_), "tune");
+    for (int i = 0; i < 2; ++i) rbgs(l, d.X, d.B, false);
+}
+
+void ViscDevice::tune_levels() {
+    const int L = int(lv_.size());
+    tune_ms_[0].assign(size_t(L), -1.f);
+    tune_ms_[1].assign(size_t(L), -1.f);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return;
+    constexpr int kReps = 6;
+    for (int l = 1; l + 1 < L; ++l) {
+        if (!use_small_[size_t(l)]) continue;
+        float ms[2] = {0.f, 0.f};
+        for (int variant = 0; variant < 2; ++variant) {
+            half_visit_pair(l, variant == 0);
+            check(hipEventRecord(e0, stream_), "event");
+            for (int r = 0; r < kReps; ++r) half_visit_pair(l, variant == 0);
+            check(hipEventRecord(e1, stream_), "event");
+            check(hipEventSynchronize(e1), "event");
+            check(hipEventElapsedTime(&ms[variant], e0, e1), "event");
+            ms[variant] /= kReps;
+        }
+        tune_ms_[0][size_t(l)] = ms[0];
+        tune_ms_[1][size_t(l)] = ms[1];
+        use_small_[size_t(l)] = ms[0] < ms[1] ? 1 : 0;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 int ViscDevice::upload_spai0(int level) {
@@ -272,9 +332,7 @@ void ViscDevice::mucycle(int level, float* x, const float* b, bool precon) {
         return;
     }
     const ViscLevelView& v = lv_[level].v;
-    const ViscLevelHost& hl = host_.levels[size_t(level)];
-    const size_t widest = std::max(hl.chan_leaves[0].size(), std::max(hl.chan_leaves[1].size(), hl.chan_leaves[2].size()));
-    if (level >= 1 && int64_t(widest) <= small_max_leaves_) {
+    if (level >= 1 && use_small_[size_t(level)]) {
         DevLevel& c = lv_[level + 1];
         check(launch_visc_small_pre(v, c.v, x, b, lv_[level].T, c.B, precon, stream_), "small pre");
         mucycle(level + 1, c.X, c.B, true);

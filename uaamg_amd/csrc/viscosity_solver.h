@@ -21,7 +21,8 @@ struct ViscOptions {
     int max_iterations = 100;       // m_max_iter
     int device = 0;
     int64_t max_coarsest_dof = 4000;
-    int flags = 0;                  // bit 1: no graph capture (A-B checks)
+    int flags = 0;                  // bit 1: no graph capture, bit 3: untrimmed level 0, bit 5: single-workgroup
+                                    // half-visits up to 16 leaves, bit 7: no single-workgroup half-visits
     hipStream_t stream = nullptr;
 };
 
@@ -60,6 +61,14 @@ public:
 
     float* staging(int i) { return staging_[i]; }
 
+    // level-to-kernel mapping chosen at acquire time (1 = single-workgroup
+    // half-visits) and the timings behind it (-1 where not timed)
+    const std::vector<uint8_t>& small_levels() const { return use_small_; }
+    float tune_ms(int level, int variant) const {
+        return level < int(tune_ms_[variant].size()) ? tune_ms_[variant][size_t(level)] : -1.f;
+    }
+    static constexpr int64_t kSmallTuneLeaves = 64;
+
     // bytes one pass of `mode` over channel c of `level` must move (each
     // stream once per DOF, coefficient streams only where stored)
     double algorithmic_bytes(int level, int mode, int channel) const;
@@ -91,6 +100,8 @@ private:
     void destroy_graphs();
     int run(Graph& g, const std::function<void()>& body);
     int wait_nu(float* nu);
+    void tune_levels();
+    void half_visit_pair(int level, bool small);
     int check(hipError_t e, const char* what);
 
     ViscHierarchyHost host_;
@@ -108,7 +119,9 @@ private:
     Graph g_init_, g_first_, g_iter_;
     bool graphs_ready_ = false;
     bool use_graphs_ = true;
-    int64_t small_max_leaves_ = 0;   // coarse levels up to this width use the single-workgroup half-visits
+    int64_t small_max_leaves_ = 0;   // candidate coarse levels: widest channel up to this many leaves
+    std::vector<uint8_t> use_small_;  // per level: half-visits as one single-workgroup launch
+    std::vector<float> tune_ms_[2];   // per level: ms per half-visit pair, [0] single-workgroup, [1] per-pass
     HostMailbox* mailbox_ = nullptr;
     HostMailbox* mailbox_dev_ = nullptr;
     unsigned int nu_seq_ = 0;

@@ -129,6 +129,16 @@ class L_with_level:
     def nlevels(self) -> int:
         return int(lib().uaamg_viscosity_nlevels(self._h))
 
+    def kernel_map(self) -> list[dict]:
+        """Per level: whether half-visits run as single-workgroup launches, and the timings behind it."""
+        out = []
+        for l in range(self.nlevels):
+            b, ms, mp = C.c_int32(), C.c_float(), C.c_float()
+            _check(lib().uaamg_viscosity_kernel_map(self._h, l, C.byref(b), C.byref(ms), C.byref(mp)), "kernel_map")
+            out.append({"level": l, "small": bool(b.value), "ms_small": round(ms.value, 4),
+                        "ms_pass": round(mp.value, 4)})
+        return out
+
     def level_info(self, level: int) -> dict:
         nd, nl = C.c_int64(), C.c_int64()
         ch = np.zeros(3, np.int64)
