@@ -125,12 +125,13 @@ def test_warm_start_and_max_iterations(problem):
 
 
 def test_graph_replay_bit_identical(problem):
-    """Graph replay and the single-workgroup half-visits reorder launches, not
-    arithmetic: flags 2 issues every launch directly, 32 runs small coarse
-    levels in one workgroup."""
+    """Graph replay and the level-to-kernel mapping reorder launches, not
+    arithmetic: flags 2 issues every launch directly, 32 runs coarse levels of
+    at most 16 leaves per channel in one workgroup, 128 never does (0 chooses by
+    timing)."""
     s, lw, orc, b = problem
     out = []
-    for flags in (0, 2, 32, 34):
+    for flags in (0, 2, 32, 34, 128):
         l2 = U.L_with_level(s.viscosity, s.liquid, s.solid, s.dt, s.rho, s.dx, flags=flags)
         solver = U.simd_viscosity3d(l2, s.velocity)
         x = l2.get_zero_vec()
