@@ -64,6 +64,7 @@ def main() -> int:
     ap.add_argument("--serve-precision", type=int, default=1, help="Poisson precision in the serving pool")
     ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="server memory budget (0: 90%% of free)")
     ap.add_argument("--no-viscosity", action="store_true")
+    ap.add_argument("--no-512", action="store_true", help="skip the 512^3 object")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-concurrency", type=int, default=4)
@@ -241,6 +242,8 @@ def main() -> int:
         out["fp32_plain"]["true_rel_residual_inf"] = float(np.abs(r64_32).max() / np.abs(b_orc).max())
         out["fp32_plain"]["rel_l2_vs_cpu_fp32"] = float(np.linalg.norm(x32 - x_cpu) / np.linalg.norm(x_cpu))
         out["speedup_vs_cpu"] = round(cpu_s * 1e3 / ms_per_solve, 2)
+    if not args.no_512 and args.n != 512:
+        out["poisson_512"] = run_poisson_512(args, dev)
     if not args.no_viscosity:
         out["viscosity"] = run_viscosity(args, dev)
     if not args.no_batch:
@@ -366,6 +369,45 @@ def run_serving(args, dev) -> dict:
             "pool": [{"n": p["n"], "kind": "viscosity" if p["visc"] else "poisson", "ndof": p["ndof"]}
                      for p in pool],
             "build_s": round(t_build, 2)}
+
+
+def run_poisson_512(args, dev) -> dict:
+    """Config 1 at 512^3 (about 47M DOF): ms per solve of both forms, b in HBM."""
+    import numpy as np
+    import torch
+
+    import uaamg_amd as U
+
+    sc = U.Scene(args.config, 512, seed=args.seed)
+    out = {"n": 512}
+    b = None
+    for prec, key in ((U.PRECISION_FP32, "fp32_plain"), (U.PRECISION_BF16_CYCLE, "mixed")):
+        lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=0, precision=prec)
+        s = U.PoissonSolver(lap)
+        s.mRelativeTolerance = args.tol
+        s._push_options()
+        if b is None:
+            b = torch.from_numpy(lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)).to(dev)
+            out["ndof"] = lap.mNumDof
+            out["levels"] = lap.nlevels
+        x = torch.zeros_like(b)
+        lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(3):
+            x.zero_()
+            t = time.perf_counter()
+            rc, it = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t)
+            if rc != U.SUCCESS:
+                raise RuntimeError(f"512^3 solve returned {rc}")
+        out[key] = {"ms_per_solve": round(1e3 * float(np.mean(lat)), 3), "pcg_iterations": it,
+                    "true_rel_residual_inf_fp64": (lap.last_residual()[0] / float(b.abs().max().item())
+                                                   if prec == U.PRECISION_BF16_CYCLE else None)}
+        del lap
+    out["speedup_vs_fp32_plain"] = round(out["fp32_plain"]["ms_per_solve"] / out["mixed"]["ms_per_solve"], 3)
+    return out
 
 
 def run_batch(args, dev) -> dict:
