@@ -661,6 +661,12 @@ void build_spai0(VLevel& L) {
                 set_on(out, v, den != 0 ? num / den : 0.f, 0.f);
             }
     }
+    // the level's linearised leaf pointers predate these grids
+    for (size_t i = 0; i < L.gp.size(); ++i)
+        for (int c = 0; c < 3; ++c) {
+            const Leaf<float>* lf = L.g[kSpai0 + c].probe(L.org[i]);
+            L.gp[i][kSpai0 + c] = lf ? lf->v : nullptr;
+        }
 }
 
 // linearised leaves + neighbour table for the level operator

@@ -261,6 +261,10 @@ def test_apply_modes_vs_csr(scene):
         assert np.abs(orc.apply(l, 1, x, b) - (b64 - Ax)).max() <= 1e-5 * scale
         jac = x64 + (b64 - Ax) / d
         assert np.abs(orc.apply(l, 2, x, b) - jac).max() <= 1e-5 * np.abs(jac).max()
+        # SPAI0: m_ii = a_ii / sum_j a_ij^2 over the explicit row
+        m = d / np.asarray(A.multiply(A).sum(axis=1)).ravel()
+        spai = x64 + m * (b64 - Ax)
+        assert np.abs(orc.apply(l, 3, x, b) - spai).max() <= 1e-5 * np.abs(spai).max(), f"level {l} SPAI0"
         # 6-colour Gauss-Seidel in the reference's colour order
         xyz, ch = _dof_coords(orc, l), _channel_of_dof(orc, l)
         red = (xyz.sum(axis=1) % 2) == 0

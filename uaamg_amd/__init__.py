@@ -86,6 +86,13 @@ def lib() -> C.CDLL:
     path = library_path()
     if not path.exists():
         raise UaamgError(f"{path} not built: run `python -m uaamg_amd.build` (HIP extension missing)")
+    # One HIP runtime per process: PyTorch ships its own, under the same soname.
+    # Loaded first, it is the one this library binds to; loaded after this
+    # library, it would come in beside the system runtime and see no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(path))
     i32, i64, f32, vp = C.c_int32, C.c_int64, C.c_float, C.c_void_p
     P = C.POINTER
