@@ -65,6 +65,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="server memory budget (0: 90%% of free)")
     ap.add_argument("--no-viscosity", action="store_true")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 object")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the per-kernel timing launches (for a profiled run of the solves alone)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-concurrency", type=int, default=4)
@@ -176,12 +178,16 @@ def main() -> int:
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(name),
                 "kernel": name, "bytes_per_launch": nbytes, "ms_per_launch": round(ms, 5)}
 
-    roofline = roof(lap_mx, 6, "apply_kernel<2, true, bf16, float, bf16> (level-0 Jacobi sweep, BF16 W-cycle)")
-    roofline_fp32 = roof(lap, 2, "apply_kernel<2, true> (level-0 weighted Jacobi sweep, FP32)")
+    if args.no_kernel_timing:   # a profiled run: only the solves' own launches in the trace
+        roofline = roofline_fp32 = None
+    else:
+        roofline = roof(lap_mx, 6, "apply_kernel<2, true, bf16, float, bf16> (level-0 Jacobi sweep, BF16 W-cycle)")
+        roofline_fp32 = roof(lap, 2, "apply_kernel<2, true> (level-0 weighted Jacobi sweep, FP32)")
 
     info = [lap.level_info(l) for l in range(lap.nlevels)]
     # one Jacobi sweep per level, back-to-back launches (the coarse levels show launch latency)
-    level_sweep_us = [round(1e3 * lap.time_apply(l, 2, reps=100), 2) for l in range(lap.nlevels)]
+    level_sweep_us = ([] if args.no_kernel_timing else
+                      [round(1e3 * lap.time_apply(l, 2, reps=100), 2) for l in range(lap.nlevels)])
     out = {
         "metric": METRIC,
         "value": round(ms_per_solve, 3),
