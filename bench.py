@@ -65,6 +65,8 @@ def main() -> int:
     ap.add_argument("--serve-budget-gb", type=float, default=0.0, help="server memory budget (0: 90%% of free)")
     ap.add_argument("--no-viscosity", action="store_true")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 object")
+    ap.add_argument("--profile-form", choices=["fp32", "bf16"], default=None,
+                    help="only time the solves of one Poisson form (for a kernel trace)")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="skip the per-kernel timing launches (for a profiled run of the solves alone)")
     ap.add_argument("--no-batch", action="store_true")
@@ -152,6 +154,17 @@ def main() -> int:
         torch.cuda.synchronize()
         return 1e3 * (time.perf_counter() - t_start) / args.steps, np.array(lat) * 1e3, it
 
+    if args.profile_form:
+        # a profiled run of one form's solves alone (run it with the level-to-kernel
+        # mapping pinned, UAAMG_BLOCK_LEAVES, so that no timing launches enter the trace)
+        lp = lap if args.profile_form == "fp32" else U.LaplacianWithLevel(
+            scene.phi, scene.weights, dt=1.0, dx=scene.dx, device=0, precision=U.PRECISION_BF16_CYCLE)
+        make_solver(lp)
+        ms, lat, it = timed_solves(lp, x)
+        print(json.dumps({"profile_form": args.profile_form, "n": args.n, "ms_per_solve": round(ms, 3),
+                          "solves": args.warmup + args.steps, "pcg_iterations": it,
+                          "kernel_map": lp.kernel_map()}), flush=True)
+        return 0
     # the plain FP32 form (the reference's arithmetic)
     ms32, lat32, it32 = timed_solves(lap, x)
     x32 = x.cpu().numpy()

@@ -111,7 +111,14 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
 
     // stage the region: values of absent leaves are 0 (vectors, mask) or the
     // default stencil constants (coefficients), as the per-sweep kernels read them
-    for (int q = tid; q < N; q += kBlkThreads) {
+    // unrolled so that every thread's loads are in flight together (the
+    // region's voxels are independent; one memory latency instead of one per
+    // pass of the loop)
+    constexpr int kPasses = (N + kBlkThreads - 1) / kBlkThreads;
+#pragma unroll
+    for (int pass = 0; pass < kPasses; ++pass) {
+        const int q = tid + pass * kBlkThreads;
+        if (q >= N) break;
         const int gx = q / (S * S) - H, gy = (q / S) % S - H, gz = q % S - H;
         const int ax = gx < 0 ? 0 : (gx > 7 ? 2 : 1), ay = gy < 0 ? 0 : (gy > 7 ? 2 : 1), az = gz < 0 ? 0 : (gz > 7 ? 2 : 1);
         const int k = ax * 9 + ay * 3 + az;

@@ -13,6 +13,7 @@
 #include <cfloat>
 
 #include "poisson_kernels.h"
+#include "wave_reduce.h"
 
 namespace uaamg {
 namespace dev {
@@ -23,11 +24,7 @@ struct CgShared {
     float red[2][kThreads / 64][2];
 };
 
-__device__ __forceinline__ float cg_wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
+__device__ __forceinline__ float cg_wave_sum(float v) { return wave_sum_dpp(v); }
 
 // Sums (a, b) over the block with one barrier; red alternates between two
 // buffers, so the next call needs no barrier before it writes.

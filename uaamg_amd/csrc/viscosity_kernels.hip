@@ -9,6 +9,7 @@
 
 #include "viscosity_host.h"
 #include "viscosity_kernels.h"
+#include "wave_reduce.h"
 #include "xcd_map.h"
 
 namespace uaamg {
@@ -400,11 +401,7 @@ __global__ __launch_bounds__(kVThreads) void visc_zero_kernel(float4* v, size_t 
 constexpr int kCgT = 512;
 constexpr int kCgPer = kViscMaxCoarsest / kCgT;
 
-__device__ __forceinline__ float wsum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
+__device__ __forceinline__ float wsum(float v) { return dev::wave_sum_dpp(v); }
 
 __device__ __forceinline__ float2 bsum2(float a, float b, float (*red)[kCgT / 64][2], int& par) {
     a = wsum(a);
