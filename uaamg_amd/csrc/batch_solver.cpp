@@ -191,6 +191,9 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
                 const int32_t f = H[size_t(l + 1)].child[c * 8 + size_t(b)];
                 if (f >= 0) H[size_t(l)].parent[size_t(f)] = int32_t(c * 8 + size_t(b));
             }
+    for (int l = 0; l < L; ++l)
+        complete_leaf_records(H[size_t(l)].slot.data(), H[size_t(l)].nbr.data(), H[size_t(l)].parent.data(),
+                              size_t(nleaf[size_t(l)]));
     // coarsest systems, merged order
     std::vector<std::vector<int32_t>> cpos(static_cast<size_t>(K)), cnb(static_cast<size_t>(K));
     std::vector<std::vector<float>> ccoef(static_cast<size_t>(K));

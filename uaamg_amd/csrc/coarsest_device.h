@@ -4,10 +4,12 @@
 // simd_vdb_poisson.cpp:2114-2126; Eigen ConjugateGradient.h conjugate_gradient()).
 //
 // Each thread keeps up to kPer DOFs' matrix rows and x, r, p, q in registers;
-// only p is shared through LDS for the neighbour reads. Per iteration two
-// block reductions: <p, Ap>, then ||r||^2 and <r, z> together (z is formed
-// right after r is updated; when ||r||^2 passes the stopping test z is simply
-// unused, as in Eigen's loop).
+// only p is shared through LDS for the neighbour reads. An absent neighbour
+// reads slot kMaxCoarsestDof of the shared p, which holds zero, so the six
+// reads of a row are unconditional. Per iteration two block reductions:
+// <p, Ap>, then ||r||^2 and <r, z> together (z is formed right after r is
+// updated; when ||r||^2 passes the stopping test z is simply unused, as in
+// Eigen's loop).
 #pragma once
 
 #include <cfloat>
@@ -20,7 +22,7 @@ namespace dev {
 
 template <int kThreads>
 struct CgShared {
-    float sp[kMaxCoarsestDof];
+    float sp[kMaxCoarsestDof + 1];   // the last slot stays zero
     float red[2][kThreads / 64][2];
 };
 
@@ -59,19 +61,29 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
     int nb[kPer][6];
     float cf[kPer][8], x[kPer], r[kPer], p[kPer], q[kPer];
     float part = 0.f;
+    constexpr int kZero = kMaxCoarsestDof;
+    if (tid == 0) sh.sp[kZero] = 0.f;   // published by the first reduction's barrier
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int d = tid + j * kThreads;
         x[j] = r[j] = p[j] = q[j] = 0.f;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) nb[j][k] = -1;
+        for (int k = 0; k < 6; ++k) nb[j][k] = kZero;
 #pragma unroll
         for (int k = 0; k < 8; ++k) cf[j][k] = 0.f;
         if (d < n) {
+            // rows are 24 B (nb) and 32 B (coef) at 256-byte aligned bases
+            const int2* nrow = reinterpret_cast<const int2*>(C.nb) + size_t(d) * 3;
+            const float4* crow = reinterpret_cast<const float4*>(C.coef) + size_t(d) * 2;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) nb[j][k] = C.nb[size_t(d) * 6 + k];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) cf[j][k] = C.coef[size_t(d) * 8 + k];
+            for (int k = 0; k < 3; ++k) {
+                const int2 t = nrow[k];
+                nb[j][2 * k] = t.x >= 0 ? t.x : kZero;
+                nb[j][2 * k + 1] = t.y >= 0 ? t.y : kZero;
+            }
+            const float4 c0 = crow[0], c1 = crow[1];
+            cf[j][0] = c0.x; cf[j][1] = c0.y; cf[j][2] = c0.z; cf[j][3] = c0.w;
+            cf[j][4] = c1.x; cf[j][5] = c1.y; cf[j][6] = c1.z; cf[j][7] = c1.w;
             r[j] = float(B[C.pos[d]]);
             part += r[j] * r[j];
         }
@@ -100,7 +112,7 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
                 if (d < n) {
                     float v[6];
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) v[k] = nb[j][k] >= 0 ? sh.sp[nb[j][k]] : 0.f;
+                    for (int k = 0; k < 6; ++k) v[k] = sh.sp[nb[j][k]];
                     const float f1 = v[0] * cf[j][1], f2 = v[1] * cf[j][2], f3 = v[2] * cf[j][3];
                     const float f4 = v[3] * cf[j][4], f5 = v[4] * cf[j][5], f6 = v[5] * cf[j][6];
                     const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);

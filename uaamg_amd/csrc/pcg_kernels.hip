@@ -81,18 +81,17 @@ __global__ __launch_bounds__(kBlockThreads) void laplacian_dot_kernel(LevelView 
     double acc = 0;
     if (leaf < L.nleaf) {
         const uint32_t rm = row_mask(L, leaf, lane);
-        if (rm) {
-            float xc[8], s[8], dg[8], y[8];
-            stencil_offdiag(L, PlainSrc{P}, leaf, lane, xc, s);
-            ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
+        const LeafRec R = leaf_rec(L, leaf);
+        float xc[8], s[8], dg[8], y[8];
+        stencil_offdiag(L, R, PlainSrc{P}, leaf, lane, xc, s);
+        ld8_slot(L.diag, R.sd, lane, L.dflt_diag, dg);
 #pragma unroll
-            for (int z = 0; z < 8; ++z) {
-                const bool on = (rm >> z) & 1u;
-                y[z] = on ? fmaf(xc[z], dg[z], s[z]) : 0.f;
-                if (on) acc += double(xc[z]) * double(y[z]);
-            }
-            st8(Z + size_t(leaf) * 512 + lane * 8, y);
+        for (int z = 0; z < 8; ++z) {
+            const bool on = (rm >> z) & 1u;
+            y[z] = on ? fmaf(xc[z], dg[z], s[z]) : 0.f;
+            if (on) acc += double(xc[z]) * double(y[z]);
         }
+        if (rm) st8(Z + size_t(leaf) * 512 + lane * 8, y);
     }
     write_partial(partials, block_sum<kBlockThreads>(acc));
 }
@@ -107,17 +106,16 @@ __global__ __launch_bounds__(kBlockThreads) void jacobi_dot_kernel(LevelView L, 
     double acc = 0;
     if (leaf < L.nleaf) {
         const uint32_t rm = row_mask(L, leaf, lane);
-        if (rm) {
-            float xc[8], s[8], y[8], b[8];
-            stencil_offdiag(L, PlainSrcT<VX>{X}, leaf, lane, xc, s);
-            jacobi_row(L, leaf, lane, xc, s, B, w, y);
-            mask_row(rm, y);
-            st8(Y + size_t(leaf) * 512 + lane * 8, y);
-            ld8(B + size_t(leaf) * 512 + lane * 8, b);
+        const LeafRec R = leaf_rec(L, leaf);
+        float xc[8], s[8], y[8], b[8];
+        stencil_offdiag(L, R, PlainSrcT<VX>{X}, leaf, lane, xc, s);
+        jacobi_row(L, R, leaf, lane, xc, s, B, w, y);
+        mask_row(rm, y);
+        if (rm) st8(Y + size_t(leaf) * 512 + lane * 8, y);
+        ld8(B + size_t(leaf) * 512 + lane * 8, b);
 #pragma unroll
-            for (int z = 0; z < 8; ++z)
-                if ((rm >> z) & 1u) acc += double(y[z]) * double(b[z]);
-        }
+        for (int z = 0; z < 8; ++z)
+            if ((rm >> z) & 1u) acc += double(y[z]) * double(b[z]);
     }
     write_partial(partials, block_sum<kBlockThreads>(acc));
 }
@@ -134,30 +132,29 @@ __global__ __launch_bounds__(kBlockThreads) void update_laplacian_dot_kernel(Lev
     double acc = 0;
     if (leaf < L.nleaf) {
         const uint32_t rm = row_mask(L, leaf, lane);
-        if (rm) {
-            const float alpha = sc->alpha, beta = sc->beta;
-            const size_t row = size_t(leaf) * 512 + lane * 8;
-            float x[8], p[8];
-            ld8(X + row, x);
-            ld8(P + row, p);
+        const LeafRec R = leaf_rec(L, leaf);
+        const float alpha = sc->alpha, beta = sc->beta;
+        const size_t row = size_t(leaf) * 512 + lane * 8;
+        float x[8], p[8];
+        ld8(X + row, x);
+        ld8(P + row, p);
+        float pc[8], s[8], dg[8], q[8];
+        stencil_offdiag(L, R, UpdateSrc{Z, P, beta}, leaf, lane, pc, s);
+        ld8_slot(L.diag, R.sd, lane, L.dflt_diag, dg);
+        mask_row(rm, pc);
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if ((rm >> k) & 1u) {
-                    const float t = alpha * p[k];
-                    x[k] = x[k] + t;
-                }
-            st8(X + row, x);
-            float pc[8], s[8], dg[8], q[8];
-            stencil_offdiag(L, UpdateSrc{Z, P, beta}, leaf, lane, pc, s);
-            mask_row(rm, pc);
-            st8(Pn + row, pc);
-            ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
-#pragma unroll
-            for (int z = 0; z < 8; ++z) {
-                const bool on = (rm >> z) & 1u;
-                q[z] = on ? fmaf(pc[z], dg[z], s[z]) : 0.f;
-                if (on) acc += double(pc[z]) * double(q[z]);
+        for (int z = 0; z < 8; ++z) {
+            const bool on = (rm >> z) & 1u;
+            if (on) {
+                const float t = alpha * p[z];
+                x[z] = x[z] + t;
             }
+            q[z] = on ? fmaf(pc[z], dg[z], s[z]) : 0.f;
+            if (on) acc += double(pc[z]) * double(q[z]);
+        }
+        if (rm) {
+            st8(X + row, x);
+            st8(Pn + row, pc);
             st8(Q + row, q);
         }
     }
@@ -337,21 +334,18 @@ __global__ __launch_bounds__(kBlockThreads) void residual64_kernel(LevelView L, 
     float m = 0.f, bad = 0.f;
     if (leaf < L.nleaf) {
         const uint32_t rm = row_mask(L, leaf, lane);
-        float r[8];
-        fill8(r, 0.f);
-        if (rm) {
-            float xc[8], dg[8], b[8];
-            double s[8];
-            stencil_offdiag64(L, PlainSrc{X}, leaf, lane, xc, s);
-            ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
-            ld8(B + size_t(leaf) * 512 + lane * 8, b);
+        const LeafRec Rec = leaf_rec(L, leaf);
+        float r[8], xc[8], dg[8], b[8];
+        double s[8];
+        stencil_offdiag64(L, Rec, PlainSrc{X}, leaf, lane, xc, s);
+        ld8_slot(L.diag, Rec.sd, lane, L.dflt_diag, dg);
+        ld8(B + size_t(leaf) * 512 + lane * 8, b);
 #pragma unroll
-            for (int z = 0; z < 8; ++z) {
-                const double ax = double(xc[z]) * double(dg[z]) + s[z];
-                r[z] = float(double(b[z]) - ax);
-            }
-            mask_row(rm, r);
+        for (int z = 0; z < 8; ++z) {
+            const double ax = double(xc[z]) * double(dg[z]) + s[z];
+            r[z] = float(double(b[z]) - ax);
         }
+        mask_row(rm, r);
         st8(R + size_t(leaf) * 512 + lane * 8, r);
 #pragma unroll
         for (int z = 0; z < 8; ++z)
@@ -376,9 +370,10 @@ __global__ __launch_bounds__(kBlockThreads) void bupdate_laplacian_dot_kernel(Le
     double acc = 0;
     if (leaf < L.nleaf) {
         const uint32_t rm = row_mask(L, leaf, lane);
+        const LeafRec R = leaf_rec(L, leaf);
         const BatchPcg& sp = st[leaf_problem(leafprob, leaf)];
         const int state = sp.state;
-        if (rm && (state == kBActive || state == kBConverging)) {
+        if (state == kBActive || state == kBConverging) {
             const float alpha = sp.alpha, beta = sp.beta;
             const size_t row = size_t(leaf) * 512 + lane * 8;
             float x[8], p[8];
@@ -390,20 +385,22 @@ __global__ __launch_bounds__(kBlockThreads) void bupdate_laplacian_dot_kernel(Le
                     const float t = alpha * p[k];
                     x[k] = x[k] + t;
                 }
-            st8(X + row, x);
+            if (rm) st8(X + row, x);
             if (state == kBActive) {
                 float pc[8], s[8], dg[8], q[8];
-                stencil_offdiag(L, UpdateSrc{Z, P, beta}, leaf, lane, pc, s);
+                stencil_offdiag(L, R, UpdateSrc{Z, P, beta}, leaf, lane, pc, s);
                 mask_row(rm, pc);
-                st8(Pn + row, pc);
-                ld8_slot(L.diag, L.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, L.dflt_diag, dg);
+                ld8_slot(L.diag, R.sd, lane, L.dflt_diag, dg);
 #pragma unroll
                 for (int z = 0; z < 8; ++z) {
                     const bool on = (rm >> z) & 1u;
                     q[z] = on ? fmaf(pc[z], dg[z], s[z]) : 0.f;
                     if (on) acc += double(pc[z]) * double(q[z]);
                 }
-                st8(Q + row, q);
+                if (rm) {
+                    st8(Pn + row, pc);
+                    st8(Q + row, q);
+                }
             }
         }
     }

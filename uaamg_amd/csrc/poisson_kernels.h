@@ -18,8 +18,49 @@ namespace uaamg {
 // storage type of the W-cycle vectors in the mixed-precision form (arithmetic stays FP32)
 using bf16 = __bf16;
 
-// slot indices per leaf
-enum { kSlotDiag = 0, kSlotX = 1, kSlotY = 2, kSlotZ = 3, kSlotSpai0 = 4, kSlotStride = 8 };
+// Per-leaf record of 32 int32 (128 bytes, scalar loads issued together at
+// the top of a kernel): the coefficient-brick slots of the leaf, the slots
+// the +x/+y/+z faces read from the neighbour leaf (-1: neighbour absent or
+// its brick trimmed, both read the default constant), the six face
+// neighbours, the leaf's parent (coarse leaf * 8 + octant, -1 on the coarsest
+// level) and each face neighbour's diagonal slot and parent. With them every
+// data load of a stencil kernel depends on one memory round trip only.
+enum {
+    kSlotDiag = 0,
+    kSlotX = 1,
+    kSlotY = 2,
+    kSlotZ = 3,
+    kSlotSpai0 = 4,
+    kSlotXp = 5,          // kSlotX of the +x neighbour
+    kSlotYp = 6,          // kSlotY of the +y neighbour
+    kSlotZp = 7,          // kSlotZ of the +z neighbour
+    kSlotNb = 8,          // 8..13: face neighbours -x,+x,-y,+y,-z,+z (as nbr)
+    kSlotParent = 14,
+    kSlotNbDiag = 16,     // 16..21: kSlotDiag of each face neighbour
+    kSlotNbParent = 22,   // 22..27: parent of each face neighbour
+    kSlotStride = 32
+};
+
+// Fills the neighbour and parent fields of a record table whose own-slot
+// fields (kSlotDiag .. kSlotSpai0) are set; nbr is [n][6], parent [n] or null
+// (not known yet: -1).
+inline void complete_leaf_records(int32_t* rec, const int32_t* nbr, const int32_t* parent, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        int32_t* r = rec + i * kSlotStride;
+        const int32_t* nb = nbr + i * 6;
+        r[kSlotXp] = nb[1] >= 0 ? rec[size_t(nb[1]) * kSlotStride + kSlotX] : -1;
+        r[kSlotYp] = nb[3] >= 0 ? rec[size_t(nb[3]) * kSlotStride + kSlotY] : -1;
+        r[kSlotZp] = nb[5] >= 0 ? rec[size_t(nb[5]) * kSlotStride + kSlotZ] : -1;
+        r[kSlotParent] = parent ? parent[i] : -1;
+        r[15] = -1;
+        for (int f = 0; f < 6; ++f) {
+            r[kSlotNb + f] = nb[f];
+            r[kSlotNbDiag + f] = nb[f] >= 0 ? rec[size_t(nb[f]) * kSlotStride + kSlotDiag] : -1;
+            r[kSlotNbParent + f] = (parent && nb[f] >= 0) ? parent[nb[f]] : -1;
+        }
+        for (int k = 28; k < kSlotStride; ++k) r[k] = -1;
+    }
+}
 
 // smoothing / apply modes of LaplacianApplySIMD (simd_vdb_poisson.cpp:683-690)
 enum ApplyMode : int {
@@ -38,7 +79,7 @@ struct LevelView {
     float term;
     const uint64_t* mask;     // [nleaf][8]
     const int32_t* nbr;       // [nleaf][6]  -x,+x,-y,+y,-z,+z
-    const int32_t* slot;      // [nleaf][kSlotStride]
+    const int32_t* slot;      // [nleaf][kSlotStride] leaf records
     const int64_t* dofbase;   // [nleaf]
     const int32_t* parent;    // [nleaf] coarse leaf * 8 + octant (fine side of a transfer)
     const int32_t* child;     // [nleaf][8] fine leaf per octant (coarse side of a transfer)

@@ -20,11 +20,13 @@ __global__ __launch_bounds__(kBlockThreads) void apply_kernel(LevelView L, const
     const int leaf = wave_leaf();
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
+    // every load is issued unconditionally (all in bounds) and rows without a
+    // DOF skip only the store, so no load waits for the mask
     const uint32_t rm = row_mask(L, leaf, lane);
-    if (rm == 0) return;
+    const LeafRec R = leaf_rec(L, leaf);
     float xc[8], s[8], y[8];
-    stencil_offdiag(L, PlainSrcT<VX>{X}, leaf, lane, xc, s);
-    const int sd = L.slot[size_t(leaf) * kSlotStride + kSlotDiag];
+    stencil_offdiag(L, R, PlainSrcT<VX>{X}, leaf, lane, xc, s);
+    const int sd = R.sd;
     const size_t row = size_t(leaf) * 512 + lane * 8;
     if (MODE == kModeRedGS || MODE == kModeBlackGS) {
         float b[8], inv[8];
@@ -66,7 +68,7 @@ __global__ __launch_bounds__(kBlockThreads) void apply_kernel(LevelView L, const
                 }
             } else {   // SPAI0
                 float sp[8];
-                ld8_slot(L.spai0, L.slot[size_t(leaf) * kSlotStride + kSlotSpai0], lane, L.dflt_spai0, sp);
+                ld8_slot(L.spai0, R.ssp, lane, L.dflt_spai0, sp);
 #pragma unroll
                 for (int z = 0; z < 8; ++z) {
                     const float r = b[z] - ax[z];
@@ -75,6 +77,7 @@ __global__ __launch_bounds__(kBlockThreads) void apply_kernel(LevelView L, const
             }
         }
     }
+    if (rm == 0) return;
 #pragma unroll
     for (int z = 0; z < 8; ++z)
         if (!((rm >> z) & 1u)) y[z] = 0.f;
@@ -88,17 +91,16 @@ __global__ __launch_bounds__(kBlockThreads) void first_sweep_kernel(LevelView L,
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(L, leaf, lane);
-    if (rm == 0) return;
+    const LeafRec R = leaf_rec(L, leaf);
     const size_t row = size_t(leaf) * 512 + lane * 8;
     float b[8], c[8], y[8];
     ld8(B + row, b);
-    const int32_t* sl = L.slot + size_t(leaf) * kSlotStride;
     if (kind == 1) {
-        ld8_slot(L.spai0, sl[kSlotSpai0], lane, L.dflt_spai0, c);
+        ld8_slot(L.spai0, R.ssp, lane, L.dflt_spai0, c);
 #pragma unroll
         for (int z = 0; z < 8; ++z) y[z] = c[z] * b[z];
     } else {
-        ld8_slot(L.invdiag, sl[kSlotDiag], lane, L.dflt_invdiag, c);
+        ld8_slot(L.invdiag, R.sd, lane, L.dflt_invdiag, c);
         const int par = ((lane >> 3) + (lane & 7)) & 1;
 #pragma unroll
         for (int z = 0; z < 8; ++z) {
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(kBlockThreads) void first_sweep_kernel(LevelView L,
             y[z] = v;
         }
     }
+    if (rm == 0) return;
 #pragma unroll
     for (int z = 0; z < 8; ++z)
         if (!((rm >> z) & 1u)) y[z] = 0.f;
@@ -119,7 +122,6 @@ __global__ __launch_bounds__(kBlockThreads) void restrict_kernel(LevelView C, co
     if (leaf >= C.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(C, leaf, lane);
-    if (rm == 0) return;
     const int cx = lane >> 3, cy = lane & 7;
     float out[8];
 #pragma unroll
@@ -149,6 +151,7 @@ __global__ __launch_bounds__(kBlockThreads) void restrict_kernel(LevelView C, co
             out[4 * bz + c] = sum * 0.125f;
         }
     }
+    if (rm == 0) return;
 #pragma unroll
     for (int z = 0; z < 8; ++z)
         if (!((rm >> z) & 1u)) out[z] = 0.f;
@@ -161,9 +164,8 @@ __global__ __launch_bounds__(kBlockThreads) void prolong_add_kernel(LevelView Fl
     if (leaf >= Fl.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(Fl, leaf, lane);
-    if (rm == 0) return;
     const int p = Fl.parent[leaf];
-    if (p < 0) return;
+    if (p < 0 || rm == 0) return;
     const int cl = p >> 3, oct = p & 7;
     const int bx = (oct >> 2) & 1, by = (oct >> 1) & 1, bz = oct & 1;
     const int crow = (((lane >> 3) >> 1) + 4 * bx) * 8 + (((lane & 7) >> 1) + 4 * by);
@@ -187,11 +189,12 @@ __global__ __launch_bounds__(kBlockThreads) void jacobi_first2_kernel(LevelView 
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(L, leaf, lane);
-    if (rm == 0) return;
-    const FirstJacobiSrcT<VB> src{B, L.invdiag, L.slot, L.dflt_invdiag, w0};
+    const LeafRec R = leaf_rec(L, leaf);
+    const FirstJacobiSrcT<VB> src{B, L.invdiag, L.dflt_invdiag, w0};
     float xc[8], s[8], y[8];
-    stencil_offdiag(L, src, leaf, lane, xc, s);
-    jacobi_row(L, leaf, lane, xc, s, B, w1, y);
+    stencil_offdiag(L, R, src, leaf, lane, xc, s);
+    jacobi_row(L, R, leaf, lane, xc, s, B, w1, y);
+    if (rm == 0) return;
     mask_row(rm, y);
     st8(Y + size_t(leaf) * 512 + lane * 8, y);
 }
@@ -204,11 +207,12 @@ __global__ __launch_bounds__(kBlockThreads) void prolong_jacobi_kernel(LevelView
     if (leaf >= L.nleaf) return;
     const int lane = threadIdx.x & 63;
     const uint32_t rm = row_mask(L, leaf, lane);
-    if (rm == 0) return;
-    const ProlongSrcT<V> src{X, Xc, L.mask, L.parent};
+    const LeafRec R = leaf_rec(L, leaf);
+    const ProlongSrcT<V> src{X, Xc, L.mask};
     float xc[8], s[8], y[8];
-    stencil_offdiag(L, src, leaf, lane, xc, s);
-    jacobi_row(L, leaf, lane, xc, s, B, w, y);
+    stencil_offdiag(L, R, src, leaf, lane, xc, s);
+    jacobi_row(L, R, leaf, lane, xc, s, B, w, y);
+    if (rm == 0) return;
     mask_row(rm, y);
     st8(Y + size_t(leaf) * 512 + lane * 8, y);
 }
@@ -227,17 +231,18 @@ __global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelV
     const bool valid = leaf < F.nleaf;
     float r[8];
     fill8(r, 0.f);
+    int par = -1;
     if (valid) {
         const uint32_t rm = row_mask(F, leaf, lane);
-        if (rm) {
-            float xc[8], s[8], dg[8], b[8];
-            stencil_offdiag(F, PlainSrcT<V>{X}, leaf, lane, xc, s);
-            ld8_slot(F.diag, F.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, F.dflt_diag, dg);
-            ld8(B + size_t(leaf) * 512 + lane * 8, b);
+        const LeafRec R = leaf_rec(F, leaf);
+        par = R.par;
+        float xc[8], s[8], dg[8], b[8];
+        stencil_offdiag(F, R, PlainSrcT<V>{X}, leaf, lane, xc, s);
+        ld8_slot(F.diag, R.sd, lane, F.dflt_diag, dg);
+        ld8(B + size_t(leaf) * 512 + lane * 8, b);
 #pragma unroll
-            for (int z = 0; z < 8; ++z) r[z] = b[z] - fmaf(xc[z], dg[z], s[z]);
-            mask_row(rm, r);
-        }
+        for (int z = 0; z < 8; ++z) r[z] = b[z] - fmaf(xc[z], dg[z], s[z]);
+        mask_row(rm, r);
     }
 #pragma unroll
     for (int z = 0; z < 8; ++z) rs[wave][lane * 8 + z] = r[z];
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelV
         for (int jj = 0; jj < 2; ++jj)
 #pragma unroll
             for (int kk = 0; kk < 2; ++kk) sum += rs[wave][((2 * cx + ii) * 8 + (2 * cy + jj)) * 8 + 2 * cz + kk];
-    const int p = F.parent[leaf];
+    const int p = par;
     if (p < 0) return;   // every DOF leaf has a parent; kept as a bounds guard
     const int cl = p >> 3, oct = p & 7;
     const int off = ((cx + 4 * ((oct >> 2) & 1)) * 8 + (cy + 4 * ((oct >> 1) & 1))) * 8 + cz + 4 * (oct & 1);

@@ -239,6 +239,20 @@ size_t PoissonDevice::device_bytes() const {
     return bytes;
 }
 
+// Leaf records of a level (layout at kSlotDiag .. kSlotNbParent): the slots
+// number each coefficient kind's kept bricks in leaf order; parent null
+// leaves the parent fields -1.
+static std::vector<int32_t> level_records(const PoissonLevelHost& h, const int32_t* parent) {
+    const size_t n = h.nleaf();
+    std::vector<int32_t> rec(n * kSlotStride, -1);
+    int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < kNumCoef; ++k)
+            if (h.keep[k][i]) rec[i * kSlotStride + k] = count[k]++;
+    complete_leaf_records(rec.data(), h.nbr.data(), parent, n);
+    return rec;
+}
+
 int PoissonDevice::acquire_device() {
     if (on_device()) return 0;
     if (host_.levels.empty()) {
@@ -276,7 +290,12 @@ int PoissonDevice::acquire_device() {
                                                     parent.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
                                      "upload parent"))
             return -3;
-        // the host vector dies at the end of this iteration
+        const std::vector<int32_t> rec = level_records(host_.levels[l], parent.data());
+        if (!rec.empty() && check(hipMemcpyAsync(const_cast<int32_t*>(lv_[l].v.slot), rec.data(),
+                                                 rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
+                                  "upload leaf records"))
+            return -3;
+        // the host vectors die at the end of this iteration
         if (check(hipStreamSynchronize(stream_), "upload parent")) return -3;
     }
     auto vec = [&](size_t nleaf) { return static_cast<float*>(arena_.take(nleaf * kLeafSize * sizeof(float))); };
@@ -481,11 +500,11 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
         v.nb27 = static_cast<const int32_t*>(up(nb27.data(), nb27.size() * sizeof(int32_t)));
         (void)check(hipStreamSynchronize(stream_), "upload sync");   // nb27 dies at scope end
     }
-    std::vector<int32_t> slot(n * kSlotStride, -1);
     int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < kNumCoef; ++k)
-            if (h.keep[k][i]) slot[i * kSlotStride + k] = count[k]++;
+        for (int k = 0; k < kNumCoef; ++k) count[k] += h.keep[k][i] ? 1 : 0;
+    // parents are filled in once the coarser level is known (acquire_device)
+    const std::vector<int32_t> slot = level_records(h, nullptr);
     v.slot = static_cast<const int32_t*>(up(slot.data(), slot.size() * sizeof(int32_t)));
     v.dofbase = static_cast<const int64_t*>(up(h.dofbase.data(), n * sizeof(int64_t)));
     v.parent = static_cast<const int32_t*>(arena_.take(std::max<size_t>(n * sizeof(int32_t), 4)));

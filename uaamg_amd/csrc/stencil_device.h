@@ -82,14 +82,50 @@ __device__ __forceinline__ void ld8_slot(const float* base, int slot, int row, f
     else fill8(v, dflt);
 }
 
+// A leaf's record (slots, neighbour slots, face neighbours, parents; layout
+// at kSlotDiag), loaded once per wavefront at the top of a kernel.
+struct LeafRec {
+    int sd, sx, sy, sz, ssp, sxp, syp, szp;
+    int nb[6];
+    int par;
+    int nbsd[6], nbpar[6];
+};
+
+__device__ __forceinline__ LeafRec leaf_rec(const LevelView& L, int leaf) {
+    const int4* p = reinterpret_cast<const int4*>(L.slot + size_t(leaf) * kSlotStride);
+    const int4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4], f = p[5], g = p[6];
+    LeafRec r;
+    r.sd = a.x; r.sx = a.y; r.sy = a.z; r.sz = a.w;
+    r.ssp = b.x; r.sxp = b.y; r.syp = b.z; r.szp = b.w;
+    r.nb[0] = c.x; r.nb[1] = c.y; r.nb[2] = c.z; r.nb[3] = c.w;
+    r.nb[4] = d.x; r.nb[5] = d.y; r.par = d.z;
+    r.nbsd[0] = e.x; r.nbsd[1] = e.y; r.nbsd[2] = e.z; r.nbsd[3] = e.w;
+    r.nbsd[4] = f.x; r.nbsd[5] = f.y; r.nbpar[0] = f.z; r.nbpar[1] = f.w;
+    r.nbpar[2] = g.x; r.nbpar[3] = g.y; r.nbpar[4] = g.z; r.nbpar[5] = g.w;
+    // One use of every field here, in the kernel's first basic block: the
+    // record's scalar loads are then issued together and waited for once
+    // (otherwise each is issued in the branch that first reads it, one
+    // memory round trip after the other). The empty asm emits no instruction.
+    const int all = (((a.x ^ a.y) ^ (a.z ^ a.w)) ^ ((b.x ^ b.y) ^ (b.z ^ b.w))) ^
+                    (((c.x ^ c.y) ^ (c.z ^ c.w)) ^ ((d.x ^ d.y) ^ (d.z ^ d.w))) ^
+                    (((e.x ^ e.y) ^ (e.z ^ e.w)) ^ ((f.x ^ f.y) ^ (f.z ^ f.w))) ^ ((g.x ^ g.y) ^ (g.z ^ g.w));
+    asm volatile("" ::"s"(all));
+    return r;
+}
+
 // Sources of the vector a stencil is applied to. row() fills the 8 values of
 // one z-row of a leaf, at() returns one value; both are called only for leaves
-// that exist (an absent neighbour leaf contributes 0).
+// that exist (an absent neighbour leaf contributes 0), with that leaf's
+// diagonal slot sd and parent par from the leaf records.
 template <class V>
 struct PlainSrcT {   // the stored brick vector
     const V* X;
-    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const { ld8(X + size_t(leaf) * 512 + r * 8, v); }
-    __device__ __forceinline__ float at(int leaf, int r, int z) const { return ld1(X + size_t(leaf) * 512 + r * 8 + z); }
+    __device__ __forceinline__ void row(int leaf, int, int, int r, float v[8]) const {
+        ld8(X + size_t(leaf) * 512 + r * 8, v);
+    }
+    __device__ __forceinline__ float at(int leaf, int, int, int r, int z) const {
+        return ld1(X + size_t(leaf) * 512 + r * 8 + z);
+    }
 };
 using PlainSrc = PlainSrcT<float>;
 
@@ -101,18 +137,16 @@ template <class VB>
 struct FirstJacobiSrcT {
     const VB* B;
     const float* invdiag;
-    const int32_t* slot;
     float dflt_inv;
     float w0;
-    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
+    __device__ __forceinline__ void row(int leaf, int sd, int, int r, float v[8]) const {
         float b[8], inv[8];
         ld8(B + size_t(leaf) * 512 + r * 8, b);
-        ld8_slot(invdiag, slot[size_t(leaf) * kSlotStride + kSlotDiag], r, dflt_inv, inv);
+        ld8_slot(invdiag, sd, r, dflt_inv, inv);
 #pragma unroll
         for (int z = 0; z < 8; ++z) v[z] = w0 * (inv[z] * b[z]);
     }
-    __device__ __forceinline__ float at(int leaf, int r, int z) const {
-        const int sd = slot[size_t(leaf) * kSlotStride + kSlotDiag];
+    __device__ __forceinline__ float at(int leaf, int sd, int, int r, int z) const {
         const float inv = sd >= 0 ? invdiag[size_t(sd) * 512 + r * 8 + z] : dflt_inv;
         return w0 * (inv * ld1(B + size_t(leaf) * 512 + r * 8 + z));
     }
@@ -120,39 +154,40 @@ struct FirstJacobiSrcT {
 using FirstJacobiSrc = FirstJacobiSrcT<float>;
 
 // x + P x_coarse on on-voxels (prolongation<true> with alpha = 1,
-// simd_vdb_poisson.cpp:1686-1754), evaluated where the following sweep needs it.
+// simd_vdb_poisson.cpp:1686-1754), evaluated where the following sweep needs
+// it. The coarse row is loaded whenever the leaf has a parent and added on
+// the row's DOF voxels only.
 template <class V>
 struct ProlongSrcT {
     const V* X;
     const V* Xc;
     const uint64_t* mask;
-    const int32_t* parent;
     __device__ __forceinline__ uint32_t bits(int leaf, int r) const {
         return uint32_t(mask[size_t(leaf) * 8 + (r >> 3)] >> ((r & 7) * 8)) & 0xffu;
     }
-    __device__ __forceinline__ float4 coarse4(int leaf, int r) const {
-        const int p = parent[leaf];
-        const int cl = p >> 3, oct = p & 7;
+    __device__ __forceinline__ float4 coarse4(int par, int r) const {
+        const int cl = par >> 3, oct = par & 7;
         const int crow = (((r >> 3) >> 1) + 4 * ((oct >> 2) & 1)) * 8 + (((r & 7) >> 1) + 4 * ((oct >> 1) & 1));
         return ld4(Xc + size_t(cl) * 512 + crow * 8 + 4 * (oct & 1));
     }
-    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
+    __device__ __forceinline__ void row(int leaf, int, int par, int r, float v[8]) const {
         ld8(X + size_t(leaf) * 512 + r * 8, v);
-        const uint32_t m = parent[leaf] >= 0 ? bits(leaf, r) : 0u;   // parent < 0: bounds guard
-        if (m) {
-            const float4 c = coarse4(leaf, r);
+        if (par >= 0) {   // bounds guard: every DOF leaf has a parent
+            const uint32_t m = bits(leaf, r);
+            const float4 c = coarse4(par, r);
             const float cc[4] = {c.x, c.y, c.z, c.w};
 #pragma unroll
             for (int z = 0; z < 8; ++z)
                 if ((m >> z) & 1u) v[z] += cc[z >> 1];
         }
     }
-    __device__ __forceinline__ float at(int leaf, int r, int z) const {
+    __device__ __forceinline__ float at(int leaf, int, int par, int r, int z) const {
         float v = ld1(X + size_t(leaf) * 512 + r * 8 + z);
-        if (parent[leaf] >= 0 && ((bits(leaf, r) >> z) & 1u)) {
-            const float4 c = coarse4(leaf, r);
+        if (par >= 0) {
+            const uint32_t m = bits(leaf, r);
+            const float4 c = coarse4(par, r);
             const float cc[4] = {c.x, c.y, c.z, c.w};
-            v += cc[z >> 1];
+            if ((m >> z) & 1u) v += cc[z >> 1];
         }
         return v;
     }
@@ -164,7 +199,7 @@ struct UpdateSrc {
     const float* Z;
     const float* P;
     float beta;
-    __device__ __forceinline__ void row(int leaf, int r, float v[8]) const {
+    __device__ __forceinline__ void row(int leaf, int, int, int r, float v[8]) const {
         float z[8], p[8];
         ld8(Z + size_t(leaf) * 512 + r * 8, z);
         ld8(P + size_t(leaf) * 512 + r * 8, p);
@@ -174,7 +209,7 @@ struct UpdateSrc {
             v[k] = z[k] + t;
         }
     }
-    __device__ __forceinline__ float at(int leaf, int r, int z) const {
+    __device__ __forceinline__ float at(int leaf, int, int, int r, int z) const {
         const size_t i = size_t(leaf) * 512 + r * 8 + z;
         const float t = beta * P[i];
         return Z[i] + t;
@@ -192,45 +227,32 @@ struct StencilRow {
 
 // Own row of the source and the off-diagonal operands of one row.
 template <class Src>
-__device__ __forceinline__ void stencil_gather(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
-                                               StencilRow& o) {
+__device__ __forceinline__ void stencil_gather(const LevelView& L, const LeafRec& R, const Src& src, int leaf, int lane,
+                                               float xc[8], StencilRow& o) {
     const int lx = lane >> 3, ly = lane & 7;
-    const int32_t* nb = L.nbr + size_t(leaf) * 6;
-    const int nmx = nb[0], npx = nb[1], nmy = nb[2], npy = nb[3], nmz = nb[4], npz = nb[5];
-    src.row(leaf, lane, xc);
+    const int nmz = R.nb[4], npz = R.nb[5];
+    src.row(leaf, R.sd, R.par, lane, xc);
 
     float* xm = o.xm;
     float* xp = o.xp;
     float* ym = o.ym;
     float* yp = o.yp;
-    {
-        const int sl = lx > 0 ? leaf : nmx, sr = lx > 0 ? lane - 8 : lane + 56;
-        if (sl >= 0) src.row(sl, sr, xm);
-        else fill8(xm, 0.f);
-    }
-    {
-        const int sl = lx < 7 ? leaf : npx, sr = lx < 7 ? lane + 8 : lane - 56;
-        if (sl >= 0) src.row(sl, sr, xp);
-        else fill8(xp, 0.f);
-    }
-    {
-        const int sl = ly > 0 ? leaf : nmy, sr = ly > 0 ? lane - 1 : lane + 7;
-        if (sl >= 0) src.row(sl, sr, ym);
-        else fill8(ym, 0.f);
-    }
-    {
-        const int sl = ly < 7 ? leaf : npy, sr = ly < 7 ? lane + 1 : lane - 7;
-        if (sl >= 0) src.row(sl, sr, yp);
-        else fill8(yp, 0.f);
-    }
-    o.zlo = nmz >= 0 ? src.at(nmz, lane, 7) : 0.f;
-    o.zhi = npz >= 0 ? src.at(npz, lane, 0) : 0.f;
+    // rows of the own leaf or, on the brick face, of the face neighbour f
+    auto side = [&](bool own, int f, int rown, int rnb, float v[8]) {
+        const int sl = own ? leaf : R.nb[f];
+        const int ssd = own ? R.sd : R.nbsd[f];
+        const int spar = own ? R.par : R.nbpar[f];
+        if (sl >= 0) src.row(sl, ssd, spar, own ? rown : rnb, v);
+        else fill8(v, 0.f);
+    };
+    side(lx > 0, 0, lane - 8, lane + 56, xm);
+    side(lx < 7, 1, lane + 8, lane - 56, xp);
+    side(ly > 0, 2, lane - 1, lane + 7, ym);
+    side(ly < 7, 3, lane + 1, lane - 7, yp);
+    o.zlo = nmz >= 0 ? src.at(nmz, R.nbsd[4], R.nbpar[4], lane, 7) : 0.f;
+    o.zhi = npz >= 0 ? src.at(npz, R.nbsd[5], R.nbpar[5], lane, 0) : 0.f;
 
-    const int32_t* sl = L.slot + size_t(leaf) * kSlotStride;
-    const int sx = sl[kSlotX], sy = sl[kSlotY], sz = sl[kSlotZ];
-    const int sxp = npx >= 0 ? L.slot[size_t(npx) * kSlotStride + kSlotX] : -1;
-    const int syp = npy >= 0 ? L.slot[size_t(npy) * kSlotStride + kSlotY] : -1;
-    const int szp = npz >= 0 ? L.slot[size_t(npz) * kSlotStride + kSlotZ] : -1;
+    const int sx = R.sx, sy = R.sy, sz = R.sz, sxp = R.sxp, syp = R.syp, szp = R.szp;
     const float d = L.dflt_off;
 
     ld8_slot(L.cx, sx, lane, d, o.cxm);
@@ -243,10 +265,10 @@ __device__ __forceinline__ void stencil_gather(const LevelView& L, const Src& sr
 
 // Own row of the source and the off-diagonal sum of the 7-point stencil for one row.
 template <class Src>
-__device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
-                                                float s[8]) {
+__device__ __forceinline__ void stencil_offdiag(const LevelView& L, const LeafRec& R, const Src& src, int leaf,
+                                                int lane, float xc[8], float s[8]) {
     StencilRow o;
-    stencil_gather(L, src, leaf, lane, xc, o);
+    stencil_gather(L, R, src, leaf, lane, xc, o);
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
         const float zpv = z < 7 ? xc[z + 1] : o.zhi;
@@ -265,10 +287,10 @@ __device__ __forceinline__ void stencil_offdiag(const LevelView& L, const Src& s
 // The same off-diagonal sum with every product and sum in FP64 (exact
 // products of FP32 operands), for the true residual of the refinement step.
 template <class Src>
-__device__ __forceinline__ void stencil_offdiag64(const LevelView& L, const Src& src, int leaf, int lane, float xc[8],
-                                                  double s[8]) {
+__device__ __forceinline__ void stencil_offdiag64(const LevelView& L, const LeafRec& R, const Src& src, int leaf,
+                                                  int lane, float xc[8], double s[8]) {
     StencilRow o;
-    stencil_gather(L, src, leaf, lane, xc, o);
+    stencil_gather(L, R, src, leaf, lane, xc, o);
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
         const double zpv = z < 7 ? xc[z + 1] : o.zhi;
@@ -287,9 +309,9 @@ __device__ __forceinline__ void stencil_offdiag64(const LevelView& L, const Src&
 
 // Weighted Jacobi on one row: y = fma(w, (b - (fma(x, diag, s))) * D^-1, x).
 template <class VB>
-__device__ __forceinline__ void jacobi_row(const LevelView& L, int leaf, int lane, const float xc[8], const float s[8],
-                                           const VB* B, float w, float y[8]) {
-    const int sd = L.slot[size_t(leaf) * kSlotStride + kSlotDiag];
+__device__ __forceinline__ void jacobi_row(const LevelView& L, const LeafRec& R, int leaf, int lane, const float xc[8],
+                                           const float s[8], const VB* B, float w, float y[8]) {
+    const int sd = R.sd;
     float dg[8], inv[8], b[8];
     ld8_slot(L.diag, sd, lane, L.dflt_diag, dg);
     ld8_slot(L.invdiag, sd, lane, L.dflt_invdiag, inv);

@@ -40,9 +40,10 @@ __device__ void block_jacobi(const LevelView& L, const Src& src, const float* B,
     for (int leaf = first_wave_leaf(); leaf < L.nleaf; leaf += kTailWaves) {
         const uint32_t rm = row_mask(L, leaf, lane);
         if (rm) {
+            const LeafRec R = leaf_rec(L, leaf);
             float xc[8], s[8], y[8];
-            stencil_offdiag(L, src, leaf, lane, xc, s);
-            jacobi_row(L, leaf, lane, xc, s, B, w, y);
+            stencil_offdiag(L, R, src, leaf, lane, xc, s);
+            jacobi_row(L, R, leaf, lane, xc, s, B, w, y);
             mask_row(rm, y);
             st8(Y + size_t(leaf) * 512 + lane * 8, y);
         }
@@ -60,9 +61,10 @@ __device__ void block_residual_restrict(const LevelView& F, const float* X, cons
         fill8(r, 0.f);
         const uint32_t rm = row_mask(F, leaf, lane);
         if (rm) {
+            const LeafRec R = leaf_rec(F, leaf);
             float xc[8], s[8], dg[8], b[8];
-            stencil_offdiag(F, PlainSrc{X}, leaf, lane, xc, s);
-            ld8_slot(F.diag, F.slot[size_t(leaf) * kSlotStride + kSlotDiag], lane, F.dflt_diag, dg);
+            stencil_offdiag(F, R, PlainSrc{X}, leaf, lane, xc, s);
+            ld8_slot(F.diag, R.sd, lane, F.dflt_diag, dg);
             ld8(B + size_t(leaf) * 512 + lane * 8, b);
 #pragma unroll
             for (int z = 0; z < 8; ++z) r[z] = b[z] - fmaf(xc[z], dg[z], s[z]);
@@ -97,7 +99,7 @@ __global__ __launch_bounds__(kTailThreads) void tail_kernel(TailArgs A) {
     for (int v = 0; v < A.nvisits; ++v) {
         if (v == 0 && A.first_skip) {
             float* dst = next();
-            block_jacobi(L, FirstJacobiSrc{A.B, L.invdiag, L.slot, L.dflt_invdiag, A.wpre[0]}, A.B, dst, A.wpre[1]);
+            block_jacobi(L, FirstJacobiSrc{A.B, L.invdiag, L.dflt_invdiag, A.wpre[0]}, A.B, dst, A.wpre[1]);
             __syncthreads();
             cur = dst;
             dst = next();
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(kTailThreads) void tail_kernel(TailArgs A) {
         __syncthreads();
         {
             float* dst = next();
-            block_jacobi(L, ProlongSrc{cur, A.Xc, L.mask, L.parent}, A.B, dst, A.wpost[0]);
+            block_jacobi(L, ProlongSrc{cur, A.Xc, L.mask}, A.B, dst, A.wpost[0]);
             __syncthreads();
             cur = dst;
             dst = next();
