@@ -29,11 +29,14 @@ using namespace dev;
 enum BlockMode { kPreZero = 0, kPreAgain = 1, kPost = 2 };
 constexpr int kBlkThreads = 512;
 
+// LDS per region voxel: b, x, diag, invdiag, three face coefficients and the
+// mask byte (29 bytes; sweeps update x in place through registers), so two
+// H = 3 workgroups fit one CU's 160 KiB.
 template <int H>
 struct Region {
     static constexpr int S = 8 + 2 * H;
     static constexpr int N = S * S * S;
-    static constexpr size_t lds_bytes = size_t(N) * (8 * sizeof(float) + 1);
+    static constexpr size_t lds_bytes = size_t(N) * (7 * sizeof(float) + 1);
 };
 
 template <int H>
@@ -49,35 +52,50 @@ __device__ __forceinline__ float stored(float v) {
     else return float(V(v));
 }
 
-// One Jacobi sweep y = J_w(x) on [lo, S - lo)^3 of the region, each output
-// rounded to the storage type V.
-template <int H, class V>
-__device__ __forceinline__ void sweep(int lo, const float* x, float* y, const float* b, const float* dg,
-                                      const float* inv, const float* cx, const float* cy, const float* cz,
-                                      const uint8_t* m, float w) {
+// One Jacobi sweep x <- J_w(x) on [LO, S - LO)^3 of the region, each output
+// rounded to the storage type V: every thread computes its voxels into
+// registers, then, after a barrier, stores them (the caller's barrier
+// publishes the stores).
+template <int H, int LO, class V>
+__device__ __forceinline__ void sweep(float* x, const float* b, const float* dg, const float* inv, const float* cx,
+                                      const float* cy, const float* cz, const uint8_t* m, float w) {
     constexpr int S = Region<H>::S;
-    const int e = S - 2 * lo;
-    const int cnt = e * e * e;
-    for (int q = threadIdx.x; q < cnt; q += kBlkThreads) {
-        const int vx = lo + q / (e * e), vy = lo + (q / e) % e, vz = lo + q % e;
-        const int i = ridx<H>(vx, vy, vz);
-        float r = 0.f;
-        if (m[i]) {
-            const int ix = S * S, iy = S;
-            const float f1 = x[i + ix] * cx[i + ix];
-            const float f2 = x[i - ix] * cx[i];
-            const float f3 = x[i + iy] * cy[i + iy];
-            const float f4 = x[i - iy] * cy[i];
-            const float f5 = x[i + 1] * cz[i + 1];
-            const float f6 = x[i - 1] * cz[i];
-            const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
-            const float ax = fmaf(x[i], dg[i], s);
-            float t = b[i] - ax;
-            t = t * inv[i];
-            r = fmaf(w, t, x[i]);
+    constexpr int e = S - 2 * LO;
+    constexpr int cnt = e * e * e;
+    constexpr int kPer = (cnt + kBlkThreads - 1) / kBlkThreads;
+    float out[kPer];
+    int at[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int q = threadIdx.x + k * kBlkThreads;
+        at[k] = -1;
+        out[k] = 0.f;
+        if (q < cnt) {
+            const int vx = LO + q / (e * e), vy = LO + (q / e) % e, vz = LO + q % e;
+            const int i = ridx<H>(vx, vy, vz);
+            at[k] = i;
+            float r = 0.f;
+            if (m[i]) {
+                const int ix = S * S, iy = S;
+                const float f1 = x[i + ix] * cx[i + ix];
+                const float f2 = x[i - ix] * cx[i];
+                const float f3 = x[i + iy] * cy[i + iy];
+                const float f4 = x[i - iy] * cy[i];
+                const float f5 = x[i + 1] * cz[i + 1];
+                const float f6 = x[i - 1] * cz[i];
+                const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
+                const float ax = fmaf(x[i], dg[i], s);
+                float t = b[i] - ax;
+                t = t * inv[i];
+                r = fmaf(w, t, x[i]);
+            }
+            out[k] = stored<V>(r);
         }
-        y[i] = stored<V>(r);
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (at[k] >= 0) x[at[k]] = out[k];
 }
 
 template <int H, int MODE, class V>
@@ -87,8 +105,7 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
     extern __shared__ float lds[];
     float* b = lds;
     float* xa = b + N;
-    float* xb = xa + N;
-    float* dg = xb + N;
+    float* dg = xa + N;
     float* inv = dg + N;
     float* cx = inv + N;
     float* cy = cx + N;
@@ -125,23 +142,44 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
         const int n = nl[k];
         const int lx = gx & 7, ly = gy & 7, lz = gz & 7;
         const int off = (lx << 6) | (ly << 3) | lz;
-        float vb = 0.f, vx = 0.f;
-        uint8_t vm = 0;
+        // every load is unconditional (absent leaf or slot: leaf 0 / slot 0,
+        // both exist, and the value is replaced below), so the loads of all
+        // passes are in flight together
+        const size_t nv = size_t(n >= 0 ? n : 0);
         const int s0 = sd[k], s1 = sx[k], s2 = sy[k], s3 = sz[k];
-        const float vdg = s0 >= 0 ? L.diag[size_t(s0) * 512 + off] : L.dflt_diag;
-        const float vinv = s0 >= 0 ? L.invdiag[size_t(s0) * 512 + off] : L.dflt_invdiag;
-        if (n >= 0) {
-            vm = uint8_t((L.mask[size_t(n) * 8 + lx] >> ((ly << 3) | lz)) & 1u);
-            vb = float(B[size_t(n) * 512 + off]);
-            if (MODE == kPreZero) {
-                vx = w0 * (vinv * vb);
-            } else {
-                vx = float(Xin[size_t(n) * 512 + off]);
-                if (MODE == kPost && vm && par[k] >= 0) {
-                    const int p = par[k], cl = p >> 3, oct = p & 7;
-                    const int crow = ((lx >> 1) + 4 * ((oct >> 2) & 1)) * 8 + ((ly >> 1) + 4 * ((oct >> 1) & 1));
-                    vx += float(Xc[size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1)]);
-                }
+        const float tdg = L.diag[size_t(s0 >= 0 ? s0 : 0) * 512 + off];
+        const float tinv = L.invdiag[size_t(s0 >= 0 ? s0 : 0) * 512 + off];
+        const float tcx = L.cx[size_t(s1 >= 0 ? s1 : 0) * 512 + off];
+        const float tcy = L.cy[size_t(s2 >= 0 ? s2 : 0) * 512 + off];
+        const float tcz = L.cz[size_t(s3 >= 0 ? s3 : 0) * 512 + off];
+        uint64_t tm = L.mask[nv * 8 + lx];
+        float tb = float(B[nv * 512 + off]);
+        float tx = 0.f, tc = 0.f;
+        int pp = -1;
+        if (MODE != kPreZero) tx = float(Xin[nv * 512 + off]);
+        if (MODE == kPost) {
+            pp = par[k];
+            const int p = pp >= 0 ? pp : 0, cl = p >> 3, oct = p & 7;
+            const int crow = ((lx >> 1) + 4 * ((oct >> 2) & 1)) * 8 + ((ly >> 1) + 4 * ((oct >> 1) & 1));
+            tc = float(Xc[size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1)]);
+        }
+        // selects, no branches: the empty asm statements (no instruction)
+        // keep the compiler from sinking these loads into a branch on n
+        asm volatile("" : "+v"(tm), "+v"(tb), "+v"(tx), "+v"(tc));
+        const bool has = n >= 0;
+        const float vdg = s0 >= 0 ? tdg : L.dflt_diag;
+        const float vinv = s0 >= 0 ? tinv : L.dflt_invdiag;
+        const uint8_t vm = has ? uint8_t((tm >> ((ly << 3) | lz)) & 1u) : uint8_t(0);
+        const float vb = has ? tb : 0.f;
+        float vx;
+        if (MODE == kPreZero) {
+            const float t = w0 * (vinv * vb);
+            vx = has ? t : 0.f;
+        } else {
+            vx = has ? tx : 0.f;
+            if (MODE == kPost) {
+                const float t = vx + tc;
+                vx = (vm && pp >= 0) ? t : vx;
             }
         }
         b[q] = vb;
@@ -149,28 +187,26 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
         m[q] = vm;
         dg[q] = vdg;
         inv[q] = vinv;
-        cx[q] = s1 >= 0 ? L.cx[size_t(s1) * 512 + off] : L.dflt_off;
-        cy[q] = s2 >= 0 ? L.cy[size_t(s2) * 512 + off] : L.dflt_off;
-        cz[q] = s3 >= 0 ? L.cz[size_t(s3) * 512 + off] : L.dflt_off;
+        cx[q] = s1 >= 0 ? tcx : L.dflt_off;
+        cy[q] = s2 >= 0 ? tcy : L.dflt_off;
+        cz[q] = s3 >= 0 ? tcz : L.dflt_off;
     }
     __syncthreads();
 
-    const float* fin;   // region array holding the final sweep
     if (MODE == kPreZero) {
-        sweep<H, V>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w1);
+        sweep<H, 1, V>(xa, b, dg, inv, cx, cy, cz, m, w1);
         __syncthreads();
-        sweep<H, V>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w2);
+        sweep<H, 2, V>(xa, b, dg, inv, cx, cy, cz, m, w2);
         __syncthreads();
-        fin = xa;
     } else {
-        sweep<H, V>(1, xa, xb, b, dg, inv, cx, cy, cz, m, w0);
+        sweep<H, 1, V>(xa, b, dg, inv, cx, cy, cz, m, w0);
         __syncthreads();
-        sweep<H, V>(2, xb, xa, b, dg, inv, cx, cy, cz, m, w1);
+        sweep<H, 2, V>(xa, b, dg, inv, cx, cy, cz, m, w1);
         __syncthreads();
-        sweep<H, V>(3, xa, xb, b, dg, inv, cx, cy, cz, m, w2);
+        sweep<H, 3, V>(xa, b, dg, inv, cx, cy, cz, m, w2);
         __syncthreads();
-        fin = xb;
     }
+    const float* fin = xa;   // the final sweep
 
     // the leaf's own values
     for (int q = tid; q < 512; q += kBlkThreads) {
@@ -179,11 +215,14 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
     }
     if (MODE == kPost) return;
 
-    // residual on the own brick, then restriction into the parent's octant
-    float* rs = (fin == xa) ? xb : xa;
-    for (int q = tid; q < 512; q += kBlkThreads) {
+    // residual on the own brick (into registers; then the first 512 entries
+    // of the diagonal array, no longer read, hold it), then restriction into
+    // the parent's octant
+    static_assert(kBlkThreads == 512, "one own-brick voxel per thread");
+    float rv = 0.f;
+    {
+        const int q = tid;
         const int i = ridx<H>(H + (q >> 6), H + ((q >> 3) & 7), H + (q & 7));
-        float r = 0.f;
         if (m[i]) {
             const int ix = S * S, iy = S;
             const float* x = fin;
@@ -194,10 +233,12 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
             const float f5 = x[i + 1] * cz[i + 1];
             const float f6 = x[i - 1] * cz[i];
             const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
-            r = b[i] - fmaf(x[i], dg[i], s);
+            rv = b[i] - fmaf(x[i], dg[i], s);
         }
-        rs[q] = r;
     }
+    __syncthreads();
+    float* rs = dg;
+    rs[tid] = rv;
     __syncthreads();
     if (tid < 64) {
         const int ccx = tid >> 4, ccy = (tid >> 2) & 3, ccz = tid & 3;

@@ -225,8 +225,8 @@ size_t PoissonDevice::device_bytes() const {
         size_t nslot[kNumCoef] = {0, 0, 0, 0, 0};
         for (int k = 0; k < kNumCoef; ++k)
             for (size_t i = 0; i < n; ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
-        for (int k = 0; k < kNumCoef; ++k) add(nslot[k] * kLeafSize * sizeof(float));
-        add(nslot[kCoefDiag] * kLeafSize * sizeof(float));   // invdiag
+        for (int k = 0; k < kNumCoef; ++k) add(std::max<size_t>(nslot[k], 1) * kLeafSize * sizeof(float));
+        add(std::max<size_t>(nslot[kCoefDiag], 1) * kLeafSize * sizeof(float));   // invdiag
         add(4 * n * kLeafSize * sizeof(float));               // X, B, T, U
         if (opt_.precision == 1) add(4 * n * kLeafSize * sizeof(bf16));
     }
@@ -522,8 +522,10 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
             return -3;
     }
     // coefficient bricks of kept leaves only
+    // at least one brick per array (zeros when no leaf keeps one), so a kernel
+    // may read slot 0 unconditionally and select the default afterwards
     auto pack = [&](const std::vector<float>& src, int kind, const std::vector<uint8_t>& keep) -> const float* {
-        std::vector<float> buf(size_t(count[kind]) * kLeafSize);
+        std::vector<float> buf(std::max<size_t>(size_t(count[kind]), 1) * kLeafSize, 0.f);
         size_t j = 0;
         for (size_t i = 0; i < n; ++i)
             if (keep[i]) {

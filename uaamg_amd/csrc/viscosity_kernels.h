@@ -18,6 +18,20 @@
 
 namespace uaamg {
 
+// Wave record: one per entry of chan_leaves (a (channel c, DOF leaf) pair,
+// one wavefront's work), 64 int32 = 256 bytes, loaded as one burst of scalar
+// loads so that every data load of the stencil row depends on one memory
+// round trip:
+//   [kVRecLeaf]      the leaf
+//   [kVRecNb + k]    its 27-leaf neighbourhood, k = (ax+1)*9 + (ay+1)*3 + (az+1) (-1 none; centre = leaf)
+//   [kVRecOwn + i]   the leaf's slots of the grids channel c reads: neg(c, 0..2), cross(d2, 0..3),
+//                    cross(d1, 0..3), diag c, invdiag c, SPAI0 c   (d1 = c+1, d2 = c+2 mod 3)
+//   [kVRecNbr + j]   slots read from a neighbour leaf: neg(c,0) of +x, neg(c,1) of +y, neg(c,2) of +z,
+//                    cross(d2,2), cross(d2,3), cross(d1,1), cross(d1,3) of the -e_c neighbour
+// (-1: leaf or brick absent, the level default is read).
+constexpr int kVRec = 64;
+enum { kVRecLeaf = 0, kVRecNb = 1, kVRecOwn = 28, kVRecNbr = 42 };
+
 struct ViscLevelView {
     int32_t nleaf;             // leaves of the table (DOF leaves + coefficient-only leaves)
     int32_t nleaf_dof;         // leaves holding vector bricks
@@ -26,6 +40,7 @@ struct ViscLevelView {
     const int32_t* slot;       // [nleaf][32]
     const float* pool;         // [nslot][512]
     const int32_t* chan_leaves;   // DOF leaves of channel 0, then 1, then 2
+    const int32_t* wrec;          // [chan_off[3]][kVRec] wave records, chan_leaves order
     int32_t chan_off[4];
     const int32_t* parent;     // [nleaf_dof] coarse leaf * 8 + octant
     const int32_t* child;      // [nleaf_dof][8]
@@ -43,6 +58,12 @@ struct ViscCoarsestView {
     const float* val;
     const float* invdiag;      // Eigen DiagonalPreconditioner: 1/diag or 1
     const int32_t* pos;        // [n] linear index of the DOF in the level's vector
+    // the same rows as ELL, entry k of row d at [k * n + d] (columns ascending,
+    // rows padded to kViscMaxRow with value 0 and column d)
+    const int32_t* ecol;
+    const float* eval;
+    const int32_t* elen;       // [n] entries of each row
+    int32_t ewidth;            // longest row, <= kViscMaxRow
     int32_t max_iters;
     float tol;
 };

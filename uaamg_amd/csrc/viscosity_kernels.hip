@@ -18,15 +18,7 @@ namespace {
 constexpr int kVWaves = 4;
 constexpr int kVThreads = 64 * kVWaves;
 
-__device__ __forceinline__ int nb_slot(int ax, int ay, int az) { return (ax + 1) * 9 + (ay + 1) * 3 + (az + 1); }
-
-// leaf holding leaf-local row (X, Y) in [-1, 8]^2, shifted by az leaves in z
-__device__ __forceinline__ int row_leaf(const ViscLevelView& L, int leaf, int X, int Y, int az) {
-    const int ax = X < 0 ? -1 : (X > 7 ? 1 : 0);
-    const int ay = Y < 0 ? -1 : (Y > 7 ? 1 : 0);
-    if (!ax && !ay && !az) return leaf;
-    return L.nb27[size_t(leaf) * 27 + nb_slot(ax, ay, az)];
-}
+constexpr int nb_slot(int ax, int ay, int az) { return (ax + 1) * 9 + (ay + 1) * 3 + (az + 1); }
 
 __device__ __forceinline__ int row_off(int X, int Y) { return ((X & 7) * 8 + (Y & 7)) * 8; }
 
@@ -41,35 +33,95 @@ __device__ __forceinline__ void st8v(float* p, const float v[8]) {
     *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
 }
 
-// value source: a level vector channel (missing leaf = 0) or a coefficient
-// grid (missing leaf or slot = the level default)
-struct VecSrc {
-    const float* v;
-    int ch;
-    __device__ __forceinline__ const float* base(const ViscLevelView& L, int lf) const {
-        return (lf >= 0 && lf < L.nleaf_dof) ? v + (size_t(lf) * 3 + ch) * 512 : nullptr;
-    }
-    __device__ __forceinline__ float missing(const ViscLevelView&) const { return 0.f; }
-};
-template <int GID>
-struct CoefSrc {
-    __device__ __forceinline__ const float* base(const ViscLevelView& L, int lf) const {
-        if (lf < 0) return nullptr;
-        const int s = L.slot[size_t(lf) * 32 + GID];
-        return s >= 0 ? L.pool + size_t(s) * 512 : nullptr;
-    }
-    __device__ __forceinline__ float missing(const ViscLevelView& L) const { return L.dflt[GID]; }
+// A wave record (layout at kVRecLeaf) in scalar registers.
+struct VRec {
+    int leaf;
+    int nb[27];
+    int own[14];
+    int nbs[7];
 };
 
-// the 8 values at (lx + OX, ly + OY, z + OZ), z = 0..7
-template <int OX, int OY, int OZ, class Src>
-__device__ __forceinline__ void fetch(const ViscLevelView& L, const Src& S, int leaf, int lx, int ly, float out[8]) {
+__device__ __forceinline__ VRec load_vrec(const ViscLevelView& L, int e) {
+    const int4* p = reinterpret_cast<const int4*>(L.wrec + size_t(e) * kVRec);
+    int v[52];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) {
+        const int4 q = p[i];
+        v[4 * i] = q.x; v[4 * i + 1] = q.y; v[4 * i + 2] = q.z; v[4 * i + 3] = q.w;
+    }
+    VRec r;
+    r.leaf = v[kVRecLeaf];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) r.nb[k] = v[kVRecNb + k];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) r.own[k] = v[kVRecOwn + k];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r.nbs[k] = v[kVRecNbr + k];
+    // One use of every field here, in the kernel's first basic block: the
+    // record's scalar loads are then issued together and waited for once
+    // (otherwise each is issued in the branch that first reads it). The
+    // empty asm emits no instruction.
+    int all = 0;
+#pragma unroll
+    for (int k = 0; k < 49; ++k) all ^= v[k];
+    asm volatile("" ::"s"(all));
+    return r;
+}
+
+// Leaf holding leaf-local row (lx + OX, ly + OY), shifted AZ leaves in z:
+// one of four record entries chosen per lane.
+template <int OX, int OY, int AZ>
+__device__ __forceinline__ int vleaf(const VRec& R, int lx, int ly) {
+    constexpr int sx = (OX > 0) - (OX < 0), sy = (OY > 0) - (OY < 0);
     const int X = lx + OX, Y = ly + OY;
-    const float* b = S.base(L, row_leaf(L, leaf, X, Y, 0));
+    const bool ox = X < 0 || X > 7, oy = Y < 0 || Y > 7;
+    const int l00 = R.nb[nb_slot(0, 0, AZ)], l10 = R.nb[nb_slot(sx, 0, AZ)];
+    const int l01 = R.nb[nb_slot(0, sy, AZ)], l11 = R.nb[nb_slot(sx, sy, AZ)];
+    return ox ? (oy ? l11 : l10) : (oy ? l01 : l00);
+}
+
+// channel ch of vector v at (lx + OX, ly + OY, z + OZ), z = 0..7 (a missing
+// leaf, or one without vector bricks, reads 0)
+template <int OX, int OY, int OZ>
+__device__ __forceinline__ void vfetch(const ViscLevelView& L, const float* v, int ch, const VRec& R, int lx, int ly,
+                                       float out[8]) {
+    const int X = lx + OX, Y = ly + OY;
+    const int lf = vleaf<OX, OY, 0>(R, lx, ly);
     float row[8];
-    if (b) ld8v(b + row_off(X, Y), row);
+    if (lf >= 0 && lf < L.nleaf_dof) ld8v(v + (size_t(lf) * 3 + ch) * 512 + row_off(X, Y), row);
     else {
-        const float m = S.missing(L);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) row[k] = 0.f;
+    }
+    if (OZ == 0) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out[k] = row[k];
+    } else if (OZ > 0) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) out[k] = row[k + 1];
+        const int lz = vleaf<OX, OY, 1>(R, lx, ly);
+        out[7] = (lz >= 0 && lz < L.nleaf_dof) ? v[(size_t(lz) * 3 + ch) * 512 + row_off(X, Y)] : 0.f;
+    } else {
+#pragma unroll
+        for (int k = 7; k > 0; --k) out[k] = row[k - 1];
+        const int lz = vleaf<OX, OY, -1>(R, lx, ly);
+        out[0] = (lz >= 0 && lz < L.nleaf_dof) ? v[(size_t(lz) * 3 + ch) * 512 + row_off(X, Y) + 7] : 0.f;
+    }
+}
+
+// coefficient grid GID at (lx + OX, ly + OY, z + OZ), at most one offset
+// nonzero: rows inside the leaf read slot s_own, rows (or the shifted z
+// value) beyond the face read s_nb; slot -1 reads the level default
+template <int GID, int OX, int OY, int OZ>
+__device__ __forceinline__ void cfetch(const ViscLevelView& L, int s_own, int s_nb, int lx, int ly, float out[8]) {
+    static_assert((OX != 0) + (OY != 0) + (OZ != 0) <= 1, "single-axis coefficient offsets");
+    const int X = lx + OX, Y = ly + OY;
+    const bool beyond = X < 0 || X > 7 || Y < 0 || Y > 7;
+    const int s = beyond ? s_nb : s_own;
+    const float m = L.dflt[GID];
+    float row[8];
+    if (s >= 0) ld8v(L.pool + size_t(s) * 512 + row_off(X, Y), row);
+    else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) row[k] = m;
     }
@@ -79,13 +131,11 @@ __device__ __forceinline__ void fetch(const ViscLevelView& L, const Src& S, int 
     } else if (OZ > 0) {
 #pragma unroll
         for (int k = 0; k < 7; ++k) out[k] = row[k + 1];
-        const float* nb = S.base(L, row_leaf(L, leaf, X, Y, 1));
-        out[7] = nb ? nb[row_off(X, Y)] : S.missing(L);
+        out[7] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y)] : m;
     } else {
 #pragma unroll
         for (int k = 7; k > 0; --k) out[k] = row[k - 1];
-        const float* nb = S.base(L, row_leaf(L, leaf, X, Y, -1));
-        out[0] = nb ? nb[row_off(X, Y) + 7] : S.missing(L);
+        out[0] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y) + 7] : m;
     }
 }
 
@@ -94,45 +144,48 @@ struct Unit {
     static constexpr int x = A == 0, y = A == 1, z = A == 2;
 };
 
-// One row (lane) of the applier on channel C of `leaf`: Y = op(X, B).
+// One row (lane) of the applier on channel C for wave record e (a DOF leaf
+// of channel C): Y = op(X, B). Every load is issued unconditionally (all in
+// bounds); rows without a DOF skip only the store.
 template <int MODE, int C>
-__device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int lane, const float* Xv, const float* Bv,
+__device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane, const float* Xv, const float* Bv,
                                          float* Yv) {
     const int lx = lane >> 3, ly = lane & 7;
+    const VRec R = load_vrec(L, e);
+    const int leaf = R.leaf;
     const uint32_t rm = uint32_t(L.mask[(size_t(leaf) * 3 + C) * 8 + lx] >> (ly * 8)) & 0xffu;
-    if (!rm) return;   // rows without a DOF are not written
     constexpr int d0 = C, d1 = (C + 1) % 3, d2 = (C + 2) % 3;
     using E0 = Unit<d0>;
     using E1 = Unit<d1>;
     using E2 = Unit<d2>;
-    const VecSrc xc{Xv, C}, x1{Xv, d1}, x2{Xv, d2};
+    constexpr int kN0 = kVNeg + 3 * C, kX2 = kVCross + 4 * d2, kX1 = kVCross + 4 * d1;
 
     float s[8];
     {
         float a[8], wa[8], b[8], wb[8];
         float f12[8], f34[8];
-        fetch<1, 0, 0>(L, xc, leaf, lx, ly, a);
-        fetch<1, 0, 0>(L, CoefSrc<kVNeg + 3 * C + 0>{}, leaf, lx, ly, wa);
-        fetch<-1, 0, 0>(L, xc, leaf, lx, ly, b);
-        fetch<0, 0, 0>(L, CoefSrc<kVNeg + 3 * C + 0>{}, leaf, lx, ly, wb);
+        vfetch<1, 0, 0>(L, Xv, C, R, lx, ly, a);
+        cfetch<kN0 + 0, 1, 0, 0>(L, R.own[0], R.nbs[0], lx, ly, wa);
+        vfetch<-1, 0, 0>(L, Xv, C, R, lx, ly, b);
+        cfetch<kN0 + 0, 0, 0, 0>(L, R.own[0], -1, lx, ly, wb);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = a[k] * wa[k], q = b[k] * wb[k];
             f12[k] = p + q;
         }
-        fetch<0, 1, 0>(L, xc, leaf, lx, ly, a);
-        fetch<0, 1, 0>(L, CoefSrc<kVNeg + 3 * C + 1>{}, leaf, lx, ly, wa);
-        fetch<0, -1, 0>(L, xc, leaf, lx, ly, b);
-        fetch<0, 0, 0>(L, CoefSrc<kVNeg + 3 * C + 1>{}, leaf, lx, ly, wb);
+        vfetch<0, 1, 0>(L, Xv, C, R, lx, ly, a);
+        cfetch<kN0 + 1, 0, 1, 0>(L, R.own[1], R.nbs[1], lx, ly, wa);
+        vfetch<0, -1, 0>(L, Xv, C, R, lx, ly, b);
+        cfetch<kN0 + 1, 0, 0, 0>(L, R.own[1], -1, lx, ly, wb);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = a[k] * wa[k], q = b[k] * wb[k];
             f34[k] = p + q;
         }
-        fetch<0, 0, 1>(L, xc, leaf, lx, ly, a);
-        fetch<0, 0, 1>(L, CoefSrc<kVNeg + 3 * C + 2>{}, leaf, lx, ly, wa);
-        fetch<0, 0, -1>(L, xc, leaf, lx, ly, b);
-        fetch<0, 0, 0>(L, CoefSrc<kVNeg + 3 * C + 2>{}, leaf, lx, ly, wb);
+        vfetch<0, 0, 1>(L, Xv, C, R, lx, ly, a);
+        cfetch<kN0 + 2, 0, 0, 1>(L, R.own[2], R.nbs[2], lx, ly, wa);
+        vfetch<0, 0, -1>(L, Xv, C, R, lx, ly, b);
+        cfetch<kN0 + 2, 0, 0, 0>(L, R.own[2], -1, lx, ly, wb);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = a[k] * wa[k], q = b[k] * wb[k];
@@ -144,24 +197,24 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
     {
         // normal d2: channel d1 at v, v + e1, v - e0, v - e0 + e1
         float u[8], c[8], t01[8];
-        fetch<0, 0, 0>(L, x1, leaf, lx, ly, u);
-        fetch<0, 0, 0>(L, CoefSrc<kVCross + 4 * d2 + 0>{}, leaf, lx, ly, c);
+        vfetch<0, 0, 0>(L, Xv, d1, R, lx, ly, u);
+        cfetch<kX2 + 0, 0, 0, 0>(L, R.own[3], -1, lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) t01[k] = u[k] * c[k];
-        fetch<E1::x, E1::y, E1::z>(L, x1, leaf, lx, ly, u);
-        fetch<0, 0, 0>(L, CoefSrc<kVCross + 4 * d2 + 1>{}, leaf, lx, ly, c);
+        vfetch<E1::x, E1::y, E1::z>(L, Xv, d1, R, lx, ly, u);
+        cfetch<kX2 + 1, 0, 0, 0>(L, R.own[4], -1, lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = c[k] * u[k];
             t01[k] = t01[k] + p;
         }
         float t23[8];
-        fetch<-E0::x, -E0::y, -E0::z>(L, x1, leaf, lx, ly, u);
-        fetch<-E0::x, -E0::y, -E0::z>(L, CoefSrc<kVCross + 4 * d2 + 2>{}, leaf, lx, ly, c);
+        vfetch<-E0::x, -E0::y, -E0::z>(L, Xv, d1, R, lx, ly, u);
+        cfetch<kX2 + 2, -E0::x, -E0::y, -E0::z>(L, R.own[5], R.nbs[3], lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) t23[k] = c[k] * u[k];
-        fetch<E1::x - E0::x, E1::y - E0::y, E1::z - E0::z>(L, x1, leaf, lx, ly, u);
-        fetch<-E0::x, -E0::y, -E0::z>(L, CoefSrc<kVCross + 4 * d2 + 3>{}, leaf, lx, ly, c);
+        vfetch<E1::x - E0::x, E1::y - E0::y, E1::z - E0::z>(L, Xv, d1, R, lx, ly, u);
+        cfetch<kX2 + 3, -E0::x, -E0::y, -E0::z>(L, R.own[6], R.nbs[4], lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = c[k] * u[k];
@@ -173,24 +226,24 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
     {
         // normal d1: channel d2 at v, v - e0, v + e2, v - e0 + e2
         float u[8], c[8], t01[8];
-        fetch<0, 0, 0>(L, x2, leaf, lx, ly, u);
-        fetch<0, 0, 0>(L, CoefSrc<kVCross + 4 * d1 + 0>{}, leaf, lx, ly, c);
+        vfetch<0, 0, 0>(L, Xv, d2, R, lx, ly, u);
+        cfetch<kX1 + 0, 0, 0, 0>(L, R.own[7], -1, lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) t01[k] = c[k] * u[k];
-        fetch<-E0::x, -E0::y, -E0::z>(L, x2, leaf, lx, ly, u);
-        fetch<-E0::x, -E0::y, -E0::z>(L, CoefSrc<kVCross + 4 * d1 + 1>{}, leaf, lx, ly, c);
+        vfetch<-E0::x, -E0::y, -E0::z>(L, Xv, d2, R, lx, ly, u);
+        cfetch<kX1 + 1, -E0::x, -E0::y, -E0::z>(L, R.own[8], R.nbs[5], lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = c[k] * u[k];
             t01[k] = t01[k] + p;
         }
         float t23[8];
-        fetch<E2::x, E2::y, E2::z>(L, x2, leaf, lx, ly, u);
-        fetch<0, 0, 0>(L, CoefSrc<kVCross + 4 * d1 + 2>{}, leaf, lx, ly, c);
+        vfetch<E2::x, E2::y, E2::z>(L, Xv, d2, R, lx, ly, u);
+        cfetch<kX1 + 2, 0, 0, 0>(L, R.own[9], -1, lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) t23[k] = c[k] * u[k];
-        fetch<E2::x - E0::x, E2::y - E0::y, E2::z - E0::z>(L, x2, leaf, lx, ly, u);
-        fetch<-E0::x, -E0::y, -E0::z>(L, CoefSrc<kVCross + 4 * d1 + 3>{}, leaf, lx, ly, c);
+        vfetch<E2::x - E0::x, E2::y - E0::y, E2::z - E0::z>(L, Xv, d2, R, lx, ly, u);
+        cfetch<kX1 + 3, -E0::x, -E0::y, -E0::z>(L, R.own[10], R.nbs[6], lx, ly, c);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float p = c[k] * u[k];
@@ -206,7 +259,7 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
     if (MODE == kVRedGS || MODE == kVBlackGS) {
         float b[8], inv[8];
         ld8v(Bv + row, b);
-        fetch<0, 0, 0>(L, CoefSrc<kVInvDiag + C>{}, leaf, lx, ly, inv);
+        cfetch<kVInvDiag + C, 0, 0, 0>(L, R.own[12], -1, lx, ly, inv);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             float t = b[k] - s[k];
@@ -218,7 +271,7 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
         }
     } else {
         float dg[8];
-        fetch<0, 0, 0>(L, CoefSrc<kVDiag + C>{}, leaf, lx, ly, dg);
+        cfetch<kVDiag + C, 0, 0, 0>(L, R.own[11], -1, lx, ly, dg);
         if (MODE == kVNormal) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) y[k] = fmaf(lhs[k], dg[k], s[k]);
@@ -230,8 +283,8 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
                 for (int k = 0; k < 8; ++k) y[k] = b[k] - fmaf(lhs[k], dg[k], s[k]);
             } else {
                 float sc[8];
-                if (MODE == kVJacobi) fetch<0, 0, 0>(L, CoefSrc<kVInvDiag + C>{}, leaf, lx, ly, sc);
-                else fetch<0, 0, 0>(L, CoefSrc<kVSpai0 + C>{}, leaf, lx, ly, sc);
+                if (MODE == kVJacobi) cfetch<kVInvDiag + C, 0, 0, 0>(L, R.own[12], -1, lx, ly, sc);
+                else cfetch<kVSpai0 + C, 0, 0, 0>(L, R.own[13], -1, lx, ly, sc);
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     const float a = fmaf(lhs[k], dg[k], s[k]);
@@ -242,6 +295,7 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int leaf, int l
             }
         }
     }
+    if (!rm) return;   // rows without a DOF are not written
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         if (!((rm >> k) & 1u)) y[k] = 0.f;
@@ -253,26 +307,43 @@ __global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, 
                                                                float* Yv) {
     const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= L.chan_off[C + 1] - L.chan_off[C]) return;
-    visc_row<MODE, C>(L, L.chan_leaves[L.chan_off[C] + w], threadIdx.x & 63, Xv, Bv, Yv);
+    visc_row<MODE, C>(L, L.chan_off[C] + w, threadIdx.x & 63, Xv, Bv, Yv);
 }
 
-// restriction of one (coarse leaf, channel) row: 1/8 of the on fine children
+// restriction of one (coarse leaf, channel) row: 1/8 of the on fine children.
+// The row's two children (z < 4 and z >= 4) and their rows are loaded
+// unconditionally (an absent child reads leaf 0 and is dropped), so the
+// loads depend on one round trip; sums run in the reference's order.
 __device__ __forceinline__ void visc_restrict_row(const ViscLevelView& Cl, const ViscLevelView& F, int leaf, int c,
                                                   int lane, const float* fine, float* coarse) {
     const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(Cl.mask[(size_t(leaf) * 3 + c) * 8 + lx] >> (ly * 8)) & 0xffu;
-    if (!rm) return;
+    const int fx = 2 * (lx & 3), fy = 2 * (ly & 3);
+    int f[2];
+    float v[2][2][2][8];      // [child][a][b][fine z]
+    uint64_t mw[2][2];        // [child][a] fine mask word of fine x = fx + a
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int oct = ((lx >> 2) << 2) | ((ly >> 2) << 1) | h;
+        f[h] = Cl.child[size_t(leaf) * 8 + oct];
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const size_t fl = size_t(f[h] >= 0 ? f[h] : 0);
+        const uint64_t* fm = F.mask + (fl * 3 + c) * 8;
+        const float* fv = fine + (fl * 3 + c) * 512;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            mw[h][a] = fm[fx + a];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) ld8v(fv + (fx + a) * 64 + (fy + b) * 8, v[h][a][b]);
+        }
+    }
     float out[8];
 #pragma unroll
     for (int z = 0; z < 8; ++z) {
-        out[z] = 0.f;
-        if (!((rm >> z) & 1u)) continue;
-        const int oct = ((lx >> 2) << 2) | ((ly >> 2) << 1) | (z >> 2);
-        const int f = Cl.child[size_t(leaf) * 8 + oct];
-        if (f < 0) continue;
-        const uint64_t* fm = F.mask + (size_t(f) * 3 + c) * 8;
-        const float* fv = fine + (size_t(f) * 3 + c) * 512;
-        const int fx = 2 * (lx & 3), fy = 2 * (ly & 3), fz = 2 * (z & 3);
+        const int h = z >> 2;
+        const int fz = 2 * (z & 3);
         float sum = 0.f;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -280,11 +351,13 @@ __device__ __forceinline__ void visc_restrict_row(const ViscLevelView& Cl, const
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int d = 0; d < 2; ++d) {
-                    const int fo = (fx + a) * 64 + (fy + b) * 8 + fz + d;
-                    if ((fm[fo >> 6] >> (fo & 63)) & 1u) sum += fv[fo];
+                    const int bit = (fy + b) * 8 + fz + d;
+                    if ((mw[h][a] >> bit) & 1u) sum += v[h][a][b][fz + d];
                 }
-        out[z] = sum * 0.125f * 0.5f * 2.0f;
+        const float o = sum * 0.125f * 0.5f * 2.0f;
+        out[z] = (((rm >> z) & 1u) && f[h] >= 0) ? o : 0.f;
     }
+    if (!rm) return;   // rows without a DOF are not written
     st8v(coarse + (size_t(leaf) * 3 + c) * 512 + size_t(lx * 8 + ly) * 8, out);
 }
 
@@ -293,9 +366,8 @@ __device__ __forceinline__ void visc_prolong_row(const ViscLevelView& F, int lea
                                                  float* fine) {
     const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(F.mask[(size_t(leaf) * 3 + c) * 8 + lx] >> (ly * 8)) & 0xffu;
-    if (!rm) return;
     const int p = F.parent[leaf];
-    if (p < 0) return;
+    if (p < 0 || !rm) return;
     const int cl = p >> 3, oct = p & 7;
     const int cx = ((oct >> 2) & 1) * 4 + (lx >> 1), cy = ((oct >> 1) & 1) * 4 + (ly >> 1), cz0 = (oct & 1) * 4;
     const float* cv = coarse + (size_t(cl) * 3 + c) * 512 + cx * 64 + cy * 8 + cz0;
@@ -321,7 +393,7 @@ template <int MODE, int C>
 __device__ __forceinline__ void small_pass(const ViscLevelView& L, const float* X, const float* B, float* Y) {
     const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = L.chan_off[C + 1] - L.chan_off[C];
-    for (int w = w0; w < n; w += kSmallWaves) visc_row<MODE, C>(L, L.chan_leaves[L.chan_off[C] + w], lane, X, B, Y);
+    for (int w = w0; w < n; w += kSmallWaves) visc_row<MODE, C>(L, L.chan_off[C] + w, lane, X, B, Y);
     __syncthreads();
 }
 
@@ -424,6 +496,7 @@ __device__ __forceinline__ float2 bsum2(float a, float b, float (*red)[kCgT / 64
 
 __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C, const float* B, float* X) {
     __shared__ float sp[kViscMaxCoarsest];
+    __shared__ float sq[kViscMaxCoarsest];   // A p of the thread's own rows
     __shared__ float red[2][kCgT / 64][2];
     int par = 0;
     const int n = C.n, tid = threadIdx.x;
@@ -454,17 +527,31 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
         }
         float abs_new = bsum2(part, 0.f, red, par).x;
         for (int it = 0; it < C.max_iters; ++it) {
+            // q = A p row by row: a row's entries are loaded together
+            // (padding is in bounds) and summed in column order up to the
+            // row's length; the rows loop is not unrolled (registers), so the
+            // sums go through LDS, each read back by the thread that wrote it
+#pragma unroll 1
+            for (int j = 0; j < kCgPer; ++j) {
+                const int d = tid + j * kCgT;
+                if (d >= n) break;
+                const int len = C.elen[d];
+                float t = 0.f;
+#pragma unroll
+                for (int k = 0; k < kViscMaxRow; ++k) {
+                    const size_t at = size_t(k) * size_t(n) + size_t(d);
+                    const float m = C.eval[at] * sp[C.ecol[at]];
+                    const float u = t + m;
+                    t = k < len ? u : t;
+                }
+                sq[d] = t;
+            }
             part = 0.f;
 #pragma unroll
             for (int j = 0; j < kCgPer; ++j) {
                 const int d = tid + j * kCgT;
                 if (d < n) {
-                    float t = 0.f;
-                    for (int64_t k = C.ptr[d]; k < C.ptr[d + 1]; ++k) {
-                        const float m = C.val[k] * sp[C.col[k]];
-                        t = t + m;
-                    }
-                    q[j] = t;
+                    q[j] = sq[d];
                     part += p[j] * q[j];
                 }
             }

@@ -53,6 +53,41 @@ void ViscDevice::init_host_only(ViscHierarchyHost&& h, const ViscOptions& opt) {
     opt_ = opt;
 }
 
+// Wave records of a level (layout at kVRecLeaf), chan_leaves order.
+static std::vector<int32_t> wave_records(const ViscLevelHost& h) {
+    std::vector<int32_t> rec;
+    rec.reserve((h.chan_leaves[0].size() + h.chan_leaves[1].size() + h.chan_leaves[2].size()) * kVRec);
+    auto slot_of = [&](int32_t lf, int gid) { return lf < 0 ? -1 : h.slot[size_t(lf) * kVSlots + size_t(gid)]; };
+    auto nbk = [](int ax, int ay, int az) { return (ax + 1) * 9 + (ay + 1) * 3 + (az + 1); };
+    for (int c = 0; c < 3; ++c) {
+        const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
+        const int own[14] = {vneg(c, 0),         vneg(c, 1),         vneg(c, 2),         vcross(d2, 0, 0),
+                             vcross(d2, 0, 1),   vcross(d2, 1, 0),   vcross(d2, 1, 1),   vcross(d1, 0, 0),
+                             vcross(d1, 0, 1),   vcross(d1, 1, 0),   vcross(d1, 1, 1),   kVDiag + c,
+                             kVInvDiag + c,      kVSpai0 + c};
+        const int mx = c == 0 ? -1 : 0, my = c == 1 ? -1 : 0, mz = c == 2 ? -1 : 0;   // -e_c
+        for (int32_t leaf : h.chan_leaves[c]) {
+            int32_t r[kVRec];
+            std::fill(r, r + kVRec, -1);
+            const int32_t* nb = &h.nb27[size_t(leaf) * 27];
+            r[kVRecLeaf] = leaf;
+            for (int k = 0; k < 27; ++k) r[kVRecNb + k] = nb[k];
+            r[kVRecNb + nbk(0, 0, 0)] = leaf;
+            for (int i = 0; i < 14; ++i) r[kVRecOwn + i] = slot_of(leaf, own[i]);
+            const int32_t nm = nb[nbk(mx, my, mz)];
+            r[kVRecNbr + 0] = slot_of(nb[nbk(1, 0, 0)], vneg(c, 0));
+            r[kVRecNbr + 1] = slot_of(nb[nbk(0, 1, 0)], vneg(c, 1));
+            r[kVRecNbr + 2] = slot_of(nb[nbk(0, 0, 1)], vneg(c, 2));
+            r[kVRecNbr + 3] = slot_of(nm, vcross(d2, 1, 0));
+            r[kVRecNbr + 4] = slot_of(nm, vcross(d2, 1, 1));
+            r[kVRecNbr + 5] = slot_of(nm, vcross(d1, 0, 1));
+            r[kVRecNbr + 6] = slot_of(nm, vcross(d1, 1, 1));
+            rec.insert(rec.end(), r, r + kVRec);
+        }
+    }
+    return rec;
+}
+
 int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
     auto put = [&](const void* src, size_t bytes) -> void* {
         void* p = arena_.take(std::max<size_t>(bytes, 4));
@@ -73,6 +108,8 @@ int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
         v.chan_off[c + 1] = int32_t(cl.size());
     }
     v.chan_leaves = static_cast<const int32_t*>(put(cl.data(), cl.size() * 4));
+    const std::vector<int32_t> wrec = wave_records(h);
+    v.wrec = static_cast<const int32_t*>(put(wrec.data(), wrec.size() * 4));
     v.parent = h.parent.empty() ? nullptr : static_cast<const int32_t*>(put(h.parent.data(), h.parent.size() * 4));
     v.child = h.child.empty() ? nullptr : static_cast<const int32_t*>(put(h.child.data(), h.child.size() * 4));
     std::copy(h.dflt, h.dflt + kVSlots, v.dflt);
@@ -86,7 +123,8 @@ int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
     d.X = static_cast<float*>(arena_.take(vec));
     d.B = static_cast<float*>(arena_.take(vec));
     d.T = static_cast<float*>(arena_.take(vec));
-    if (!v.nb27 || !v.mask || !v.slot || !v.pool || !v.chan_leaves || !f.dofbase || !d.X || !d.B || !d.T) {
+    if (!v.nb27 || !v.mask || !v.slot || !v.pool || !v.chan_leaves || !v.wrec || !f.dofbase || !d.X || !d.B ||
+        !d.T) {
         err_ = "device arena too small";
         return -2;
     }
@@ -125,10 +163,33 @@ int ViscDevice::build_coarsest() {
     cv_.col = static_cast<const int32_t*>(put(host_.ccol.data(), host_.ccol.size() * 4));
     cv_.val = static_cast<const float*>(put(host_.cval.data(), host_.cval.size() * 4));
     cv_.invdiag = static_cast<const float*>(put(inv.data(), inv.size() * 4));
+    int32_t width = 0;
+    for (int64_t i = 0; i < n; ++i) width = std::max<int32_t>(width, int32_t(host_.cptr[i + 1] - host_.cptr[i]));
+    if (width > kViscMaxRow) {
+        err_ = "coarsest row longer than the stencil allows";
+        return -1;
+    }
+    // stored at the full kViscMaxRow width, so the kernel's loads need no bound
+    std::vector<int32_t> ecol(static_cast<size_t>(n) * kViscMaxRow);
+    std::vector<int32_t> elen(static_cast<size_t>(n));
+    std::vector<float> eval(size_t(n) * kViscMaxRow, 0.f);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t b = host_.cptr[i], len = host_.cptr[i + 1] - b;
+        elen[size_t(i)] = int32_t(len);
+        for (int32_t k = 0; k < kViscMaxRow; ++k) {
+            const size_t at = size_t(k) * size_t(n) + size_t(i);
+            ecol[at] = k < len ? host_.ccol[size_t(b + k)] : int32_t(i);
+            eval[at] = k < len ? host_.cval[size_t(b + k)] : 0.f;
+        }
+    }
+    cv_.ewidth = width;
+    cv_.ecol = static_cast<const int32_t*>(put(ecol.data(), ecol.size() * 4));
+    cv_.eval = static_cast<const float*>(put(eval.data(), eval.size() * 4));
+    cv_.elen = static_cast<const int32_t*>(put(elen.data(), elen.size() * 4));
     cv_.pos = static_cast<const int32_t*>(put(pos.data(), pos.size() * 4));
     cv_.max_iters = 10;
     cv_.tol = FLT_EPSILON;
-    if (!cv_.ptr || !cv_.col || !cv_.val || !cv_.invdiag || !cv_.pos) {
+    if (!cv_.ptr || !cv_.col || !cv_.val || !cv_.invdiag || !cv_.pos || !cv_.ecol || !cv_.eval || !cv_.elen) {
         err_ = "device arena too small";
         return -2;
     }
@@ -150,6 +211,8 @@ size_t ViscDevice::device_bytes() const {
     if (host_.levels.empty()) return 0;
     size_t bytes = 1 << 20;
     for (const auto& l : host_.levels) {
+        const size_t nwave = l.chan_leaves[0].size() + l.chan_leaves[1].size() + l.chan_leaves[2].size();
+        bytes += align_up(nwave * kVRec * 4);
         bytes += align_up(l.nb27.size() * 4) + align_up(l.mask.size() * 8) + align_up(l.slot.size() * 4) +
                  align_up(l.pool.size() * 4) + align_up(l.dofbase.size() * 8) + 3 * align_up(l.nleaf_dof * 4) +
                  align_up(l.parent.size() * 4) + align_up(l.child.size() * 4) + 8 * kAlign;
@@ -161,6 +224,8 @@ size_t ViscDevice::device_bytes() const {
     bytes += 5 * align_up(vec0) + 3 * align_up(ndof0 * 4 + 4);
     bytes += align_up(host_.cptr.size() * 8) + align_up(host_.ccol.size() * 4) + 3 * align_up(host_.cval.size() * 4) +
              8 * kAlign;
+    const size_t nc = size_t(host_.levels.back().ndof);
+    bytes += 2 * align_up(nc * kViscMaxRow * 4) + align_up(nc * 4) + 2 * kAlign;   // ELL rows
     bytes += align_up(sizeof(PcgScalars)) + align_up((l0.nleaf_dof * 3 / kWavesPerBlock + 2) * 8);
     return bytes;
 }
@@ -296,8 +361,16 @@ int ViscDevice::upload_spai0(int level) {
     spai0_bufs_.push_back(reinterpret_cast<float*>(slot));
     check(hipMemcpyAsync(pool, h.pool.data(), h.pool.size() * 4, hipMemcpyHostToDevice, stream_), "spai0 upload");
     check(hipMemcpyAsync(slot, h.slot.data(), h.slot.size() * 4, hipMemcpyHostToDevice, stream_), "spai0 upload");
+    // the wave records carry the new SPAI0 slots
+    const std::vector<int32_t> rec = wave_records(h);
+    int32_t* wrec = nullptr;
+    if (check(hipMalloc(reinterpret_cast<void**>(&wrec), std::max<size_t>(rec.size() * 4, 4)), "spai0 alloc"))
+        return -3;
+    spai0_bufs_.push_back(reinterpret_cast<float*>(wrec));
+    check(hipMemcpyAsync(wrec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice, stream_), "spai0 upload");
     lv_[level].v.pool = pool;
     lv_[level].v.slot = slot;
+    lv_[level].v.wrec = wrec;
     return check(hipStreamSynchronize(stream_), "spai0 sync");
 }
 
