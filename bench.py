@@ -6,7 +6,10 @@ pool plus 8 seeded liquid spheres (r in [0.05, 0.2]) around a static solid
 sphere (r = 0.2 at the centre) that gives fractional face weights; rhs = the
 divergence of seeded uniform[-1,1] face velocities; zero initial guess;
 solveMultigridPCG to 1e-7 relative residual (infinity norm) with the
-scheduled-relaxed-Jacobi W-cycle, FP32 throughout.
+scheduled-relaxed-Jacobi W-cycle. Three forms are timed: plain FP32 (the
+reference's arithmetic), FP32 with the FP64 residual replacement, and the
+measured form - BF16-stored W-cycle vectors with FP32 arithmetic, FP64 dots
+and the FP64 residual replacement.
 
 A step is one full solve with the hierarchy built and b already resident in
 HBM; the timed region covers K solves (each zeroes x on the device first).

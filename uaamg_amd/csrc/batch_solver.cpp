@@ -209,7 +209,8 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
     for (int l = 0; l < L; ++l) {
         const HostLevel& h = H[size_t(l)];
         const size_t n = size_t(nleaf[size_t(l)]);
-        bytes += al(h.mask.size() * 8) + al(h.nbr.size() * 4) + al(h.nb27.size() * 4) + al(h.slot.size() * 4) +
+        bytes += al(h.mask.size() * 8) + al(h.nbr.size() * 4) + (l >= 1 ? al(h.nb27.size() * kNb27Stride * 4) : 0) +
+                 al(h.slot.size() * 4) +
                  al(h.child.size() * 4) + al(h.parent.size() * 4) + al(h.dofbase.size() * 8);
         for (const auto& p : h.pool) bytes += al(p.size() * 4);
         bytes += 4 * al(n * kLeafSize * 4);
@@ -257,7 +258,15 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
         v.dflt_spai0 = s0.default_spai0_apply();
         v.mask = static_cast<const uint64_t*>(up(h.mask.data(), h.mask.size() * 8));
         v.nbr = static_cast<const int32_t*>(up(h.nbr.data(), h.nbr.size() * 4));
-        v.nb27 = static_cast<const int32_t*>(up(h.nb27.data(), h.nb27.size() * 4));
+        std::vector<int32_t> nbrec;
+        if (l >= 1) {
+            nbrec.resize(h.nb27.size() * kNb27Stride);
+            build_nb27_records(nbrec.data(), h.nb27.data(), h.slot.data(), h.parent.data(), n);
+            v.nb27 = static_cast<const int32_t*>(up(nbrec.data(), nbrec.size() * 4));
+            (void)check(hipStreamSynchronize(stream_), "upload sync");   // nbrec dies at scope end
+        } else {
+            v.nb27 = nullptr;
+        }
         v.slot = static_cast<const int32_t*>(up(h.slot.data(), h.slot.size() * 4));
         v.dofbase = static_cast<const int64_t*>(up(h.dofbase.data(), h.dofbase.size() * 8));
         v.parent = static_cast<const int32_t*>(up(h.parent.data(), h.parent.size() * 4));

@@ -116,13 +116,14 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
     const int leaf = logical_block();
     const int tid = threadIdx.x;
     if (tid < 27) {
-        const int n = L.nb27[size_t(leaf) * 27 + tid];
-        nl[tid] = n;
-        sd[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotDiag] : -1;
-        sx[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotX] : -1;
-        sy[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotY] : -1;
-        sz[tid] = n >= 0 ? L.slot[size_t(n) * kSlotStride + kSlotZ] : -1;
-        par[tid] = (MODE == kPost && n >= 0) ? L.parent[n] : -1;
+        const int4* r = reinterpret_cast<const int4*>(L.nb27 + (size_t(leaf) * 27 + tid) * kNb27Stride);
+        const int4 a = r[0], b = r[1];
+        nl[tid] = a.x;
+        sd[tid] = a.y;
+        sx[tid] = a.z;
+        sy[tid] = a.w;
+        sz[tid] = b.x;
+        par[tid] = MODE == kPost ? b.y : -1;
     }
     __syncthreads();
 

@@ -62,6 +62,25 @@ inline void complete_leaf_records(int32_t* rec, const int32_t* nbr, const int32_
     }
 }
 
+// Neighbourhood records of the block-visit kernels: per leaf and offset
+// (ax,ay,az) in {-1,0,1}^3 (k = (ax+1)*9 + (ay+1)*3 + az+1) the neighbour leaf
+// (-1 none), its diagonal / x / y / z slots and its parent (-1 when the leaf
+// is absent), so a workgroup's staging depends on one memory round trip.
+constexpr int kNb27Stride = 8;
+
+// nb27: [n][27] neighbour leaves; rec: the level's leaf records; parent [n] or null.
+inline void build_nb27_records(int32_t* out, const int32_t* nb27, const int32_t* rec, const int32_t* parent,
+                               size_t n) {
+    for (size_t i = 0; i < n * 27; ++i) {
+        const int32_t m = nb27[i];
+        int32_t* r = out + i * kNb27Stride;
+        r[0] = m;
+        for (int k = 0; k < 4; ++k) r[1 + k] = m >= 0 ? rec[size_t(m) * kSlotStride + size_t(kSlotDiag + k)] : -1;
+        r[5] = (m >= 0 && parent) ? parent[m] : -1;
+        r[6] = r[7] = -1;
+    }
+}
+
 // smoothing / apply modes of LaplacianApplySIMD (simd_vdb_poisson.cpp:683-690)
 enum ApplyMode : int {
     kModeLaplacian = 0,
@@ -90,7 +109,7 @@ struct LevelView {
     const float* cz;
     const float* spai0;
     float dflt_diag, dflt_off, dflt_invdiag, dflt_spai0;
-    const int32_t* nb27;      // [nleaf][27] leaf at offset (ax,ay,az) in {-1,0,1}^3 (block-visit kernels)
+    const int32_t* nb27;      // [nleaf][27][kNb27Stride] neighbourhood records (block-visit kernels; null on level 0)
 };
 
 // Scalars of the outer PCG, kept in device memory so a whole iteration runs

@@ -221,7 +221,7 @@ size_t PoissonDevice::device_bytes() const {
         add(n * sizeof(int64_t));
         add(n * sizeof(int32_t));         // parent
         add(n * 8 * sizeof(int32_t));     // child
-        add(n * 27 * sizeof(int32_t));    // 3x3x3 leaf neighbourhood
+        if (l >= 1) add(n * 27 * kNb27Stride * sizeof(int32_t));   // neighbourhood records
         size_t nslot[kNumCoef] = {0, 0, 0, 0, 0};
         for (int k = 0; k < kNumCoef; ++k)
             for (size_t i = 0; i < n; ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
@@ -295,6 +295,23 @@ int PoissonDevice::acquire_device() {
                                                  rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
                                   "upload leaf records"))
             return -3;
+        std::vector<int32_t> nbrec;
+        if (l >= 1 && lv_[l].v.nb27 && !parent.empty()) {
+            const auto& hl = host_.levels[l];
+            const size_t n = hl.nleaf();
+            std::vector<int32_t> ids(n * 27);
+            for (size_t i = 0; i < n; ++i)
+                for (int k = 0; k < 27; ++k)
+                    ids[i * 27 + size_t(k)] =
+                        hl.lookup.find({hl.origin[i].x + 8 * (k / 9 - 1), hl.origin[i].y + 8 * ((k / 3) % 3 - 1),
+                                        hl.origin[i].z + 8 * (k % 3 - 1)});
+            nbrec.resize(n * 27 * kNb27Stride);
+            build_nb27_records(nbrec.data(), ids.data(), rec.data(), parent.data(), n);
+            if (check(hipMemcpyAsync(const_cast<int32_t*>(lv_[l].v.nb27), nbrec.data(), nbrec.size() * sizeof(int32_t),
+                                     hipMemcpyHostToDevice, stream_),
+                      "upload neighbourhood records"))
+                return -3;
+        }
         // the host vectors die at the end of this iteration
         if (check(hipStreamSynchronize(stream_), "upload parent")) return -3;
     }
@@ -491,15 +508,10 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
     };
     v.mask = static_cast<const uint64_t*>(up(h.mask.data(), n * 8 * sizeof(uint64_t)));
     v.nbr = static_cast<const int32_t*>(up(h.nbr.data(), n * 6 * sizeof(int32_t)));
-    {
-        std::vector<int32_t> nb27(n * 27, -1);
-        for (size_t i = 0; i < n; ++i)
-            for (int k = 0; k < 27; ++k)
-                nb27[i * 27 + k] = h.lookup.find({h.origin[i].x + 8 * (k / 9 - 1), h.origin[i].y + 8 * ((k / 3) % 3 - 1),
-                                                  h.origin[i].z + 8 * (k % 3 - 1)});
-        v.nb27 = static_cast<const int32_t*>(up(nb27.data(), nb27.size() * sizeof(int32_t)));
-        (void)check(hipStreamSynchronize(stream_), "upload sync");   // nb27 dies at scope end
-    }
+    // neighbourhood records of the block-visit kernels (levels >= 1), filled
+    // once the parents are known (acquire_device)
+    v.nb27 = h.level >= 1 ? static_cast<const int32_t*>(arena_.take(std::max<size_t>(n * 27 * kNb27Stride * 4, 4)))
+                          : nullptr;
     int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < n; ++i)
         for (int k = 0; k < kNumCoef; ++k) count[k] += h.keep[k][i] ? 1 : 0;
