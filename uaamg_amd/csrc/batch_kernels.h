@@ -80,7 +80,9 @@ hipError_t launch_bfinalize(int op, int nprob, const int32_t* blk_begin, const d
 // Publishes (sequence << 32 | number of problems still iterating) to the mailbox.
 hipError_t launch_bstatus(int nprob, const BatchPcg* st, unsigned int* seq, HostMailbox* mb, hipStream_t s);
 // Coarsest solves of a group of problems, one workgroup each.
-hipError_t launch_bcoarsest(const CoarsestView* views, int n, const float* b, float* x, hipStream_t s);
-hipError_t launch_bcoarsest(const CoarsestView* views, int n, const bf16* b, bf16* x, hipStream_t s);
+hipError_t launch_bcoarsest(const CoarsestView* views, int n, int max_ndof, const float* b, float* x,
+                            hipStream_t s);
+hipError_t launch_bcoarsest(const CoarsestView* views, int n, int max_ndof, const bf16* b, bf16* x,
+                            hipStream_t s);
 
 }  // namespace uaamg

@@ -200,6 +200,7 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
     for (int k = 0; k < K; ++k) {
         const int c = nlev_[size_t(k)] - 1;
         coarsest_system(hl(k, c), off[size_t(c)][size_t(k)], cpos[size_t(k)], cnb[size_t(k)], ccoef[size_t(k)]);
+        lv_[size_t(c)].coarse_max_ndof = std::max(lv_[size_t(c)].coarse_max_ndof, int(cpos[size_t(k)].size()));
     }
 
     // one arena for everything
@@ -450,7 +451,8 @@ void PoissonBatch::tune_levels() {
 void PoissonBatch::precond(int l, float* X, const float* B, bool first, int dot_op) {
     Level& d = lv_[size_t(l)];
     if (first && d.coarse_count)
-        check(launch_bcoarsest(views_ + d.coarse_first, d.coarse_count, B, X, stream_), "coarsest");
+        check(launch_bcoarsest(views_ + d.coarse_first, d.coarse_count, d.coarse_max_ndof, B, X, stream_),
+              "coarsest");
     const LevelView& v = d.sv;
     if (v.nleaf > 0) {
         const bool weighted = opt_.smoother == kWeightedJacobi;
@@ -502,7 +504,8 @@ template <class TX, class TB>
 void PoissonBatch::precond_mixed(int l, TX* X, const TB* B, bool first, int dot_op) {
     Level& d = lv_[size_t(l)];
     if (first && d.coarse_count)
-        check(launch_bcoarsest(views_ + d.coarse_first, d.coarse_count, B, X, stream_), "coarsest");
+        check(launch_bcoarsest(views_ + d.coarse_first, d.coarse_count, d.coarse_max_ndof, B, X, stream_),
+              "coarsest");
     const LevelView& v = d.sv;
     if (v.nleaf > 0) {
         const bool weighted = opt_.smoother == kWeightedJacobi;

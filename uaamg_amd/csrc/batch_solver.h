@@ -66,6 +66,7 @@ private:
         float *X = nullptr, *B = nullptr, *T = nullptr, *U = nullptr;
         bf16 *Xh = nullptr, *Bh = nullptr, *Th = nullptr, *Uh = nullptr;
         int coarse_first = 0, coarse_count = 0;   // problems whose coarsest level this is (merged order)
+        int coarse_max_ndof = 0;                  // largest of their coarsest systems
     };
     struct Graph {
         hipGraph_t g = nullptr;

@@ -138,7 +138,8 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
         const int q = tid + pass * kBlkThreads;
         if (q >= N) break;
         const int gx = q / (S * S) - H, gy = (q / S) % S - H, gz = q % S - H;
-        const int ax = gx < 0 ? 0 : (gx > 7 ? 2 : 1), ay = gy < 0 ? 0 : (gy > 7 ? 2 : 1), az = gz < 0 ? 0 : (gz > 7 ? 2 : 1);
+        const int ax = gx < 0 ? 0 : (gx > 7 ? 2 : 1), ay = gy < 0 ? 0 : (gy > 7 ? 2 : 1);
+        const int az = gz < 0 ? 0 : (gz > 7 ? 2 : 1);
         const int k = ax * 9 + ay * 3 + az;
         const int n = nl[k];
         const int lx = gx & 7, ly = gy & 7, lz = gz & 7;

@@ -51,10 +51,13 @@ __device__ __forceinline__ float2 cg_sum2(float a, float b, CgShared<kThreads>& 
 }
 
 // Solves the coarsest system for right-hand side B (bricks) into X (bricks,
-// DOF positions only). Every thread of the block must call it.
-template <int kThreads, class V = float>
+// DOF positions only); C.ndof <= kMaxDof (thread t owns rows t + j*kThreads,
+// so a smaller kMaxDof only drops row slots that would be empty: the results
+// do not depend on it). Every thread of the block must call it.
+template <int kThreads, class V = float, int kMaxDof = kMaxCoarsestDof>
 __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgShared<kThreads>& sh) {
-    constexpr int kPer = kMaxCoarsestDof / kThreads;
+    static_assert(kMaxDof <= kMaxCoarsestDof && kMaxDof % kThreads == 0, "row slots");
+    constexpr int kPer = kMaxDof / kThreads;
     int parity = 0;
     const int n = C.ndof;
     const int tid = threadIdx.x;
