@@ -108,7 +108,9 @@ def main() -> int:
     t0 = time.perf_counter()
     lap = U.LaplacianWithLevel(scene.phi, scene.weights, dt=1.0, dx=scene.dx, device=0)
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
     b_host = lap.createPressurePoissonRightHandSide(scene.weights, scene.vx, scene.vy, scene.vz)
+    t_rhs_host = time.perf_counter() - t0
 
     def make_solver(l):
         sol = U.PoissonSolver(l)
@@ -255,6 +257,8 @@ def main() -> int:
         "roofline_fp32": roofline_fp32,
     }
 
+    out["rhs_build"] = rhs_build(lap, scene, b_host, t_rhs_host, dev)
+
     if not args.no_cpu_baseline:
         import oracle as O
         orc = O.OraclePoisson(scene.phi, scene.weights, 1.0, scene.dx)
@@ -290,6 +294,39 @@ def main() -> int:
         out["serving"] = run_serving(args, dev)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def rhs_build(lap, scene, b_host, t_host_s: float, dev) -> dict:
+    """The right-hand side built on the host (uaamg_poisson_build_rhs) and on the
+    device (uaamg_poisson_build_rhs_device) from host-resident grids (values
+    uploaded in the call) and from device-resident grid values; the device
+    results are checked bit for bit against the host's."""
+    import numpy as np
+    import torch
+
+    grids = [scene.weights, scene.vx, scene.vy, scene.vz]
+    out = torch.empty(lap.mNumDof, dtype=torch.float32, device=dev)
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    ms_upload = timed(lambda: lap.createPressurePoissonRightHandSide_device(out.data_ptr(), *grids))
+    same_upload = bool(np.array_equal(out.cpu().numpy(), b_host))
+    dvals = [torch.from_numpy(np.ascontiguousarray(g.values, dtype=np.float32)).to(dev) for g in grids]
+    ptrs = [t.data_ptr() for t in dvals]
+    out.zero_()
+    ms_resident = timed(lambda: lap.createPressurePoissonRightHandSide_device(out.data_ptr(), *grids, values_ptrs=ptrs))
+    same_resident = bool(np.array_equal(out.cpu().numpy(), b_host))
+    return {"host_ms": round(1e3 * t_host_s, 2), "device_ms_values_uploaded": round(ms_upload, 2),
+            "device_ms_values_resident": round(ms_resident, 2),
+            "input_mb": round(sum(g.values.nbytes for g in grids) / 1e6, 1),
+            "identical_to_host": same_upload and same_resident}
 
 
 def run_serving(args, dev) -> dict:

@@ -166,6 +166,22 @@ int uaamg_poisson_build_rhs(const uaamg_poisson* h, const uaamg_grid* face_weigh
                             float* rhs);
 
 /*
+ * The same right-hand side built on the device into rhs_dev (ndof floats, DOF order, device memory),
+ * bit-identical to uaamg_poisson_build_rhs: one wavefront per level-0 DOF leaf reads the face weights,
+ * liquid and solid velocities at each DOF voxel and at its +x/+y/+z neighbour voxel. The grids' leaves
+ * are matched to the DOF leaves by origin on the host (leaf tables, O(leaves)); values_on_device = 0:
+ * each grid's values are host memory, uploaded for the call; 1: they are device memory
+ * ([nleaf][512][ncomp] in the grid's own leaf order; origins stay host memory). solid_velocity may be
+ * NULL (zero). Allocates device memory for the call (values when uploaded, the leaf tables) and
+ * returns after rhs_dev is written.
+ * Replaces LaplacianWithLevel::createPressurePoissonRightHandSide (simd_vdb_poisson.h:59-65, .cpp:101-127)
+ * for callers whose velocities live on the GPU, or whose host cannot afford the per-voxel loop.
+ */
+int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                                   const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
+                                   int32_t values_on_device, float* rhs_dev);
+
+/*
  * PoissonSolver::solveMultigridPCG (simd_vdb_poisson.h:162, .cpp:2155-2226) on host DOF-ordered
  * vectors: x holds the initial guess on entry and the pressure on exit. Returns UAAMG_SUCCESS or
  * UAAMG_FAILED; *iterations receives mIterationTaken.

@@ -110,6 +110,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_level_topology": (C.c_int, [vp, C.c_int, vp, vp, vp]),
         "uaamg_poisson_level_coefficients": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "uaamg_poisson_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
+        "uaamg_poisson_build_rhs_device": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), i32, vp]),
         "uaamg_poisson_solve_pcg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_pcg_device": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_last_residual": (C.c_int, [vp, P(f32), P(i32)]),
@@ -205,15 +206,16 @@ class Grid:
     def ncomp(self) -> int:
         return 1 if self.values.ndim == 2 else 3
 
-    def _struct(self) -> _Grid:
+    def _struct(self, values_ptr: int | None = None) -> _Grid:
+        """values_ptr: address of the values in device memory (same layout) instead of self.values"""
         o = np.ascontiguousarray(self.origins, dtype=np.int32)
-        v = np.ascontiguousarray(self.values, dtype=np.float32)
+        v = None if values_ptr is not None else np.ascontiguousarray(self.values, dtype=np.float32)
         a = None if self.active is None else np.ascontiguousarray(self.active, dtype=np.uint64)
         self._keep = (o, v, a)   # keep alive while the struct is in use
         g = _Grid()
         g.nleaf = o.shape[0]
         g.origins = o.ctypes.data
-        g.values = v.ctypes.data
+        g.values = values_ptr if values_ptr is not None else v.ctypes.data
         g.active = None if a is None else a.ctypes.data
         g.ncomp = self.ncomp
         bg = self.background if isinstance(self.background, (tuple, list)) else (self.background,) * 3
@@ -394,6 +396,21 @@ class LaplacianWithLevel:
         sv = None if solid_velocity is None else C.byref(solid_velocity._struct())
         _check(lib().uaamg_poisson_build_rhs(self._h, *[C.byref(x) for x in g], sv, _ptr(rhs)), "build_rhs")
         return rhs
+
+    def createPressurePoissonRightHandSide_device(self, rhs_ptr: int, face_weights: Grid, vx: Grid, vy: Grid,
+                                                  vz: Grid, solid_velocity: Grid | None = None,
+                                                  values_ptrs: list | None = None) -> None:
+        """The right-hand side built on the device into rhs_ptr (ndof floats, DOF order, device memory).
+        values_ptrs: device addresses of the grids' values, in argument order (None: upload
+        each grid's host values for the call)."""
+        grids = [face_weights, vx, vy, vz] + ([] if solid_velocity is None else [solid_velocity])
+        if values_ptrs is not None and len(values_ptrs) != len(grids):
+            raise ValueError("one device values pointer per grid")
+        st = [g._struct(None if values_ptrs is None else values_ptrs[i]) for i, g in enumerate(grids)]
+        sv = None if solid_velocity is None else C.byref(st[4])
+        _check(lib().uaamg_poisson_build_rhs_device(self._h, *[C.byref(x) for x in st[:4]], sv,
+                                                    0 if values_ptrs is None else 1, C.c_void_p(rhs_ptr)),
+               "build_rhs_device")
 
     def getZeroVectorGrid(self) -> np.ndarray:
         return np.zeros(self.mNumDof, dtype=np.float32)

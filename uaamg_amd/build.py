@@ -33,6 +33,8 @@ SOURCES = [
     "batch_solver.cpp",
     "server.cpp",
     "capi_server.cpp",
+    "rhs_kernels.hip",
+    "rhs_device.cpp",
 ]
 
 ARCH = os.environ.get("UAAMG_OFFLOAD_ARCH", "gfx950")

@@ -15,6 +15,7 @@
 #include "batch_solver.h"
 #include "poisson_host.h"
 #include "poisson_solver.h"
+#include "rhs_device.h"
 #include "capi_common.h"
 #include "scene.h"
 
@@ -276,6 +277,38 @@ int uaamg_poisson_build_rhs(const uaamg_poisson* h, const uaamg_grid* face_weigh
         sv.lookup.build(sv.origin);
     }
     build_poisson_rhs(h->dev.host().levels[0], w, gx, gy, gz, sv, rhs);
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                                   const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
+                                   int32_t values_on_device, float* rhs_dev) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !rhs_dev) return fail(UAAMG_EINVAL, "null argument");
+    const uaamg_grid* in[kRhsGrids] = {face_weights, vx, vy, vz, solid_velocity};
+    const int ncomp[kRhsGrids] = {3, 1, 1, 1, 3};
+    const char* names[kRhsGrids] = {"face_weights", "vx", "vy", "vz", "solid_velocity"};
+    RhsGridRef refs[kRhsGrids];
+    for (int g = 0; g < kRhsGrids; ++g) {
+        RhsGridRef& r = refs[g];
+        r.ncomp = ncomp[g];
+        if (!in[g]) {
+            if (g != kRhsSolid) return fail(UAAMG_EINVAL, std::string(names[g]) + ": null grid");
+            continue;   // zero solid velocity: an empty grid with background 0
+        }
+        if (in[g]->ncomp != ncomp[g]) return fail(UAAMG_EINVAL, std::string(names[g]) + ": wrong component count");
+        if (in[g]->nleaf < 0 || (in[g]->nleaf > 0 && (!in[g]->origins || !in[g]->values)))
+            return fail(UAAMG_EINVAL, std::string(names[g]) + ": null arrays");
+        r.nleaf = in[g]->nleaf;
+        r.origins = in[g]->origins;
+        r.values = in[g]->values;
+        r.values_on_device = values_on_device != 0;
+        for (int c = 0; c < 3; ++c) r.bg[c] = in[g]->background[c];
+    }
+    std::string err;
+    const int rc = build_poisson_rhs_device(h->dev, refs, rhs_dev, err);
+    if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
     return 0;
     UAAMG_GUARD_END
 }

@@ -263,14 +263,6 @@ __global__ __launch_bounds__(kBlockThreads) void residual_restrict_kernel(LevelV
     st1(Cv + size_t(cl) * 512 + off, sum * 0.125f);
 }
 
-__device__ __forceinline__ int64_t row_dof_base(const LevelView& L, int leaf, int lane) {
-    const uint64_t* m = L.mask + size_t(leaf) * 8;
-    const int lx = lane >> 3, ly = lane & 7;
-    int64_t base = L.dofbase[leaf];
-    for (int w = 0; w < lx; ++w) base += __popcll(m[w]);
-    const uint64_t below = ly == 0 ? 0ull : (m[lx] & ((uint64_t(1) << (ly * 8)) - 1));
-    return base + __popcll(below);
-}
 
 __global__ __launch_bounds__(kBlockThreads) void scatter_dof_kernel(LevelView L, const float* dv, float* Y) {
     const int leaf = wave_leaf();

@@ -78,6 +78,9 @@ public:
     const PoissonHierarchyHost& host() const { return host_; }
     SolverOptions& options() { return opt_; }
     hipStream_t stream() const { return stream_; }
+    int device() const { return opt_.device; }
+    // device view of a level (on_device() only)
+    const LevelView& level_view(int l) const { return lv_[size_t(l)].v; }
     const std::string& error() const { return err_; }
 
     // Solves with DOF-ordered vectors in device memory. x is the initial guess
@@ -104,8 +107,6 @@ public:
     // the solver's stream, timed with HIP events on that stream.
     int time_apply(int level, int mode, int reps, float* ms_per_launch);
 
-    // ||b - A x||_inf of the last solve's result (FP64 residual) when residual
-    // replacement is on, and how many replacements it made; -1 otherwise
     // Level-to-kernel mapping chosen at acquire time: per level, whether a
     // W-cycle visit runs as two block-visit launches, and the two timings the
     // choice came from (ms per visit pair; -1 where the level was not timed).
@@ -114,6 +115,8 @@ public:
         return level < int(tune_ms_[variant].size()) ? tune_ms_[variant][size_t(level)] : -1.f;
     }
 
+    // ||b - A x||_inf of the last solve's result (FP64 residual) when residual
+    // replacement is on, and how many replacements it made; -1 otherwise
     float last_true_residual() const { return true_nu_; }
     int last_replacements() const { return replacements_; }
 

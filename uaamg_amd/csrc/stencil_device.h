@@ -331,6 +331,16 @@ __device__ __forceinline__ void mask_row(uint32_t rm, float y[8]) {
         if (!((rm >> z) & 1u)) y[z] = 0.f;
 }
 
+// index of the first DOF of a leaf's z-row (DOFs numbered in leaf order, then voxel offset)
+__device__ __forceinline__ int64_t row_dof_base(const LevelView& L, int leaf, int lane) {
+    const uint64_t* m = L.mask + size_t(leaf) * 8;
+    const int lx = lane >> 3, ly = lane & 7;
+    int64_t base = L.dofbase[leaf];
+    for (int w = 0; w < lx; ++w) base += __popcll(m[w]);
+    const uint64_t below = ly == 0 ? 0ull : (m[lx] & ((uint64_t(1) << (ly * 8)) - 1));
+    return base + __popcll(below);
+}
+
 inline unsigned grid_for(int32_t nleaf) { return unsigned((nleaf + kWavesPerBlock - 1) / kWavesPerBlock); }
 
 }  // namespace dev
