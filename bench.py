@@ -258,6 +258,7 @@ def main() -> int:
     }
 
     out["rhs_build"] = rhs_build(lap, scene, b_host, t_rhs_host, dev)
+    out["pressure_gradient"] = pressure_gradient(lap, scene, x_mx)
 
     if not args.no_cpu_baseline:
         import oracle as O
@@ -327,6 +328,25 @@ def rhs_build(lap, scene, b_host, t_host_s: float, dev) -> dict:
             "device_ms_values_resident": round(ms_resident, 2),
             "input_mb": round(sum(g.values.nbytes for g in grids) / 1e6, 1),
             "identical_to_host": same_upload and same_resident}
+
+
+def pressure_gradient(lap, scene, pressure) -> dict:
+    """The step after the solve on the device (uaamg_poisson_apply_pressure_gradient_device) on the
+    solved pressure, velocity grids in host memory (uploaded and downloaded in the call): ms per call
+    and the number of faces it updated."""
+    import numpy as np
+    import torch
+
+    vel = [scene.vx, scene.vy, scene.vz]
+    lap.apply_pressure_gradient_device(pressure.data_ptr(), scene.phi, scene.weights, None, vel, 0.04, 0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = lap.apply_pressure_gradient_device(pressure.data_ptr(), scene.phi, scene.weights, None, vel, 0.04, 0.0)
+    ms = 1e3 * (time.perf_counter() - t0)
+    faces = int(sum(np.unpackbits(a.view(np.uint8)).sum() for a, _, _, _ in out))
+    mb = sum(g.values.nbytes for g in vel) / 1e6
+    return {"ms_per_call": round(ms, 2), "faces_updated": faces, "velocity_mb": round(mb, 1),
+            "io": "velocity, update and reflected grids through host memory"}
 
 
 def run_serving(args, dev) -> dict:

@@ -182,6 +182,38 @@ int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weig
                                    int32_t values_on_device, float* rhs_dev);
 
 /*
+ * One face-velocity channel of the caller's grid, updated in place: leaves in any order, active
+ * masks [nleaf][8] (host memory, in/out), values [nleaf][512] (in/out) and the optional outputs
+ * update / reflected [nleaf][512] (NULL: not produced). values, update and reflected are host or
+ * device memory as the call's values_on_device says.
+ */
+typedef struct uaamg_face_channel {
+    int64_t nleaf;
+    const int32_t* origins;
+    uint64_t* active;
+    float* values;
+    float* update;
+    float* reflected;
+} uaamg_face_channel;
+
+/*
+ * The step after the solve on the device: the pressure gradient applied to the face velocities.
+ * For every active face of channel c with weight w > 0 where the pressure has a DOF on either side,
+ *   du = -dt (p(v) - p(v - e_c)) / dx / theta, theta = fraction_inside(phi(v - e_c), phi(v))
+ *   clamped to >= 0.02 when either phi >= 0 (else 1); u += du; reflected = u + du; and where w < 1
+ *   u = (1 - s) u + s u_solid with s = (1 - w) friction;
+ * every other active face is turned off. update receives du on updated faces and the incoming
+ * velocity elsewhere (the reference's deep copy), reflected the reflected velocity likewise; the
+ * active masks end as the updated faces. pressure_dev: the solve's DOF-ordered result (device).
+ * Replaces FLIP_Solver_OpenVDB::apply_pressure_gradient (FLIP_vdb.cpp:2571-2667); bit-identical to
+ * its restatement. channels: 3 entries (x, y, z); solid_velocity may be NULL (zero).
+ */
+int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* pressure_dev,
+                                                 const uaamg_grid* liquid_phi, const uaamg_grid* face_weights,
+                                                 const uaamg_grid* solid_velocity, uaamg_face_channel* channels,
+                                                 float dt, float friction, int32_t values_on_device);
+
+/*
  * PoissonSolver::solveMultigridPCG (simd_vdb_poisson.h:162, .cpp:2155-2226) on host DOF-ordered
  * vectors: x holds the initial guess on entry and the pressure on exit. Returns UAAMG_SUCCESS or
  * UAAMG_FAILED; *iterations receives mIterationTaken.

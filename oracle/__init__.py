@@ -47,6 +47,8 @@ def lib() -> C.CDLL:
         "orc_level_coefficients": (None, [vp, i32, i32, vp, vp]),
         "orc_build_rhs": (None, [vp, i64, vp, vp, vp, i64, vp, vp, vp, vp, f32, vp]),
         "orc_apply": (i32, [vp, i32, i32, f32, vp, vp, vp]),
+        "orc_apply_pressure_gradient": (None, [vp, vp, i64, vp, vp, f32, i64, vp, vp, vp, i64, vp, vp, vp, i32,
+                                               i64, vp, vp, vp, vp, vp, f32, f32]),
         "orc_first_sweep": (i32, [vp, i32, i32, f32, vp, vp]),
         "orc_restrict": (i32, [vp, i32, vp, vp]),
         "orc_prolong": (i32, [vp, i32, vp, vp, f32]),
@@ -159,6 +161,37 @@ class OraclePoisson:
         lib().orc_build_rhs(self.h, wo.shape[0], _p(wo), _p(wv), _p(wbg), vo.shape[0], _p(vo), *[_p(x) for x in xs],
                             vbg, _p(rhs))
         return rhs
+
+    def apply_pressure_gradient(self, pressure, phi, weights, solid, velocity, dt: float, friction: float):
+        """FLIP_Solver_OpenVDB::apply_pressure_gradient restated (FLIP_vdb.cpp:2571-2667).
+        velocity: 3 (origins [n,3], active [n,8], values [n,512]) per channel; returns per channel
+        (active, values, update, reflected) after the update."""
+        pr = np.ascontiguousarray(pressure, np.float32)
+        pa = np.ascontiguousarray(phi.origins, np.int32)
+        pv = np.ascontiguousarray(phi.values, np.float32)
+        wa = np.ascontiguousarray(weights.origins, np.int32)
+        wv = np.ascontiguousarray(weights.values, np.float32)
+        wb = np.asarray(weights.background, np.float32).reshape(-1)
+        if solid is None:
+            sa, sv, sb = np.zeros((0, 3), np.int32), np.zeros((0, 512, 3), np.float32), np.zeros(3, np.float32)
+        else:
+            sa = np.ascontiguousarray(solid.origins, np.int32)
+            sv = np.ascontiguousarray(solid.values, np.float32)
+            sb = np.asarray(solid.background, np.float32).reshape(-1)
+        out = []
+        for c, (org, act, val) in enumerate(velocity):
+            org = np.ascontiguousarray(org, np.int32)
+            act = np.array(act, dtype=np.uint64, copy=True)
+            val = np.array(val, dtype=np.float32, copy=True)
+            upd = np.zeros_like(val)
+            refl = np.zeros_like(val)
+            bg = phi.background[0] if isinstance(phi.background, (tuple, list)) else phi.background
+            lib().orc_apply_pressure_gradient(self.h, _p(pr), pa.shape[0], _p(pa), _p(pv), float(bg), wa.shape[0],
+                                              _p(wa), _p(wv), _p(wb), sa.shape[0], _p(sa), _p(sv), _p(sb), c,
+                                              org.shape[0], _p(org), _p(act), _p(val), _p(upd), _p(refl),
+                                              float(dt), float(friction))
+            out.append((act, val, upd, refl))
+        return out
 
     def apply(self, level, mode, x, b=None, weight=0.0):
         x = np.ascontiguousarray(x, np.float32)

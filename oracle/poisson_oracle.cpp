@@ -1090,6 +1090,75 @@ void orc_build_rhs(void* p, int64_t nw, const int32_t* w_org, const float* w_val
     }
 }
 
+// FLIP_Solver_OpenVDB::apply_pressure_gradient (FLIP_vdb.cpp:2571-2667), one
+// channel c: the pressure grid is the level-0 DOF topology (TopologyCopy of
+// the DOF index grid, 0 off the DOFs) holding the DOF-ordered pressure. The
+// velocity leaves come in the caller's order; act / val are updated in place,
+// upd / refl receive the update and reflected grids (the incoming velocity
+// where no update happens, as after the reference's deepCopy).
+void orc_apply_pressure_gradient(void* p, const float* pressure, int64_t nphi, const int32_t* phi_org,
+                                 const float* phi_val, float phi_bg, int64_t nw, const int32_t* w_org,
+                                 const float* w_val, const float* w_bg, int64_t ns, const int32_t* s_org,
+                                 const float* s_val, const float* s_bg, int c, int64_t nv, const int32_t* v_org,
+                                 uint64_t* act, float* val, float* upd, float* refl, float dt, float friction) {
+    Handle* h = static_cast<Handle*>(p);
+    const Level& L = h->levels[0];
+    FGrid phi;
+    VGrid w, sv;
+    load_grid<float>(nphi, phi_org, nullptr, phi_val, 1, &phi_bg, phi);
+    load_grid<Vec3>(nw, w_org, nullptr, w_val, 3, w_bg, w);
+    const float zero3[3] = {0.f, 0.f, 0.f};
+    load_grid<Vec3>(ns, s_org, nullptr, s_val, 3, s_bg ? s_bg : zero3, sv);
+    auto p_on = [&](Coord g) { return L.dof.is_on(g); };
+    auto p_val = [&](Coord g) { return L.dof.is_on(g) ? pressure[L.dof.get(g)] : 0.f; };
+    for (int64_t i = 0; i < nv; ++i) {
+        const Coord o{v_org[3 * i], v_org[3 * i + 1], v_org[3 * i + 2]};
+        for (int off = 0; off < 512; ++off) {
+            float& u = val[i * 512 + off];
+            const float u0 = u;
+            upd[i * 512 + off] = u0;
+            refl[i * 512 + off] = u0;
+            uint64_t& word = act[i * 8 + (off >> 6)];
+            const uint64_t bit = uint64_t(1) << (off & 63);
+            if (!(word & bit)) continue;
+            const Coord lc = local_of(off);
+            const Coord g{o.x + lc.x, o.y + lc.y, o.z + lc.z};
+            bool has_update = false;
+            float updated = u0, reflected = u0, du = 0.f;
+            const float weight = w.get(g).v[c];
+            Coord lower = g;
+            lower[c] -= 1;
+            if (weight > 0.0f) {
+                if (p_on(g) || p_on(lower)) has_update = true;
+                if (has_update) {
+                    const float phi_this = phi.get(g), phi_below = phi.get(lower);
+                    const float p_this = p_val(g), p_below = p_val(lower);
+                    float theta = 1.0f;
+                    if (phi_this >= 0 || phi_below >= 0) {
+                        theta = fraction_inside(phi_below, phi_this);
+                        if (theta < 0.02f) theta = 0.02f;
+                    }
+                    du = -dt * (p_this - p_below) / L.dx / theta;
+                    updated += du;
+                    reflected = updated + du;
+                    if (weight < 1.0f) {
+                        const float svel = sv.get(g).v[c];
+                        const float sf = (1.0f - weight) * friction;
+                        updated = (1.0f - sf) * updated + sf * svel;
+                    }
+                }
+            }
+            if (has_update) {
+                u = updated;
+                upd[i * 512 + off] = du;
+                refl[i * 512 + off] = reflected;
+            } else {
+                word &= ~bit;
+            }
+        }
+    }
+}
+
 int orc_apply(void* p, int l, int mode, float w, const float* x, const float* b, float* y) {
     Handle* h = static_cast<Handle*>(p);
     Level& L = h->levels[l];

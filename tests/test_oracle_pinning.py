@@ -343,3 +343,93 @@ def test_scene_system_matches_scipy():
     r = b - orc.apply64(x64)
     assert np.abs(r).max() <= 1e-9 * np.abs(b).max()
     assert np.linalg.norm(x - x64) <= 1e-5 * np.linalg.norm(x64)
+
+
+def _grid_lookup(grid):
+    idx = {tuple(int(v) for v in o): i for i, o in enumerate(grid.origins)}
+    bg = grid.background if isinstance(grid.background, (tuple, list)) else (grid.background,) * 3
+
+    def get(g, comp=None):
+        o = tuple(int(v) & ~7 for v in g)
+        i = idx.get(o)
+        off = ((g[0] & 7) << 6) | ((g[1] & 7) << 3) | (g[2] & 7)
+        if i is None:
+            return np.float32(bg[0 if comp is None else comp])
+        v = grid.values[i, off]
+        return np.float32(v if comp is None else v[comp])
+    return get
+
+
+@pytest.mark.parametrize("friction", [0.0, 0.5])
+def test_pressure_gradient_closed_forms(friction):
+    """The oracle's apply_pressure_gradient restatement (FLIP_vdb.cpp:2571-2667) on a
+    linear pressure p = 2x + 3y - 5z (integer, exact in FP32) against the update rule
+    evaluated face by face from coordinates: interior liquid faces (theta = 1) get
+    du = -dt * a_c / dx exactly; faces to an air voxel use theta = phi_below /
+    (phi_below - phi_this) clamped at 0.02 with the air pressure 0; weight-0 faces
+    and faces with no DOF on either side are turned off and keep their value;
+    partially solid faces blend towards the solid velocity."""
+    import uaamg_amd as U
+    sc = U.Scene(1, 32, seed=1)   # the solid sphere gives fractional face weights
+    lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=-1)
+    orc = O.OraclePoisson(sc.phi, sc.weights, 1.0, sc.dx)
+    org, mask, base = lap.level_topology(0)
+    dof = {}
+    pres = np.zeros(lap.mNumDof, np.float32)
+    for i, o in enumerate(org):
+        k = int(base[i])
+        for off in range(512):
+            if (int(mask[i, off >> 6]) >> (off & 63)) & 1:
+                g = (int(o[0]) + (off >> 6), int(o[1]) + ((off >> 3) & 7), int(o[2]) + (off & 7))
+                dof[g] = k
+                pres[k] = 2 * g[0] + 3 * g[1] - 5 * g[2]
+                k += 1
+    rng = np.random.default_rng(3)
+    solid = U.Grid(sc.weights.origins.copy(), rng.uniform(-1, 1, sc.weights.values.shape).astype(np.float32), None,
+                   (0.0, 0.0, 0.0))
+    vel = [sc.vx, sc.vy, sc.vz]
+    ones = [np.full((g.origins.shape[0], 8), np.uint64(0xFFFFFFFFFFFFFFFF), np.uint64) for g in vel]
+    dt, dx = np.float32(0.5), np.float32(sc.dx)
+    out = orc.apply_pressure_gradient(pres, sc.phi, sc.weights, solid,
+                                      [(g.origins, a, g.values) for g, a in zip(vel, ones)], float(dt), friction)
+    phi, w, sv = _grid_lookup(sc.phi), _grid_lookup(sc.weights), _grid_lookup(solid)
+    slope = (2, 3, -5)
+    counts = {"interior": 0, "air": 0, "off": 0, "blend": 0}
+    for c in range(3):
+        act, val, upd, refl = out[c]
+        for i in range(0, vel[c].origins.shape[0], 3):   # a third of the leaves keeps the test short
+            o = vel[c].origins[i]
+            for off in range(0, 512, 7):
+                g = (int(o[0]) + (off >> 6), int(o[1]) + ((off >> 3) & 7), int(o[2]) + (off & 7))
+                lo = list(g)
+                lo[c] -= 1
+                lo = tuple(lo)
+                u0 = np.float32(vel[c].values[i, off])
+                on = (int(act[i, off >> 6]) >> (off & 63)) & 1
+                wt = w(g, c)
+                if wt <= 0 or (g not in dof and lo not in dof):
+                    assert not on and upd[i, off] == u0 and val[i, off] == u0
+                    counts["off"] += 1
+                    continue
+                assert on
+                pt = np.float32(pres[dof[g]]) if g in dof else np.float32(0)
+                pb = np.float32(pres[dof[lo]]) if lo in dof else np.float32(0)
+                ft, fb = phi(g), phi(lo)
+                theta = np.float32(1)
+                if ft >= 0 or fb >= 0:
+                    theta = np.float32(O.fraction_inside2(float(fb), float(ft)))
+                    theta = max(theta, np.float32(0.02))
+                    counts["air"] += 1
+                else:
+                    assert pt - pb == slope[c]
+                    counts["interior"] += 1
+                du = np.float32(np.float32(np.float32(-dt * (pt - pb)) / dx) / theta)
+                assert upd[i, off] == du
+                up = np.float32(u0 + du)
+                assert refl[i, off] == np.float32(up + du)
+                if wt < 1:
+                    s = np.float32(np.float32(1 - wt) * np.float32(friction))
+                    up = np.float32(np.float32(np.float32(1 - s) * up) + np.float32(s * sv(g, c)))
+                    counts["blend"] += 1
+                assert val[i, off] == up
+    assert min(counts.values()) > 0, counts

@@ -47,6 +47,17 @@ class _Grid(C.Structure):
     ]
 
 
+class _FaceChannel(C.Structure):
+    _fields_ = [
+        ("nleaf", C.c_int64),
+        ("origins", C.c_void_p),
+        ("active", C.c_void_p),
+        ("values", C.c_void_p),
+        ("update", C.c_void_p),
+        ("reflected", C.c_void_p),
+    ]
+
+
 class _Options(C.Structure):
     _fields_ = [
         ("relative_tolerance", C.c_float),
@@ -111,6 +122,8 @@ def lib() -> C.CDLL:
         "uaamg_poisson_level_coefficients": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "uaamg_poisson_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
         "uaamg_poisson_build_rhs_device": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), i32, vp]),
+        "uaamg_poisson_apply_pressure_gradient_device": (C.c_int, [vp, vp, P(_Grid), P(_Grid), P(_Grid),
+                                                                   P(_FaceChannel), f32, f32, i32]),
         "uaamg_poisson_solve_pcg": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_solve_pcg_device": (C.c_int, [vp, vp, vp, P(i32)]),
         "uaamg_poisson_last_residual": (C.c_int, [vp, P(f32), P(i32)]),
@@ -377,6 +390,7 @@ class LaplacianWithLevel:
         _check(L.uaamg_poisson_create(C.byref(gphi), C.byref(gw), float(dt), float(dx), C.byref(o), C.byref(h)),
                "uaamg_poisson_create")
         self._h = h
+        self._device = device
         self.mDt = float(dt)
         self.mDxThisLevel = float(dx)
         self.mLevel = 0
@@ -411,6 +425,57 @@ class LaplacianWithLevel:
         _check(lib().uaamg_poisson_build_rhs_device(self._h, *[C.byref(x) for x in st[:4]], sv,
                                                     0 if values_ptrs is None else 1, C.c_void_p(rhs_ptr)),
                "build_rhs_device")
+
+    def apply_pressure_gradient_device(self, pressure_ptr: int, liquid_phi: Grid, face_weights: Grid,
+                                       solid_velocity: Grid | None, velocity: list, dt: float, friction: float,
+                                       values_on_device: bool = False):
+        """FLIP_Solver_OpenVDB::apply_pressure_gradient (FLIP_vdb.cpp:2571-2667) on the device, the
+        pressure being the solve's DOF-ordered result in device memory (pressure_ptr). velocity: three
+        Grids (x, y, z faces; values [n,512], active [n,8]). Returns per channel (active, values,
+        update, reflected) after the update, in the channel's leaf order; the inputs are not modified.
+        values_on_device: stage every values array in device memory (torch) and let the library work
+        on it there (the device-resident calling form)."""
+        keep = []
+
+        def place(a: np.ndarray) -> int:
+            if not values_on_device:
+                return a.ctypes.data
+            import torch
+            t = torch.from_numpy(a).to(torch.device("cuda", self._device))
+            keep.append((a, t))
+            return t.data_ptr()
+
+        grids = [liquid_phi, face_weights] + ([] if solid_velocity is None else [solid_velocity])
+        st = [g._struct(place(np.ascontiguousarray(g.values, dtype=np.float32)) if values_on_device else None)
+              for g in grids]
+        sv = None if solid_velocity is None else C.byref(st[2])
+        chans = (_FaceChannel * 3)()
+        out = []
+        for c, g in enumerate(velocity):
+            org = np.ascontiguousarray(g.origins, dtype=np.int32)
+            n = org.shape[0]
+            act = (np.full((n, 8), np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64) if g.active is None
+                   else np.array(g.active, dtype=np.uint64, copy=True))
+            val = np.array(g.values, dtype=np.float32, copy=True).reshape(n, 512)
+            upd = np.zeros_like(val)
+            refl = np.zeros_like(val)
+            chans[c].nleaf = n
+            chans[c].origins = org.ctypes.data
+            chans[c].active = act.ctypes.data
+            chans[c].values = place(val)
+            chans[c].update = place(upd)
+            chans[c].reflected = place(refl)
+            out.append((org, act, val, upd, refl))
+        _check(lib().uaamg_poisson_apply_pressure_gradient_device(self._h, C.c_void_p(pressure_ptr), C.byref(st[0]),
+                                                                  C.byref(st[1]), sv, chans, float(dt),
+                                                                  float(friction), 1 if values_on_device else 0),
+               "apply_pressure_gradient_device")
+        if values_on_device:
+            import torch
+            torch.cuda.synchronize()
+            for a, t in keep:
+                a[...] = t.cpu().numpy()   # values / update / reflected come back; inputs are copies
+        return [(a, v, u, r) for (_, a, v, u, r) in out]
 
     def getZeroVectorGrid(self) -> np.ndarray:
         return np.zeros(self.mNumDof, dtype=np.float32)

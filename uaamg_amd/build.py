@@ -35,6 +35,8 @@ SOURCES = [
     "capi_server.cpp",
     "rhs_kernels.hip",
     "rhs_device.cpp",
+    "face_update_kernels.hip",
+    "face_update.cpp",
 ]
 
 ARCH = os.environ.get("UAAMG_OFFLOAD_ARCH", "gfx950")

@@ -16,6 +16,7 @@
 #include "poisson_host.h"
 #include "poisson_solver.h"
 #include "rhs_device.h"
+#include "face_update.h"
 #include "capi_common.h"
 #include "scene.h"
 
@@ -308,6 +309,51 @@ int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weig
     }
     std::string err;
     const int rc = build_poisson_rhs_device(h->dev, refs, rhs_dev, err);
+    if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* pressure_dev,
+                                                 const uaamg_grid* liquid_phi, const uaamg_grid* face_weights,
+                                                 const uaamg_grid* solid_velocity, uaamg_face_channel* channels,
+                                                 float dt, float friction, int32_t values_on_device) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !pressure_dev || !channels) return fail(UAAMG_EINVAL, "null argument");
+    const uaamg_grid* in[3] = {liquid_phi, face_weights, solid_velocity};
+    const int ncomp[3] = {1, 3, 3};
+    const char* names[3] = {"liquid_phi", "face_weights", "solid_velocity"};
+    FaceGridRef refs[3];
+    for (int g = 0; g < 3; ++g) {
+        FaceGridRef& r = refs[g];
+        r.ncomp = ncomp[g];
+        if (!in[g]) {
+            if (g != 2) return fail(UAAMG_EINVAL, std::string(names[g]) + ": null grid");
+            continue;   // zero solid velocity
+        }
+        if (in[g]->ncomp != ncomp[g]) return fail(UAAMG_EINVAL, std::string(names[g]) + ": wrong component count");
+        if (in[g]->nleaf < 0 || (in[g]->nleaf > 0 && (!in[g]->origins || !in[g]->values)))
+            return fail(UAAMG_EINVAL, std::string(names[g]) + ": null arrays");
+        r.nleaf = in[g]->nleaf;
+        r.origins = in[g]->origins;
+        r.values = in[g]->values;
+        for (int k = 0; k < 3; ++k) r.bg[k] = in[g]->background[k];
+    }
+    FaceChannelRef ch[3];
+    for (int c = 0; c < 3; ++c) {
+        const uaamg_face_channel& f = channels[c];
+        if (f.nleaf < 0 || (f.nleaf > 0 && (!f.origins || !f.active || !f.values)))
+            return fail(UAAMG_EINVAL, "velocity channel: null arrays");
+        ch[c].nleaf = f.nleaf;
+        ch[c].origins = f.origins;
+        ch[c].active = f.active;
+        ch[c].values = f.values;
+        ch[c].update = f.update;
+        ch[c].reflected = f.reflected;
+    }
+    std::string err;
+    const int rc = apply_pressure_gradient_device(h->dev, pressure_dev, refs, ch, dt, friction, values_on_device != 0,
+                                                  err);
     if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
     return 0;
     UAAMG_GUARD_END
