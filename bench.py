@@ -259,6 +259,11 @@ def main() -> int:
 
     out["rhs_build"] = rhs_build(lap, scene, b_host, t_rhs_host, dev)
     out["pressure_gradient"] = pressure_gradient(lap, scene, x_mx)
+    out["face_weights"] = face_weights(scene)
+    # the hierarchy build with its per-voxel level-0 stage on the host threads instead of the device
+    t0 = time.perf_counter()
+    U.LaplacianWithLevel(scene.phi, scene.weights, dt=1.0, dx=scene.dx, device=-1)
+    out["build_s_host_stage"] = round(time.perf_counter() - t0, 3)
 
     if not args.no_cpu_baseline:
         import oracle as O
@@ -347,6 +352,30 @@ def pressure_gradient(lap, scene, pressure) -> dict:
     mb = sum(g.values.nbytes for g in vel) / 1e6
     return {"ms_per_call": round(ms, 2), "faces_updated": faces, "velocity_mb": round(mb, 1),
             "io": "velocity, update and reflected grids through host memory"}
+
+
+def face_weights(scene) -> dict:
+    """Face weights from a solid SDF on the device (uaamg_face_weights): the scene's liquid SDF
+    topology dilated twice, a solid sphere SDF sampled on the weight grid's leaves; ms per call
+    including the host topology pass and the transfers."""
+    import numpy as np
+
+    import uaamg_amd as U
+
+    org = scene.weights.origins
+    n = int(round(1.0 / scene.dx))
+    off = np.arange(512)
+    lx, ly, lz = off >> 6, (off >> 3) & 7, off & 7
+    vals = np.empty((org.shape[0], 512), np.float32)
+    for i, o in enumerate(org):
+        vals[i] = np.sqrt((o[0] + lx - 0.5 * n) ** 2 + (o[1] + ly - 0.5 * n) ** 2 + (o[2] + lz - 0.5 * n) ** 2) - 0.2 * n
+    solid = U.Grid(org, vals, None, (3.0,))
+    U.calculate_face_weights(scene.phi, solid)
+    t0 = time.perf_counter()
+    g = U.calculate_face_weights(scene.phi, solid)
+    ms = 1e3 * (time.perf_counter() - t0)
+    return {"ms_per_call": round(ms, 2), "leaves": int(g.origins.shape[0]),
+            "io": "liquid topology and solid SDF from host memory, weights back to host memory"}
 
 
 def run_serving(args, dev) -> dict:

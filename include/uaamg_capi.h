@@ -214,6 +214,18 @@ int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* 
                                                  float dt, float friction, int32_t values_on_device);
 
 /*
+ * The face weights of the pressure solve from the solid SDF, computed on HIP device `device`.
+ * The weight grid has the liquid SDF's topology dilated twice over the 26 neighbours; at each active
+ * voxel v the weight of face c is 1 - fraction_inside(four solid SDF values at the corners v + {0,1}^2
+ * of that face's plane), clamped to [0, 1]; every other voxel of the grid holds 1 (the background).
+ * Two-phase: *nleaf receives the leaf count; when origins ([nleaf][3]), active ([nleaf][8]) and values
+ * ([nleaf][512][3], host memory) are non-NULL they receive the grid, leaves in OpenVDB tree order.
+ * Replaces FLIP_Solver_OpenVDB::calculate_face_weights (FLIP_vdb.cpp:1990-2068).
+ */
+int uaamg_face_weights(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf, int32_t device, int64_t* nleaf,
+                       int32_t* origins, uint64_t* active, float* values);
+
+/*
  * PoissonSolver::solveMultigridPCG (simd_vdb_poisson.h:162, .cpp:2155-2226) on host DOF-ordered
  * vectors: x holds the initial guess on entry and the pressure on exit. Returns UAAMG_SUCCESS or
  * UAAMG_FAILED; *iterations receives mIterationTaken.

@@ -1090,6 +1090,68 @@ void orc_build_rhs(void* p, int64_t nw, const int32_t* w_org, const float* w_val
     }
 }
 
+// FLIP_Solver_OpenVDB::calculate_face_weights (FLIP_vdb.cpp:1990-2068): the
+// liquid topology dilated twice over the 26 neighbours (voxel by voxel: every
+// voxel within Chebyshev distance 2 of an active one), weights from the solid
+// SDF at the face corners; inactive voxels hold 1. Two-phase like the library:
+// *n receives the leaf count, the arrays (tree order) are filled when non-null.
+void orc_face_weights(int64_t nl, const int32_t* l_org, const uint64_t* l_act, int64_t ns, const int32_t* s_org,
+                      const float* s_val, float s_bg, int64_t* n, int32_t* org, uint64_t* act, float* val) {
+    std::map<Coord, Leaf<Vec3>, TreeLess> out;
+    Vec3 one;
+    one.v[0] = one.v[1] = one.v[2] = 1.f;
+    auto touch_leaf = [&](Coord o) -> Leaf<Vec3>& {
+        auto it = out.find(o);
+        if (it != out.end()) return it->second;
+        Leaf<Vec3>& l = out[o];
+        for (int i = 0; i < 512; ++i) l.v[i] = one;
+        return l;
+    };
+    for (int64_t i = 0; i < nl; ++i) touch_leaf(Coord{l_org[3 * i], l_org[3 * i + 1], l_org[3 * i + 2]});
+    for (int64_t i = 0; i < nl; ++i) {
+        const Coord o{l_org[3 * i], l_org[3 * i + 1], l_org[3 * i + 2]};
+        for (int off = 0; off < 512; ++off) {
+            if (!((l_act[8 * i + (off >> 6)] >> (off & 63)) & 1u)) continue;
+            const Coord lc = local_of(off);
+            for (int dx = -2; dx <= 2; ++dx)
+                for (int dy = -2; dy <= 2; ++dy)
+                    for (int dz = -2; dz <= 2; ++dz) {
+                        const Coord g{o.x + lc.x + dx, o.y + lc.y + dy, o.z + lc.z + dz};
+                        touch_leaf(leaf_of(g)).set_on(offset_of(g), true);
+                    }
+        }
+    }
+    *n = int64_t(out.size());
+    if (!org || !act || !val) return;
+    FGrid solid;
+    load_grid<float>(ns, s_org, nullptr, s_val, 1, &s_bg, solid);
+    size_t k = 0;
+    for (auto& kv : out) {
+        Leaf<Vec3>& l = kv.second;
+        for (int off = 0; off < 512; ++off) {
+            if (!l.is_on(off)) continue;
+            const Coord lc = local_of(off);
+            const Coord g{kv.first.x + lc.x, kv.first.y + lc.y, kv.first.z + lc.z};
+            float sd[2][2][2];
+            for (int ii = 0; ii < 2; ++ii)
+                for (int jj = 0; jj < 2; ++jj)
+                    for (int kk = 0; kk < 2; ++kk) sd[ii][jj][kk] = solid.get(Coord{g.x + ii, g.y + jj, g.z + kk});
+            float w[3];
+            w[0] = 1.0f - fraction_inside4(sd[0][0][0], sd[0][1][0], sd[0][0][1], sd[0][1][1]);
+            w[1] = 1.0f - fraction_inside4(sd[0][0][0], sd[0][0][1], sd[1][0][0], sd[1][0][1]);
+            w[2] = 1.0f - fraction_inside4(sd[0][0][0], sd[1][0][0], sd[0][1][0], sd[1][1][0]);
+            for (int c = 0; c < 3; ++c) l.v[off].v[c] = std::max(0.f, std::min(w[c], 1.f));
+        }
+        org[3 * k] = kv.first.x;
+        org[3 * k + 1] = kv.first.y;
+        org[3 * k + 2] = kv.first.z;
+        std::memcpy(act + 8 * k, l.on, sizeof(l.on));
+        for (int off = 0; off < 512; ++off)
+            for (int c = 0; c < 3; ++c) val[(k * 512 + size_t(off)) * 3 + size_t(c)] = l.v[off].v[c];
+        ++k;
+    }
+}
+
 // FLIP_Solver_OpenVDB::apply_pressure_gradient (FLIP_vdb.cpp:2571-2667), one
 // channel c: the pressure grid is the level-0 DOF topology (TopologyCopy of
 // the DOF index grid, 0 off the DOFs) holding the DOF-ordered pressure. The

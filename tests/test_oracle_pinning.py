@@ -433,3 +433,88 @@ def test_pressure_gradient_closed_forms(friction):
                     counts["blend"] += 1
                 assert val[i, off] == up
     assert min(counts.values()) > 0, counts
+
+
+def _random_topology(rng, nleaf=6, span=3):
+    """leaves at random (possibly negative) origins with random active masks"""
+    seen = set()
+    org = []
+    while len(org) < nleaf:
+        o = tuple(int(v) * 8 for v in rng.integers(-span, span, 3))
+        if o not in seen:
+            seen.add(o)
+            org.append(o)
+    org = np.array(org, np.int32)
+    act = np.zeros((nleaf, 8), np.uint64)
+    for i in range(nleaf):
+        bits = rng.uniform(size=512) < 0.03
+        for off in np.nonzero(bits)[0]:
+            act[i, off >> 6] |= np.uint64(1) << np.uint64(off & 63)
+    return org, act
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_face_weight_topology_is_box_dilation(seed):
+    """The weight grid's topology (dilateActiveValues(2, NN_FACE_EDGE_VERTEX) of the liquid SDF):
+    oracle and library both against a dense scipy binary dilation with a 5x5x5 box."""
+    import scipy.ndimage as ndi
+    import uaamg_amd as U
+    rng = np.random.default_rng(seed)
+    org, act = _random_topology(rng)
+    liquid = U.Grid(org, np.zeros((org.shape[0], 512), np.float32), act, (0.0,))
+    lo = org.min(axis=0) - 16
+    dims = tuple(int(v) for v in org.max(axis=0) + 24 - lo)
+    vol = np.zeros(dims, bool)
+    for i, o in enumerate(org):
+        for off in range(512):
+            if (int(act[i, off >> 6]) >> (off & 63)) & 1:
+                vol[o[0] - lo[0] + (off >> 6), o[1] - lo[1] + ((off >> 3) & 7), o[2] - lo[2] + (off & 7)] = True
+    dil = ndi.binary_dilation(vol, structure=np.ones((5, 5, 5), bool))
+    want = {tuple(int(v) for v in o) for o in org}
+    for x, y, z in zip(*np.nonzero(dil)):
+        want.add((int(x + lo[0]) & ~7, int(y + lo[1]) & ~7, int(z + lo[2]) & ~7))
+    o_org, o_act, o_val = O.face_weights(liquid, U.Grid(np.zeros((0, 3), np.int32), np.zeros((0, 512), np.float32),
+                                                        None, (1.0,)))
+    assert {tuple(int(v) for v in o) for o in o_org} == want
+    for i, o in enumerate(o_org):
+        for off in range(0, 512, 3):
+            g = (o[0] - lo[0] + (off >> 6), o[1] - lo[1] + ((off >> 3) & 7), o[2] - lo[2] + (off & 7))
+            inside = all(0 <= g[k] < dims[k] for k in range(3)) and dil[g]
+            assert bool((int(o_act[i, off >> 6]) >> (off & 63)) & 1) == inside
+    # the library's host topology pass (no device needed without output values)
+    n = __import__("ctypes").c_int64()
+    C = __import__("ctypes")
+    gl, gs = liquid._struct(), U.Grid(np.zeros((0, 3), np.int32), np.zeros((0, 512), np.float32), None, (1.0,))._struct()
+    assert U.lib().uaamg_face_weights(C.byref(gl), C.byref(gs), -1, C.byref(n), None, None, None) == 0
+    assert n.value == o_org.shape[0]
+
+
+def test_face_weights_closed_form_plane():
+    """A solid half-space z < h (SDF z - h, h = 3.25): x- and y-faces are cut at height h - z,
+    so w = 1 - clamp(h - z, 0, 1); a z-face lies in one plane: w = 0 below h, 1 above;
+    inactive voxels hold 1."""
+    import uaamg_amd as U
+    h = np.float32(3.25)
+    org = np.array([[0, 0, 0], [0, 0, 8]], np.int32)
+    act = np.zeros((2, 8), np.uint64)
+    act[0, 3] = np.uint64(1) << np.uint64(4 * 8 + 2)     # voxel (3, 4, 2) of the first leaf
+    liquid = U.Grid(org, np.zeros((2, 512), np.float32), act, (0.0,))
+    sorg = np.array([[x, y, z] for x in range(-8, 24, 8) for y in range(-8, 24, 8) for z in range(-8, 24, 8)], np.int32)
+    sval = np.zeros((sorg.shape[0], 512), np.float32)
+    for i, o in enumerate(sorg):
+        for off in range(512):
+            sval[i, off] = np.float32(o[2] + (off & 7)) - h
+    org_o, act_o, val_o = O.face_weights(liquid, U.Grid(sorg, sval, None, (1.0,)))
+    checked = 0
+    for i, o in enumerate(org_o):
+        for off in range(512):
+            z = int(o[2]) + (off & 7)
+            w = val_o[i, off]
+            if not (int(act_o[i, off >> 6]) >> (off & 63)) & 1:
+                assert tuple(w) == (1.0, 1.0, 1.0)
+                continue
+            cut = np.float32(min(max(h - np.float32(z), np.float32(0)), np.float32(1)))
+            assert w[0] == np.float32(1) - cut and w[1] == np.float32(1) - cut
+            assert w[2] == (np.float32(0) if z < h else np.float32(1))
+            checked += 1
+    assert checked == 125   # the 5x5x5 box around the one active voxel

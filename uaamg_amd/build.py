@@ -37,6 +37,10 @@ SOURCES = [
     "rhs_device.cpp",
     "face_update_kernels.hip",
     "face_update.cpp",
+    "face_weights_kernels.hip",
+    "face_weights.cpp",
+    "finest_kernels.hip",
+    "finest_device.cpp",
 ]
 
 ARCH = os.environ.get("UAAMG_OFFLOAD_ARCH", "gfx950")

@@ -17,6 +17,7 @@
 #include "poisson_solver.h"
 #include "rhs_device.h"
 #include "face_update.h"
+#include "face_weights.h"
 #include "capi_common.h"
 #include "scene.h"
 
@@ -132,7 +133,11 @@ int prepare_poisson(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights
     HostGrid phi, w;
     if (!to_host_grid(liquid_phi, 1, phi, "liquid_phi")) return UAAMG_EINVAL;
     if (!to_host_grid(face_weights, 3, w, "face_weights")) return UAAMG_EINVAL;
-    build_hierarchy(phi, w, dt, dx, o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000, hier);
+    // the per-voxel level-0 stage runs on the handle's device (bit-identical to the host stage)
+    std::string err;
+    if (!build_hierarchy(phi, w, dt, dx, o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000, hier,
+                         host_only ? -1 : o.device, &err))
+        return fail(err.empty() ? UAAMG_EINVAL : UAAMG_EHIP, err.empty() ? "bad grids" : err);
     if (hier.levels[0].ndof == 0) return fail(UAAMG_EINVAL, "no degrees of freedom");
     so = SolverOptions{};
     so.relative_tolerance = o.relative_tolerance;
@@ -255,6 +260,10 @@ int uaamg_poisson_level_topology(const uaamg_poisson* h, int level, int32_t* ori
 int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind, float* bricks, uint8_t* kept) {
     if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
     if (kind < 0 || kind >= kNumCoef) return fail(UAAMG_EINVAL, "bad coefficient kind");
+    // the SPAI0 coefficients are built on first use; a query is a use (the handle's
+    // logical state does not change)
+    if (kind == kCoefSpai0 && const_cast<PoissonDevice&>(h->dev).ensure_spai0())
+        return fail(UAAMG_EHIP, h->dev.error());
     const auto& l = h->dev.host().levels[level];
     if (bricks && !l.coef[kind].empty())
         std::memcpy(bricks, l.coef[kind].data(), l.coef[kind].size() * sizeof(float));
@@ -355,6 +364,35 @@ int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* 
     const int rc = apply_pressure_gradient_device(h->dev, pressure_dev, refs, ch, dt, friction, values_on_device != 0,
                                                   err);
     if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_face_weights(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf, int32_t device, int64_t* nleaf,
+                       int32_t* origins, uint64_t* active, float* values) {
+    UAAMG_GUARD_BEGIN
+    if (!nleaf) return fail(UAAMG_EINVAL, "null argument");
+    HostGrid liquid;
+    if (!to_host_grid(liquid_sdf, 1, liquid, "liquid_sdf")) return UAAMG_EINVAL;
+    if (!solid_sdf || solid_sdf->ncomp != 1 || solid_sdf->nleaf < 0 ||
+        (solid_sdf->nleaf > 0 && (!solid_sdf->origins || !solid_sdf->values)))
+        return fail(UAAMG_EINVAL, "solid_sdf: null grid, arrays or wrong component count");
+    const bool fill = origins && active && values;
+    std::vector<Coord> org;
+    std::vector<uint64_t> act;
+    std::string err;
+    const int rc = face_weights_device(liquid, solid_sdf->nleaf, solid_sdf->origins, solid_sdf->values,
+                                       solid_sdf->background[0], device, org, act, fill ? values : nullptr, err);
+    if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
+    *nleaf = int64_t(org.size());
+    if (fill) {
+        for (size_t i = 0; i < org.size(); ++i) {
+            origins[3 * i] = org[i].x;
+            origins[3 * i + 1] = org[i].y;
+            origins[3 * i + 2] = org[i].z;
+        }
+        std::memcpy(active, act.data(), act.size() * sizeof(uint64_t));
+    }
     return 0;
     UAAMG_GUARD_END
 }

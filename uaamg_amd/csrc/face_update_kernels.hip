@@ -1,19 +1,12 @@
 // Device pressure-gradient update of the face velocities (face_update.h).
 #include "face_update.h"
+#include "fraction_inside.h"
 #include "stencil_device.h"
 
 namespace uaamg {
 namespace {
 
 using namespace dev;
-
-// fraction_inside(phi_left, phi_right) (levelset_util.cpp:4-13)
-__device__ __forceinline__ float fraction_inside2(float l, float r) {
-    if (l < 0 && r < 0) return 1.0f;
-    if (l < 0 && r >= 0) return l / (l - r);
-    if (l >= 0 && r < 0) return r / (r - l);
-    return 0.0f;
-}
 
 // pressure at a voxel of level-0 DOF leaf dl: (is a DOF, value; 0 off the DOFs)
 __device__ __forceinline__ bool pressure_at(const GradArgs& a, int32_t dl, int off, float& p) {

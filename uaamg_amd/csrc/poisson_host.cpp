@@ -1,6 +1,8 @@
 // Host construction of the UAAMG Poisson level hierarchy (see poisson_host.h).
 #include "poisson_host.h"
 
+#include "finest_device.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -192,7 +194,8 @@ float PoissonLevelHost::default_coef(int kind) const {
     }
 }
 
-bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, float dx, PoissonLevelHost& l) {
+bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, float dx, PoissonLevelHost& l,
+                        int device, std::string* err) {
     if (phi.ncomp != 1 || weights.ncomp != 3 || !(dx > 0.f)) return false;
     l = PoissonLevelHost();
     l.level = 0;
@@ -208,7 +211,15 @@ bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, 
     std::vector<float> cd(nphi * kLeafSize), cxyz[3];
     for (int c = 0; c < 3; ++c) cxyz[c].resize(nphi * kLeafSize);
 
-    parallel_for(int64_t(nphi), [&](int64_t b, int64_t e) {
+    // the per-voxel stage on the device (finest_device.h) or on the host threads
+    if (device >= 0) {
+        std::string e;
+        if (finest_coefficients_device(phi, weights, dt_over_dx2, device, dmask, cd, cxyz, e) != 0) {
+            if (err) *err = e;
+            return false;
+        }
+    }
+    if (device < 0) parallel_for(int64_t(nphi), [&](int64_t b, int64_t e) {
         GridAccessor phi_axr[6] = {GridAccessor(phi), GridAccessor(phi), GridAccessor(phi),
                                    GridAccessor(phi), GridAccessor(phi), GridAccessor(phi)};
         GridAccessor w_self(weights), w_pos[3] = {GridAccessor(weights), GridAccessor(weights), GridAccessor(weights)};
@@ -392,17 +403,18 @@ void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& l) {
     build_invdiag(l);
 }
 
-void build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, float dx, int64_t max_coarsest_dof,
-                     PoissonHierarchyHost& out) {
+bool build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, float dx, int64_t max_coarsest_dof,
+                     PoissonHierarchyHost& out, int device, std::string* err) {
     out = PoissonHierarchyHost();
     out.nleaf_phi = int64_t(phi.nleaf());
     out.levels.emplace_back();
-    build_finest_level(phi, weights, dt, dx, out.levels.back());
+    if (!build_finest_level(phi, weights, dt, dx, out.levels.back(), device, err)) return false;
     while (out.levels.back().ndof > max_coarsest_dof) {
         PoissonLevelHost next;
         build_coarse_level(out.levels.back(), next);
         out.levels.push_back(std::move(next));
     }
+    return true;
 }
 
 void build_spai0(PoissonHierarchyHost& h) {

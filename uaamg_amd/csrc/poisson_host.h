@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "leaf_topology.h"
@@ -61,15 +62,18 @@ struct PoissonHierarchyHost {
     bool spai0_ready = false;
 };
 
-// Level 0 from the liquid phi and face-weight grids. Returns false on bad input.
-bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, float dx, PoissonLevelHost& out);
+// Level 0 from the liquid phi and face-weight grids. device >= 0 runs the
+// per-voxel stage on that HIP device (bit-identical; finest_device.h). Returns
+// false on bad input or a device error (err, when given, says which).
+bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, float dx, PoissonLevelHost& out,
+                        int device = -1, std::string* err = nullptr);
 
 // Next coarser level from a finer one.
 void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& out);
 
-// Full hierarchy: coarsen while ndof > max_coarsest_dof.
-void build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, float dx, int64_t max_coarsest_dof,
-                     PoissonHierarchyHost& out);
+// Full hierarchy: coarsen while ndof > max_coarsest_dof (device: as build_finest_level).
+bool build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, float dx, int64_t max_coarsest_dof,
+                     PoissonHierarchyHost& out, int device = -1, std::string* err = nullptr);
 
 // SPAI0 coefficients of every level (init_SPAI0, simd_vdb_poisson.cpp:1113-1190).
 void build_spai0(PoissonHierarchyHost& h);

@@ -85,6 +85,9 @@ void PoissonDevice::release_device() {
     sc_ = nullptr;
     partials_ = nullptr;
     cv_ = CoarsestView{};
+    for (void* p : spai0_bufs_) (void)hipFree(p);
+    spai0_bufs_.clear();
+    spai0_on_device_ = false;
     arena_.release();
 }
 
@@ -201,7 +204,9 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
         err_ = "empty hierarchy";
         return -1;
     }
-    build_spai0(host_);
+    // SPAI0 coefficients only when that smoother is selected (ensure_spai0
+    // builds and uploads them later when it is selected afterwards)
+    if (opt_.smoother == kSpai0) build_spai0(host_);
     return acquire_device();
 }
 
@@ -224,7 +229,7 @@ size_t PoissonDevice::device_bytes() const {
         if (l >= 1) add(n * 27 * kNb27Stride * sizeof(int32_t));   // neighbourhood records
         size_t nslot[kNumCoef] = {0, 0, 0, 0, 0};
         for (int k = 0; k < kNumCoef; ++k)
-            for (size_t i = 0; i < n; ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
+            for (size_t i = 0; i < n && !hl.keep[k].empty(); ++i) nslot[k] += hl.keep[k][i] ? 1 : 0;
         for (int k = 0; k < kNumCoef; ++k) add(std::max<size_t>(nslot[k], 1) * kLeafSize * sizeof(float));
         add(std::max<size_t>(nslot[kCoefDiag], 1) * kLeafSize * sizeof(float));   // invdiag
         add(4 * n * kLeafSize * sizeof(float));               // X, B, T, U
@@ -248,7 +253,7 @@ static std::vector<int32_t> level_records(const PoissonLevelHost& h, const int32
     int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < n; ++i)
         for (int k = 0; k < kNumCoef; ++k)
-            if (h.keep[k][i]) rec[i * kSlotStride + k] = count[k]++;
+            if (!h.keep[k].empty() && h.keep[k][i]) rec[i * kSlotStride + k] = count[k]++;
     complete_leaf_records(rec.data(), h.nbr.data(), parent, n);
     return rec;
 }
@@ -277,15 +282,10 @@ int PoissonDevice::acquire_device() {
     lv_.resize(L);
     for (int l = 0; l < L; ++l)
         if (int rc = upload_level(host_.levels[l], lv_[l], true)) return rc;
+    spai0_on_device_ = host_.spai0_ready;   // packed into the arena with the other coefficients
     // parent table of each finer level from the child table of the coarser one
     for (int l = 0; l + 1 < L; ++l) {
-        const auto& coarse = host_.levels[l + 1];
-        std::vector<int32_t> parent(host_.levels[l].nleaf(), -1);
-        for (size_t c = 0; c < coarse.nleaf(); ++c)
-            for (int b = 0; b < 8; ++b) {
-                int32_t f = coarse.child[c * 8 + b];
-                if (f >= 0) parent[f] = int32_t(c * 8 + b);
-            }
+        const std::vector<int32_t> parent = level_parents(l);
         if (!parent.empty() && check(hipMemcpyAsync(const_cast<int32_t*>(lv_[l].v.parent), parent.data(),
                                                     parent.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
                                      "upload parent"))
@@ -488,7 +488,58 @@ void PoissonDevice::tune_levels() {
 void PoissonDevice::init_host_only(PoissonHierarchyHost&& h, const SolverOptions& opt) {
     host_ = std::move(h);
     opt_ = opt;
-    build_spai0(host_);
+}
+
+// Builds the SPAI0 coefficients (init_SPAI0, simd_vdb_poisson.cpp:1113-1190)
+// on first use and, on the device, uploads every level's kept SPAI0 bricks
+// and the leaf records that point to them. Captured graphs are dropped (the
+// next solve recaptures them with the new level views).
+int PoissonDevice::ensure_spai0() {
+    if (!host_.spai0_ready) build_spai0(host_);
+    if (!on_device() || spai0_on_device_) return 0;
+    for (int l = 0; l < nlevels(); ++l) {
+        const PoissonLevelHost& h = host_.levels[size_t(l)];
+        const size_t n = h.nleaf();
+        size_t kept = 0;
+        for (size_t i = 0; i < n; ++i) kept += h.keep[kCoefSpai0][i] ? 1 : 0;
+        std::vector<float> buf(std::max<size_t>(kept, 1) * kLeafSize, 0.f);
+        size_t j = 0;
+        for (size_t i = 0; i < n; ++i)
+            if (h.keep[kCoefSpai0][i]) {
+                std::memcpy(&buf[j * kLeafSize], &h.coef[kCoefSpai0][i * kLeafSize], kLeafSize * sizeof(float));
+                ++j;
+            }
+        void* d = nullptr;
+        if (check(hipMalloc(&d, buf.size() * sizeof(float)), "spai0 alloc")) return -3;
+        spai0_bufs_.push_back(d);
+        if (check(hipMemcpyAsync(d, buf.data(), buf.size() * sizeof(float), hipMemcpyHostToDevice, stream_),
+                  "spai0 upload"))
+            return -3;
+        lv_[size_t(l)].v.spai0 = static_cast<const float*>(d);
+        const std::vector<int32_t> parent = level_parents(l);
+        const std::vector<int32_t> rec = level_records(h, parent.empty() ? nullptr : parent.data());
+        if (!rec.empty() && check(hipMemcpyAsync(const_cast<int32_t*>(lv_[size_t(l)].v.slot), rec.data(),
+                                                 rec.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
+                                  "spai0 leaf records"))
+            return -3;
+        if (check(hipStreamSynchronize(stream_), "spai0 sync")) return -3;   // host buffers die here
+    }
+    spai0_on_device_ = true;
+    destroy_graphs();
+    return first_error_ == hipSuccess ? 0 : -3;
+}
+
+// parent of every leaf of level l (coarse leaf * 8 + octant), empty on the coarsest level
+std::vector<int32_t> PoissonDevice::level_parents(int l) const {
+    if (l + 1 >= nlevels()) return {};
+    const auto& coarse = host_.levels[size_t(l + 1)];
+    std::vector<int32_t> parent(host_.levels[size_t(l)].nleaf(), -1);
+    for (size_t c = 0; c < coarse.nleaf(); ++c)
+        for (int b = 0; b < 8; ++b) {
+            const int32_t f = coarse.child[c * 8 + size_t(b)];
+            if (f >= 0) parent[size_t(f)] = int32_t(c * 8 + size_t(b));
+        }
+    return parent;
 }
 
 int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spai0) {
@@ -514,7 +565,7 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
                           : nullptr;
     int32_t count[kNumCoef] = {0, 0, 0, 0, 0};
     for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < kNumCoef; ++k) count[k] += h.keep[k][i] ? 1 : 0;
+        for (int k = 0; k < kNumCoef; ++k) count[k] += (!h.keep[k].empty() && h.keep[k][i]) ? 1 : 0;
     // parents are filled in once the coarser level is known (acquire_device)
     const std::vector<int32_t> slot = level_records(h, nullptr);
     v.slot = static_cast<const int32_t*>(up(slot.data(), slot.size() * sizeof(int32_t)));
@@ -554,7 +605,7 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
     v.cx = pack(h.coef[kCoefX], kCoefX, h.keep[kCoefX]);
     v.cy = pack(h.coef[kCoefY], kCoefY, h.keep[kCoefY]);
     v.cz = pack(h.coef[kCoefZ], kCoefZ, h.keep[kCoefZ]);
-    if (spai0) v.spai0 = pack(h.coef[kCoefSpai0], kCoefSpai0, h.keep[kCoefSpai0]);
+    if (spai0 && host_.spai0_ready) v.spai0 = pack(h.coef[kCoefSpai0], kCoefSpai0, h.keep[kCoefSpai0]);
     d.X = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
     d.B = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
     d.T = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
@@ -918,6 +969,7 @@ void PoissonDevice::mg_cycle(int l, float* X, const float* B) {
 int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
+    if (opt_.smoother == kSpai0 && ensure_spai0()) return -3;
     if (use_graphs_ && ensure_graphs()) return -3;
     const LevelView& v0 = lv_[0].v;
     int it = 0;
@@ -997,6 +1049,7 @@ int PoissonDevice::pcg_iterate(float& nu, float numax, int& it) {
 int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
+    if (opt_.smoother == kSpai0 && ensure_spai0()) return -3;
     const LevelView& v0 = lv_[0].v;
     int it = 0;
     if (iterations) *iterations = 0;
@@ -1035,6 +1088,7 @@ int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iteration
 
 int PoissonDevice::apply_dev(int l, int mode, float w, const float* x_dof, const float* b_dof, float* y_dof) {
     if (l < 0 || l >= nlevels()) return -1;
+    if (mode == kModeSpai0 && ensure_spai0()) return -3;
     const LevelView& v = lv_[l].v;
     float *xb = staging_[0], *bb = staging_[1], *yb = staging_[2];
     check(launch_scatter_dof(v, x_dof, xb, stream_), "scatter");
@@ -1049,6 +1103,7 @@ int PoissonDevice::apply_dev(int l, int mode, float w, const float* x_dof, const
 
 int PoissonDevice::first_sweep_dev(int l, int kind, float w, const float* b_dof, float* y_dof) {
     if (l < 0 || l >= nlevels() || kind < 0 || kind > 2) return -1;
+    if (kind == 1 && ensure_spai0()) return -3;
     const LevelView& v = lv_[l].v;
     float *bb = staging_[1], *yb = staging_[2];
     check(launch_scatter_dof(v, b_dof, bb, stream_), "scatter");
@@ -1082,6 +1137,7 @@ int PoissonDevice::prolong_dev(int fl, const float* coarse_dof, float* fine_dof,
 }
 
 int PoissonDevice::precondition_dev(const float* r_dof, float* z_dof) {
+    if (opt_.smoother == kSpai0 && ensure_spai0()) return -3;
     const LevelView& v0 = lv_[0].v;
     check(launch_scatter_dof(v0, r_dof, r0_, stream_), "scatter");
     precond(0, z0_, r0_, true, -1);
@@ -1116,6 +1172,7 @@ double PoissonDevice::algorithmic_bytes(int l, int mode) const {
 int PoissonDevice::time_apply(int l, int mode, int reps, float* ms) {
     if (!on_device() || l < 0 || l >= nlevels() || reps <= 0 || mode < 0 || mode > kModeJacobiBf16) return -1;
     if (mode == kModeJacobiBf16 && !mixed_) return -1;
+    if (mode == kModeSpai0 && ensure_spai0()) return -3;
     const LevelView& v = lv_[l].v;
     const float* X = l == 0 ? x0_ : lv_[l].X;
     const float* B = l == 0 ? b0_ : lv_[l].B;

@@ -118,6 +118,8 @@ public:
     // ||b - A x||_inf of the last solve's result (FP64 residual) when residual
     // replacement is on, and how many replacements it made; -1 otherwise
     float last_true_residual() const { return true_nu_; }
+    // SPAI0 coefficients, built (host) and uploaded (device) on first use
+    int ensure_spai0();
     int last_replacements() const { return replacements_; }
 
 private:
@@ -148,6 +150,7 @@ private:
     int wait_nu(float* nu);
 
     int upload_level(const PoissonLevelHost& h, DevLevel& d, bool spai0);
+    std::vector<int32_t> level_parents(int l) const;
     int build_coarsest();
     void precond(int level, float* X, const float* B, bool skip_first, int dot_target);
     void precond_fused(int level, float* X, const float* B, bool skip_first, int dot_target);
@@ -178,6 +181,8 @@ private:
     // captured PCG pieces, valid for graph_smoother_ / graph_sor_
     Graph g_init_, g_first_, g_iter_[2];
     int graph_smoother_ = -1;
+    std::vector<void*> spai0_bufs_;   // SPAI0 bricks uploaded after acquire_device
+    bool spai0_on_device_ = false;
     std::vector<float> graph_sor_;
     bool use_graphs_ = true;
     bool fuse_ = true;                     // fused W-cycle / PCG kernels

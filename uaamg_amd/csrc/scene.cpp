@@ -1,4 +1,5 @@
 #include "scene.h"
+#include "fraction_inside.h"
 
 #include <algorithm>
 #include <cmath>
@@ -27,68 +28,10 @@ float hash_uniform(uint64_t seed, uint64_t a, uint64_t b, uint64_t c, uint64_t d
     return float(h >> 40) * (1.0f / 16777216.0f);
 }
 
-float fraction_inside(float phi_left, float phi_right) {
-    if (phi_left < 0 && phi_right < 0) return 1;
-    if (phi_left < 0 && phi_right >= 0) return phi_left / (phi_left - phi_right);
-    if (phi_left >= 0 && phi_right < 0) return phi_right / (phi_right - phi_left);
-    return 0;
-}
-
-namespace {
-void rotate4(float* a) {
-    float t = a[0];
-    a[0] = a[1];
-    a[1] = a[2];
-    a[2] = a[3];
-    a[3] = t;
-}
-}  // namespace
+float fraction_inside(float phi_left, float phi_right) { return fraction_inside2(phi_left, phi_right); }
 
 float fraction_inside(float phi_bl, float phi_br, float phi_tl, float phi_tr) {
-    const int inside = (phi_bl < 0) + (phi_tl < 0) + (phi_br < 0) + (phi_tr < 0);
-    float q[4] = {phi_bl, phi_br, phi_tr, phi_tl};   // counter-clockwise corner list
-    if (inside == 4) return 1;
-    if (inside == 3) {
-        while (q[0] < 0) rotate4(q);   // the single outside corner goes first
-        float s0 = 1 - fraction_inside(q[0], q[3]);
-        float s1 = 1 - fraction_inside(q[0], q[1]);
-        return 1 - 0.5f * s0 * s1;
-    }
-    if (inside == 2) {
-        while (q[0] >= 0 || !(q[1] < 0 || q[2] < 0)) rotate4(q);
-        if (q[1] < 0) {   // the two inside corners share an edge
-            float left = fraction_inside(q[0], q[3]);
-            float right = fraction_inside(q[1], q[2]);
-            return 0.5f * (left + right);
-        }
-        // diagonal pair: the centre value picks the topology
-        float mid = 0.25f * (q[0] + q[1] + q[2] + q[3]);
-        if (mid < 0) {
-            float area = 0;
-            float s1 = 1 - fraction_inside(q[0], q[3]);
-            float s3 = 1 - fraction_inside(q[2], q[3]);
-            area += 0.5f * s1 * s3;
-            float s2 = 1 - fraction_inside(q[2], q[1]);
-            float s0 = 1 - fraction_inside(q[0], q[1]);
-            area += 0.5f * s0 * s2;
-            return 1 - area;
-        }
-        float area = 0;
-        float s0 = fraction_inside(q[0], q[1]);
-        float s1 = fraction_inside(q[0], q[3]);
-        area += 0.5f * s0 * s1;
-        float s2 = fraction_inside(q[2], q[1]);
-        float s3 = fraction_inside(q[2], q[3]);
-        area += 0.5f * s2 * s3;
-        return area;
-    }
-    if (inside == 1) {
-        while (q[0] >= 0) rotate4(q);
-        float s0 = fraction_inside(q[0], q[3]);
-        float s1 = fraction_inside(q[0], q[1]);
-        return 0.5f * s0 * s1;
-    }
-    return 0;
+    return fraction_inside4(phi_bl, phi_br, phi_tl, phi_tr);
 }
 
 namespace {
