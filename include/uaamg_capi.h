@@ -90,7 +90,8 @@ typedef struct uaamg_options {
     int32_t flags;            /* 0 normally; bit 0 = unfused kernels, bit 1 = no graph replay,
                                  bit 2 = bottom levels in one single-workgroup launch, bit 4 = no
                                  block-visit kernels (A/B checks); bit 6 = FP64 residual replacement
-                                 at FP32 precision */
+                                 at FP32 precision; bit 8 = level-0 per-voxel build stage on the host
+                                 threads instead of the device (same result) */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
 } uaamg_options;
 

@@ -135,8 +135,9 @@ int prepare_poisson(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights
     if (!to_host_grid(face_weights, 3, w, "face_weights")) return UAAMG_EINVAL;
     // the per-voxel level-0 stage runs on the handle's device (bit-identical to the host stage)
     std::string err;
+    const bool host_stage = host_only || (o.flags & 256);   // flags bit 8: per-voxel stage on the host threads
     if (!build_hierarchy(phi, w, dt, dx, o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000, hier,
-                         host_only ? -1 : o.device, &err))
+                         host_stage ? -1 : o.device, &err))
         return fail(err.empty() ? UAAMG_EINVAL : UAAMG_EHIP, err.empty() ? "bad grids" : err);
     if (hier.levels[0].ndof == 0) return fail(UAAMG_EINVAL, "no degrees of freedom");
     so = SolverOptions{};
