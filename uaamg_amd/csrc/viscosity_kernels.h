@@ -74,6 +74,10 @@ constexpr int kViscMaxRow = 16;    // 1 + 6 + 8 entries per row at most
 // y = op(x, b) on channel c of a level (Gauss-Seidel modes in place: y == x)
 hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int channel, const float* x, const float* b, float* y,
                              hipStream_t s);
+// y = op(x, b) on all three channels in one launch, for the modes with y != x
+// (NORMAL, RESIDUAL, JACOBI, SPAI0)
+hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const float* x, const float* b, float* y,
+                                 hipStream_t s);
 // coarse = restriction of fine (restriction, simd_viscosity3d.cpp:2353-2413), all channels
 hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
                                 float* coarse_v, hipStream_t s);

@@ -302,6 +302,18 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane
     st8v(Yv + row, y);
 }
 
+// All three channels in one launch (modes that write y != x: the channels
+// are independent); the wave's record index selects the channel.
+template <int MODE>
+__global__ __launch_bounds__(kVThreads) void visc_apply_all_kernel(ViscLevelView L, const float* Xv, const float* Bv,
+                                                                   float* Yv) {
+    const int e = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e < L.chan_off[1]) visc_row<MODE, 0>(L, e, lane, Xv, Bv, Yv);
+    else if (e < L.chan_off[2]) visc_row<MODE, 1>(L, e, lane, Xv, Bv, Yv);
+    else if (e < L.chan_off[3]) visc_row<MODE, 2>(L, e, lane, Xv, Bv, Yv);
+}
+
 template <int MODE, int C>
 __global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, const float* Xv, const float* Bv,
                                                                float* Yv) {
@@ -618,6 +630,21 @@ hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int c, const floa
         case kVBlackGS: return apply_mode<kVBlackGS>(L, c, x, b, y, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const float* x, const float* b, float* y,
+                                 hipStream_t s) {
+    const int n = L.chan_off[3];
+    if (n <= 0) return hipSuccess;
+    const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
+    switch (mode) {
+        case kVNormal: visc_apply_all_kernel<kVNormal><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVResidual: visc_apply_all_kernel<kVResidual><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVJacobi: visc_apply_all_kernel<kVJacobi><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVSpai0Mode: visc_apply_all_kernel<kVSpai0Mode><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        default: return hipErrorInvalidValue;   // Gauss-Seidel passes update in place, channel by channel
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,

@@ -376,6 +376,10 @@ int ViscDevice::upload_spai0(int level) {
 
 // ------------------------------------------------------------ W-cycle
 void ViscDevice::apply_all(int level, int mode, const float* x, const float* b, float* y) {
+    if (mode != kVRedGS && mode != kVBlackGS && x != y) {
+        check(launch_visc_apply_all(lv_[level].v, mode, x, b, y, stream_), "apply");
+        return;
+    }
     for (int c = 0; c < 3; ++c) check(launch_visc_apply(lv_[level].v, mode, c, x, b, y, stream_), "apply");
 }
 
