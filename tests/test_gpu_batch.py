@@ -77,7 +77,7 @@ def test_batch_equals_each_problem_alone(batch_inputs, precision, flags):
         assert s1 == s2 == U.SUCCESS, f"problem {i}"
         assert i1 == i2, f"problem {i}: {i1} vs {i2}"
         assert np.array_equal(x1, x2), f"problem {i}"
-        # the residual replacement (BF16 form) restarts the same problems the same number of times
+        # the residual replacement (BF16 form) replaces the same residuals of the same problems
         assert r1[1] == res[i][1] and (r1[0] == res[i][0] or precision == 0), f"problem {i}: {r1} vs {res[i]}"
 
 

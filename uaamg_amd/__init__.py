@@ -360,7 +360,7 @@ class PoissonBatch:
         return st.tolist(), it.tolist()
 
     def last_residual(self):
-        """Per problem: (FP64 true residual, restarts) of the last solve; -1s without replacement."""
+        """Per problem: (FP64 true residual, residual replacements) of the last solve; -1s without replacement."""
         r = np.zeros(self.n, np.float32)
         k = np.zeros(self.n, np.int32)
         _check(lib().uaamg_poisson_batch_last_residual(self._h, _ptr(r), _ptr(k)), "batch last_residual")

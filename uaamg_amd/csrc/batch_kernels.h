@@ -31,8 +31,11 @@ struct BatchPcg {
     int32_t it0;           // iterations at the start of the current recurrence (growth rule)
     float true_nu;         // FP64 true residual of the last replacement check (-1 without)
     float prev_true;
-    int32_t replacements;  // PCG restarts from the true residual (-1 without replacement)
+    int32_t replacements;  // residuals replaced by the true residual: steps and restarts (-1 without replacement)
     int32_t refine;        // 1 while the replacement step may still restart this problem
+    int32_t restarts;      // PCG restarts from the true residual
+    int32_t replacing;     // the next step forms r as the true residual of its iterate
+    int32_t stalled;       // the true residual stopped halving in this recurrence
 };
 
 enum BatchState : int32_t { kBActive = 0, kBConverging = 1, kBSuccess = 2, kBFailed = 3 };
@@ -68,8 +71,11 @@ hipError_t launch_jacobi_dot_partials(const LevelView& L, const bf16* x, const f
 hipError_t launch_bupdate_laplacian_dot(const LevelView& L, const int32_t* leafprob, const BatchPcg* st,
                                         const float* z, const float* p, float* p_new, float* x, float* q,
                                         double* partials, hipStream_t s);
+// r -= alpha q, or for problems with `replacing` set r = b - A (x + alpha p)
+// in FP64 (the single-problem replacement step); partial max|r|
 hipError_t launch_baxpy_absmax(const LevelView& L, const int32_t* leafprob, const BatchPcg* st, const float* q,
-                               float* r, double* partials, hipStream_t s);
+                               const float* x, const float* p, const float* b, float* r, double* partials,
+                               hipStream_t s);
 hipError_t launch_bupdate_x(const LevelView& L, const int32_t* leafprob, BatchPcg* st, const float* p, float* x,
                             hipStream_t s);
 // One workgroup per problem: sums its blocks' partials in the single-problem

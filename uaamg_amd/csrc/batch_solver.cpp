@@ -600,7 +600,7 @@ void PoissonBatch::issue_first() {
     else precond(0, p_[0], r0_, true, kBFinRho);
     check(launch_laplacian_dot_partials(v0, p_[0], q0_, partials_, stream_), "A p");
     check(launch_bfinalize(kBFinSigma, K, blk_begin_, partials_, st_, stream_), "finalize");
-    check(launch_baxpy_absmax(v0, leafprob_, st_, q0_, r0_, partials_, stream_), "axpy");
+    check(launch_baxpy_absmax(v0, leafprob_, st_, q0_, x0_, p_[0], b0_, r0_, partials_, stream_), "axpy");
     check(launch_bfinalize(kBFinNuFirst, K, blk_begin_, partials_, st_, stream_), "finalize");
     check(launch_bstatus(K, st_, seq_dev_, mailbox_dev_, stream_), "status");
 }
@@ -613,7 +613,7 @@ void PoissonBatch::issue_iter(int k) {
     check(launch_bupdate_laplacian_dot(v0, leafprob_, st_, z0_, p_[k], p_[k ^ 1], x0_, q0_, partials_, stream_),
           "update A p");
     check(launch_bfinalize(kBFinSigma, K, blk_begin_, partials_, st_, stream_), "finalize");
-    check(launch_baxpy_absmax(v0, leafprob_, st_, q0_, r0_, partials_, stream_), "axpy");
+    check(launch_baxpy_absmax(v0, leafprob_, st_, q0_, x0_, p_[k ^ 1], b0_, r0_, partials_, stream_), "axpy");
     check(launch_bfinalize(kBFinNuIter, K, blk_begin_, partials_, st_, stream_), "finalize");
     check(launch_bstatus(K, st_, seq_dev_, mailbox_dev_, stream_), "status");
 }

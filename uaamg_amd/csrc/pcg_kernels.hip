@@ -331,31 +331,50 @@ __global__ __launch_bounds__(kCgThreads) void coarsest_cg_kernel(CoarsestView C,
 // r = b - A x with FP64 products and sums (fma(x, diag, s) of the operator
 // becomes x * diag + s in FP64), rounded once to FP32, and the partial
 // maxima of |r| as absmax_kernel forms them.
-__global__ __launch_bounds__(kBlockThreads) void residual64_kernel(LevelView L, const float* X, const float* B,
-                                                                   float* R, double* partials) {
+template <class Src>
+__device__ __forceinline__ void residual64_row(const LevelView& L, const Src& src, const float* B, float* R, int leaf,
+                                               int lane, uint32_t rm, float& m, float& bad) {
+    const LeafRec Rec = leaf_rec(L, leaf);
+    float r[8], xc[8], dg[8], b[8];
+    double s[8];
+    stencil_offdiag64(L, Rec, src, leaf, lane, xc, s);
+    ld8_slot(L.diag, Rec.sd, lane, L.dflt_diag, dg);
+    ld8(B + size_t(leaf) * 512 + lane * 8, b);
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+        const double ax = double(xc[z]) * double(dg[z]) + s[z];
+        r[z] = float(double(b[z]) - ax);
+    }
+    mask_row(rm, r);
+    st8(R + size_t(leaf) * 512 + lane * 8, r);
+#pragma unroll
+    for (int z = 0; z < 8; ++z)
+        if ((rm >> z) & 1u) absmax_acc(r[z], m, bad);
+}
+
+template <class Src>
+__device__ __forceinline__ void residual64_body(const LevelView& L, const Src& src, const float* B, float* R,
+                                                double* partials) {
     const int leaf = wave_leaf();
     const int lane = threadIdx.x & 63;
     float m = 0.f, bad = 0.f;
-    if (leaf < L.nleaf) {
-        const uint32_t rm = row_mask(L, leaf, lane);
-        const LeafRec Rec = leaf_rec(L, leaf);
-        float r[8], xc[8], dg[8], b[8];
-        double s[8];
-        stencil_offdiag64(L, Rec, PlainSrc{X}, leaf, lane, xc, s);
-        ld8_slot(L.diag, Rec.sd, lane, L.dflt_diag, dg);
-        ld8(B + size_t(leaf) * 512 + lane * 8, b);
-#pragma unroll
-        for (int z = 0; z < 8; ++z) {
-            const double ax = double(xc[z]) * double(dg[z]) + s[z];
-            r[z] = float(double(b[z]) - ax);
-        }
-        mask_row(rm, r);
-        st8(R + size_t(leaf) * 512 + lane * 8, r);
-#pragma unroll
-        for (int z = 0; z < 8; ++z)
-            if ((rm >> z) & 1u) absmax_acc(r[z], m, bad);
-    }
+    if (leaf < L.nleaf) residual64_row(L, src, B, R, leaf, lane, row_mask(L, leaf, lane), m, bad);
     write_absmax_partial(m, bad, partials);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void residual64_kernel(LevelView L, const float* X, const float* B,
+                                                                   float* R, double* partials) {
+    residual64_body(L, PlainSrc{X}, B, R, partials);
+}
+
+// The residual replacement inside a PCG recurrence: r = b - A (x + alpha p)
+// as residual64_kernel forms it, for the iterate whose update x += alpha p is
+// still pending (the next step or the final update stores exactly these
+// values). Takes the place of r -= alpha A p.
+__global__ __launch_bounds__(kBlockThreads) void replace64_kernel(LevelView L, const float* X, const float* P,
+                                                                  const float* B, float* R, const PcgScalars* sc,
+                                                                  double* partials) {
+    residual64_body(L, StepSrc{X, P, sc->alpha}, B, R, partials);
 }
 
 // ---------------------------------------------------------- batched solve
@@ -412,15 +431,18 @@ __global__ __launch_bounds__(kBlockThreads) void bupdate_laplacian_dot_kernel(Le
 }
 
 __global__ __launch_bounds__(kBlockThreads) void baxpy_absmax_kernel(LevelView L, const int32_t* leafprob,
-                                                                     const BatchPcg* st, const float* Zv, float* R,
-                                                                     double* partials) {
+                                                                     const BatchPcg* st, const float* Zv,
+                                                                     const float* X, const float* P, const float* B,
+                                                                     float* R, double* partials) {
     const int leaf = wave_leaf();
     const int lane = threadIdx.x & 63;
     float m = 0.f, bad = 0.f;
     if (leaf < L.nleaf) {
         const uint32_t rm = row_mask(L, leaf, lane);
         const BatchPcg& sp = st[leaf_problem(leafprob, leaf)];
-        if (rm && sp.state == kBActive) {
+        if (sp.state == kBActive && sp.replacing) {
+            residual64_row(L, StepSrc{X, P, sp.alpha}, B, R, leaf, lane, rm, m, bad);
+        } else if (rm && sp.state == kBActive) {
             const size_t row = size_t(leaf) * 512 + lane * 8;
             float r[8], z[8];
             ld8(R + row, r);
@@ -504,15 +526,18 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
         if (op == kBFinReplace) {   // the restart rule of solve_pcg_dev's residual replacement
             if (sp.state != kBSuccess || !sp.refine) return;
             sp.true_nu = nu;
-            if (nu <= sp.numax || !(nu < 0.5f * sp.prev_true) || sp.replacements >= kBMaxReplacements ||
+            if (nu <= sp.numax || !(nu < 0.5f * sp.prev_true) || sp.restarts >= kBMaxReplacements ||
                 sp.iters >= sp.max_iters) {
                 sp.refine = 0;
                 return;
             }
             sp.prev_true = nu;
+            sp.restarts += 1;
             sp.replacements += 1;
             sp.it0 = sp.iters;
             sp.nu = sp.nu_prev = nu;
+            sp.replacing = 0;
+            sp.stalled = 0;
             sp.state = kBActive;
             return;
         }
@@ -525,8 +550,28 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
                 return;
             }
         }
-        if (nu <= sp.numax) sp.state = kBConverging;
-        else if (nu > sp.nu_prev && sp.iters - sp.it0 > 3) sp.state = kBFailed;
+        // the step rules of pcg_iterate, the in-recurrence replacement included
+        const int replaced = sp.replacing;
+        sp.replacing = 0;
+        if (nu <= sp.numax) {
+            sp.state = kBConverging;
+        } else {
+            if (replaced && !(nu < 0.5f * sp.nu_prev)) {
+                // the true residual stopped halving: the recurrence ends for
+                // a restart when the restart rule takes one (kBFinReplace
+                // then finds this nu again), else goes on with the recursive residual
+                if (sp.refine && sp.restarts < kBMaxReplacements && nu < 0.5f * sp.prev_true)
+                    sp.state = kBConverging;
+                else
+                    sp.stalled = 1;
+            } else if (nu > sp.nu_prev && sp.iters - sp.it0 > 3) {
+                sp.state = kBFailed;
+            }
+            if (sp.state == kBActive && sp.refine && !sp.stalled && nu <= kReplaceBand * sp.numax) {
+                sp.replacing = 1;
+                sp.replacements += 1;
+            }
+        }
         sp.nu_prev = nu;
         return;
     }
@@ -613,6 +658,14 @@ hipError_t launch_residual64_absmax(const LevelView& L, const float* x, const fl
                                     double* partials, HostMailbox* mb, hipStream_t s) {
     const int g = reduction_blocks(L);
     residual64_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, x, b, r, partials);
+    if (hipError_t e = hipGetLastError()) return e;
+    return finalize(kFinNu, g, partials, sc, mb, s);
+}
+
+hipError_t launch_replace64_absmax(const LevelView& L, const float* x, const float* p, const float* b, float* r,
+                                  PcgScalars* sc, double* partials, HostMailbox* mb, hipStream_t s) {
+    const int g = reduction_blocks(L);
+    replace64_kernel<<<unsigned(g), kBlockThreads, 0, s>>>(L, x, p, b, r, sc, partials);
     if (hipError_t e = hipGetLastError()) return e;
     return finalize(kFinNu, g, partials, sc, mb, s);
 }
@@ -739,9 +792,10 @@ hipError_t launch_bupdate_laplacian_dot(const LevelView& L, const int32_t* leafp
 }
 
 hipError_t launch_baxpy_absmax(const LevelView& L, const int32_t* leafprob, const BatchPcg* st, const float* q,
-                               float* r, double* partials, hipStream_t s) {
+                               const float* x, const float* p, const float* b, float* r, double* partials,
+                               hipStream_t s) {
     if (L.nleaf <= 0) return hipSuccess;
-    baxpy_absmax_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, q, r, partials);
+    baxpy_absmax_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, q, x, p, b, r, partials);
     return hipGetLastError();
 }
 

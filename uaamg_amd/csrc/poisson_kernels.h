@@ -264,5 +264,14 @@ hipError_t launch_block_post(const LevelView& L, const bf16* x, const bf16* xc, 
 // FP64, stored as FP32, then nu = ||r||_inf through the mailbox
 hipError_t launch_residual64_absmax(const LevelView& L, const float* x, const float* b, float* r, PcgScalars* sc,
                                     double* partials, HostMailbox* mb, hipStream_t s);
+// With FP64 residual replacement, the PCG steps started while ||r||_inf <=
+// kReplaceBand * the tolerance form r as the true residual of the new iterate
+// (about the last three steps at a tenfold reduction per step), so the
+// recursive residual's drift never builds up near convergence.
+constexpr float kReplaceBand = 1000.f;
+// the same for the iterate with its pending update: r = b - A (x + alpha p),
+// alpha from the scalars (the replacement taking the place of r -= alpha A p)
+hipError_t launch_replace64_absmax(const LevelView& L, const float* x, const float* p, const float* b, float* r,
+                                  PcgScalars* sc, double* partials, HostMailbox* mb, hipStream_t s);
 
 }  // namespace uaamg

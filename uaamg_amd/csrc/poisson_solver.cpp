@@ -92,7 +92,7 @@ void PoissonDevice::release_device() {
 }
 
 void PoissonDevice::destroy_graphs() {
-    for (Graph* g : {&g_init_, &g_first_, &g_iter_[0], &g_iter_[1]}) {
+    for (Graph* g : {&g_init_, &g_first_, &g_iter_[0], &g_iter_[1], &g_repl_[0], &g_repl_[1]}) {
         if (g->e) (void)hipGraphExecDestroy(g->e);
         if (g->g) (void)hipGraphDestroy(g->g);
         g->e = nullptr;
@@ -132,26 +132,33 @@ void PoissonDevice::issue_first() {
     check(launch_axpy_absmax(v0, q0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
 }
 
-void PoissonDevice::issue_iter(int k) {
+void PoissonDevice::issue_iter(int k, bool replace) {
     const LevelView& v0 = lv_[0].v;
     precond(0, z0_, r0_, true, kDotRhoNew);
     check(launch_update_laplacian_dot(v0, z0_, p_[k], p_[k ^ 1], x0_, q0_, sc_, partials_, stream_), "update A p");
-    check(launch_axpy_absmax(v0, q0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
+    if (replace)
+        check(launch_replace64_absmax(v0, x0_, p_[k ^ 1], b0_, r0_, sc_, partials_, mailbox_dev_, stream_),
+              "replace");
+    else
+        check(launch_axpy_absmax(v0, q0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
 }
 
 int PoissonDevice::ensure_graphs() {
-    if (graph_smoother_ == opt_.smoother && graph_sor_ == sor_w_) return 0;
+    if (graph_smoother_ == opt_.smoother && graph_sor_ == sor_w_ && graph_refine_ == refine_) return 0;
     destroy_graphs();
     int rc = capture(g_init_, [&] { issue_init(); });
     if (!rc) rc = capture(g_first_, [&] { issue_first(); });
     if (!rc) rc = capture(g_iter_[0], [&] { issue_iter(0); });
     if (!rc) rc = capture(g_iter_[1], [&] { issue_iter(1); });
+    if (!rc && refine_) rc = capture(g_repl_[0], [&] { issue_iter(0, true); });
+    if (!rc && refine_) rc = capture(g_repl_[1], [&] { issue_iter(1, true); });
     if (rc) {
         destroy_graphs();
         return rc;
     }
     graph_smoother_ = opt_.smoother;
     graph_sor_ = sor_w_;
+    graph_refine_ = refine_;
     return 0;
 }
 
@@ -982,32 +989,42 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
     const float numax = opt_.relative_tolerance * nu;
     int status = kFailed;
     true_nu_ = -1.f;
-    replacements_ = -1;
+    replacements_ = refine_ ? 0 : -1;
+    bool nu_true = false;
+    float prev = INFINITY;   // true residual at the last restart
+    int restarts = 0;
+    // a recurrence whose true residual stops halving ends for a restart when
+    // the restart rule below would take one
+    auto restart_below = [&] { return refine_ && restarts < kMaxReplacements ? 0.5f * prev : 0.f; };
     if (nu <= numax) {
         status = kSuccess;
     } else if (opt_.max_iterations > 0) {
-        status = pcg_iterate(nu, numax, it);
+        status = pcg_iterate(nu, numax, it, nu_true, restart_below());
         if (status < 0) return status;
     }
     if (refine_ && status == kSuccess) {
-        // FP64 residual replacement: the recursive residual met the tolerance;
-        // recompute r = b - A x in FP64 and, while the true residual misses
-        // it, restart the PCG recurrence from x on that residual (the
-        // iteration count continues). Restarts stop when one no longer halves
-        // the true residual (the FP32 iterate's attainable floor); the status
-        // then stays the recursive criterion's, and last_true_residual() says
-        // how far the true residual got.
-        replacements_ = 0;
-        float prev = INFINITY;
+        // FP64 residual replacement. Near convergence the steps already
+        // replace r by the true residual (pcg_iterate); when the recursive
+        // residual met the tolerance without that, recompute r = b - A x in
+        // FP64 and, while the true residual misses the tolerance, restart the
+        // PCG recurrence from x on that residual (the iteration count
+        // continues). Restarts stop when one no longer halves the true
+        // residual (the FP32 iterate's attainable floor); the status then
+        // stays the recursive criterion's, and last_true_residual() says how
+        // far the true residual got.
         for (;;) {
-            check(launch_residual64_absmax(v0, x0_, b0_, r0_, sc_, partials_, mailbox_dev_, stream_), "residual64");
-            if (wait_nu(&nu)) return -3;
+            if (!nu_true) {
+                check(launch_residual64_absmax(v0, x0_, b0_, r0_, sc_, partials_, mailbox_dev_, stream_),
+                      "residual64");
+                if (wait_nu(&nu)) return -3;
+            }
             true_nu_ = nu;
             if (nu <= numax || !(nu < 0.5f * prev)) break;
-            if (replacements_ >= kMaxReplacements || it >= opt_.max_iterations) break;
+            if (restarts >= kMaxReplacements || it >= opt_.max_iterations) break;
             prev = nu;
+            ++restarts;
             ++replacements_;
-            status = pcg_iterate(nu, numax, it);
+            status = pcg_iterate(nu, numax, it, nu_true, restart_below());
             if (status < 0) return status;
             if (status != kSuccess) break;
         }
@@ -1022,20 +1039,48 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
 // ||r||_inf): p = M r, then the iteration pieces until ||r||_inf <= numax
 // (x updated, kSuccess), the residual grows after this recurrence's third
 // iteration or `it` reaches max_iterations (kFailed), or an error (< 0).
-int PoissonDevice::pcg_iterate(float& nu, float numax, int& it) {
+// With refine_, a piece started while ||r||_inf <= kReplaceBand * numax
+// forms r as the FP64 true residual b - A x of the new iterate instead of
+// r - alpha A p (counted in replacements_). When such a true residual does
+// not halve, the recurrence ends (x updated, kSuccess with nu above numax)
+// if nu < restart_below, so the caller restarts it; otherwise it goes on
+// with the recursive residual. nu_true says that the final nu is the FP64
+// true residual of the returned x.
+int PoissonDevice::pcg_iterate(float& nu, float numax, int& it, bool& nu_true, float restart_below) {
     const LevelView& v0 = lv_[0].v;
     const int it0 = it;   // the growth test counts this recurrence's own iterations
+    nu_true = false;
     if (run(g_first_, [&] { issue_first(); })) return -3;
     int k = 0;   // p_[k] holds the current direction
+    bool replaced = false, stalled = false;
     for (;;) {
         const float nu_old = nu;
         if (wait_nu(&nu)) return -3;
         if (nu <= numax) {
             check(launch_update_x(v0, p_[k], x0_, sc_, stream_), "update x");
+            nu_true = replaced;
             return kSuccess;
         }
-        if (nu > nu_old && it - it0 > 3) return kFailed;
-        if (run(g_iter_[k], [&] { issue_iter(k); })) return -3;
+        if (replaced && !(nu < 0.5f * nu_old)) {
+            // the true residual stopped halving: a fresh recurrence from it
+            // when the restart rule allows one, else this one goes on with
+            // the recursive residual
+            if (nu < restart_below) {
+                check(launch_update_x(v0, p_[k], x0_, sc_, stream_), "update x");
+                nu_true = true;
+                return kSuccess;
+            }
+            stalled = true;
+        } else if (nu > nu_old && it - it0 > 3) {
+            return kFailed;
+        }
+        replaced = refine_ && !stalled && nu <= kReplaceBand * numax;
+        if (replaced) {
+            ++replacements_;
+            if (run(g_repl_[k], [&] { issue_iter(k, true); })) return -3;
+        } else if (run(g_iter_[k], [&] { issue_iter(k); })) {
+            return -3;
+        }
         k ^= 1;
         if (++it >= opt_.max_iterations) {
             if (wait_nu(&nu)) return -3;   // the piece's own reduction

@@ -143,7 +143,7 @@ private:
     int capture(Graph& g, const std::function<void()>& body);
     void issue_init();
     void issue_first();
-    void issue_iter(int k);
+    void issue_iter(int k, bool replace = false);
     int ensure_graphs();
     void destroy_graphs();
     int run(Graph& g, const std::function<void()>& body);
@@ -156,7 +156,7 @@ private:
     void precond_fused(int level, float* X, const float* B, bool skip_first, int dot_target);
     template <class TX, class TB>
     void precond_mixed(int level, TX* X, const TB* B, bool skip_first, int dot_target);
-    int pcg_iterate(float& nu, float numax, int& it);
+    int pcg_iterate(float& nu, float numax, int& it, bool& nu_true, float restart_below);
     void tune_levels();
     void visit_pair(int level, bool block);
     void mg_cycle(int level, float* X, const float* B);
@@ -180,6 +180,7 @@ private:
     CoarsestView cv_{};
     // captured PCG pieces, valid for graph_smoother_ / graph_sor_
     Graph g_init_, g_first_, g_iter_[2];
+    Graph g_repl_[2];   // g_iter_ with the FP64 residual replacement (captured with refine_)
     int graph_smoother_ = -1;
     std::vector<void*> spai0_bufs_;   // SPAI0 bricks uploaded after acquire_device
     bool spai0_on_device_ = false;
@@ -191,7 +192,8 @@ private:
     std::vector<uint8_t> use_block_;       // per level: visits run as two block-visit launches
     std::vector<float> tune_ms_[2];        // per level: measured ms of a visit pair, [0] block-visit, [1] per-sweep
     bool mixed_ = false;                   // BF16-stored W-cycle vectors (Jacobi-family smoothers)
-    bool refine_ = false;                  // FP64 residual replacement after convergence
+    bool refine_ = false;                  // FP64 residual replacement
+    bool graph_refine_ = false;
     float true_nu_ = -1.f;
     int replacements_ = -1;
     HostMailbox* mailbox_ = nullptr;       // pinned host memory

@@ -216,6 +216,30 @@ struct UpdateSrc {
     }
 };
 
+// The PCG iterate with its pending update, x + alpha p, formed as the update
+// kernels form it (product, then sum; contraction is off), so that values
+// read here equal the ones the deferred update later stores.
+struct StepSrc {
+    const float* X;
+    const float* P;
+    float alpha;
+    __device__ __forceinline__ void row(int leaf, int, int, int r, float v[8]) const {
+        float x[8], p[8];
+        ld8(X + size_t(leaf) * 512 + r * 8, x);
+        ld8(P + size_t(leaf) * 512 + r * 8, p);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float t = alpha * p[k];
+            v[k] = x[k] + t;
+        }
+    }
+    __device__ __forceinline__ float at(int leaf, int, int, int r, int z) const {
+        const size_t i = size_t(leaf) * 512 + r * 8 + z;
+        const float t = alpha * P[i];
+        return X[i] + t;
+    }
+};
+
 // The operands of the 7-point stencil for one row: neighbour values and the
 // coefficients of the faces they couple through.
 struct StencilRow {
