@@ -3,13 +3,14 @@
 // tolerance = float epsilon, restated (constructCoarsestLevelExactSolver,
 // simd_vdb_poisson.cpp:2114-2126; Eigen ConjugateGradient.h conjugate_gradient()).
 //
-// Each thread keeps up to kPer DOFs' matrix rows and x, r, p, q in registers;
-// only p is shared through LDS for the neighbour reads. An absent neighbour
-// reads slot kMaxCoarsestDof of the shared p, which holds zero, so the six
-// reads of a row are unconditional. Per iteration two block reductions:
-// <p, Ap>, then ||r||^2 and <r, z> together (z is formed right after r is
-// updated; when ||r||^2 passes the stopping test z is simply unused, as in
-// Eigen's loop).
+// Each thread keeps up to kPer DOFs' matrix rows and x, r, z, p, q = A p in
+// registers; only z is shared through LDS for the neighbour reads. An absent
+// neighbour reads slot kMaxCoarsestDof of the shared vector, which holds
+// zero, so the six reads of a row are unconditional. Two barriers per
+// iteration: the reduction of <p, Ap>, and the reduction of ||r||^2 and
+// <r, z>, whose barrier also publishes the new z. The new direction's
+// product is then formed as A p = A z + beta A p_old (the same vector as
+// Eigen's A p, rounded differently), so no third barrier publishes p.
 #pragma once
 
 #include <cfloat>
@@ -62,14 +63,13 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
     const int n = C.ndof;
     const int tid = threadIdx.x;
     int nb[kPer][6];
-    float cf[kPer][8], x[kPer], r[kPer], p[kPer], q[kPer];
-    float part = 0.f;
+    float cf[kPer][8], x[kPer], r[kPer], z[kPer], p[kPer], q[kPer];
     constexpr int kZero = kMaxCoarsestDof;
     if (tid == 0) sh.sp[kZero] = 0.f;   // published by the first reduction's barrier
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int d = tid + j * kThreads;
-        x[j] = r[j] = p[j] = q[j] = 0.f;
+        x[j] = r[j] = z[j] = p[j] = q[j] = 0.f;
 #pragma unroll
         for (int k = 0; k < 6; ++k) nb[j][k] = kZero;
 #pragma unroll
@@ -88,27 +88,28 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
             cf[j][0] = c0.x; cf[j][1] = c0.y; cf[j][2] = c0.z; cf[j][3] = c0.w;
             cf[j][4] = c1.x; cf[j][5] = c1.y; cf[j][6] = c1.z; cf[j][7] = c1.w;
             r[j] = float(B[C.pos[d]]);
-            part += r[j] * r[j];
         }
     }
-    const float rhs_norm2 = cg_sum2<kThreads>(part, 0.f, sh, parity).x;
-    bool done = rhs_norm2 == 0.f;
-    const float threshold = fmaxf(C.tol * C.tol * rhs_norm2, FLT_MIN);
-    if (!done && rhs_norm2 < threshold) done = true;
-    if (!done) {
-        part = 0.f;
+    // ||b||^2 and <r, z> of z = M r in one reduction, whose barrier publishes z
+    float rr = 0.f, rz = 0.f;
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) {
-            const int d = tid + j * kThreads;
-            if (d < n) {
-                p[j] = cf[j][7] * r[j];
-                sh.sp[d] = p[j];
-                part += r[j] * p[j];
-            }
+    for (int j = 0; j < kPer; ++j) {
+        const int d = tid + j * kThreads;
+        if (d < n) {
+            rr += r[j] * r[j];
+            z[j] = cf[j][7] * r[j];
+            sh.sp[d] = z[j];
+            rz += r[j] * z[j];
         }
-        float abs_new = cg_sum2<kThreads>(part, 0.f, sh, parity).x;   // its barrier also publishes sp
+    }
+    const float2 s0 = cg_sum2<kThreads>(rr, rz, sh, parity);
+    const float rhs_norm2 = s0.x;
+    const float threshold = fmaxf(C.tol * C.tol * rhs_norm2, FLT_MIN);
+    if (rhs_norm2 != 0.f && !(rhs_norm2 < threshold)) {
+        float abs_new = s0.y, beta = 0.f;
         for (int it = 0; it < C.max_iters; ++it) {
-            part = 0.f;
+            // p = z + beta p and A p = A z + beta A p (beta = 0 on the first step)
+            float pq = 0.f;
 #pragma unroll
             for (int j = 0; j < kPer; ++j) {
                 const int d = tid + j * kThreads;
@@ -119,12 +120,16 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
                     const float f1 = v[0] * cf[j][1], f2 = v[1] * cf[j][2], f3 = v[2] * cf[j][3];
                     const float f4 = v[3] * cf[j][4], f5 = v[4] * cf[j][5], f6 = v[5] * cf[j][6];
                     const float s = ((f1 + f2) + (f3 + f4)) + (f5 + f6);
-                    q[j] = fmaf(p[j], cf[j][0], s);
-                    part += p[j] * q[j];
+                    const float az = fmaf(z[j], cf[j][0], s);
+                    p[j] = z[j] + beta * p[j];
+                    q[j] = az + beta * q[j];
+                    pq += p[j] * q[j];
                 }
             }
-            const float alpha = abs_new / cg_sum2<kThreads>(part, 0.f, sh, parity).x;
-            float rr = 0.f, rz = 0.f;
+            // the barrier also orders this step's reads of z before the writes below
+            const float alpha = abs_new / cg_sum2<kThreads>(pq, 0.f, sh, parity).x;
+            rr = 0.f;
+            rz = 0.f;
 #pragma unroll
             for (int j = 0; j < kPer; ++j) {
                 const int d = tid + j * kThreads;
@@ -132,24 +137,16 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
                     x[j] = x[j] + alpha * p[j];
                     r[j] = r[j] - alpha * q[j];
                     rr += r[j] * r[j];
-                    q[j] = cf[j][7] * r[j];   // z
-                    rz += r[j] * q[j];
+                    z[j] = cf[j][7] * r[j];
+                    sh.sp[d] = z[j];
+                    rz += r[j] * z[j];
                 }
             }
             const float2 sums = cg_sum2<kThreads>(rr, rz, sh, parity);
             if (sums.x < threshold) break;
             const float abs_old = abs_new;
             abs_new = sums.y;
-            const float beta = abs_new / abs_old;
-#pragma unroll
-            for (int j = 0; j < kPer; ++j) {
-                const int d = tid + j * kThreads;
-                if (d < n) {
-                    p[j] = q[j] + beta * p[j];
-                    sh.sp[d] = p[j];
-                }
-            }
-            __syncthreads();
+            beta = abs_new / abs_old;
         }
     }
 #pragma unroll
