@@ -675,6 +675,7 @@ int PoissonBatch::solve(float* const* x, const float* const* b, int* status, int
         }
         graphs_ready_ = true;
     }
+    if (own_stream_ && check(caller_.follow(stream_), "order after the caller's stream")) return -3;
     const bool refine = mixed_ || (opt_.flags & 64);
     for (int k = 0; k < K; ++k) {
         BatchPcg& s = st_host_[size_t(k)];

@@ -90,6 +90,7 @@ private:
 
     SolverOptions opt_;
     hipStream_t stream_ = nullptr;
+    CallerOrder caller_;
     bool own_stream_ = false;
     DeviceArena arena_;
     std::vector<Level> lv_;

@@ -49,6 +49,10 @@ int apply_pressure_gradient_device(PoissonDevice& d, const float* pressure_dev, 
         err = "hipSetDevice failed";
         return -3;
     }
+    if (d.follow_caller()) {
+        err = d.error();
+        return -3;
+    }
     auto hip_fail = [&](hipError_t e, const char* what) {
         err = std::string(what) + ": " + hipGetErrorString(e);
         (void)hipStreamSynchronize(s);

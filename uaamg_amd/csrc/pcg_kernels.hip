@@ -318,9 +318,6 @@ __global__ __launch_bounds__(kBlockThreads) void update_kernel(LevelView L, cons
 // ---------------------------------------------------------- coarsest solve
 constexpr int kCgThreads = 512;
 
-// kMaxDof = 2048 (four row slots per thread) when the system fits: fewer
-// registers, about 8% faster at ~1700 DOF; the same values
-constexpr int kCgSmallDof = 2048;
 
 template <class V, int kMaxDof>
 __global__ __launch_bounds__(kCgThreads) void coarsest_cg_kernel(CoarsestView C, const V* B, V* X) {

@@ -421,16 +421,36 @@ void PoissonDevice::visit_pair(int l, bool block) {
     const bool weighted = opt_.smoother == kWeightedJacobi;
     auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
     auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    // above the coarsest level the block path solves it inside its pre
+    // launch, so both paths time the coarsest solve there
+    const bool with_coarsest = l + 1 == nlevels() - 1 && cv_.ndof <= kCgSmallDof;
     for (int first = 1; first >= 0; --first) {
         if (block) {
             if (mixed_) {
-                check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                if (with_coarsest)
+                    check(launch_block_pre_coarsest(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), cv_,
+                                                    c.Xh, &sc_->counter[0], stream_),
+                          "tune");
+                else
+                    check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_),
+                          "tune");
                 check(launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, wpost(0), wpost(1), wpost(2), stream_), "tune");
             } else {
-                check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                if (with_coarsest)
+                    check(launch_block_pre_coarsest(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), cv_, c.X,
+                                                    &sc_->counter[0], stream_),
+                          "tune");
+                else
+                    check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
                 check(launch_block_post(v, d.T, c.X, d.B, d.X, wpost(0), wpost(1), wpost(2), stream_), "tune");
             }
-        } else if (mixed_) {
+            continue;
+        }
+        if (with_coarsest) {
+            if (mixed_) check(launch_coarsest_cg(cv_, c.Bh, c.Xh, stream_), "tune");
+            else check(launch_coarsest_cg(cv_, c.B, c.X, stream_), "tune");
+        }
+        if (mixed_) {
             if (first) {
                 check(launch_jacobi_first2(v, d.Bh, d.Th, wpre(0), wpre(1), stream_), "tune");
                 check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(2), stream_), "tune");
@@ -810,9 +830,15 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
         // two launches per visit: pre-smoothing + residual + restriction, then
         // prolongation + post-smoothing (the coarser visits run in between)
         DevLevel& c = lv_[l + 1];
-        check(launch_block_pre(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
-        precond(l + 1, c.X, c.B, true, -1);
-        if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
+        if (l + 1 == L - 1 && cv_.ndof <= kCgSmallDof) {   // the coarsest solve inside the pre launch
+            check(launch_block_pre_coarsest(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), cv_, c.X,
+                                            &sc_->counter[0], stream_),
+                  "block pre + coarsest");
+        } else {
+            check(launch_block_pre(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+            precond(l + 1, c.X, c.B, true, -1);
+            if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
+        }
         check(launch_block_post(v, d.T, c.X, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
         return;
     }
@@ -897,8 +923,15 @@ void PoissonDevice::precond_mixed(int l, TX* X, const TB* B, bool skip_first, in
     };
     if constexpr (std::is_same_v<TX, bf16>) {
         if (l >= 1 && use_block_[size_t(l)]) {
-            check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "block pre");
-            coarse();
+            if (l + 1 == L - 1 && cv_.ndof <= kCgSmallDof) {   // the coarsest solve inside the pre launch
+                check(launch_block_pre_coarsest(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), cv_, c.Xh,
+                                                &sc_->counter[0], stream_),
+                      "block pre + coarsest");
+            } else {
+                check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_),
+                      "block pre");
+                coarse();
+            }
             check(launch_block_post(v, d.Th, c.Xh, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
             return;
         }
@@ -973,11 +1006,17 @@ void PoissonDevice::mg_cycle(int l, float* X, const float* B) {
 // iteration runs the reference's lines 13-16 of iteration k and lines 6-8 of
 // iteration k+1; the host reads ||r||_inf from the mailbox and applies the
 // convergence and failure tests of lines 9-12 between pieces.
+int PoissonDevice::follow_caller() {
+    if (!own_stream_) return 0;
+    return check(caller_.follow(stream_), "order after the caller's stream");
+}
+
 int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
     if (opt_.smoother == kSpai0 && ensure_spai0()) return -3;
     if (use_graphs_ && ensure_graphs()) return -3;
+    if (follow_caller()) return -3;
     const LevelView& v0 = lv_[0].v;
     int it = 0;
     if (iterations) *iterations = 0;
@@ -1094,6 +1133,7 @@ int PoissonDevice::pcg_iterate(float& nu, float numax, int& it, bool& nu_true, f
 int PoissonDevice::solve_mg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
+    if (follow_caller()) return -3;
     if (opt_.smoother == kSpai0 && ensure_spai0()) return -3;
     const LevelView& v0 = lv_[0].v;
     int it = 0;

@@ -11,6 +11,7 @@
 
 #include "poisson_host.h"
 #include "poisson_kernels.h"
+#include "stream_order.h"
 
 namespace uaamg {
 
@@ -78,6 +79,8 @@ public:
     const PoissonHierarchyHost& host() const { return host_; }
     SolverOptions& options() { return opt_; }
     hipStream_t stream() const { return stream_; }
+    // orders the solver's stream after the caller's null-stream work (own stream only)
+    int follow_caller();
     int device() const { return opt_.device; }
     // device view of a level (on_device() only)
     const LevelView& level_view(int l) const { return lv_[size_t(l)].v; }
@@ -166,6 +169,7 @@ private:
     PoissonHierarchyHost host_;
     SolverOptions opt_;
     hipStream_t stream_ = nullptr;
+    CallerOrder caller_;
     bool own_stream_ = false;
     DeviceArena arena_;
     std::vector<DevLevel> lv_;

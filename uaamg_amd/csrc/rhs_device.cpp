@@ -32,6 +32,10 @@ int build_poisson_rhs_device(PoissonDevice& d, const RhsGridRef* grids, float* r
         err = "hipSetDevice failed";
         return -3;
     }
+    if (d.follow_caller()) {
+        err = d.error();
+        return -3;
+    }
     auto hip_fail = [&](hipError_t e, const char* what) {
         err = std::string(what) + ": " + hipGetErrorString(e);
         (void)hipStreamSynchronize(s);   // no copy still reads a buffer that goes out of scope

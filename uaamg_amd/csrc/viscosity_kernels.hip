@@ -506,43 +506,41 @@ __device__ __forceinline__ float2 bsum2(float a, float b, float (*red)[kCgT / 64
     return t;
 }
 
+// Two barriers per iteration: the reduction of <p, Ap>, and the reduction of
+// ||r||^2 and <r, z>, whose barrier also publishes the new z; the new
+// direction's product is formed as A p = A z + beta A p_old (the same vector
+// as Eigen's A p, rounded differently), so no third barrier publishes p.
 __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C, const float* B, float* X) {
-    __shared__ float sp[kViscMaxCoarsest];
-    __shared__ float sq[kViscMaxCoarsest];   // A p of the thread's own rows
+    __shared__ float sz[kViscMaxCoarsest];
+    __shared__ float sq[kViscMaxCoarsest];   // A z of the thread's own rows
     __shared__ float red[2][kCgT / 64][2];
     int par = 0;
     const int n = C.n, tid = threadIdx.x;
-    float x[kCgPer], r[kCgPer], p[kCgPer], q[kCgPer], inv[kCgPer];
-    float part = 0.f;
+    float x[kCgPer], r[kCgPer], z[kCgPer], p[kCgPer], q[kCgPer], inv[kCgPer];
+    float rr = 0.f, rz = 0.f;
 #pragma unroll
     for (int j = 0; j < kCgPer; ++j) {
         const int d = tid + j * kCgT;
-        x[j] = r[j] = p[j] = q[j] = inv[j] = 0.f;
+        x[j] = r[j] = z[j] = p[j] = q[j] = inv[j] = 0.f;
         if (d < n) {
             r[j] = B[C.pos[d]];
             inv[j] = C.invdiag[d];
-            part += r[j] * r[j];
+            rr += r[j] * r[j];
+            z[j] = inv[j] * r[j];
+            sz[d] = z[j];
+            rz += r[j] * z[j];
         }
     }
-    const float rhs2 = bsum2(part, 0.f, red, par).x;
+    const float2 s0 = bsum2(rr, rz, red, par);   // its barrier publishes z
+    const float rhs2 = s0.x;
     const float thr = fmaxf(C.tol * C.tol * rhs2, FLT_MIN);
     if (rhs2 != 0.f && !(rhs2 < thr)) {
-        part = 0.f;
-#pragma unroll
-        for (int j = 0; j < kCgPer; ++j) {
-            const int d = tid + j * kCgT;
-            if (d < n) {
-                p[j] = inv[j] * r[j];
-                sp[d] = p[j];
-                part += r[j] * p[j];
-            }
-        }
-        float abs_new = bsum2(part, 0.f, red, par).x;
+        float abs_new = s0.y, beta = 0.f;
         for (int it = 0; it < C.max_iters; ++it) {
-            // q = A p row by row: a row's entries are loaded together
-            // (padding is in bounds) and summed in column order up to the
-            // row's length; the rows loop is not unrolled (registers), so the
-            // sums go through LDS, each read back by the thread that wrote it
+            // A z row by row: a row's entries are loaded together (padding is
+            // in bounds) and summed in column order up to the row's length;
+            // the rows loop is not unrolled (registers), so the sums go
+            // through LDS, each read back by the thread that wrote it
 #pragma unroll 1
             for (int j = 0; j < kCgPer; ++j) {
                 const int d = tid + j * kCgT;
@@ -552,23 +550,27 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
 #pragma unroll
                 for (int k = 0; k < kViscMaxRow; ++k) {
                     const size_t at = size_t(k) * size_t(n) + size_t(d);
-                    const float m = C.eval[at] * sp[C.ecol[at]];
+                    const float m = C.eval[at] * sz[C.ecol[at]];
                     const float u = t + m;
                     t = k < len ? u : t;
                 }
                 sq[d] = t;
             }
-            part = 0.f;
+            // p = z + beta p and A p = A z + beta A p (beta = 0 on the first step)
+            float pq = 0.f;
 #pragma unroll
             for (int j = 0; j < kCgPer; ++j) {
                 const int d = tid + j * kCgT;
                 if (d < n) {
-                    q[j] = sq[d];
-                    part += p[j] * q[j];
+                    p[j] = z[j] + beta * p[j];
+                    q[j] = sq[d] + beta * q[j];
+                    pq += p[j] * q[j];
                 }
             }
-            const float alpha = abs_new / bsum2(part, 0.f, red, par).x;
-            float rr = 0.f, rz = 0.f;
+            // the barrier also orders this step's reads of z before the writes below
+            const float alpha = abs_new / bsum2(pq, 0.f, red, par).x;
+            rr = 0.f;
+            rz = 0.f;
 #pragma unroll
             for (int j = 0; j < kCgPer; ++j) {
                 const int d = tid + j * kCgT;
@@ -576,24 +578,16 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
                     x[j] = x[j] + alpha * p[j];
                     r[j] = r[j] - alpha * q[j];
                     rr += r[j] * r[j];
-                    q[j] = inv[j] * r[j];
-                    rz += r[j] * q[j];
+                    z[j] = inv[j] * r[j];
+                    sz[d] = z[j];
+                    rz += r[j] * z[j];
                 }
             }
             const float2 sums = bsum2(rr, rz, red, par);
             if (sums.x < thr) break;
             const float abs_old = abs_new;
             abs_new = sums.y;
-            const float beta = abs_new / abs_old;
-#pragma unroll
-            for (int j = 0; j < kCgPer; ++j) {
-                const int d = tid + j * kCgT;
-                if (d < n) {
-                    p[j] = q[j] + beta * p[j];
-                    sp[d] = p[j];
-                }
-            }
-            __syncthreads();
+            beta = abs_new / abs_old;
         }
     }
 #pragma unroll

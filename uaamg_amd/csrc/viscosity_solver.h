@@ -11,6 +11,7 @@
 
 #include "poisson_kernels.h"
 #include "poisson_solver.h"
+#include "stream_order.h"
 #include "viscosity_host.h"
 #include "viscosity_kernels.h"
 
@@ -107,6 +108,7 @@ private:
     ViscHierarchyHost host_;
     ViscOptions opt_;
     hipStream_t stream_ = nullptr;
+    CallerOrder caller_;
     bool own_stream_ = false;
     DeviceArena arena_;
     std::vector<DevLevel> lv_;
