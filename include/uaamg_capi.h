@@ -92,7 +92,8 @@ typedef struct uaamg_options {
                                  block-visit kernels (A/B checks); bit 6 = FP64 residual replacement
                                  at FP32 precision; bit 8 = level-0 per-voxel build stage on the host
                                  threads instead of the device (same result) */
-    void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
+    void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle, which each device
+                                 entry point first orders after the work queued on the null stream */
 } uaamg_options;
 
 typedef struct uaamg_poisson uaamg_poisson;
