@@ -133,6 +133,29 @@ def test_mixed_warm_start_and_failure(problem):
     assert solver.solveMultigridPCG(x4, bb) == U.FAILED
 
 
+def test_mixed_warm_started_stream_of_right_hand_sides(problem):
+    """A serving stream: each solve starts from the previous right-hand
+    side's solution (the residual is then mostly the 5% perturbation, so the
+    tolerance lies near the FP32 iterate's floor). Every solve succeeds, and
+    the FP64 true residual it reports is the host's."""
+    s, orc, b, x64 = problem
+    lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, precision=BF16)
+    solver = U.PoissonSolver(lap)
+    solver.mRelativeTolerance = TOL
+    rng = np.random.default_rng(21)
+    x = lap.getZeroVectorGrid()
+    for k in range(6):
+        bk = (b * (1.0 + 0.05 * rng.standard_normal(b.shape))).astype(np.float32)
+        st = solver.solveMultigridPCG(x, bk)
+        true_nu, reps = lap.last_residual()
+        r = bk.astype(np.float64) - orc.apply64(x.astype(np.float64))
+        nu_host = float(np.abs(r).max())
+        _say(f"  warm solve {k}: status {st}, it {solver.mIterationTaken}, replacements {reps}, "
+             f"true residual {nu_host:.3e}")
+        assert st == U.SUCCESS, f"solve {k}"
+        assert abs(true_nu - nu_host) <= 1e-3 * nu_host + 1e-30
+
+
 def test_mixed_rejects_unknown_precision(problem):
     s, orc, b, x64 = problem
     with pytest.raises(U.UaamgError):

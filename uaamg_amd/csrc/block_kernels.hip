@@ -19,7 +19,6 @@
 //   kPost     (H = 3): x + P x_coarse, three sweeps
 #include <type_traits>
 
-#include "coarsest_device.h"
 #include "stencil_device.h"
 
 namespace uaamg {
@@ -99,16 +98,9 @@ __device__ __forceinline__ void sweep(float* x, const float* b, const float* dg,
         if (at[k] >= 0) x[at[k]] = out[k];
 }
 
-//
-// kCoarsest (pre modes on level L-2): the workgroup that finishes last then
-// solves the coarsest system (right-hand side Cv, solution Xcs) in the LDS
-// the region no longer needs. Each workgroup's coarse values are released to
-// the device before it counts itself done on `done`; the last one acquires
-// them all, resets the counter and runs the solve.
-template <int H, int MODE, class V, bool kCoarsest = false>
+template <int H, int MODE, class V>
 __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, const V* Xin, const V* Xc, const V* B,
-                                                                  V* Xout, V* Cv, float w0, float w1, float w2,
-                                                                  CoarsestView C, V* Xcs, unsigned int* done) {
+                                                                  V* Xout, V* Cv, float w0, float w1, float w2) {
     constexpr int S = Region<H>::S, N = Region<H>::N;
     extern __shared__ float lds[];
     float* b = lds;
@@ -257,33 +249,18 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
             for (int jj = 0; jj < 2; ++jj)
                 for (int kk = 0; kk < 2; ++kk) sum += rs[((2 * ccx + ii) * 8 + (2 * ccy + jj)) * 8 + 2 * ccz + kk];
         const int p = L.parent[leaf];
-        if (p >= 0) {
-            const int cl = p >> 3, oct = p & 7;
-            const int off =
-                ((ccx + 4 * ((oct >> 2) & 1)) * 8 + (ccy + 4 * ((oct >> 1) & 1))) * 8 + ccz + 4 * (oct & 1);
-            Cv[size_t(cl) * 512 + off] = V(sum * 0.125f);
-        }
-    }
-    if constexpr (kCoarsest) {
-        __shared__ unsigned int last;
-        __threadfence();   // release this workgroup's coarse values
-        __syncthreads();
-        if (tid == 0) last = atomicAdd(done, 1u) == gridDim.x - 1 ? 1u : 0u;
-        __syncthreads();
-        if (!last) return;
-        __threadfence();   // acquire every workgroup's values
-        if (tid == 0) *done = 0u;   // seen by the next launch (kernel boundary)
-        static_assert(sizeof(CgShared<kBlkThreads>) <= Region<H>::lds_bytes, "coarsest solve fits the region's LDS");
-        coarsest_cg_block<kBlkThreads, V, kCgSmallDof>(C, Cv, Xcs, *reinterpret_cast<CgShared<kBlkThreads>*>(lds));
+        if (p < 0) return;
+        const int cl = p >> 3, oct = p & 7;
+        const int off = ((ccx + 4 * ((oct >> 2) & 1)) * 8 + (ccy + 4 * ((oct >> 1) & 1))) * 8 + ccz + 4 * (oct & 1);
+        Cv[size_t(cl) * 512 + off] = V(sum * 0.125f);
     }
 }
 
-template <int H, int MODE, class V, bool kCoarsest = false>
+template <int H, int MODE, class V>
 hipError_t launch_mode(const LevelView& L, const V* xin, const V* xc, const V* b, V* xout, V* cv, float w0, float w1,
-                       float w2, hipStream_t s, const CoarsestView& C = CoarsestView{}, V* xcs = nullptr,
-                       unsigned int* done = nullptr) {
-    block_visit_kernel<H, MODE, V, kCoarsest><<<unsigned(L.nleaf), kBlkThreads, Region<H>::lds_bytes, s>>>(
-        L, xin, xc, b, xout, cv, w0, w1, w2, C, xcs, done);
+                       float w2, hipStream_t s) {
+    block_visit_kernel<H, MODE, V><<<unsigned(L.nleaf), kBlkThreads, Region<H>::lds_bytes, s>>>(L, xin, xc, b, xout,
+                                                                                                cv, w0, w1, w2);
     return hipGetLastError();
 }
 
@@ -297,12 +274,6 @@ hipError_t set_lds_limits() {
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPost, V>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPreZero, V, true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<4, kPreAgain, V, true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<4>::lds_bytes));
     return e;
 }
 
@@ -313,16 +284,6 @@ hipError_t block_pre_t(const LevelView& L, bool from_zero, const V* x, const V* 
     if (!L.nb27) return hipErrorInvalidValue;
     if (from_zero) return launch_mode<3, kPreZero, V>(L, nullptr, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
     return launch_mode<4, kPreAgain, V>(L, x, nullptr, b, x_out, coarse_b, w0, w1, w2, s);
-}
-
-template <class V>
-hipError_t block_pre_coarsest_t(const LevelView& L, bool from_zero, const V* x, const V* b, V* x_out, V* coarse_b,
-                                float w0, float w1, float w2, const CoarsestView& C, V* xc, unsigned int* done,
-                                hipStream_t s) {
-    if (L.nleaf <= 0 || !L.nb27 || !done || C.ndof > kCgSmallDof) return hipErrorInvalidValue;
-    if (from_zero)
-        return launch_mode<3, kPreZero, V, true>(L, nullptr, nullptr, b, x_out, coarse_b, w0, w1, w2, s, C, xc, done);
-    return launch_mode<4, kPreAgain, V, true>(L, x, nullptr, b, x_out, coarse_b, w0, w1, w2, s, C, xc, done);
 }
 
 template <class V>
@@ -359,18 +320,6 @@ hipError_t launch_block_pre(const LevelView& L, bool from_zero, const bf16* x, c
 hipError_t launch_block_post(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* x_out,
                              float w0, float w1, float w2, hipStream_t s) {
     return block_post_t<bf16>(L, x, xc, b, x_out, w0, w1, w2, s);
-}
-
-hipError_t launch_block_pre_coarsest(const LevelView& L, bool from_zero, const float* x, const float* b, float* x_out,
-                                     float* coarse_b, float w0, float w1, float w2, const CoarsestView& C, float* xc,
-                                     unsigned int* done, hipStream_t s) {
-    return block_pre_coarsest_t<float>(L, from_zero, x, b, x_out, coarse_b, w0, w1, w2, C, xc, done, s);
-}
-
-hipError_t launch_block_pre_coarsest(const LevelView& L, bool from_zero, const bf16* x, const bf16* b, bf16* x_out,
-                                     bf16* coarse_b, float w0, float w1, float w2, const CoarsestView& C, bf16* xc,
-                                     unsigned int* done, hipStream_t s) {
-    return block_pre_coarsest_t<bf16>(L, from_zero, x, b, x_out, coarse_b, w0, w1, w2, C, xc, done, s);
 }
 
 }  // namespace uaamg

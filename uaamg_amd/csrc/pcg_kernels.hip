@@ -547,7 +547,8 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
                 return;
             }
         }
-        // the step rules of pcg_iterate, the in-recurrence replacement included
+        // the step rules of pcg_iterate, the in-recurrence replacement (first
+        // recurrence only) included
         const int replaced = sp.replacing;
         sp.replacing = 0;
         if (nu <= sp.numax) {
@@ -564,7 +565,8 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
             } else if (nu > sp.nu_prev && sp.iters - sp.it0 > 3) {
                 sp.state = kBFailed;
             }
-            if (sp.state == kBActive && sp.refine && !sp.stalled && nu <= kReplaceBand * sp.numax) {
+            if (sp.state == kBActive && sp.refine && sp.restarts == 0 && !sp.stalled &&
+                nu <= kReplaceBand * sp.numax) {
                 sp.replacing = 1;
                 sp.replacements += 1;
             }

@@ -123,7 +123,7 @@ struct PcgScalars {
     float beta;
     float nu;          // ||r||_inf (NaN if any entry is not finite)
     unsigned int seq;  // number of completed ||r||_inf reductions
-    unsigned int counter[8];   // [0]: workgroups of a block pre visit done (the last one solves the coarsest level)
+    unsigned int counter[8];   // completion counters of the reduction kernels
 };
 
 // Host-mapped pinned mailbox: each ||r||_inf reduction publishes
@@ -235,13 +235,6 @@ hipError_t launch_block_pre(const LevelView& L, bool from_zero, const float* x, 
 // x_out = three sweeps (w0, w1, w2) from x + P x_coarse
 hipError_t launch_block_post(const LevelView& L, const float* x, const float* xc, const float* b, float* x_out,
                              float w0, float w1, float w2, hipStream_t s);
-// launch_block_pre on level L-2 followed by the coarsest solve of coarse_b
-// into xc (launch_coarsest_cg's arithmetic) in the workgroup that finishes
-// last, so the solve needs no launch of its own. C.ndof <= kCgSmallDof;
-// `done` is a device counter that is zero before and after the launch.
-hipError_t launch_block_pre_coarsest(const LevelView& L, bool from_zero, const float* x, const float* b, float* x_out,
-                                     float* coarse_b, float w0, float w1, float w2, const CoarsestView& C, float* xc,
-                                     unsigned int* done, hipStream_t s);
 
 // ---- mixed-precision W-cycle: the same passes with the level vectors stored
 // in BF16 and every operation computed in FP32. Level 0's right-hand side is
@@ -269,10 +262,6 @@ hipError_t launch_block_pre(const LevelView& L, bool from_zero, const bf16* x, c
                             bf16* coarse_b, float w0, float w1, float w2, hipStream_t s);
 hipError_t launch_block_post(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* x_out,
                              float w0, float w1, float w2, hipStream_t s);
-hipError_t launch_block_pre_coarsest(const LevelView& L, bool from_zero, const bf16* x, const bf16* b, bf16* x_out,
-                                     bf16* coarse_b, float w0, float w1, float w2, const CoarsestView& C, bf16* xc,
-                                     unsigned int* done, hipStream_t s);
-
 // ---- FP64 residual replacement: r = b - A x with every product and sum in
 // FP64, stored as FP32, then nu = ||r||_inf through the mailbox
 hipError_t launch_residual64_absmax(const LevelView& L, const float* x, const float* b, float* r, PcgScalars* sc,

@@ -421,36 +421,16 @@ void PoissonDevice::visit_pair(int l, bool block) {
     const bool weighted = opt_.smoother == kWeightedJacobi;
     auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
     auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
-    // above the coarsest level the block path solves it inside its pre
-    // launch, so both paths time the coarsest solve there
-    const bool with_coarsest = l + 1 == nlevels() - 1 && cv_.ndof <= kCgSmallDof;
     for (int first = 1; first >= 0; --first) {
         if (block) {
             if (mixed_) {
-                if (with_coarsest)
-                    check(launch_block_pre_coarsest(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), cv_,
-                                                    c.Xh, &sc_->counter[0], stream_),
-                          "tune");
-                else
-                    check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_),
-                          "tune");
+                check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "tune");
                 check(launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, wpost(0), wpost(1), wpost(2), stream_), "tune");
             } else {
-                if (with_coarsest)
-                    check(launch_block_pre_coarsest(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), cv_, c.X,
-                                                    &sc_->counter[0], stream_),
-                          "tune");
-                else
-                    check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
+                check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
                 check(launch_block_post(v, d.T, c.X, d.B, d.X, wpost(0), wpost(1), wpost(2), stream_), "tune");
             }
-            continue;
-        }
-        if (with_coarsest) {
-            if (mixed_) check(launch_coarsest_cg(cv_, c.Bh, c.Xh, stream_), "tune");
-            else check(launch_coarsest_cg(cv_, c.B, c.X, stream_), "tune");
-        }
-        if (mixed_) {
+        } else if (mixed_) {
             if (first) {
                 check(launch_jacobi_first2(v, d.Bh, d.Th, wpre(0), wpre(1), stream_), "tune");
                 check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(2), stream_), "tune");
@@ -830,15 +810,9 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
         // two launches per visit: pre-smoothing + residual + restriction, then
         // prolongation + post-smoothing (the coarser visits run in between)
         DevLevel& c = lv_[l + 1];
-        if (l + 1 == L - 1 && cv_.ndof <= kCgSmallDof) {   // the coarsest solve inside the pre launch
-            check(launch_block_pre_coarsest(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), cv_, c.X,
-                                            &sc_->counter[0], stream_),
-                  "block pre + coarsest");
-        } else {
-            check(launch_block_pre(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
-            precond(l + 1, c.X, c.B, true, -1);
-            if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
-        }
+        check(launch_block_pre(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+        precond(l + 1, c.X, c.B, true, -1);
+        if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
         check(launch_block_post(v, d.T, c.X, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
         return;
     }
@@ -923,15 +897,8 @@ void PoissonDevice::precond_mixed(int l, TX* X, const TB* B, bool skip_first, in
     };
     if constexpr (std::is_same_v<TX, bf16>) {
         if (l >= 1 && use_block_[size_t(l)]) {
-            if (l + 1 == L - 1 && cv_.ndof <= kCgSmallDof) {   // the coarsest solve inside the pre launch
-                check(launch_block_pre_coarsest(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), cv_, c.Xh,
-                                                &sc_->counter[0], stream_),
-                      "block pre + coarsest");
-            } else {
-                check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_),
-                      "block pre");
-                coarse();
-            }
+            check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+            coarse();
             check(launch_block_post(v, d.Th, c.Xh, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
             return;
         }
@@ -1002,15 +969,15 @@ void PoissonDevice::mg_cycle(int l, float* X, const float* B) {
     for (int i = 0; i < kSweeps; ++i) smooth(l, cur, oth, B, i, false, true);
 }
 
-// solveMultigridPCG (simd_vdb_poisson.cpp:2155-2226). One captured piece per
-// iteration runs the reference's lines 13-16 of iteration k and lines 6-8 of
-// iteration k+1; the host reads ||r||_inf from the mailbox and applies the
-// convergence and failure tests of lines 9-12 between pieces.
 int PoissonDevice::follow_caller() {
     if (!own_stream_) return 0;
     return check(caller_.follow(stream_), "order after the caller's stream");
 }
 
+// solveMultigridPCG (simd_vdb_poisson.cpp:2155-2226). One captured piece per
+// iteration runs the reference's lines 13-16 of iteration k and lines 6-8 of
+// iteration k+1; the host reads ||r||_inf from the mailbox and applies the
+// convergence and failure tests of lines 9-12 between pieces.
 int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
@@ -1038,16 +1005,18 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
     if (nu <= numax) {
         status = kSuccess;
     } else if (opt_.max_iterations > 0) {
-        status = pcg_iterate(nu, numax, it, nu_true, restart_below());
+        status = pcg_iterate(nu, numax, it, nu_true, restart_below(), refine_);
         if (status < 0) return status;
     }
     if (refine_ && status == kSuccess) {
-        // FP64 residual replacement. Near convergence the steps already
-        // replace r by the true residual (pcg_iterate); when the recursive
-        // residual met the tolerance without that, recompute r = b - A x in
-        // FP64 and, while the true residual misses the tolerance, restart the
-        // PCG recurrence from x on that residual (the iteration count
-        // continues). Restarts stop when one no longer halves the true
+        // FP64 residual replacement. Near convergence the first
+        // recurrence's steps already replace r by the true residual
+        // (pcg_iterate), ending it for a restart when that stops halving;
+        // when the recursive residual met the tolerance without that,
+        // recompute r = b - A x in FP64. While the true residual misses the
+        // tolerance, restart the PCG recurrence from x on it (the iteration
+        // count continues; restarted recurrences use the recursive
+        // residual). Restarts stop when one no longer halves the true
         // residual (the FP32 iterate's attainable floor); the status then
         // stays the recursive criterion's, and last_true_residual() says how
         // far the true residual got.
@@ -1063,7 +1032,7 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
             prev = nu;
             ++restarts;
             ++replacements_;
-            status = pcg_iterate(nu, numax, it, nu_true, restart_below());
+            status = pcg_iterate(nu, numax, it, nu_true, restart_below(), false);
             if (status < 0) return status;
             if (status != kSuccess) break;
         }
@@ -1078,14 +1047,14 @@ int PoissonDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iteratio
 // ||r||_inf): p = M r, then the iteration pieces until ||r||_inf <= numax
 // (x updated, kSuccess), the residual grows after this recurrence's third
 // iteration or `it` reaches max_iterations (kFailed), or an error (< 0).
-// With refine_, a piece started while ||r||_inf <= kReplaceBand * numax
+// With `replace`, a piece started while ||r||_inf <= kReplaceBand * numax
 // forms r as the FP64 true residual b - A x of the new iterate instead of
 // r - alpha A p (counted in replacements_). When such a true residual does
 // not halve, the recurrence ends (x updated, kSuccess with nu above numax)
 // if nu < restart_below, so the caller restarts it; otherwise it goes on
 // with the recursive residual. nu_true says that the final nu is the FP64
 // true residual of the returned x.
-int PoissonDevice::pcg_iterate(float& nu, float numax, int& it, bool& nu_true, float restart_below) {
+int PoissonDevice::pcg_iterate(float& nu, float numax, int& it, bool& nu_true, float restart_below, bool replace) {
     const LevelView& v0 = lv_[0].v;
     const int it0 = it;   // the growth test counts this recurrence's own iterations
     nu_true = false;
@@ -1113,7 +1082,7 @@ int PoissonDevice::pcg_iterate(float& nu, float numax, int& it, bool& nu_true, f
         } else if (nu > nu_old && it - it0 > 3) {
             return kFailed;
         }
-        replaced = refine_ && !stalled && nu <= kReplaceBand * numax;
+        replaced = replace && !stalled && nu <= kReplaceBand * numax;
         if (replaced) {
             ++replacements_;
             if (run(g_repl_[k], [&] { issue_iter(k, true); })) return -3;

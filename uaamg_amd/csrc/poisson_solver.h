@@ -159,7 +159,7 @@ private:
     void precond_fused(int level, float* X, const float* B, bool skip_first, int dot_target);
     template <class TX, class TB>
     void precond_mixed(int level, TX* X, const TB* B, bool skip_first, int dot_target);
-    int pcg_iterate(float& nu, float numax, int& it, bool& nu_true, float restart_below);
+    int pcg_iterate(float& nu, float numax, int& it, bool& nu_true, float restart_below, bool replace);
     void tune_levels();
     void visit_pair(int level, bool block);
     void mg_cycle(int level, float* X, const float* B);
