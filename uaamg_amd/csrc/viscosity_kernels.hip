@@ -597,6 +597,100 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
     }
 }
 
+// Systems of at most kCgSmallN rows keep their ELL rows in registers (two
+// rows per thread), so an iteration reads only the shared z: the same
+// products and sums, in the same order, as visc_coarsest_kernel.
+constexpr int kCgSmallN = 1024;
+
+__global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestView C, const float* B, float* X) {
+    constexpr int kPer = kCgSmallN / kCgT;
+    __shared__ float sz[kCgSmallN];
+    __shared__ float red[2][kCgT / 64][2];
+    int par = 0;
+    const int n = C.n, tid = threadIdx.x;
+    float x[kPer], r[kPer], z[kPer], p[kPer], q[kPer], inv[kPer];
+    float ev[kPer][kViscMaxRow];
+    int ec[kPer][kViscMaxRow], len[kPer];
+    float rr = 0.f, rz = 0.f;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int d = tid + j * kCgT;
+        x[j] = r[j] = z[j] = p[j] = q[j] = inv[j] = 0.f;
+        len[j] = 0;
+#pragma unroll
+        for (int k = 0; k < kViscMaxRow; ++k) {
+            ev[j][k] = 0.f;
+            ec[j][k] = 0;
+        }
+        if (d < n) {
+            len[j] = C.elen[d];
+#pragma unroll
+            for (int k = 0; k < kViscMaxRow; ++k) {
+                const size_t at = size_t(k) * size_t(n) + size_t(d);
+                ev[j][k] = C.eval[at];
+                ec[j][k] = C.ecol[at];
+            }
+            r[j] = B[C.pos[d]];
+            inv[j] = C.invdiag[d];
+            rr += r[j] * r[j];
+            z[j] = inv[j] * r[j];
+            sz[d] = z[j];
+            rz += r[j] * z[j];
+        }
+    }
+    const float2 s0 = bsum2(rr, rz, red, par);   // its barrier publishes z
+    const float rhs2 = s0.x;
+    const float thr = fmaxf(C.tol * C.tol * rhs2, FLT_MIN);
+    if (rhs2 != 0.f && !(rhs2 < thr)) {
+        float abs_new = s0.y, beta = 0.f;
+        for (int it = 0; it < C.max_iters; ++it) {
+            float pq = 0.f;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int d = tid + j * kCgT;
+                if (d < n) {
+                    float t = 0.f;
+#pragma unroll
+                    for (int k = 0; k < kViscMaxRow; ++k) {
+                        const float m = ev[j][k] * sz[ec[j][k]];
+                        const float u = t + m;
+                        t = k < len[j] ? u : t;
+                    }
+                    p[j] = z[j] + beta * p[j];
+                    q[j] = t + beta * q[j];
+                    pq += p[j] * q[j];
+                }
+            }
+            // the barrier also orders this step's reads of z before the writes below
+            const float alpha = abs_new / bsum2(pq, 0.f, red, par).x;
+            rr = 0.f;
+            rz = 0.f;
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const int d = tid + j * kCgT;
+                if (d < n) {
+                    x[j] = x[j] + alpha * p[j];
+                    r[j] = r[j] - alpha * q[j];
+                    rr += r[j] * r[j];
+                    z[j] = inv[j] * r[j];
+                    sz[d] = z[j];
+                    rz += r[j] * z[j];
+                }
+            }
+            const float2 sums = bsum2(rr, rz, red, par);
+            if (sums.x < thr) break;
+            const float abs_old = abs_new;
+            abs_new = sums.y;
+            beta = abs_new / abs_old;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int d = tid + j * kCgT;
+        if (d < n) X[C.pos[d]] = x[j];
+    }
+}
+
 template <int MODE>
 hipError_t apply_mode(const ViscLevelView& L, int c, const float* x, const float* b, float* y, hipStream_t s) {
     const int n = L.chan_off[c + 1] - L.chan_off[c];
@@ -660,7 +754,8 @@ hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView&, 
 hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float* x, hipStream_t s) {
     if (C.n <= 0) return hipSuccess;
     if (C.n > kViscMaxCoarsest) return hipErrorInvalidValue;
-    visc_coarsest_kernel<<<1, kCgT, 0, s>>>(C, b, x);
+    if (C.n <= kCgSmallN) visc_coarsest_small_kernel<<<1, kCgT, 0, s>>>(C, b, x);
+    else visc_coarsest_kernel<<<1, kCgT, 0, s>>>(C, b, x);
     return hipGetLastError();
 }
 
