@@ -240,8 +240,9 @@ int uaamg_poisson_solve_pcg_device(uaamg_poisson* h, float* x_dev, const float* 
 /*
  * After a solve with FP64 residual replacement (UAAMG_PRECISION_BF16_CYCLE, or flags bit 6):
  * ||b - A x||_inf of the returned x evaluated in FP64, and the number of times the PCG residual
- * was replaced by that true residual: the steps taken within 1000x of the tolerance, plus restarts
- * of the recurrence when the recursive residual met the tolerance and the true one did not.
+ * was replaced by that true residual: the first recurrence's steps taken within 1000x of the
+ * tolerance, plus restarts of the recurrence when the true residual missed the tolerance (after
+ * the recursive one met it, or when a replaced step stopped halving it).
  * Both are -1 when the last solve ran without replacement. No reference counterpart.
  */
 int uaamg_poisson_last_residual(const uaamg_poisson* h, float* true_residual_inf, int32_t* replacements);
