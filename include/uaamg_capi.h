@@ -90,8 +90,11 @@ typedef struct uaamg_options {
     int32_t flags;            /* 0 normally; bit 0 = unfused kernels, bit 1 = no graph replay,
                                  bit 2 = bottom levels in one single-workgroup launch, bit 4 = no
                                  block-visit kernels (A/B checks); bit 6 = FP64 residual replacement
-                                 at FP32 precision; bit 8 = level-0 per-voxel build stage on the host
-                                 threads instead of the device (same result) */
+                                 at FP32 precision; bit 7 = pure-multigrid fallback in the merged batch
+                                 and the solve server: a problem whose PCG returns FAILED is zeroed and
+                                 solved by solvePureMultigrid, as FLIP_vdb.cpp:3150-3155 does; bit 8 =
+                                 level-0 per-voxel build stage on the host threads instead of the
+                                 device (same result) */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle, which each device
                                  entry point first orders after the work queued on the null stream */
 } uaamg_options;
@@ -289,6 +292,11 @@ int uaamg_poisson_batch_solve_device(uaamg_poisson_batch* h, float* const* x_dev
                                      int32_t* status, int32_t* iterations);
 /* Per problem after the last solve, as uaamg_poisson_last_residual (arrays of nproblems entries). */
 int uaamg_poisson_batch_last_residual(const uaamg_poisson_batch* h, float* true_residual_inf, int32_t* replacements);
+/* After a solve with flags bit 7 (FLIP's fallback, FLIP_vdb.cpp:3150-3155): per input problem, the
+ * status and iteration count (mIterationTaken) of its solvePureMultigrid run, -1 where its PCG
+ * succeeded. The PCG's FAILED status and iterations stay in uaamg_poisson_batch_solve_device's
+ * outputs; x holds the pure-multigrid result. */
+int uaamg_poisson_batch_last_fallback(const uaamg_poisson_batch* h, int32_t* mg_status, int32_t* mg_iterations);
 
 /* PoissonSolver::solvePureMultigrid (simd_vdb_poisson.h:163, .cpp:2228-2267), host vectors. */
 int uaamg_poisson_solve_mg(uaamg_poisson* h, float* x, const float* b, int32_t* iterations);
@@ -551,6 +559,12 @@ int uaamg_server_submit(uaamg_server* s, int64_t problem, const float* b_dev, fl
  * UAAMG_FAILED or a negative error), the times are in ms from submission (queueing, and total). */
 int uaamg_server_wait(uaamg_server* s, int64_t ticket, int32_t* status, int32_t* iterations, double* queue_ms,
                       double* total_ms);
+/* uaamg_server_wait plus the pure-multigrid fallback of a Poisson problem registered with flags bit 7:
+ * when its PCG returned FAILED (*status = UAAMG_FAILED, *iterations its PCG iterations), the server
+ * zeroed the solution and ran solvePureMultigrid (FLIP_vdb.cpp:3150-3155); *mg_status and
+ * *mg_iterations receive that run's status and iterations, -1 when it did not run. */
+int uaamg_server_wait_fallback(uaamg_server* s, int64_t ticket, int32_t* status, int32_t* iterations,
+                               int32_t* mg_status, int32_t* mg_iterations, double* queue_ms, double* total_ms);
 int uaamg_server_stats(uaamg_server* s, uaamg_server_counters* out);
 
 #ifdef __cplusplus

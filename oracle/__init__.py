@@ -45,7 +45,7 @@ def lib() -> C.CDLL:
         "orc_level_nleaf_dof": (i64, [vp, i32]),
         "orc_level_topology": (None, [vp, i32, vp, vp, vp]),
         "orc_level_coefficients": (None, [vp, i32, i32, vp, vp]),
-        "orc_build_rhs": (None, [vp, i64, vp, vp, vp, i64, vp, vp, vp, vp, f32, vp]),
+        "orc_build_rhs": (None, [vp, i64, vp, vp, vp, i64, vp, vp, vp, vp, f32, i64, vp, vp, vp, vp]),
         "orc_apply": (i32, [vp, i32, i32, f32, vp, vp, vp]),
         "orc_face_weights": (None, [i64, vp, vp, i64, vp, vp, f32, C.POINTER(i64), vp, vp, vp]),
         "orc_apply_pressure_gradient": (None, [vp, vp, i64, vp, vp, f32, i64, vp, vp, vp, i64, vp, vp, vp, i32,
@@ -151,7 +151,9 @@ class OraclePoisson:
         lib().orc_level_coefficients(self.h, level, kind, _p(br), _p(kept))
         return br, kept
 
-    def build_rhs(self, weights, vx, vy, vz) -> np.ndarray:
+    def build_rhs(self, weights, vx, vy, vz, solid_velocity=None) -> np.ndarray:
+        """createPressurePoissonRightHandSide (simd_vdb_poisson.cpp:95-127). vx, vy, vz share one leaf
+        topology; solid_velocity is a Vec3 grid (None: zero everywhere)."""
         rhs = np.zeros(self.ndof(0), np.float32)
         wo = np.ascontiguousarray(weights.origins, np.int32)
         wv = np.ascontiguousarray(weights.values, np.float32)
@@ -159,8 +161,15 @@ class OraclePoisson:
         vo = np.ascontiguousarray(vx.origins, np.int32)
         xs = [np.ascontiguousarray(v.values, np.float32) for v in (vx, vy, vz)]
         vbg = float(np.array(vx.background).reshape(-1)[0])
+        if solid_velocity is None:
+            so, sv, sbg = np.zeros((0, 3), np.int32), np.zeros((0, 512, 3), np.float32), np.zeros(3, np.float32)
+        else:
+            so = np.ascontiguousarray(solid_velocity.origins, np.int32)
+            sv = np.ascontiguousarray(solid_velocity.values, np.float32)
+            sbg = np.array(solid_velocity.background, dtype=np.float32).reshape(-1)
+            sbg = np.resize(sbg, 3).astype(np.float32)
         lib().orc_build_rhs(self.h, wo.shape[0], _p(wo), _p(wv), _p(wbg), vo.shape[0], _p(vo), *[_p(x) for x in xs],
-                            vbg, _p(rhs))
+                            vbg, so.shape[0], _p(so), _p(sv), _p(sbg), _p(rhs))
         return rhs
 
     def apply_pressure_gradient(self, pressure, phi, weights, solid, velocity, dt: float, friction: float):

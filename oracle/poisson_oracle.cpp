@@ -1053,14 +1053,23 @@ void orc_level_coefficients(void* p, int l, int kind, float* bricks, uint8_t* ke
     }
 }
 
-// BuildPoissonRhs (:19-93) with solid velocity = 0 when s_val is null
+// BuildPoissonRhs (simd_vdb_poisson.cpp:19-93, driven by
+// createPressurePoissonRightHandSide :95-127): per DOF voxel, over the faces
+// x+ x- y+ y- z+ z- (face f of channel c read at the voxel, +1 along c for the
+// + face), rhs -= / += invdx * (w * u + (1 - w) * u_solid); a voxel whose six
+// face weights are all 0 gets 0. The solid velocity is a Vec3 grid like the
+// weights (ns = 0: no leaves, every lookup reads s_bg). Pinned by
+// tests/test_oracle_rhs.py (closed forms of uniform, linear, quadratic and
+// blended fields, and a dense brute-force divergence).
 void orc_build_rhs(void* p, int64_t nw, const int32_t* w_org, const float* w_val, const float* w_bg, int64_t nv,
-                   const int32_t* v_org, const float* vx, const float* vy, const float* vz, float v_bg, float* rhs) {
+                   const int32_t* v_org, const float* vx, const float* vy, const float* vz, float v_bg, int64_t ns,
+                   const int32_t* s_org, const float* s_val, const float* s_bg, float* rhs) {
     Handle* h = static_cast<Handle*>(p);
     const Level& L = h->levels[0];
-    VGrid w;
+    VGrid w, sv;
     FGrid v[3];
     load_grid<Vec3>(nw, w_org, nullptr, w_val, 3, w_bg, w);
+    load_grid<Vec3>(ns, s_org, nullptr, s_val, 3, s_bg, sv);
     const float* vv[3] = {vx, vy, vz};
     for (int c = 0; c < 3; ++c) load_grid<float>(nv, v_org, nullptr, vv[c], 1, &v_bg, v[c]);
     const float invdx = 1.0f / L.dx;
@@ -1080,7 +1089,7 @@ void orc_build_rhs(void* p, int64_t nw, const int32_t* w_org, const float* w_val
                 const float wt = w.get(nc).v[c];
                 if (wt != 0.f) nz = true;
                 const float vel = v[c].get(nc);
-                const float svel = 0.f;
+                const float svel = sv.get(nc).v[c];
                 const float term = invdx * (wt * vel + (1.0f - wt) * svel);
                 if (pos) r -= term;
                 else r += term;

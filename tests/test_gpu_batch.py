@@ -1,13 +1,19 @@
 """GPU parity of the merged batch (uaamg_poisson_batch_*): every problem's
+status, iteration count (within 1) and solution (1e-5 relative L2) agree with
+the oracle's FP32 solveMultigridPCG and with its FP64 twin; every problem's
 result, status and iteration count are bit-identical to solving it alone with
 its own handle, for a batch of uneven problems (one-, two- and three-level
 hierarchies, given out of depth order), in FP32 and in the BF16 W-cycle form,
 with and without graph replay; per-problem failure (iteration limit, a NaN
-right-hand side) and a zero right-hand side stay confined to their problem.
+right-hand side) and a zero right-hand side stay confined to their problem;
+FLIP's pure-multigrid fallback (FLIP_vdb.cpp:3150-3155) reproduces the
+oracle's solvePureMultigrid on the problems whose PCG fails, including the
+seeded 36^3 problem on which the reference's own PCG rule fails.
 """
 import numpy as np
 import pytest
 
+import oracle as O
 import uaamg_amd as U
 
 pytestmark = pytest.mark.gpu
@@ -125,3 +131,95 @@ def test_batch_warm_start_and_repeat(batch_inputs):
         torch.cuda.synchronize()
         assert st[i] == rc and it[i] == it1
         assert torch.equal(xs[i], x), f"problem {i}"
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_oracle(batch_inputs, precision):
+    scenes, rhs = batch_inputs
+    _, batched = _batched(scenes, rhs, precision)
+    for i, (s, b) in enumerate(zip(scenes, rhs)):
+        orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+        bn = b.cpu().numpy()
+        st_o, it_o, xo = orc.solve_pcg(bn)
+        _, _, x64 = orc.solve_pcg64(bn)
+        st, it, x = batched[i]
+        _say(f"  problem {i} precision {precision}: status {st} / oracle {st_o}, iterations {it} / {it_o}, "
+             f"rel L2 vs FP32 oracle {_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}")
+        assert st == st_o == U.SUCCESS, f"problem {i}"
+        assert abs(it - it_o) <= 1, f"problem {i}"
+        assert _rel(x, x64) <= 1e-5, f"problem {i}"
+        if precision == 0:
+            assert _rel(x, xo) <= 1e-5, f"problem {i}"
+
+
+# Scene(3, 0, seed=22): a 36^3 problem (24084 DOF, 2 levels) on which the
+# reference's PCG rule fails (||r||_inf grows at iteration 5 after iteration 3)
+# and so does FLIP's pure-multigrid fallback (growth after iteration 8, at 14);
+# the batch reports both and returns the pure-multigrid result.
+FAILING = (3, 0, 22)
+
+
+@pytest.fixture(scope="module")
+def failing_inputs(hip_available):
+    import torch
+    dev = torch.device("cuda:0")
+    scenes, rhs = [], []
+    for cfg, n, seed in (FAILING, (3, 40, 3), (0, 32, 0)):
+        s = U.Scene(cfg, n, seed=seed)
+        lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, device=-1)
+        scenes.append(s)
+        rhs.append(torch.from_numpy(lap.createPressurePoissonRightHandSide(s.weights, s.vx, s.vy, s.vz)).to(dev))
+    return scenes, rhs
+
+
+def test_failing_problem_is_the_references(failing_inputs):
+    """the oracle's PCG and pure multigrid both fail on it, at iterations 5 and 14"""
+    scenes, rhs = failing_inputs
+    s = scenes[0]
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    b = rhs[0].cpu().numpy()
+    st, it, _ = orc.solve_pcg(b)
+    assert (st, it) == (U.FAILED, 5)
+    st, it, _ = orc.solve_mg(b)
+    assert (st, it) == (U.FAILED, 14)
+
+
+@pytest.mark.parametrize("max_iterations", [100, 1])
+def test_batch_pure_multigrid_fallback(failing_inputs, max_iterations):
+    """flags bit 7: the failed problem is zeroed and solved by solvePureMultigrid with the same
+    tolerance and iteration limit; the others keep their PCG result. max_iterations = 1 makes
+    every PCG fail."""
+    import torch
+    scenes, rhs = failing_inputs
+    bt = U.PoissonBatch([(s.phi, s.weights, 1.0, s.dx) for s in scenes], flags=128,
+                        max_iterations=max_iterations)
+    xs = [torch.zeros_like(b) for b in rhs]
+    st, it = bt.solve_device([x.data_ptr() for x in xs], [b.data_ptr() for b in rhs])
+    torch.cuda.synchronize()
+    fb = bt.last_fallback()
+    for i, (s, b) in enumerate(zip(scenes, rhs)):
+        orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+        bn = b.cpu().numpy()
+        st_o, it_o, xo = orc.solve_pcg(bn, maxit=max_iterations)
+        x = xs[i].cpu().numpy()
+        _say(f"  problem {i} max_it {max_iterations}: PCG {st[i]}/{it[i]} (oracle {st_o}/{it_o}), "
+             f"fallback {fb[i]}")
+        assert st[i] == st_o and abs(it[i] - it_o) <= 1, f"problem {i}"
+        if st_o == U.SUCCESS:
+            assert fb[i] == (-1, -1)
+            assert _rel(x, xo) <= 1e-5
+            continue
+        mst, mit, xm = orc.solve_mg(bn, maxit=max_iterations)
+        assert fb[i][0] == mst and abs(fb[i][1] - mit) <= 1, f"problem {i}: {fb[i]} vs {(mst, mit)}"
+        assert _rel(x, xm) <= 1e-5, f"problem {i}: {_rel(x, xm):.2e}"
+    # the same problems without the flag: FAILED results stay the PCG's
+    bt0 = U.PoissonBatch([(s.phi, s.weights, 1.0, s.dx) for s in scenes], max_iterations=max_iterations)
+    ys = [torch.zeros_like(b) for b in rhs]
+    st0, it0 = bt0.solve_device([y.data_ptr() for y in ys], [b.data_ptr() for b in rhs])
+    torch.cuda.synchronize()
+    assert list(st0) == list(st) and list(it0) == list(it)
+    assert all(f == (-1, -1) for f in bt0.last_fallback())

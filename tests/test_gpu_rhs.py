@@ -1,12 +1,14 @@
 """GPU parity of the device right-hand-side build (uaamg_poisson_build_rhs_device):
-bit-identical to the host build (uaamg_poisson_build_rhs, itself pinned against the
-oracle) for host-resident and device-resident grid values, with a solid velocity,
-grids in shuffled leaf order, velocity leaves missing (background lookups with a
-non-zero background), and the all-zero-weight rule.
+bit-identical to the oracle's BuildPoissonRhs (simd_vdb_poisson.cpp:19-93) and to the
+host build (uaamg_poisson_build_rhs) for host-resident and device-resident grid
+values, with a solid velocity and fractional face weights (the solid sphere of the
+256^3 scene family), grids in shuffled leaf order, velocity leaves missing
+(background lookups with a non-zero background), and the all-zero-weight rule.
 """
 import numpy as np
 import pytest
 
+import oracle as O
 import uaamg_amd as U
 
 pytestmark = pytest.mark.gpu
@@ -49,14 +51,23 @@ def test_device_rhs_equals_host(hip_available, cfg, n, seed):
                                                      _shuffled(s.vy, seed + 1), _thinned(s.vz, seed + 2, -0.5),
                                                      _shuffled(solid, seed + 3))),
     ]
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    if cfg == 1:
+        frac = s.weights.values[(s.weights.values > 0) & (s.weights.values < 1)].size
+        assert frac > 0, "the solid sphere gives fractional face weights"
     for name, (w, vx, vy, vz, sv) in cases:
         ref = lap.createPressurePoissonRightHandSide(w, vx, vy, vz, sv)
+        if not name.startswith("shuffled"):
+            # the oracle takes the three velocity channels on one topology (the scene's)
+            assert np.array_equal(ref, orc.build_rhs(w, vx, vy, vz, sv)), f"{name}: host build vs oracle"
         out = torch.full((lap.mNumDof,), float("nan"), device=dev)
         lap.createPressurePoissonRightHandSide_device(out.data_ptr(), w, vx, vy, vz, sv)
         torch.cuda.synchronize()
         got = out.cpu().numpy()
         _say(f"  {name}: ndof {lap.mNumDof}, nonzero {int(np.count_nonzero(ref))}")
         assert np.array_equal(got, ref), f"{name}: max diff {np.abs(got - ref).max()}"
+        if not name.startswith("shuffled"):
+            assert np.array_equal(got, orc.build_rhs(w, vx, vy, vz, sv)), f"{name}: device build vs oracle"
         # the same grids with their values already in device memory
         grids = [w, vx, vy, vz] + ([] if sv is None else [sv])
         dvals = [torch.from_numpy(np.ascontiguousarray(g.values, dtype=np.float32)).to(dev) for g in grids]

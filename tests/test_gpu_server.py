@@ -1,11 +1,17 @@
-"""GPU checks of the C++ solve server: each result is the same alone and in the
-mix, under a memory budget that forces evictions too; warm starts come from
-the resident (or evicted and restored) solution; the server runs the same
-solver as a direct handle (bit-identical to uaamg_poisson_solve_pcg_device).
+"""GPU checks of the C++ solve server: every result agrees with the oracle
+(Poisson: solveMultigridPCG's status, iterations within 1 and 1e-5 relative L2
+against the FP32 oracle and its FP64 twin; viscosity: pcg_solve's iterations
+within 1 and 1e-5 against both); each result is the same alone and in the mix,
+under a memory budget that forces evictions too; warm starts come from the
+resident (or evicted and restored) solution; the server runs the same solver
+as a direct handle (bit-identical to uaamg_poisson_solve_pcg_device); a
+problem registered with FLIP's fallback (flags bit 7) is solved again by pure
+multigrid when its PCG fails, as the oracle's solvePureMultigrid.
 """
 import numpy as np
 import pytest
 
+import oracle as O
 import uaamg_amd as U
 
 pytestmark = pytest.mark.gpu
@@ -173,3 +179,73 @@ def test_server_errors(pool):
         with pytest.raises(U.UaamgError):
             srv.wait(12345)
     torch.cuda.synchronize()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_server_results_match_oracle(pool):
+    import torch
+    from uaamg_amd.server import SolveServer
+    with SolveServer(slots=4) as srv:
+        pids = [_register(srv, p) for p in pool]
+        outs = [[torch.zeros_like(p["rhs"][r]) for r in range(2)] for p in pool]
+        # x0 = none: request 0 starts from zero, request 1 from request 0's resident solution
+        tickets = [[srv.submit(pids[i], p["rhs"][r].data_ptr(), outs[i][r].data_ptr(), 0, p["tol"])
+                    for r in range(2)] for i, p in enumerate(pool)]
+        res = [[srv.wait(t) for t in ts] for ts in tickets]
+    torch.cuda.synchronize()
+    for i, p in enumerate(pool):
+        sc = p["scene"]
+        for r in range(2):
+            b = p["rhs"][r].cpu().numpy()
+            st, it = res[i][r][0], res[i][r][1]
+            x = outs[i][r].cpu().numpy()
+            # request r = 1 is warm-started from request 0's solution (the problem's resident one)
+            x0 = None if r == 0 else outs[i][0].cpu().numpy()
+            if p["kind"] == "poisson":
+                orc = O.OraclePoisson(sc.phi, sc.weights, 1.0, sc.dx)
+                st_o, it_o, xo = orc.solve_pcg(b, x0=x0)
+                _, _, x64 = orc.solve_pcg64(b)
+                assert st == st_o == U.SUCCESS
+            else:
+                orc = O.OracleViscosity(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx)
+                it_o, xo, _ = orc.solve(b, x0=x0, tol=p["tol"])
+                _, x64 = orc.solve64(b, tol=1e-12, maxit=200)
+                assert st == U.SUCCESS
+            _say(f"  {p['kind']} {i} request {r}: iterations {it} / oracle {it_o}, rel L2 vs FP32 oracle "
+                 f"{_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}")
+            assert abs(it - it_o) <= 1, (i, r)
+            assert _rel(x, x64) <= 1e-5, (i, r)
+            if p["prec"] == 0:
+                assert _rel(x, xo) <= 1e-5, (i, r)
+
+
+def test_server_pure_multigrid_fallback(hip_available):
+    """Scene(3, 0, seed=22): the reference's PCG fails at iteration 5 and its pure multigrid at 14;
+    registered with flags bit 7 the server returns the PCG's FAILED status and iterations, the
+    fallback's status and iterations, and the pure-multigrid solution."""
+    import torch
+    from uaamg_amd.server import SolveServer
+    dev = torch.device("cuda:0")
+    sc = U.Scene(3, 0, seed=22)
+    lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=-1)
+    bn = lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)
+    b = torch.from_numpy(bn).to(dev)
+    orc = O.OraclePoisson(sc.phi, sc.weights, 1.0, sc.dx)
+    st_o, it_o, _ = orc.solve_pcg(bn)
+    mst, mit, xm = orc.solve_mg(bn)
+    with SolveServer(slots=2) as srv:
+        pid = srv.add_poisson(sc.phi, sc.weights, 1.0, sc.dx, flags=128)
+        plain = srv.add_poisson(sc.phi, sc.weights, 1.0, sc.dx)
+        out = torch.zeros_like(b)
+        out0 = torch.zeros_like(b)
+        st, it, fst, fit, _, _ = srv.wait_fallback(srv.submit(pid, b.data_ptr(), out.data_ptr(), 0))
+        st0, it0, fst0, fit0, _, _ = srv.wait_fallback(srv.submit(plain, b.data_ptr(), out0.data_ptr(), 0))
+    torch.cuda.synchronize()
+    _say(f"  server fallback: PCG {st}/{it} (oracle {st_o}/{it_o}), MG {fst}/{fit} (oracle {mst}/{mit})")
+    assert st == st_o == U.FAILED and abs(it - it_o) <= 1
+    assert fst == mst and abs(fit - mit) <= 1
+    assert _rel(out.cpu().numpy(), xm) <= 1e-5
+    assert (st0, fst0, fit0) == (U.FAILED, -1, -1) and it0 == it

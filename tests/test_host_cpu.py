@@ -4,7 +4,8 @@
 - the product's host hierarchy build (run through a device=-1 handle, which
   only inspects) equals the oracle's bit for bit: per-level leaf origins, DOF
   masks, DOF numbering, the four stencil coefficient grids with their trimmed
-  leaves, SPAI0, and the pressure right-hand side;
+  leaves, SPAI0, and the pressure right-hand side (with and without a solid
+  velocity);
 - solves on a device=-1 handle fail loudly (no CPU fallback);
 - scene face weights obey their closed forms.
 """
@@ -51,6 +52,14 @@ def test_host_hierarchy_bit_exact_vs_oracle(cfg, n, seed):
             assert np.array_equal(pc, oc), f"level {l} kind {kind}: coefficients differ"
     rp = lap.createPressurePoissonRightHandSide(s.weights, s.vx, s.vy, s.vz)
     ro = orc.build_rhs(s.weights, s.vx, s.vy, s.vz)
+    assert np.array_equal(rp, ro)
+    # with a solid velocity (non-zero background, leaves on the weights' topology)
+    rng = np.random.default_rng(seed + 100)
+    sv = U.Grid(s.weights.origins.copy(),
+                rng.uniform(-1, 1, (s.weights.origins.shape[0], 512, 3)).astype(np.float32), None,
+                (0.5, -0.25, 0.125))
+    rp = lap.createPressurePoissonRightHandSide(s.weights, s.vx, s.vy, s.vz, sv)
+    ro = orc.build_rhs(s.weights, s.vx, s.vy, s.vz, sv)
     assert np.array_equal(rp, ro)
 
 

@@ -139,6 +139,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_batch_ndof": (i64, [vp, i32]),
         "uaamg_poisson_batch_solve_device": (C.c_int, [vp, vp, vp, vp, vp]),
         "uaamg_poisson_batch_last_residual": (C.c_int, [vp, vp, vp]),
+        "uaamg_poisson_batch_last_fallback": (C.c_int, [vp, vp, vp]),
         "uaamg_viscosity_solve_batch_device": (C.c_int, [vp, i32, f32, vp, vp, vp, i32]),
         "uaamg_poisson_apply": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp, vp]),
         "uaamg_poisson_first_sweep": (C.c_int, [vp, C.c_int, C.c_int, f32, vp, vp]),
@@ -365,6 +366,14 @@ class PoissonBatch:
         k = np.zeros(self.n, np.int32)
         _check(lib().uaamg_poisson_batch_last_residual(self._h, _ptr(r), _ptr(k)), "batch last_residual")
         return list(zip(r.tolist(), k.tolist()))
+
+    def last_fallback(self):
+        """Per problem: (status, iterations) of the pure-multigrid fallback of the last solve (flags
+        bit 7, FLIP_vdb.cpp:3150-3155); (-1, -1) where the PCG succeeded."""
+        st = np.zeros(self.n, np.int32)
+        it = np.zeros(self.n, np.int32)
+        _check(lib().uaamg_poisson_batch_last_fallback(self._h, _ptr(st), _ptr(it)), "batch last_fallback")
+        return list(zip(st.tolist(), it.tolist()))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -17,6 +17,7 @@ constexpr int kCoefs = 4;   // diag (with invdiag), x, y, z
 
 PoissonBatch::~PoissonBatch() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    fb_dev_.clear();   // the fallback solvers run on stream_
     destroy_graphs();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
     if (mailbox_) (void)hipHostFree(mailbox_);
@@ -64,6 +65,10 @@ int PoissonBatch::init(std::vector<PoissonHierarchyHost>&& probs, const SolverOp
         return -1;
     }
     mixed_ = opt_.precision == 1;
+    if (opt_.flags & 128) {   // FLIP's fallback needs each problem alone: keep the host hierarchies
+        fb_host_ = probs;
+        fb_dev_.resize(size_t(K));
+    }
     use_graphs_ = !(opt_.flags & 2);
     block_max_leaves_ = (opt_.flags & 16) ? 0 : PoissonDevice::kTuneMaxLeaves;
     if (const char* env = std::getenv("UAAMG_BLOCK_LEAVES")) block_max_leaves_ = std::atoi(env);
@@ -743,7 +748,48 @@ int PoissonBatch::solve(float* const* x, const float* const* b, int* status, int
     }
     last_.resize(size_t(K));
     for (int kk = 0; kk < K; ++kk) last_[size_t(order_[size_t(kk)])] = st_host_[size_t(kk)];
+    mg_status_.assign(size_t(K), -1);
+    mg_iters_.assign(size_t(K), -1);
+    if (!fb_host_.empty())
+        for (int i = 0; i < K; ++i)
+            if (last_[size_t(i)].state != kBSuccess)
+                if (int rc = fallback(i, x[i], b[i])) return rc;
     return first_error_ == hipSuccess ? 0 : -3;
+}
+
+// FLIP_vdb.cpp:3150-3155: when solveMultigridPCG fails, FLIP zeroes the
+// pressure and calls solvePureMultigrid on the same hierarchy. The merged
+// batch has no pure-multigrid cycle of its own, so the failed problem is
+// solved alone by a single-problem solver on the batch's stream (made from the
+// kept host hierarchy on the first failure, then reused).
+int PoissonBatch::fallback(int i, float* x, const float* b) {
+    std::unique_ptr<PoissonDevice>& d = fb_dev_[size_t(i)];
+    if (!d) {
+        SolverOptions o = opt_;
+        o.stream = stream_;
+        o.flags &= ~128;
+        o.precision = 0;   // solvePureMultigrid runs the FP32 W-cycle
+        d.reset(new PoissonDevice());
+        PoissonHierarchyHost h = fb_host_[size_t(i)];
+        if (int rc = d->init(std::move(h), o)) {
+            err_ = "fallback solver of problem " + std::to_string(i) + ": " + d->error();
+            d.reset();
+            return rc;
+        }
+    }
+    d->options().relative_tolerance = opt_.relative_tolerance;
+    d->options().max_iterations = opt_.max_iterations;
+    const int64_t n = ndof(i);
+    if (check(hipMemsetAsync(x, 0, size_t(n) * sizeof(float), stream_), "zero the fallback's pressure")) return -3;
+    int its = 0;
+    const int rc = d->solve_mg_dev(x, b, &its);
+    if (rc < 0) {
+        err_ = "fallback of problem " + std::to_string(i) + ": " + d->error();
+        return rc;
+    }
+    mg_status_[size_t(i)] = rc;
+    mg_iters_[size_t(i)] = its;
+    return 0;
 }
 
 }  // namespace uaamg

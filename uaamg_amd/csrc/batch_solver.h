@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -39,7 +40,9 @@ public:
     ~PoissonBatch();
     // Merges and uploads the problems. opt: tolerance, max_iterations,
     // smoother (Jacobi family only), precision, device, flags (bit 1: no graph
-    // replay, bit 4: no block-visit kernels). 0 or a negative uaamg status.
+    // replay, bit 4: no block-visit kernels, bit 7: a problem whose PCG fails
+    // is solved again from zero by pure multigrid, as FLIP does). 0 or a
+    // negative uaamg status.
     int init(std::vector<PoissonHierarchyHost>&& problems, const SolverOptions& opt);
 
     int nproblems() const { return int(order_.size()); }
@@ -58,6 +61,11 @@ public:
     // input problem i's FP64 true residual and restart count (-1 without)
     float last_true_residual(int i) const { return i < int(last_.size()) ? last_[size_t(i)].true_nu : -1.f; }
     int last_replacements(int i) const { return i < int(last_.size()) ? last_[size_t(i)].replacements : -1; }
+    // after a solve with the pure-multigrid fallback (flags bit 7): input
+    // problem i's solvePureMultigrid status and iterations, -1 where its PCG
+    // did not fail (or the fallback is off)
+    int last_mg_status(int i) const { return i < int(mg_status_.size()) ? mg_status_[size_t(i)] : -1; }
+    int last_mg_iterations(int i) const { return i < int(mg_iters_.size()) ? mg_iters_[size_t(i)] : -1; }
 
 private:
     struct Level {
@@ -87,6 +95,13 @@ private:
     void destroy_graphs();
     void tune_levels();
     void visit_pair(int level, bool block);
+    int fallback(int i, float* x, const float* b);
+
+    // pure-multigrid fallback (flags bit 7): each problem's host hierarchy,
+    // kept at init, and a single-problem solver made on its first failure
+    std::vector<PoissonHierarchyHost> fb_host_;
+    std::vector<std::unique_ptr<PoissonDevice>> fb_dev_;
+    std::vector<int> mg_status_, mg_iters_;
 
     SolverOptions opt_;
     hipStream_t stream_ = nullptr;

@@ -670,6 +670,15 @@ int uaamg_poisson_batch_last_residual(const uaamg_poisson_batch* h, float* true_
     return 0;
 }
 
+int uaamg_poisson_batch_last_fallback(const uaamg_poisson_batch* h, int32_t* mg_status, int32_t* mg_iterations) {
+    if (!h) return fail(UAAMG_EINVAL, "null handle");
+    for (int i = 0; i < h->b.nproblems(); ++i) {
+        if (mg_status) mg_status[i] = h->b.last_mg_status(i);
+        if (mg_iterations) mg_iterations[i] = h->b.last_mg_iterations(i);
+    }
+    return 0;
+}
+
 int uaamg_poisson_batch_solve_device(uaamg_poisson_batch* h, float* const* x_dev, const float* const* b_dev,
                                      int32_t* status, int32_t* iterations) {
     UAAMG_GUARD_BEGIN

@@ -111,6 +111,20 @@ int uaamg_server_wait(uaamg_server* h, int64_t ticket, int32_t* status, int32_t*
     UAAMG_GUARD_END
 }
 
+int uaamg_server_wait_fallback(uaamg_server* h, int64_t ticket, int32_t* status, int32_t* iterations,
+                               int32_t* mg_status, int32_t* mg_iterations, double* queue_ms, double* total_ms) {
+    UAAMG_GUARD_BEGIN
+    if (!h) return fail(UAAMG_EINVAL, "null handle");
+    int st = 0, it = 0, ms = -1, mi = -1;
+    if (int rc = h->s->wait(ticket, &st, &it, queue_ms, total_ms, &ms, &mi)) return server_status(rc, *h->s);
+    if (status) *status = st;
+    if (iterations) *iterations = it;
+    if (mg_status) *mg_status = ms;
+    if (mg_iterations) *mg_iterations = mi;
+    return 0;
+    UAAMG_GUARD_END
+}
+
 int uaamg_server_stats(uaamg_server* h, uaamg_server_counters* out) {
     if (!h || !out) return fail(UAAMG_EINVAL, "null argument");
     const ServerStats s = h->s->stats();

@@ -121,7 +121,8 @@ int Server::submit(int64_t pid, const float* b, float* x_out, const float* x0, f
     return 0;
 }
 
-int Server::wait(int64_t ticket, int* status, int* iterations, double* queue_ms, double* total_ms) {
+int Server::wait(int64_t ticket, int* status, int* iterations, double* queue_ms, double* total_ms, int* mg_status,
+                 int* mg_iterations) {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = results_.find(ticket);
     if (it == results_.end()) {
@@ -135,6 +136,8 @@ int Server::wait(int64_t ticket, int* status, int* iterations, double* queue_ms,
     if (iterations) *iterations = r.iterations;
     if (queue_ms) *queue_ms = r.queue_ms;
     if (total_ms) *total_ms = r.total_ms;
+    if (mg_status) *mg_status = r.mg_status;
+    if (mg_iterations) *mg_iterations = r.mg_iterations;
     return 0;
 }
 
@@ -195,9 +198,12 @@ int Server::make_resident(Problem& p) {
     return e == hipSuccess ? 0 : -3;
 }
 
-int Server::run(Problem& p, const Request& r, int* iterations) {
+int Server::run(Problem& p, const Request& r, int* iterations, int* mg_status, int* mg_iterations) {
     const size_t nb = size_t(p.ndof) * sizeof(float);
     hipStream_t s = p.visc ? p.vd->stream() : p.pd->stream();
+    // the caller's x0 and b may still be in production on its null stream:
+    // the copy of x0 below is the first read of either
+    if ((p.visc ? p.vd->follow_caller() : p.pd->follow_caller()) != 0) return -3;
     if (r.x0 && nb && hipMemcpyAsync(p.x, r.x0, nb, hipMemcpyDeviceToDevice, s) != hipSuccess) return -3;
     int rc;
     if (p.visc) {
@@ -206,6 +212,14 @@ int Server::run(Problem& p, const Request& r, int* iterations) {
     } else {
         p.pd->options().relative_tolerance = r.tol;
         rc = p.pd->solve_pcg_dev(p.x, r.b, iterations);
+        if (rc == PoissonDevice::kFailed && (p.pd->options().flags & 128)) {
+            // FLIP_vdb.cpp:3150-3155: zero the pressure and solve by pure
+            // multigrid; the request keeps the PCG's FAILED status
+            if (hipMemsetAsync(p.x, 0, nb, s) != hipSuccess) return -3;
+            const int mg = p.pd->solve_mg_dev(p.x, r.b, mg_iterations);
+            if (mg < 0) return mg;
+            *mg_status = mg;
+        }
     }
     if (rc < 0) return rc;
     p.has_solution = true;
@@ -258,8 +272,8 @@ void Server::worker() {
         int rc = acquire ? make_resident(*p) : 0;
         const bool acquire_failed = acquire && rc != 0;
         if (acquire_failed) evict(*p);   // back to host-only
-        int its = 0;
-        if (rc == 0) rc = run(*p, r, &its);
+        int its = 0, mg_st = -1, mg_its = -1;
+        if (rc == 0) rc = run(*p, r, &its, &mg_st, &mg_its);
         const auto t_end = clock::now();
 
         lk.lock();
@@ -278,6 +292,8 @@ void Server::worker() {
         res.done = true;
         res.status = rc;
         res.iterations = its;
+        res.mg_status = mg_st;
+        res.mg_iterations = mg_st < 0 ? -1 : mg_its;
         res.queue_ms = ms_between(r.submitted, t_start);
         res.total_ms = ms_between(r.submitted, t_end);
         ++st_.completed;

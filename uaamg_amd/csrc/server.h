@@ -67,7 +67,8 @@ public:
     int submit(int64_t pid, const float* b, float* x_out, const float* x0, float tolerance, int64_t* ticket);
     // Blocks until the ticket completes; status is the solver's (0 SUCCESS,
     // 1 FAILED, < 0 error); times in milliseconds from submission.
-    int wait(int64_t ticket, int* status, int* iterations, double* queue_ms, double* total_ms);
+    int wait(int64_t ticket, int* status, int* iterations, double* queue_ms, double* total_ms,
+             int* mg_status = nullptr, int* mg_iterations = nullptr);
     ServerStats stats();
     const std::string& error() const { return err_; }
 
@@ -97,11 +98,12 @@ private:
     struct Result {
         bool done = false;
         int status = 0, iterations = 0;
+        int mg_status = -1, mg_iterations = -1;   // the pure-multigrid fallback's (-1: not run)
         double queue_ms = 0, total_ms = 0;
     };
 
     void worker();
-    int run(Problem& p, const Request& r, int* iterations);
+    int run(Problem& p, const Request& r, int* iterations, int* mg_status, int* mg_iterations);
     int make_resident(Problem& p);
     void evict(Problem& p);
     // under the lock: picks victims so that `need` more bytes fit; false if impossible now

@@ -520,11 +520,16 @@ int ViscDevice::wait_nu(float* nu) {
     return 0;
 }
 
+int ViscDevice::follow_caller() {
+    if (!own_stream_) return 0;
+    return check(caller_.follow(stream_), "order after the caller's stream");
+}
+
 int ViscDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations) {
     if (!on_device()) return -4;
     if (first_error_ != hipSuccess) return -3;
     if (use_graphs_ && ensure_graphs()) return -3;
-    if (own_stream_ && check(caller_.follow(stream_), "order after the caller's stream")) return -3;
+    if (follow_caller()) return -3;
     const DevLevel& d = lv_[0];
     if (iterations) *iterations = 0;
     check(launch_scatter_dof(d.flat, x_dof, x0_, stream_), "scatter x");

@@ -61,6 +61,8 @@ public:
     int precondition_dev(const float* r_dof, float* z_dof);
 
     float* staging(int i) { return staging_[i]; }
+    // orders the solver's stream after the caller's null-stream work (own stream only)
+    int follow_caller();
 
     // level-to-kernel mapping chosen at acquire time (1 = single-workgroup
     // half-visits) and the timings behind it (-1 where not timed)

@@ -44,6 +44,8 @@ def _bind():
         "uaamg_server_problem_info": (C.c_int, [vp, i64, P(i64), P(i64), P(i32)]),
         "uaamg_server_submit": (C.c_int, [vp, i64, vp, vp, vp, f32, P(i64)]),
         "uaamg_server_wait": (C.c_int, [vp, i64, P(i32), P(i32), P(C.c_double), P(C.c_double)]),
+        "uaamg_server_wait_fallback": (C.c_int, [vp, i64, P(i32), P(i32), P(i32), P(i32), P(C.c_double),
+                                                 P(C.c_double)]),
         "uaamg_server_stats": (C.c_int, [vp, P(_ServerCounters)]),
     }
     for name, (res, args) in sig.items():
@@ -111,6 +113,15 @@ class SolveServer:
         _check(_bind().uaamg_server_wait(self._h, ticket, C.byref(st), C.byref(it), C.byref(q), C.byref(tot)),
                "wait")
         return int(st.value), int(it.value), float(q.value), float(tot.value)
+
+    def wait_fallback(self, ticket: int) -> tuple[int, int, int, int, float, float]:
+        """(status, iterations, fallback status, fallback iterations, queue ms, total ms): wait() plus
+        the pure-multigrid fallback of a problem registered with flags bit 7 (-1s when it did not run)."""
+        st, it, ms, mi = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        q, tot = C.c_double(), C.c_double()
+        _check(_bind().uaamg_server_wait_fallback(self._h, ticket, C.byref(st), C.byref(it), C.byref(ms),
+                                                  C.byref(mi), C.byref(q), C.byref(tot)), "wait_fallback")
+        return int(st.value), int(it.value), int(ms.value), int(mi.value), float(q.value), float(tot.value)
 
     def stats(self) -> dict:
         c = _ServerCounters()
