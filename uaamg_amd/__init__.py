@@ -77,6 +77,7 @@ class _ViscOptions(C.Structure):
         ("device", C.c_int32),
         ("max_coarsest_dof", C.c_int32),
         ("flags", C.c_int32),
+        ("precision", C.c_int32),
         ("stream", C.c_void_p),
     ]
 
@@ -119,6 +120,8 @@ def lib() -> C.CDLL:
         "uaamg_poisson_nlevels": (C.c_int, [vp]),
         "uaamg_poisson_level_info": (C.c_int, [vp, C.c_int, P(i64), P(i64), P(i64)]),
         "uaamg_poisson_level_topology": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+        "uaamg_poisson_grid_to_vector": (C.c_int, [vp, P(_Grid), vp]),
+        "uaamg_poisson_vector_to_grid": (C.c_int, [vp, vp, vp]),
         "uaamg_poisson_level_coefficients": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "uaamg_poisson_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
         "uaamg_poisson_build_rhs_device": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), i32, vp]),
@@ -166,6 +169,11 @@ def lib() -> C.CDLL:
         "uaamg_viscosity_level_coefficients": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "uaamg_viscosity_level_matrix": (C.c_int, [vp, C.c_int, P(i64), vp, vp, vp]),
         "uaamg_viscosity_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), vp]),
+        "uaamg_viscosity_build_rhs_packed": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
+        "uaamg_viscosity_grid_to_vector": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), vp]),
+        "uaamg_viscosity_vector_to_grid": (C.c_int, [vp, vp, vp]),
+        "uaamg_packed3_from_vec3": (C.c_int, [P(_Grid), vp]),
+        "uaamg_packed3_to_vec3": (C.c_int, [P(_Grid), P(_Grid), P(_Grid), P(i64), vp, vp, vp]),
         "uaamg_viscosity_solve_pcg": (C.c_int, [vp, f32, vp, vp, P(i32)]),
         "uaamg_viscosity_solve_pcg_device": (C.c_int, [vp, f32, vp, vp, P(i32)]),
         "uaamg_viscosity_apply": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),

@@ -33,6 +33,7 @@ SOURCES = [
     "batch_solver.cpp",
     "server.cpp",
     "capi_server.cpp",
+    "capi_packed3.cpp",
     "rhs_kernels.hip",
     "rhs_device.cpp",
     "face_update_kernels.hip",

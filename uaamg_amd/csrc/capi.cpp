@@ -258,6 +258,37 @@ int uaamg_poisson_level_topology(const uaamg_poisson* h, int level, int32_t* ori
     return 0;
 }
 
+int uaamg_poisson_grid_to_vector(const uaamg_poisson* h, const uaamg_grid* grid, float* vec) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !grid || !vec) return fail(UAAMG_EINVAL, "null argument");
+    HostGrid g;
+    if (!to_host_grid(grid, 1, g, "grid")) return UAAMG_EINVAL;
+    const auto& l = h->dev.host().levels[0];
+    for (size_t i = 0; i < l.nleaf(); ++i) {
+        const int32_t gi = g.lookup.find(l.origin[i]);
+        const uint64_t* m = &l.mask[i * 8];
+        int64_t d = l.dofbase[i];
+        for (int off = 0; off < kLeafSize; ++off)
+            if (mask_test(m, off)) vec[d++] = gi < 0 ? 0.f : g.leaf_values(size_t(gi))[off];
+    }
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_poisson_vector_to_grid(const uaamg_poisson* h, const float* vec, float* bricks) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !vec || !bricks) return fail(UAAMG_EINVAL, "null argument");
+    const auto& l = h->dev.host().levels[0];
+    for (size_t i = 0; i < l.nleaf(); ++i) {
+        const uint64_t* m = &l.mask[i * 8];
+        float* out = bricks + i * kLeafSize;
+        int64_t d = l.dofbase[i];
+        for (int off = 0; off < kLeafSize; ++off) out[off] = mask_test(m, off) ? vec[d++] : 0.f;
+    }
+    return 0;
+    UAAMG_GUARD_END
+}
+
 int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind, float* bricks, uint8_t* kept) {
     if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
     if (kind < 0 || kind >= kNumCoef) return fail(UAAMG_EINVAL, "bad coefficient kind");
@@ -292,15 +323,13 @@ int uaamg_poisson_build_rhs(const uaamg_poisson* h, const uaamg_grid* face_weigh
     UAAMG_GUARD_END
 }
 
-int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
-                                   const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
-                                   int32_t values_on_device, float* rhs_dev) {
-    UAAMG_GUARD_BEGIN
-    if (!h || !rhs_dev) return fail(UAAMG_EINVAL, "null argument");
-    const uaamg_grid* in[kRhsGrids] = {face_weights, vx, vy, vz, solid_velocity};
+namespace {
+
+// the five input grids of a right-hand-side build as RhsGridRef (kRhs* order);
+// returns 0 or UAAMG_EINVAL with the message set
+int rhs_refs(const uaamg_grid* const in[kRhsGrids], bool values_on_device, RhsGridRef refs[kRhsGrids]) {
     const int ncomp[kRhsGrids] = {3, 1, 1, 1, 3};
     const char* names[kRhsGrids] = {"face_weights", "vx", "vy", "vz", "solid_velocity"};
-    RhsGridRef refs[kRhsGrids];
     for (int g = 0; g < kRhsGrids; ++g) {
         RhsGridRef& r = refs[g];
         r.ncomp = ncomp[g];
@@ -314,9 +343,22 @@ int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weig
         r.nleaf = in[g]->nleaf;
         r.origins = in[g]->origins;
         r.values = in[g]->values;
-        r.values_on_device = values_on_device != 0;
+        r.values_on_device = values_on_device;
         for (int c = 0; c < 3; ++c) r.bg[c] = in[g]->background[c];
     }
+    return 0;
+}
+
+}  // namespace
+
+int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                                   const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
+                                   int32_t values_on_device, float* rhs_dev) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !rhs_dev) return fail(UAAMG_EINVAL, "null argument");
+    const uaamg_grid* in[kRhsGrids] = {face_weights, vx, vy, vz, solid_velocity};
+    RhsGridRef refs[kRhsGrids];
+    if (int rc = rhs_refs(in, values_on_device != 0, refs)) return rc;
     std::string err;
     const int rc = build_poisson_rhs_device(h->dev, refs, rhs_dev, err);
     if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);

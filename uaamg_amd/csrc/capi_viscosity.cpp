@@ -62,6 +62,7 @@ int uaamg_viscosity_options_default(uaamg_viscosity_options* o) {
     o->device = 0;
     o->max_coarsest_dof = 4000;
     o->flags = 0;
+    o->precision = UAAMG_PRECISION_FP32;
     o->stream = nullptr;
     return 0;
 }
@@ -79,6 +80,8 @@ int prepare_viscosity(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf,
     if (opts) o = *opts;
     if (o.max_coarsest_dof < 1 || o.max_coarsest_dof > 4096)
         return fail(UAAMG_EINVAL, "max_coarsest_dof must lie in [1, 4096]");
+    if (o.precision != UAAMG_PRECISION_FP32 && o.precision != UAAMG_PRECISION_BF16_CYCLE)
+        return fail(UAAMG_EINVAL, "unknown precision");
     if (o.device >= 0) {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o.device)
@@ -95,6 +98,7 @@ int prepare_viscosity(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf,
     vo.device = o.device;
     vo.max_coarsest_dof = o.max_coarsest_dof;
     vo.flags = o.flags;
+    vo.precision = o.precision;
     vo.stream = static_cast<hipStream_t>(o.stream);
     return 0;
 }
@@ -215,6 +219,59 @@ int uaamg_viscosity_build_rhs(const uaamg_viscosity* h, const uaamg_grid* veloci
     if (!to_host_grid(velocity, 3, u, "velocity")) return UAAMG_EINVAL;
     if (solid_velocity && !to_host_grid(solid_velocity, 3, us, "solid_velocity")) return UAAMG_EINVAL;
     build_visc_rhs(h->dev.host(), u, solid_velocity ? &us : nullptr, rhs);
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_viscosity_build_rhs_packed(const uaamg_viscosity* h, const uaamg_grid* v0, const uaamg_grid* v1,
+                                     const uaamg_grid* v2, const uaamg_grid* solid_velocity, float* rhs) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !v0 || !v1 || !v2 || !rhs) return fail(UAAMG_EINVAL, "null argument");
+    HostGrid u[3], us;
+    const uaamg_grid* in[3] = {v0, v1, v2};
+    const char* names[3] = {"velocity channel 0", "velocity channel 1", "velocity channel 2"};
+    for (int c = 0; c < 3; ++c)
+        if (!to_host_grid(in[c], 1, u[c], names[c])) return UAAMG_EINVAL;
+    if (solid_velocity && !to_host_grid(solid_velocity, 3, us, "solid_velocity")) return UAAMG_EINVAL;
+    const HostGrid* ch[3] = {&u[0], &u[1], &u[2]};
+    build_visc_rhs(h->dev.host(), ch, solid_velocity ? &us : nullptr, rhs);
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_viscosity_grid_to_vector(const uaamg_viscosity* h, const uaamg_grid* v0, const uaamg_grid* v1,
+                                   const uaamg_grid* v2, float* vec) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !v0 || !v1 || !v2 || !vec) return fail(UAAMG_EINVAL, "null argument");
+    HostGrid g[3];
+    const uaamg_grid* in[3] = {v0, v1, v2};
+    for (int c = 0; c < 3; ++c)
+        if (!to_host_grid(in[c], 1, g[c], "velocity channel")) return UAAMG_EINVAL;
+    const ViscLevelHost& L = h->dev.host().levels[0];
+    for (int c = 0; c < 3; ++c)
+        for (size_t i = 0; i < L.nleaf_dof; ++i) {
+            int64_t d = L.dofbase[i * 3 + size_t(c)];
+            if (d < 0) continue;
+            const int32_t gi = g[c].lookup.find(L.origin[i]);
+            const uint64_t* m = &L.mask[(i * 3 + size_t(c)) * 8];
+            for (int off = 0; off < kLeafSize; ++off)
+                if (mask_test(m, off)) vec[d++] = gi < 0 ? 0.f : g[c].leaf_values(size_t(gi))[off];
+        }
+    return 0;
+    UAAMG_GUARD_END
+}
+
+int uaamg_viscosity_vector_to_grid(const uaamg_viscosity* h, const float* vec, float* bricks) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !vec || !bricks) return fail(UAAMG_EINVAL, "null argument");
+    const ViscLevelHost& L = h->dev.host().levels[0];
+    for (size_t i = 0; i < L.nleaf_dof; ++i)
+        for (int c = 0; c < 3; ++c) {
+            int64_t d = L.dofbase[i * 3 + size_t(c)];
+            const uint64_t* m = &L.mask[(i * 3 + size_t(c)) * 8];
+            float* out = bricks + (i * 3 + size_t(c)) * kLeafSize;
+            for (int off = 0; off < kLeafSize; ++off) out[off] = (d >= 0 && mask_test(m, off)) ? vec[d++] : 0.f;
+        }
     return 0;
     UAAMG_GUARD_END
 }

@@ -980,7 +980,11 @@ void visc_explicit_matrix(const ViscLevelHost& L, std::vector<int64_t>& ptr, std
 }
 
 // ------------------------------------------------------------- rhs
-void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid& u, const HostGrid* solid_u, float* out) {
+namespace {
+
+// uget(c, v): the liquid velocity of channel c at voxel v as an accessor reads it
+template <class UGet>
+void build_visc_rhs_impl(const ViscHierarchyHost& h, const UGet& uget, const HostGrid* solid_u, float* out) {
     const ViscGeometry& G = h.geo;
     const ViscLevelHost& L0 = h.levels[0];
     const float eps = 1e-1f;
@@ -1020,7 +1024,7 @@ void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid& u, const HostGri
                     const Coord v{o.x + (off >> 6), o.y + ((off >> 3) & 7), o.z + (off & 7)};
                     float fv = fa.get(v);
                     if (fv < eps) fv = eps;
-                    out[idx[off]] = uval(u, v, c) * fv;
+                    out[idx[off]] = uget(c, v) * fv;
                 }
             }
         });
@@ -1120,6 +1124,24 @@ void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid& u, const HostGri
         else acc.emplace(kc.first, kc.second);
     }
     for (const auto& kv : acc) out[kv.first] = out[kv.first] + kv.second;
+}
+
+float grid_value(const HostGrid& g, const Coord& p, int comp) {
+    const int32_t i = g.lookup.find(leaf_origin_of(p));
+    if (i < 0) return g.background[comp];
+    return g.values[(size_t(i) * kLeafSize + size_t(voxel_offset(p.x & 7, p.y & 7, p.z & 7))) * size_t(g.ncomp) +
+                    size_t(comp)];
+}
+
+}  // namespace
+
+void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid& u, const HostGrid* solid_u, float* out) {
+    build_visc_rhs_impl(h, [&](int c, const Coord& v) { return grid_value(u, v, c); }, solid_u, out);
+}
+
+void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid* const channels[3], const HostGrid* solid_u,
+                    float* out) {
+    build_visc_rhs_impl(h, [&](int c, const Coord& v) { return grid_value(*channels[c], v, 0); }, solid_u, out);
 }
 
 }  // namespace uaamg

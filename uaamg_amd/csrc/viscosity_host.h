@@ -102,6 +102,9 @@ void visc_explicit_matrix(const ViscLevelHost& L, std::vector<int64_t>& ptr, std
 // Right-hand side as a DOF-ordered vector (build_rhs, :1942-2087). u and
 // solid_u are Vec3f grids with face-centred components; solid_u may be null.
 void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid& u, const HostGrid* solid_u, float* out_dof);
+// the same with the liquid velocity as three single-channel grids (packed_FloatGrid3's v[0..2])
+void build_visc_rhs(const ViscHierarchyHost& h, const HostGrid* const channels[3], const HostGrid* solid_u,
+                    float* out_dof);
 
 // volume_fraction of a cube from its 8 corner values (volume_fractions.cpp:100-126)
 float visc_volume_fraction(float p000, float p100, float p010, float p110, float p001, float p101, float p011,

@@ -16,6 +16,8 @@
 
 #include <cstdint>
 
+#include "poisson_kernels.h"   // bf16
+
 namespace uaamg {
 
 // Wave record: one per entry of chan_leaves (a (channel c, DOF leaf) pair,
@@ -71,29 +73,50 @@ struct ViscCoarsestView {
 constexpr int kViscMaxCoarsest = 4096;
 constexpr int kViscMaxRow = 16;    // 1 + 6 + 8 entries per row at most
 
+// Level vectors are FP32, or BF16 on the coarse levels of the mixed form
+// (precision 1): loads widen exactly, stores round to nearest even, and every
+// operation computes in FP32 as in the FP32 form.
+
 // y = op(x, b) on channel c of a level (Gauss-Seidel modes in place: y == x)
 hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int channel, const float* x, const float* b, float* y,
+                             hipStream_t s);
+hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int channel, const bf16* x, const bf16* b, bf16* y,
                              hipStream_t s);
 // y = op(x, b) on all three channels in one launch, for the modes with y != x
 // (NORMAL, RESIDUAL, JACOBI, SPAI0)
 hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const float* x, const float* b, float* y,
                                  hipStream_t s);
+hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const bf16* x, const bf16* b, bf16* y,
+                                 hipStream_t s);
 // coarse = restriction of fine (restriction, simd_viscosity3d.cpp:2353-2413), all channels
 hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
                                 float* coarse_v, hipStream_t s);
+hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
+                                bf16* coarse_v, hipStream_t s);
+hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const bf16* fine_v,
+                                bf16* coarse_v, hipStream_t s);
 // fine += coarse on fine DOFs (prolongation<true>, :2659-2729), all channels
 hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView& coarse, const float* coarse_v,
                                float* fine_v, hipStream_t s);
+hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView& coarse, const bf16* coarse_v,
+                               float* fine_v, hipStream_t s);
+hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView& coarse, const bf16* coarse_v,
+                               bf16* fine_v, hipStream_t s);
 // x = coarsest solve of b (vectors in level layout), one workgroup
 hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float* x, hipStream_t s);
+hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const bf16* b, bf16* x, hipStream_t s);
 // zero every brick of a level vector (set_grid_to_zero, :2304-2315)
 hipError_t launch_visc_zero(const ViscLevelView& L, float* v, hipStream_t s);
+hipError_t launch_visc_zero(const ViscLevelView& L, bf16* v, hipStream_t s);
 
 // One workgroup per half-visit of a small level (same passes, one launch):
 // pre = (zero x when precon,) 2 XYZ sweeps, residual into t, restriction into
 // coarse_b; post = x += P x_coarse, 2 ZYX sweeps.
 hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, float* x, const float* b,
                                  float* t, float* coarse_b, bool precon, hipStream_t s);
+hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, bf16* x, const bf16* b,
+                                 bf16* t, bf16* coarse_b, bool precon, hipStream_t s);
 hipError_t launch_visc_small_post(const ViscLevelView& L, float* x, const float* b, const float* xc, hipStream_t s);
+hipError_t launch_visc_small_post(const ViscLevelView& L, bf16* x, const bf16* b, const bf16* xc, hipStream_t s);
 
 }  // namespace uaamg

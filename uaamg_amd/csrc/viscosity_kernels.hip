@@ -7,6 +7,7 @@
 // _mm256_fmadd_ps (build with -ffp-contract=off).
 #include <cfloat>
 
+#include "stencil_device.h"
 #include "viscosity_host.h"
 #include "viscosity_kernels.h"
 #include "wave_reduce.h"
@@ -32,6 +33,14 @@ __device__ __forceinline__ void st8v(float* p, const float v[8]) {
     *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
     *reinterpret_cast<float4*>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
 }
+// BF16-stored level vectors of the mixed form: a z-row is 16 bytes, widened
+// exactly on load, rounded to nearest even on store; arithmetic stays FP32
+__device__ __forceinline__ void ld8v(const bf16* p, float v[8]) { dev::ld8(p, v); }
+__device__ __forceinline__ void st8v(bf16* p, const float v[8]) { dev::st8(p, v); }
+__device__ __forceinline__ float ld1v(const float* p) { return *p; }
+__device__ __forceinline__ float ld1v(const bf16* p) { return float(*p); }
+__device__ __forceinline__ void st1v(float* p, float v) { *p = v; }
+__device__ __forceinline__ void st1v(bf16* p, float v) { *p = bf16(v); }
 
 // A wave record (layout at kVRecLeaf) in scalar registers.
 struct VRec {
@@ -82,8 +91,8 @@ __device__ __forceinline__ int vleaf(const VRec& R, int lx, int ly) {
 
 // channel ch of vector v at (lx + OX, ly + OY, z + OZ), z = 0..7 (a missing
 // leaf, or one without vector bricks, reads 0)
-template <int OX, int OY, int OZ>
-__device__ __forceinline__ void vfetch(const ViscLevelView& L, const float* v, int ch, const VRec& R, int lx, int ly,
+template <int OX, int OY, int OZ, class V>
+__device__ __forceinline__ void vfetch(const ViscLevelView& L, const V* v, int ch, const VRec& R, int lx, int ly,
                                        float out[8]) {
     const int X = lx + OX, Y = ly + OY;
     const int lf = vleaf<OX, OY, 0>(R, lx, ly);
@@ -100,12 +109,12 @@ __device__ __forceinline__ void vfetch(const ViscLevelView& L, const float* v, i
 #pragma unroll
         for (int k = 0; k < 7; ++k) out[k] = row[k + 1];
         const int lz = vleaf<OX, OY, 1>(R, lx, ly);
-        out[7] = (lz >= 0 && lz < L.nleaf_dof) ? v[(size_t(lz) * 3 + ch) * 512 + row_off(X, Y)] : 0.f;
+        out[7] = (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y)) : 0.f;
     } else {
 #pragma unroll
         for (int k = 7; k > 0; --k) out[k] = row[k - 1];
         const int lz = vleaf<OX, OY, -1>(R, lx, ly);
-        out[0] = (lz >= 0 && lz < L.nleaf_dof) ? v[(size_t(lz) * 3 + ch) * 512 + row_off(X, Y) + 7] : 0.f;
+        out[0] = (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y) + 7) : 0.f;
     }
 }
 
@@ -147,9 +156,8 @@ struct Unit {
 // One row (lane) of the applier on channel C for wave record e (a DOF leaf
 // of channel C): Y = op(X, B). Every load is issued unconditionally (all in
 // bounds); rows without a DOF skip only the store.
-template <int MODE, int C>
-__device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane, const float* Xv, const float* Bv,
-                                         float* Yv) {
+template <int MODE, int C, class V>
+__device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane, const V* Xv, const V* Bv, V* Yv) {
     const int lx = lane >> 3, ly = lane & 7;
     const VRec R = load_vrec(L, e);
     const int leaf = R.leaf;
@@ -304,9 +312,8 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane
 
 // All three channels in one launch (modes that write y != x: the channels
 // are independent); the wave's record index selects the channel.
-template <int MODE>
-__global__ __launch_bounds__(kVThreads) void visc_apply_all_kernel(ViscLevelView L, const float* Xv, const float* Bv,
-                                                                   float* Yv) {
+template <int MODE, class V>
+__global__ __launch_bounds__(kVThreads) void visc_apply_all_kernel(ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
     const int e = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (e < L.chan_off[1]) visc_row<MODE, 0>(L, e, lane, Xv, Bv, Yv);
@@ -314,9 +321,8 @@ __global__ __launch_bounds__(kVThreads) void visc_apply_all_kernel(ViscLevelView
     else if (e < L.chan_off[3]) visc_row<MODE, 2>(L, e, lane, Xv, Bv, Yv);
 }
 
-template <int MODE, int C>
-__global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, const float* Xv, const float* Bv,
-                                                               float* Yv) {
+template <int MODE, int C, class V>
+__global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
     const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= L.chan_off[C + 1] - L.chan_off[C]) return;
     visc_row<MODE, C>(L, L.chan_off[C] + w, threadIdx.x & 63, Xv, Bv, Yv);
@@ -326,8 +332,9 @@ __global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, 
 // The row's two children (z < 4 and z >= 4) and their rows are loaded
 // unconditionally (an absent child reads leaf 0 and is dropped), so the
 // loads depend on one round trip; sums run in the reference's order.
+template <class VF, class VC>
 __device__ __forceinline__ void visc_restrict_row(const ViscLevelView& Cl, const ViscLevelView& F, int leaf, int c,
-                                                  int lane, const float* fine, float* coarse) {
+                                                  int lane, const VF* fine, VC* coarse) {
     const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(Cl.mask[(size_t(leaf) * 3 + c) * 8 + lx] >> (ly * 8)) & 0xffu;
     const int fx = 2 * (lx & 3), fy = 2 * (ly & 3);
@@ -343,7 +350,7 @@ __device__ __forceinline__ void visc_restrict_row(const ViscLevelView& Cl, const
     for (int h = 0; h < 2; ++h) {
         const size_t fl = size_t(f[h] >= 0 ? f[h] : 0);
         const uint64_t* fm = F.mask + (fl * 3 + c) * 8;
-        const float* fv = fine + (fl * 3 + c) * 512;
+        const VF* fv = fine + (fl * 3 + c) * 512;
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             mw[h][a] = fm[fx + a];
@@ -374,21 +381,22 @@ __device__ __forceinline__ void visc_restrict_row(const ViscLevelView& Cl, const
 }
 
 // prolongation<true> of one (fine leaf, channel) row
-__device__ __forceinline__ void visc_prolong_row(const ViscLevelView& F, int leaf, int c, int lane, const float* coarse,
-                                                 float* fine) {
+template <class VF, class VC>
+__device__ __forceinline__ void visc_prolong_row(const ViscLevelView& F, int leaf, int c, int lane, const VC* coarse,
+                                                 VF* fine) {
     const int lx = lane >> 3, ly = lane & 7;
     const uint32_t rm = uint32_t(F.mask[(size_t(leaf) * 3 + c) * 8 + lx] >> (ly * 8)) & 0xffu;
     const int p = F.parent[leaf];
     if (p < 0 || !rm) return;
     const int cl = p >> 3, oct = p & 7;
     const int cx = ((oct >> 2) & 1) * 4 + (lx >> 1), cy = ((oct >> 1) & 1) * 4 + (ly >> 1), cz0 = (oct & 1) * 4;
-    const float* cv = coarse + (size_t(cl) * 3 + c) * 512 + cx * 64 + cy * 8 + cz0;
-    float* row = fine + (size_t(leaf) * 3 + c) * 512 + size_t(lx * 8 + ly) * 8;
+    const VC* cv = coarse + (size_t(cl) * 3 + c) * 512 + cx * 64 + cy * 8 + cz0;
+    VF* row = fine + (size_t(leaf) * 3 + c) * 512 + size_t(lx * 8 + ly) * 8;
     float v[8];
     ld8v(row, v);
 #pragma unroll
     for (int z = 0; z < 8; ++z)
-        if ((rm >> z) & 1u) v[z] += cv[z >> 1];
+        if ((rm >> z) & 1u) v[z] += ld1v(cv + (z >> 1));
     st8v(row, v);
 }
 
@@ -401,54 +409,56 @@ __device__ __forceinline__ void visc_prolong_row(const ViscLevelView& F, int lea
 constexpr int kSmallThreads = 512;
 constexpr int kSmallWaves = kSmallThreads / 64;
 
-template <int MODE, int C>
-__device__ __forceinline__ void small_pass(const ViscLevelView& L, const float* X, const float* B, float* Y) {
+template <int MODE, int C, class V>
+__device__ __forceinline__ void small_pass(const ViscLevelView& L, const V* X, const V* B, V* Y) {
     const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = L.chan_off[C + 1] - L.chan_off[C];
     for (int w = w0; w < n; w += kSmallWaves) visc_row<MODE, C>(L, L.chan_off[C] + w, lane, X, B, Y);
     __syncthreads();
 }
 
-__device__ __forceinline__ void small_rbgs(const ViscLevelView& L, float* X, const float* B, bool xyz) {
+template <class V>
+__device__ __forceinline__ void small_rbgs(const ViscLevelView& L, V* X, const V* B, bool xyz) {
     if (xyz) {
-        small_pass<kVRedGS, 0>(L, X, B, X);
-        small_pass<kVBlackGS, 0>(L, X, B, X);
-        small_pass<kVRedGS, 1>(L, X, B, X);
-        small_pass<kVBlackGS, 1>(L, X, B, X);
-        small_pass<kVRedGS, 2>(L, X, B, X);
-        small_pass<kVBlackGS, 2>(L, X, B, X);
+        small_pass<kVRedGS, 0, V>(L, X, B, X);
+        small_pass<kVBlackGS, 0, V>(L, X, B, X);
+        small_pass<kVRedGS, 1, V>(L, X, B, X);
+        small_pass<kVBlackGS, 1, V>(L, X, B, X);
+        small_pass<kVRedGS, 2, V>(L, X, B, X);
+        small_pass<kVBlackGS, 2, V>(L, X, B, X);
     } else {
-        small_pass<kVBlackGS, 2>(L, X, B, X);
-        small_pass<kVRedGS, 2>(L, X, B, X);
-        small_pass<kVBlackGS, 1>(L, X, B, X);
-        small_pass<kVRedGS, 1>(L, X, B, X);
-        small_pass<kVBlackGS, 0>(L, X, B, X);
-        small_pass<kVRedGS, 0>(L, X, B, X);
+        small_pass<kVBlackGS, 2, V>(L, X, B, X);
+        small_pass<kVRedGS, 2, V>(L, X, B, X);
+        small_pass<kVBlackGS, 1, V>(L, X, B, X);
+        small_pass<kVRedGS, 1, V>(L, X, B, X);
+        small_pass<kVBlackGS, 0, V>(L, X, B, X);
+        small_pass<kVRedGS, 0, V>(L, X, B, X);
     }
 }
 
 // (zero X when precon,) 2 XYZ sweeps, residual into T, restriction into Cb
-__global__ __launch_bounds__(kSmallThreads) void visc_small_pre_kernel(ViscLevelView L, ViscLevelView Cl, float* X,
-                                                                       const float* B, float* T, float* Cb,
-                                                                       int precon) {
+template <class V>
+__global__ __launch_bounds__(kSmallThreads) void visc_small_pre_kernel(ViscLevelView L, ViscLevelView Cl, V* X,
+                                                                       const V* B, V* T, V* Cb, int precon) {
     if (precon) {
-        const size_t n4 = size_t(L.nleaf_dof) * 3 * 512 / 4;
-        float4* x4 = reinterpret_cast<float4*>(X);
-        for (size_t i = threadIdx.x; i < n4; i += kSmallThreads) x4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const size_t n16 = size_t(L.nleaf_dof) * 3 * 512 * sizeof(V) / 16;
+        uint4* x16 = reinterpret_cast<uint4*>(X);
+        for (size_t i = threadIdx.x; i < n16; i += kSmallThreads) x16[i] = make_uint4(0u, 0u, 0u, 0u);
         __syncthreads();
     }
     small_rbgs(L, X, B, true);
     small_rbgs(L, X, B, true);
-    small_pass<kVResidual, 0>(L, X, B, T);
-    small_pass<kVResidual, 1>(L, X, B, T);
-    small_pass<kVResidual, 2>(L, X, B, T);
+    small_pass<kVResidual, 0, V>(L, X, B, T);
+    small_pass<kVResidual, 1, V>(L, X, B, T);
+    small_pass<kVResidual, 2, V>(L, X, B, T);
     const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int w = w0; w < Cl.nleaf_dof * 3; w += kSmallWaves) visc_restrict_row(Cl, L, w / 3, w % 3, lane, T, Cb);
 }
 
 // X += P Xc, then 2 ZYX sweeps
-__global__ __launch_bounds__(kSmallThreads) void visc_small_post_kernel(ViscLevelView L, float* X, const float* B,
-                                                                        const float* Xc) {
+template <class V>
+__global__ __launch_bounds__(kSmallThreads) void visc_small_post_kernel(ViscLevelView L, V* X, const V* B,
+                                                                        const V* Xc) {
     const int w0 = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int w = w0; w < L.nleaf_dof * 3; w += kSmallWaves) visc_prolong_row(L, w / 3, w % 3, lane, Xc, X);
     __syncthreads();
@@ -457,23 +467,25 @@ __global__ __launch_bounds__(kSmallThreads) void visc_small_post_kernel(ViscLeve
 }
 
 // restriction: coarse (leaf, channel) waves
-__global__ __launch_bounds__(kVThreads) void visc_restrict_kernel(ViscLevelView Cl, ViscLevelView F, const float* fine,
-                                                                  float* coarse) {
+template <class VF, class VC>
+__global__ __launch_bounds__(kVThreads) void visc_restrict_kernel(ViscLevelView Cl, ViscLevelView F, const VF* fine,
+                                                                  VC* coarse) {
     const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= Cl.nleaf_dof * 3) return;
     visc_restrict_row(Cl, F, w / 3, w - 3 * (w / 3), threadIdx.x & 63, fine, coarse);
 }
 
 // prolongation<true>: fine (leaf, channel) waves add the parent coarse value
-__global__ __launch_bounds__(kVThreads) void visc_prolong_kernel(ViscLevelView F, const float* coarse, float* fine) {
+template <class VF, class VC>
+__global__ __launch_bounds__(kVThreads) void visc_prolong_kernel(ViscLevelView F, const VC* coarse, VF* fine) {
     const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
     if (w >= F.nleaf_dof * 3) return;
     visc_prolong_row(F, w / 3, w - 3 * (w / 3), threadIdx.x & 63, coarse, fine);
 }
 
-__global__ __launch_bounds__(kVThreads) void visc_zero_kernel(float4* v, size_t n4) {
+__global__ __launch_bounds__(kVThreads) void visc_zero_kernel(uint4* v, size_t n16) {
     const size_t i = size_t(blockIdx.x) * kVThreads + threadIdx.x;
-    if (i < n4) v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n16) v[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 // ---------------------------------------------------------- coarsest solve
@@ -510,7 +522,8 @@ __device__ __forceinline__ float2 bsum2(float a, float b, float (*red)[kCgT / 64
 // ||r||^2 and <r, z>, whose barrier also publishes the new z; the new
 // direction's product is formed as A p = A z + beta A p_old (the same vector
 // as Eigen's A p, rounded differently), so no third barrier publishes p.
-__global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C, const float* B, float* X) {
+template <class V>
+__global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C, const V* B, V* X) {
     __shared__ float sz[kViscMaxCoarsest];
     __shared__ float sq[kViscMaxCoarsest];   // A z of the thread's own rows
     __shared__ float red[2][kCgT / 64][2];
@@ -523,7 +536,7 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
         const int d = tid + j * kCgT;
         x[j] = r[j] = z[j] = p[j] = q[j] = inv[j] = 0.f;
         if (d < n) {
-            r[j] = B[C.pos[d]];
+            r[j] = ld1v(B + C.pos[d]);
             inv[j] = C.invdiag[d];
             rr += r[j] * r[j];
             z[j] = inv[j] * r[j];
@@ -593,7 +606,7 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
 #pragma unroll
     for (int j = 0; j < kCgPer; ++j) {
         const int d = tid + j * kCgT;
-        if (d < n) X[C.pos[d]] = x[j];
+        if (d < n) st1v(X + C.pos[d], x[j]);
     }
 }
 
@@ -602,7 +615,8 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
 // products and sums, in the same order, as visc_coarsest_kernel.
 constexpr int kCgSmallN = 1024;
 
-__global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestView C, const float* B, float* X) {
+template <class V>
+__global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestView C, const V* B, V* X) {
     constexpr int kPer = kCgSmallN / kCgT;
     __shared__ float sz[kCgSmallN];
     __shared__ float red[2][kCgT / 64][2];
@@ -630,7 +644,7 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestV
                 ev[j][k] = C.eval[at];
                 ec[j][k] = C.ecol[at];
             }
-            r[j] = B[C.pos[d]];
+            r[j] = ld1v(B + C.pos[d]);
             inv[j] = C.invdiag[d];
             rr += r[j] * r[j];
             z[j] = inv[j] * r[j];
@@ -687,27 +701,25 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestV
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int d = tid + j * kCgT;
-        if (d < n) X[C.pos[d]] = x[j];
+        if (d < n) st1v(X + C.pos[d], x[j]);
     }
 }
 
-template <int MODE>
-hipError_t apply_mode(const ViscLevelView& L, int c, const float* x, const float* b, float* y, hipStream_t s) {
+template <int MODE, class V>
+hipError_t apply_mode(const ViscLevelView& L, int c, const V* x, const V* b, V* y, hipStream_t s) {
     const int n = L.chan_off[c + 1] - L.chan_off[c];
     if (n <= 0) return hipSuccess;
     const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
     switch (c) {
-        case 0: visc_apply_kernel<MODE, 0><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
-        case 1: visc_apply_kernel<MODE, 1><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
-        default: visc_apply_kernel<MODE, 2><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case 0: visc_apply_kernel<MODE, 0, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case 1: visc_apply_kernel<MODE, 1, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        default: visc_apply_kernel<MODE, 2, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
     }
     return hipGetLastError();
 }
 
-}  // namespace
-
-hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int c, const float* x, const float* b, float* y,
-                             hipStream_t s) {
+template <class V>
+hipError_t apply_one(const ViscLevelView& L, int mode, int c, const V* x, const V* b, V* y, hipStream_t s) {
     if (c < 0 || c > 2) return hipErrorInvalidValue;
     switch (mode) {
         case kVNormal: return apply_mode<kVNormal>(L, c, x, b, y, s);
@@ -720,62 +732,131 @@ hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int c, const floa
     }
 }
 
-hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const float* x, const float* b, float* y,
-                                 hipStream_t s) {
+template <class V>
+hipError_t apply_all(const ViscLevelView& L, int mode, const V* x, const V* b, V* y, hipStream_t s) {
     const int n = L.chan_off[3];
     if (n <= 0) return hipSuccess;
     const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
     switch (mode) {
-        case kVNormal: visc_apply_all_kernel<kVNormal><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
-        case kVResidual: visc_apply_all_kernel<kVResidual><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
-        case kVJacobi: visc_apply_all_kernel<kVJacobi><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
-        case kVSpai0Mode: visc_apply_all_kernel<kVSpai0Mode><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVNormal: visc_apply_all_kernel<kVNormal, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVResidual: visc_apply_all_kernel<kVResidual, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVJacobi: visc_apply_all_kernel<kVJacobi, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        case kVSpai0Mode: visc_apply_all_kernel<kVSpai0Mode, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
         default: return hipErrorInvalidValue;   // Gauss-Seidel passes update in place, channel by channel
     }
     return hipGetLastError();
 }
 
-hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
-                                float* coarse_v, hipStream_t s) {
+template <class VF, class VC>
+hipError_t restrict_to(const ViscLevelView& coarse, const ViscLevelView& fine, const VF* fine_v, VC* coarse_v,
+                       hipStream_t s) {
     const int n = coarse.nleaf_dof * 3;
     if (n <= 0) return hipSuccess;
-    visc_restrict_kernel<<<unsigned((n + kVWaves - 1) / kVWaves), kVThreads, 0, s>>>(coarse, fine, fine_v, coarse_v);
+    visc_restrict_kernel<VF, VC><<<unsigned((n + kVWaves - 1) / kVWaves), kVThreads, 0, s>>>(coarse, fine, fine_v,
+                                                                                             coarse_v);
     return hipGetLastError();
+}
+
+template <class VF, class VC>
+hipError_t prolong_from(const ViscLevelView& fine, const VC* coarse_v, VF* fine_v, hipStream_t s) {
+    const int n = fine.nleaf_dof * 3;
+    if (n <= 0) return hipSuccess;
+    visc_prolong_kernel<VF, VC><<<unsigned((n + kVWaves - 1) / kVWaves), kVThreads, 0, s>>>(fine, coarse_v, fine_v);
+    return hipGetLastError();
+}
+
+template <class V>
+hipError_t coarsest(const ViscCoarsestView& C, const V* b, V* x, hipStream_t s) {
+    if (C.n <= 0) return hipSuccess;
+    if (C.n > kViscMaxCoarsest) return hipErrorInvalidValue;
+    if (C.n <= kCgSmallN) visc_coarsest_small_kernel<V><<<1, kCgT, 0, s>>>(C, b, x);
+    else visc_coarsest_kernel<V><<<1, kCgT, 0, s>>>(C, b, x);
+    return hipGetLastError();
+}
+
+template <class V>
+hipError_t zero(const ViscLevelView& L, V* v, hipStream_t s) {
+    const size_t n16 = size_t(L.nleaf_dof) * 3 * 512 * sizeof(V) / 16;
+    if (!n16) return hipSuccess;
+    visc_zero_kernel<<<unsigned((n16 + kVThreads - 1) / kVThreads), kVThreads, 0, s>>>(reinterpret_cast<uint4*>(v),
+                                                                                      n16);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int c, const float* x, const float* b, float* y,
+                             hipStream_t s) {
+    return apply_one(L, mode, c, x, b, y, s);
+}
+hipError_t launch_visc_apply(const ViscLevelView& L, int mode, int c, const bf16* x, const bf16* b, bf16* y,
+                             hipStream_t s) {
+    return apply_one(L, mode, c, x, b, y, s);
+}
+
+hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const float* x, const float* b, float* y,
+                                 hipStream_t s) {
+    return apply_all(L, mode, x, b, y, s);
+}
+hipError_t launch_visc_apply_all(const ViscLevelView& L, int mode, const bf16* x, const bf16* b, bf16* y,
+                                 hipStream_t s) {
+    return apply_all(L, mode, x, b, y, s);
+}
+
+hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
+                                float* coarse_v, hipStream_t s) {
+    return restrict_to(coarse, fine, fine_v, coarse_v, s);
+}
+hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const float* fine_v,
+                                bf16* coarse_v, hipStream_t s) {
+    return restrict_to(coarse, fine, fine_v, coarse_v, s);
+}
+hipError_t launch_visc_restrict(const ViscLevelView& coarse, const ViscLevelView& fine, const bf16* fine_v,
+                                bf16* coarse_v, hipStream_t s) {
+    return restrict_to(coarse, fine, fine_v, coarse_v, s);
 }
 
 hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView&, const float* coarse_v, float* fine_v,
                                hipStream_t s) {
-    const int n = fine.nleaf_dof * 3;
-    if (n <= 0) return hipSuccess;
-    visc_prolong_kernel<<<unsigned((n + kVWaves - 1) / kVWaves), kVThreads, 0, s>>>(fine, coarse_v, fine_v);
-    return hipGetLastError();
+    return prolong_from(fine, coarse_v, fine_v, s);
+}
+hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView&, const bf16* coarse_v, float* fine_v,
+                               hipStream_t s) {
+    return prolong_from(fine, coarse_v, fine_v, s);
+}
+hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView&, const bf16* coarse_v, bf16* fine_v,
+                               hipStream_t s) {
+    return prolong_from(fine, coarse_v, fine_v, s);
 }
 
 hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float* x, hipStream_t s) {
-    if (C.n <= 0) return hipSuccess;
-    if (C.n > kViscMaxCoarsest) return hipErrorInvalidValue;
-    if (C.n <= kCgSmallN) visc_coarsest_small_kernel<<<1, kCgT, 0, s>>>(C, b, x);
-    else visc_coarsest_kernel<<<1, kCgT, 0, s>>>(C, b, x);
-    return hipGetLastError();
+    return coarsest(C, b, x, s);
+}
+hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const bf16* b, bf16* x, hipStream_t s) {
+    return coarsest(C, b, x, s);
 }
 
 hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, float* x, const float* b,
                                  float* t, float* coarse_b, bool precon, hipStream_t s) {
-    visc_small_pre_kernel<<<1, kSmallThreads, 0, s>>>(L, coarse, x, b, t, coarse_b, precon ? 1 : 0);
+    visc_small_pre_kernel<float><<<1, kSmallThreads, 0, s>>>(L, coarse, x, b, t, coarse_b, precon ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, bf16* x, const bf16* b,
+                                 bf16* t, bf16* coarse_b, bool precon, hipStream_t s) {
+    visc_small_pre_kernel<bf16><<<1, kSmallThreads, 0, s>>>(L, coarse, x, b, t, coarse_b, precon ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_visc_small_post(const ViscLevelView& L, float* x, const float* b, const float* xc, hipStream_t s) {
-    visc_small_post_kernel<<<1, kSmallThreads, 0, s>>>(L, x, b, xc);
+    visc_small_post_kernel<float><<<1, kSmallThreads, 0, s>>>(L, x, b, xc);
+    return hipGetLastError();
+}
+hipError_t launch_visc_small_post(const ViscLevelView& L, bf16* x, const bf16* b, const bf16* xc, hipStream_t s) {
+    visc_small_post_kernel<bf16><<<1, kSmallThreads, 0, s>>>(L, x, b, xc);
     return hipGetLastError();
 }
 
-hipError_t launch_visc_zero(const ViscLevelView& L, float* v, hipStream_t s) {
-    const size_t n4 = size_t(L.nleaf_dof) * 3 * 512 / 4;
-    if (!n4) return hipSuccess;
-    visc_zero_kernel<<<unsigned((n4 + kVThreads - 1) / kVThreads), kVThreads, 0, s>>>(reinterpret_cast<float4*>(v),
-                                                                                     n4);
-    return hipGetLastError();
-}
+hipError_t launch_visc_zero(const ViscLevelView& L, float* v, hipStream_t s) { return zero(L, v, s); }
+hipError_t launch_visc_zero(const ViscLevelView& L, bf16* v, hipStream_t s) { return zero(L, v, s); }
 
 }  // namespace uaamg

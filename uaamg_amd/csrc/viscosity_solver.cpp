@@ -5,6 +5,7 @@
 #include <cfloat>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 
 namespace uaamg {
@@ -89,6 +90,7 @@ static std::vector<int32_t> wave_records(const ViscLevelHost& h) {
 }
 
 int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
+    d.bf = bf_level(h.level);
     auto put = [&](const void* src, size_t bytes) -> void* {
         void* p = arena_.take(std::max<size_t>(bytes, 4));
         if (p && bytes) check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload");
@@ -119,10 +121,10 @@ int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
     f.level = h.level;
     f.mask = v.mask;
     f.dofbase = static_cast<const int64_t*>(put(h.dofbase.data(), h.dofbase.size() * 8));
-    const size_t vec = std::max<size_t>(h.nleaf_dof * 3 * 512 * 4, 4);
-    d.X = static_cast<float*>(arena_.take(vec));
-    d.B = static_cast<float*>(arena_.take(vec));
-    d.T = static_cast<float*>(arena_.take(vec));
+    const size_t vec = std::max<size_t>(h.nleaf_dof * 3 * 512 * (d.bf ? 2 : 4), 16);
+    d.X = arena_.take(vec);
+    d.B = arena_.take(vec);
+    d.T = arena_.take(vec);
     if (!v.nb27 || !v.mask || !v.slot || !v.pool || !v.chan_leaves || !v.wrec || !f.dofbase || !d.X || !d.B ||
         !d.T) {
         err_ = "device arena too small";
@@ -304,20 +306,31 @@ int ViscDevice::acquire_device() {
 // One half-visit pair of level l's own work (the coarser levels' work between
 // them left out) in either mapping, on the level's zero scratch vectors (zero
 // in, zero out).
-void ViscDevice::half_visit_pair(int l, bool small) {
+template <class V>
+void ViscDevice::half_visit(int l, bool small) {
     DevLevel& d = lv_[size_t(l)];
     DevLevel& c = lv_[size_t(l + 1)];
+    V* X = static_cast<V*>(d.X);
+    V* B = static_cast<V*>(d.B);
+    V* T = static_cast<V*>(d.T);
+    V* cX = static_cast<V*>(c.X);   // tuned levels are >= 1: the coarse level has the same storage
+    V* cB = static_cast<V*>(c.B);
     if (small) {
-        check(launch_visc_small_pre(d.v, c.v, d.X, d.B, d.T, c.B, true, stream_), "tune");
-        check(launch_visc_small_post(d.v, d.X, d.B, c.X, stream_), "tune");
+        check(launch_visc_small_pre(d.v, c.v, X, B, T, cB, true, stream_), "tune");
+        check(launch_visc_small_post(d.v, X, B, cX, stream_), "tune");
         return;
     }
-    check(launch_visc_zero(d.v, d.X, stream_), "tune");
-    for (int i = 0; i < 2; ++i) rbgs(l, d.X, d.B, true);
-    apply_all(l, kVResidual, d.X, d.B, d.T);
-    check(launch_visc_restrict(c.v, d.v, d.T, c.B, stream_), "tune");
-    check(launch_visc_prolong(d.v, c.v, c.X, d.X, stream_), "tune");
-    for (int i = 0; i < 2; ++i) rbgs(l, d.X, d.B, false);
+    check(launch_visc_zero(d.v, X, stream_), "tune");
+    for (int i = 0; i < 2; ++i) rbgs<V>(l, X, B, true);
+    apply_all<V>(l, kVResidual, X, B, T);
+    check(launch_visc_restrict(c.v, d.v, static_cast<const V*>(T), cB, stream_), "tune");
+    check(launch_visc_prolong(d.v, c.v, static_cast<const V*>(cX), X, stream_), "tune");
+    for (int i = 0; i < 2; ++i) rbgs<V>(l, X, B, false);
+}
+
+void ViscDevice::half_visit_pair(int l, bool small) {
+    if (lv_[size_t(l)].bf) half_visit<bf16>(l, small);
+    else half_visit<float>(l, small);
 }
 
 void ViscDevice::tune_levels() {
@@ -375,7 +388,8 @@ int ViscDevice::upload_spai0(int level) {
 }
 
 // ------------------------------------------------------------ W-cycle
-void ViscDevice::apply_all(int level, int mode, const float* x, const float* b, float* y) {
+template <class V>
+void ViscDevice::apply_all(int level, int mode, const V* x, const V* b, V* y) {
     if (mode != kVRedGS && mode != kVBlackGS && x != y) {
         check(launch_visc_apply_all(lv_[level].v, mode, x, b, y, stream_), "apply");
         return;
@@ -384,7 +398,8 @@ void ViscDevice::apply_all(int level, int mode, const float* x, const float* b, 
 }
 
 // XYZ_RBGS_apply / ZYX_RBGS_apply (:2221-2302): six colour passes in place
-void ViscDevice::rbgs(int level, float* x, const float* b, bool xyz) {
+template <class V>
+void ViscDevice::rbgs(int level, V* x, const V* b, bool xyz) {
     const ViscLevelView& v = lv_[level].v;
     if (xyz) {
         for (int c = 0; c < 3; ++c) {
@@ -402,30 +417,44 @@ void ViscDevice::rbgs(int level, float* x, const float* b, bool xyz) {
 // simd_viscosity3d::mucycle (:3828-3884) with mu = 2, n = 2. At the level
 // above the coarsest the second visit would solve the same right-hand side
 // from zero again (solve_exact starts from a zero guess) and is skipped.
-void ViscDevice::mucycle(int level, float* x, const float* b, bool precon) {
+template <class V>
+void ViscDevice::mucycle(int level, V* x, const V* b, bool precon) {
     const int L = int(lv_.size());
     if (level == L - 1) {
         check(launch_visc_coarsest(cv_, b, x, stream_), "coarsest");
         return;
     }
     const ViscLevelView& v = lv_[level].v;
+    V* T = static_cast<V*>(lv_[level].T);
     if (level >= 1 && use_small_[size_t(level)]) {
-        DevLevel& c = lv_[level + 1];
-        check(launch_visc_small_pre(v, c.v, x, b, lv_[level].T, c.B, precon, stream_), "small pre");
-        mucycle(level + 1, c.X, c.B, true);
-        if (level + 1 < L - 1) mucycle(level + 1, c.X, c.B, false);
-        check(launch_visc_small_post(v, x, b, c.X, stream_), "small post");
+        DevLevel& c = lv_[level + 1];   // below level 0 every level has the same storage
+        V* cX = static_cast<V*>(c.X);
+        V* cB = static_cast<V*>(c.B);
+        check(launch_visc_small_pre(v, c.v, x, b, T, cB, precon, stream_), "small pre");
+        mucycle<V>(level + 1, cX, cB, true);
+        if (level + 1 < L - 1) mucycle<V>(level + 1, cX, cB, false);
+        check(launch_visc_small_post(v, x, b, cX, stream_), "small post");
         return;
     }
     if (precon) check(launch_visc_zero(v, x, stream_), "zero");
-    for (int i = 0; i < 2; ++i) rbgs(level, x, b, true);
-    apply_all(level, kVResidual, x, b, lv_[level].T);
+    for (int i = 0; i < 2; ++i) rbgs<V>(level, x, b, true);
+    apply_all<V>(level, kVResidual, x, b, T);
     DevLevel& c = lv_[level + 1];
-    check(launch_visc_restrict(c.v, v, lv_[level].T, c.B, stream_), "restrict");
-    mucycle(level + 1, c.X, c.B, true);
-    if (level + 1 < L - 1) mucycle(level + 1, c.X, c.B, false);
-    check(launch_visc_prolong(v, c.v, c.X, x, stream_), "prolong");
-    for (int i = 0; i < 2; ++i) rbgs(level, x, b, false);
+    // the coarse level's storage: BF16 below level 0 in the mixed form
+    auto below = [&](auto* cX, auto* cB) {
+        using VC = std::remove_pointer_t<decltype(cX)>;
+        check(launch_visc_restrict(c.v, v, static_cast<const V*>(T), cB, stream_), "restrict");
+        mucycle<VC>(level + 1, cX, cB, true);
+        if (level + 1 < L - 1) mucycle<VC>(level + 1, cX, cB, false);
+        check(launch_visc_prolong(v, c.v, static_cast<const VC*>(cX), x, stream_), "prolong");
+    };
+    if constexpr (std::is_same_v<V, bf16>) {
+        below(static_cast<bf16*>(c.X), static_cast<bf16*>(c.B));
+    } else {
+        if (c.bf) below(static_cast<bf16*>(c.X), static_cast<bf16*>(c.B));
+        else below(static_cast<float*>(c.X), static_cast<float*>(c.B));
+    }
+    for (int i = 0; i < 2; ++i) rbgs<V>(level, x, b, false);
 }
 
 // ---------------------------------------------------------------- PCG
@@ -436,25 +465,25 @@ void ViscDevice::mucycle(int level, float* x, const float* b, bool precon) {
 //          z = A p, alpha, r -= alpha z, ||r||_inf
 void ViscDevice::issue_init() {
     const DevLevel& d = lv_[0];
-    apply_all(0, kVResidual, x0_, b0_, r0_);
+    apply_all<float>(0, kVResidual, x0_, b0_, r0_);
     check(launch_absmax(d.flat, r0_, sc_, partials_, mailbox_dev_, stream_), "absmax");
 }
 
 void ViscDevice::issue_first() {
     const DevLevel& d = lv_[0];
-    mucycle(0, p0_, r0_, true);
+    mucycle<float>(0, p0_, r0_, true);
     check(launch_dot(d.flat, p0_, r0_, sc_, partials_, kDotRho, stream_), "rho");
-    apply_all(0, kVNormal, p0_, p0_, z0_);
+    apply_all<float>(0, kVNormal, p0_, p0_, z0_);
     check(launch_dot(d.flat, p0_, z0_, sc_, partials_, kDotSigma, stream_), "sigma");
     check(launch_axpy_absmax(d.flat, z0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
 }
 
 void ViscDevice::issue_iter() {
     const DevLevel& d = lv_[0];
-    mucycle(0, z0_, r0_, true);
+    mucycle<float>(0, z0_, r0_, true);
     check(launch_dot(d.flat, z0_, r0_, sc_, partials_, kDotRhoNew, stream_), "rho_new");
     check(launch_update_xp(d.flat, z0_, x0_, p0_, sc_, stream_), "update");
-    apply_all(0, kVNormal, p0_, p0_, z0_);
+    apply_all<float>(0, kVNormal, p0_, p0_, z0_);
     check(launch_dot(d.flat, p0_, z0_, sc_, partials_, kDotSigma, stream_), "sigma");
     check(launch_axpy_absmax(d.flat, z0_, r0_, sc_, partials_, mailbox_dev_, stream_), "axpy");
 }
@@ -561,58 +590,64 @@ int ViscDevice::solve_pcg_dev(float* x_dof, const float* b_dof, int* iterations)
 int ViscDevice::apply_dev(int level, int mode, const float* x_dof, const float* b_dof, float* y_dof) {
     if (!on_device()) return -4;
     if (level < 0 || level >= nlevels() || mode < 0 || mode > 7) return -1;
+    if (lv_[level].bf) {
+        err_ = "unit operations run on FP32 level vectors (precision 0)";
+        return -1;
+    }
     if (mode == kVSpai0Mode)
         if (int rc = upload_spai0(level)) return rc;
     DevLevel& d = lv_[level];
+    float *X = d.Xf(), *B = d.Bf(), *T = d.Tf();
     const size_t vec = size_t(d.v.nleaf_dof) * 3 * 512 * 4;
-    check(launch_scatter_dof(d.flat, x_dof, d.X, stream_), "scatter");
-    if (b_dof) check(launch_scatter_dof(d.flat, b_dof, d.B, stream_), "scatter");
-    else check(hipMemsetAsync(d.B, 0, std::max<size_t>(vec, 4), stream_), "memset");
+    check(launch_scatter_dof(d.flat, x_dof, X, stream_), "scatter");
+    if (b_dof) check(launch_scatter_dof(d.flat, b_dof, B, stream_), "scatter");
+    else check(hipMemsetAsync(B, 0, std::max<size_t>(vec, 4), stream_), "memset");
     if (mode <= kVSpai0Mode) {
-        check(hipMemsetAsync(d.T, 0, std::max<size_t>(vec, 4), stream_), "memset");
-        apply_all(level, mode, d.X, d.B, d.T);
-        check(launch_gather_dof(d.flat, d.T, y_dof, stream_), "gather");
+        check(hipMemsetAsync(T, 0, std::max<size_t>(vec, 4), stream_), "memset");
+        apply_all<float>(level, mode, X, B, T);
+        check(launch_gather_dof(d.flat, T, y_dof, stream_), "gather");
     } else {
         if (mode == kVRedGS || mode == kVBlackGS) {
-            for (int c = 0; c < 3; ++c) check(launch_visc_apply(d.v, mode, c, d.X, d.B, d.X, stream_), "gs");
+            for (int c = 0; c < 3; ++c) check(launch_visc_apply(d.v, mode, c, X, B, X, stream_), "gs");
         } else {
-            rbgs(level, d.X, d.B, mode == 6);
+            rbgs<float>(level, X, B, mode == 6);
         }
-        check(launch_gather_dof(d.flat, d.X, y_dof, stream_), "gather");
+        check(launch_gather_dof(d.flat, X, y_dof, stream_), "gather");
     }
     return check(hipStreamSynchronize(stream_), "apply sync");
 }
 
 int ViscDevice::restrict_dev(int fl, const float* fine_dof, float* coarse_dof) {
     if (!on_device()) return -4;
-    if (fl < 0 || fl + 1 >= nlevels()) return -1;
+    if (fl < 0 || fl + 1 >= nlevels() || lv_[fl + 1].bf) return -1;   // FP32 level vectors only
     DevLevel &f = lv_[fl], &c = lv_[fl + 1];
-    check(launch_scatter_dof(f.flat, fine_dof, f.T, stream_), "scatter");
+    check(launch_scatter_dof(f.flat, fine_dof, f.Tf(), stream_), "scatter");
     check(hipMemsetAsync(c.B, 0, std::max<size_t>(size_t(c.v.nleaf_dof) * 3 * 512 * 4, 4), stream_), "memset");
-    check(launch_visc_restrict(c.v, f.v, f.T, c.B, stream_), "restrict");
-    check(launch_gather_dof(c.flat, c.B, coarse_dof, stream_), "gather");
+    check(launch_visc_restrict(c.v, f.v, f.Tf(), c.Bf(), stream_), "restrict");
+    check(launch_gather_dof(c.flat, c.Bf(), coarse_dof, stream_), "gather");
     return check(hipStreamSynchronize(stream_), "restrict sync");
 }
 
 int ViscDevice::prolong_dev(int fl, const float* coarse_dof, float* fine_dof) {
     if (!on_device()) return -4;
-    if (fl < 0 || fl + 1 >= nlevels()) return -1;
+    if (fl < 0 || fl + 1 >= nlevels() || lv_[fl + 1].bf) return -1;   // FP32 level vectors only
     DevLevel &f = lv_[fl], &c = lv_[fl + 1];
-    check(launch_scatter_dof(c.flat, coarse_dof, c.X, stream_), "scatter");
-    check(launch_scatter_dof(f.flat, fine_dof, f.X, stream_), "scatter");
-    check(launch_visc_prolong(f.v, c.v, c.X, f.X, stream_), "prolong");
-    check(launch_gather_dof(f.flat, f.X, fine_dof, stream_), "gather");
+    check(launch_scatter_dof(c.flat, coarse_dof, c.Xf(), stream_), "scatter");
+    check(launch_scatter_dof(f.flat, fine_dof, f.Xf(), stream_), "scatter");
+    check(launch_visc_prolong(f.v, c.v, c.Xf(), f.Xf(), stream_), "prolong");
+    check(launch_gather_dof(f.flat, f.Xf(), fine_dof, stream_), "gather");
     return check(hipStreamSynchronize(stream_), "prolong sync");
 }
 
 int ViscDevice::coarsest_dev(const float* b_dof, float* x_dof) {
     if (!on_device()) return -4;
     DevLevel& d = lv_.back();
+    if (d.bf) return -1;   // FP32 level vectors only
     const size_t vec = std::max<size_t>(size_t(d.v.nleaf_dof) * 3 * 512 * 4, 4);
-    check(launch_scatter_dof(d.flat, b_dof, d.B, stream_), "scatter");
+    check(launch_scatter_dof(d.flat, b_dof, d.Bf(), stream_), "scatter");
     check(hipMemsetAsync(d.X, 0, vec, stream_), "memset");
-    check(launch_visc_coarsest(cv_, d.B, d.X, stream_), "coarsest");
-    check(launch_gather_dof(d.flat, d.X, x_dof, stream_), "gather");
+    check(launch_visc_coarsest(cv_, d.Bf(), d.Xf(), stream_), "coarsest");
+    check(launch_gather_dof(d.flat, d.Xf(), x_dof, stream_), "gather");
     return check(hipStreamSynchronize(stream_), "coarsest sync");
 }
 
@@ -620,7 +655,7 @@ int ViscDevice::precondition_dev(const float* r_dof, float* z_dof) {
     if (!on_device()) return -4;
     const DevLevel& d = lv_[0];
     check(launch_scatter_dof(d.flat, r_dof, r0_, stream_), "scatter");
-    mucycle(0, z0_, r0_, true);
+    mucycle<float>(0, z0_, r0_, true);
     check(launch_gather_dof(d.flat, z0_, z_dof, stream_), "gather");
     return check(hipStreamSynchronize(stream_), "precondition sync");
 }
@@ -655,13 +690,14 @@ int ViscDevice::time_apply(int level, int mode, int c, int reps, float* ms) {
     if (!on_device()) return -4;
     if (level < 0 || level >= nlevels() || mode < 0 || mode > 5 || c < 0 || c > 2 || reps <= 0) return -1;
     DevLevel& d = lv_[level];
+    if (d.bf) return -1;   // timed on FP32 level vectors
     const bool gs = mode == kVRedGS || mode == kVBlackGS;
-    float* y = gs ? d.X : d.T;
-    check(launch_visc_apply(d.v, mode, c, d.X, d.B, y, stream_), "warm");
+    float* y = gs ? d.Xf() : d.Tf();
+    check(launch_visc_apply(d.v, mode, c, d.Xf(), d.Bf(), y, stream_), "warm");
     hipEvent_t e0, e1;
     if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -3;
     check(hipEventRecord(e0, stream_), "record");
-    for (int r = 0; r < reps; ++r) check(launch_visc_apply(d.v, mode, c, d.X, d.B, y, stream_), "timed");
+    for (int r = 0; r < reps; ++r) check(launch_visc_apply(d.v, mode, c, d.Xf(), d.Bf(), y, stream_), "timed");
     check(hipEventRecord(e1, stream_), "record");
     check(hipEventSynchronize(e1), "event sync");
     float t = 0.f;

@@ -24,6 +24,7 @@ struct ViscOptions {
     int64_t max_coarsest_dof = 4000;
     int flags = 0;                  // bit 1: no graph capture, bit 3: untrimmed level 0, bit 5: single-workgroup
                                     // half-visits up to 16 leaves, bit 7: no single-workgroup half-visits
+    int precision = 0;              // 0: FP32; 1: the W-cycle's coarse-level vectors stored in BF16
     hipStream_t stream = nullptr;
 };
 
@@ -82,7 +83,13 @@ private:
     struct DevLevel {
         ViscLevelView v{};
         LevelView flat{};          // the vector as 3*nleaf_dof bricks for the PCG kernels
-        float *X = nullptr, *B = nullptr, *T = nullptr;
+        // the W-cycle's level vectors: FP32, or BF16 (bf = true) on the coarse
+        // levels of the mixed form
+        void *X = nullptr, *B = nullptr, *T = nullptr;
+        bool bf = false;
+        float* Xf() const { return static_cast<float*>(X); }
+        float* Bf() const { return static_cast<float*>(B); }
+        float* Tf() const { return static_cast<float*>(T); }
     };
     struct Graph {
         hipGraph_t g = nullptr;
@@ -92,9 +99,14 @@ private:
     int upload_level(const ViscLevelHost& h, DevLevel& d);
     int upload_spai0(int level);
     int build_coarsest();
-    void rbgs(int level, float* x, const float* b, bool xyz);
-    void mucycle(int level, float* x, const float* b, bool precon);
-    void apply_all(int level, int mode, const float* x, const float* b, float* y);
+    template <class V>
+    void rbgs(int level, V* x, const V* b, bool xyz);
+    template <class V>
+    void mucycle(int level, V* x, const V* b, bool precon);
+    template <class V>
+    void apply_all(int level, int mode, const V* x, const V* b, V* y);
+    template <class V>
+    void half_visit(int level, bool small);
     void issue_init();
     void issue_first();
     void issue_iter();
@@ -105,6 +117,7 @@ private:
     int wait_nu(float* nu);
     void tune_levels();
     void half_visit_pair(int level, bool small);
+    bool bf_level(int level) const { return opt_.precision == 1 && level >= 1; }
     int check(hipError_t e, const char* what);
 
     ViscHierarchyHost host_;

@@ -82,11 +82,11 @@ class SolveServer:
         return int(pid.value)
 
     def add_viscosity(self, viscosity: Grid, liquid_sdf: Grid, solid_sdf: Grid, dt: float, rho: float, dx: float, *,
-                      flags: int = 0, max_iterations: int = 100) -> int:
+                      flags: int = 0, max_iterations: int = 100, precision: int = 0) -> int:
         L = _bind()
         o = _ViscOptions()
         L.uaamg_viscosity_options_default(C.byref(o))
-        o.flags, o.max_iterations = flags, max_iterations
+        o.flags, o.max_iterations, o.precision = flags, max_iterations, precision
         pid = C.c_int64()
         g = [viscosity._struct(), liquid_sdf._struct(), solid_sdf._struct()]
         _check(L.uaamg_server_add_viscosity(self._h, *[C.byref(x) for x in g], float(dt), float(rho), float(dx),

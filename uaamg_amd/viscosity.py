@@ -100,17 +100,20 @@ class L_with_level:
 
     create_viscosity_matrix(viscosity, liquid_sdf, solid_sdf, dt, rho, dx) mirrors
     simd_viscosity3d.h:95-98; dx is liquid_sdf's voxel size. device=-1 keeps a
-    host-only handle (inspection; no solves).
+    host-only handle (inspection; no solves). precision=1 stores the W-cycle's
+    level vectors below level 0 in BF16 (FP32 arithmetic; no reference
+    counterpart).
     """
 
     def __init__(self, viscosity: Grid, liquid_sdf: Grid, solid_sdf: Grid, dt: float, rho: float, dx: float, *,
-                 device: int = 0, max_coarsest_dof: int = 4000, flags: int = 0):
+                 device: int = 0, max_coarsest_dof: int = 4000, flags: int = 0, precision: int = 0):
         L = lib()
         o = _ViscOptions()
         L.uaamg_viscosity_options_default(C.byref(o))
         o.device = device
         o.max_coarsest_dof = max_coarsest_dof
         o.flags = flags
+        o.precision = precision
         h = C.c_void_p()
         g = [viscosity._struct(), liquid_sdf._struct(), solid_sdf._struct()]
         _check(L.uaamg_viscosity_create(*[C.byref(x) for x in g], float(dt), float(rho), float(dx), C.byref(o),
