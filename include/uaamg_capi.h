@@ -54,6 +54,9 @@ extern "C" {
 #define UAAMG_MODE_RED_GS 4
 #define UAAMG_MODE_BLACK_GS 5
 #define UAAMG_MODE_JACOBI_BF16 6 /* algorithmic_bytes / time_apply only: the BF16 W-cycle's Jacobi sweep */
+#define UAAMG_MODE_BLOCK_POST 7  /* algorithmic_bytes / time_apply only: a coarse level's post-visit block kernel
+                                    (prolongation and three sweeps in one launch; levels with block tables) */
+#define UAAMG_MODE_COARSEST 8    /* algorithmic_bytes / time_apply only: the coarsest-level CG (its level) */
 
 /* coefficient kinds of a level */
 #define UAAMG_COEF_DIAGONAL 0
@@ -246,6 +249,15 @@ int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* 
 int uaamg_face_weights(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf, int32_t device, int64_t* nleaf,
                        int32_t* origins, uint64_t* active, float* values);
 
+/* variants of uaamg_face_weights_ex */
+#define UAAMG_FACE_WEIGHTS_STATIC 0 /* FLIP_Solver_OpenVDB::calculate_face_weights(liquid, solid), FLIP_vdb.cpp:1990-2068 */
+#define UAAMG_FACE_WEIGHTS_MEMBER 1 /* the member calculate_face_weights() FLIP's substep calls, FLIP_vdb.cpp:2669-2752:
+                                       the same weights, each one below 1e-2 after the clamp set to 0 */
+
+/* uaamg_face_weights with the variant chosen (uaamg_face_weights = variant UAAMG_FACE_WEIGHTS_STATIC). */
+int uaamg_face_weights_ex(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf, int32_t device, int32_t variant,
+                          int64_t* nleaf, int32_t* origins, uint64_t* active, float* values);
+
 /*
  * PoissonSolver::solveMultigridPCG (simd_vdb_poisson.h:162, .cpp:2155-2226) on host DOF-ordered
  * vectors: x holds the initial guess on entry and the pressure on exit. Returns UAAMG_SUCCESS or
@@ -394,7 +406,9 @@ typedef struct uaamg_viscosity_options {
     int32_t device;           /* HIP device ordinal; -1 = host-only handle (inspection, no solves) */
     int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, :3638) */
     int32_t flags;            /* 0 normally; bit 1 = no graph replay (A/B checks); bit 3 = keep level 0 untrimmed;
-                                 bit 5 = small coarse levels as single-workgroup half-visits */
+                                 bit 5 = small coarse levels as single-workgroup half-visits; bit 8 = level-0
+                                 volume stage (subcell SDF, subcell / voxel / face / edge volumes) on the host
+                                 threads instead of the device (same result) */
     int32_t precision;        /* UAAMG_PRECISION_FP32 (default, the reference's arithmetic) or
                                  UAAMG_PRECISION_BF16_CYCLE: the W-cycle's level vectors below level 0 stored
                                  in BF16 and computed in FP32; level 0, the coarsest solve's arithmetic and

@@ -47,7 +47,7 @@ def lib() -> C.CDLL:
         "orc_level_coefficients": (None, [vp, i32, i32, vp, vp]),
         "orc_build_rhs": (None, [vp, i64, vp, vp, vp, i64, vp, vp, vp, vp, f32, i64, vp, vp, vp, vp]),
         "orc_apply": (i32, [vp, i32, i32, f32, vp, vp, vp]),
-        "orc_face_weights": (None, [i64, vp, vp, i64, vp, vp, f32, C.POINTER(i64), vp, vp, vp]),
+        "orc_face_weights": (None, [i64, vp, vp, i64, vp, vp, f32, f32, C.POINTER(i64), vp, vp, vp]),
         "orc_apply_pressure_gradient": (None, [vp, vp, i64, vp, vp, f32, i64, vp, vp, vp, i64, vp, vp, vp, i32,
                                                i64, vp, vp, vp, vp, vp, f32, f32]),
         "orc_first_sweep": (i32, [vp, i32, i32, f32, vp, vp]),
@@ -272,9 +272,10 @@ class OraclePoisson:
         return y
 
 
-def face_weights(liquid, solid):
-    """FLIP_Solver_OpenVDB::calculate_face_weights restated (FLIP_vdb.cpp:1990-2068): (origins,
-    active, values) of the weight grid, leaves in tree order."""
+def face_weights(liquid, solid, cut: bool = False):
+    """FLIP_Solver_OpenVDB::calculate_face_weights restated (FLIP_vdb.cpp:1990-2068; cut=True: the
+    member variant of FLIP_vdb.cpp:2669-2752, weights below 1e-2 set to 0): (origins, active,
+    values) of the weight grid, leaves in tree order."""
     lo = np.ascontiguousarray(liquid.origins, np.int32)
     la = (np.full((lo.shape[0], 8), np.uint64(0xFFFFFFFFFFFFFFFF), np.uint64) if liquid.active is None
           else np.ascontiguousarray(liquid.active, np.uint64))
@@ -282,12 +283,13 @@ def face_weights(liquid, solid):
     sv = np.ascontiguousarray(solid.values, np.float32)
     bg = solid.background[0] if isinstance(solid.background, (tuple, list)) else solid.background
     n = C.c_int64()
-    lib().orc_face_weights(lo.shape[0], _p(lo), _p(la), so.shape[0], _p(so), _p(sv), float(bg), C.byref(n),
+    c = C.c_float(1e-2 if cut else 0.0)
+    lib().orc_face_weights(lo.shape[0], _p(lo), _p(la), so.shape[0], _p(so), _p(sv), float(bg), c, C.byref(n),
                            None, None, None)
     org = np.zeros((n.value, 3), np.int32)
     act = np.zeros((n.value, 8), np.uint64)
     val = np.zeros((n.value, 512, 3), np.float32)
-    lib().orc_face_weights(lo.shape[0], _p(lo), _p(la), so.shape[0], _p(so), _p(sv), float(bg), C.byref(n),
+    lib().orc_face_weights(lo.shape[0], _p(lo), _p(la), so.shape[0], _p(so), _p(sv), float(bg), c, C.byref(n),
                            _p(org), _p(act), _p(val))
     return org, act, val
 

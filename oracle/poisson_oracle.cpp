@@ -1102,10 +1102,13 @@ void orc_build_rhs(void* p, int64_t nw, const int32_t* w_org, const float* w_val
 // FLIP_Solver_OpenVDB::calculate_face_weights (FLIP_vdb.cpp:1990-2068): the
 // liquid topology dilated twice over the 26 neighbours (voxel by voxel: every
 // voxel within Chebyshev distance 2 of an active one), weights from the solid
-// SDF at the face corners; inactive voxels hold 1. Two-phase like the library:
-// *n receives the leaf count, the arrays (tree order) are filled when non-null.
+// SDF at the face corners; inactive voxels hold 1. cut > 0 restates the member
+// variant FLIP's substep uses (FLIP_vdb.cpp:2669-2752): after the clamp, a
+// weight below cut (1e-2 there) becomes 0. Two-phase like the library: *n
+// receives the leaf count, the arrays (tree order) are filled when non-null.
 void orc_face_weights(int64_t nl, const int32_t* l_org, const uint64_t* l_act, int64_t ns, const int32_t* s_org,
-                      const float* s_val, float s_bg, int64_t* n, int32_t* org, uint64_t* act, float* val) {
+                      const float* s_val, float s_bg, float cut, int64_t* n, int32_t* org, uint64_t* act,
+                      float* val) {
     std::map<Coord, Leaf<Vec3>, TreeLess> out;
     Vec3 one;
     one.v[0] = one.v[1] = one.v[2] = 1.f;
@@ -1149,7 +1152,10 @@ void orc_face_weights(int64_t nl, const int32_t* l_org, const uint64_t* l_act, i
             w[0] = 1.0f - fraction_inside4(sd[0][0][0], sd[0][1][0], sd[0][0][1], sd[0][1][1]);
             w[1] = 1.0f - fraction_inside4(sd[0][0][0], sd[0][0][1], sd[1][0][0], sd[1][0][1]);
             w[2] = 1.0f - fraction_inside4(sd[0][0][0], sd[1][0][0], sd[0][1][0], sd[1][1][0]);
-            for (int c = 0; c < 3; ++c) l.v[off].v[c] = std::max(0.f, std::min(w[c], 1.f));
+            for (int c = 0; c < 3; ++c) {
+                l.v[off].v[c] = std::max(0.f, std::min(w[c], 1.f));
+                if (l.v[off].v[c] < cut) l.v[off].v[c] = 0.f;
+            }
         }
         org[3 * k] = kv.first.x;
         org[3 * k + 1] = kv.first.y;

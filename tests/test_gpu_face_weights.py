@@ -1,6 +1,7 @@
-"""GPU parity of the face weights from the solid SDF (uaamg_face_weights,
-FLIP_vdb.cpp:1990-2068): topology, active masks and every weight bit-identical to
-the oracle's restatement, for a scene's liquid SDF topology and a sampled solid
+"""GPU parity of the face weights from the solid SDF (uaamg_face_weights_ex: the
+static FLIP_vdb.cpp:1990-2068 and the member variant FLIP_vdb.cpp:2669-2752 with its
+1e-2 cut): topology, active masks and every weight bit-identical to the oracle's
+restatement, for a scene's liquid SDF topology and a sampled solid
 sphere SDF (cut faces at every orientation), and for a solid grid missing leaves
 (background lookups).
 """
@@ -31,11 +32,14 @@ def test_face_weights_equal_oracle(hip_available, cfg, n, seed):
     keep = np.random.default_rng(seed).uniform(size=solid.origins.shape[0]) > 0.1
     thinned = U.Grid(solid.origins[keep].copy(), solid.values[keep].copy(), None, (3.0,))
     for sg in (solid, thinned):
-        got = U.calculate_face_weights(s.phi, sg)
-        org, act, val = O.face_weights(s.phi, sg)
-        print(f"[case] cfg={cfg} n={n} leaves {org.shape[0]}", flush=True)
-        assert np.array_equal(got.origins, org)
-        assert np.array_equal(got.active, act)
-        assert np.array_equal(got.values, val)
-        cut = ((val > 0) & (val < 1)).sum()
-        assert cut > 0   # the sphere cuts faces
+        for member in (False, True):
+            got = U.calculate_face_weights(s.phi, sg, member=member)
+            org, act, val = O.face_weights(s.phi, sg, cut=member)
+            print(f"[case] cfg={cfg} n={n} member={member} leaves {org.shape[0]}", flush=True)
+            assert np.array_equal(got.origins, org)
+            assert np.array_equal(got.active, act)
+            assert np.array_equal(got.values, val)
+            cut = ((val > 0) & (val < 1)).sum()
+            assert cut > 0   # the sphere cuts faces
+            if member:
+                assert not ((val > 0) & (val < np.float32(1e-2))).any()

@@ -518,3 +518,27 @@ def test_face_weights_closed_form_plane():
             assert w[2] == (np.float32(0) if z < h else np.float32(1))
             checked += 1
     assert checked == 125   # the 5x5x5 box around the one active voxel
+
+
+def test_face_weights_member_variant_cut():
+    """The member variant (FLIP_vdb.cpp:2669-2752) equals the static one with every weight below 1e-2
+    set to 0: a solid half-space z < 3.995 leaves x/y faces at z = 3 a sliver 0 < w < 1e-2 (cut to 0)
+    and every other weight as the static variant's."""
+    import uaamg_amd as U
+    h = np.float32(3.995)
+    org = np.array([[0, 0, 0]], np.int32)
+    act = np.zeros((1, 8), np.uint64)
+    act[0, 3] = np.uint64(1) << np.uint64(4 * 8 + 2)     # voxel (3, 4, 2)
+    liquid = U.Grid(org, np.zeros((1, 512), np.float32), act, (0.0,))
+    sorg = np.array([[x, y, z] for x in range(-8, 24, 8) for y in range(-8, 24, 8) for z in range(-8, 24, 8)], np.int32)
+    sval = np.zeros((sorg.shape[0], 512), np.float32)
+    for i, o in enumerate(sorg):
+        for off in range(512):
+            sval[i, off] = np.float32(o[2] + (off & 7)) - h
+    solid = U.Grid(sorg, sval, None, (1.0,))
+    o0, a0, v0 = O.face_weights(liquid, solid)
+    o1, a1, v1 = O.face_weights(liquid, solid, cut=True)
+    assert np.array_equal(o0, o1) and np.array_equal(a0, a1)
+    sliver = (v0 > 0) & (v0 < np.float32(1e-2))
+    assert sliver.sum() == 2 * 25          # the x and y faces of the 25 active voxels at z = 3
+    assert np.array_equal(v1, np.where(sliver, np.float32(0), v0))

@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
         "uaamg_poisson_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
         "uaamg_poisson_build_rhs_device": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), i32, vp]),
         "uaamg_face_weights": (C.c_int, [P(_Grid), P(_Grid), i32, P(i64), vp, vp, vp]),
+        "uaamg_face_weights_ex": (C.c_int, [P(_Grid), P(_Grid), i32, i32, P(i64), vp, vp, vp]),
         "uaamg_poisson_apply_pressure_gradient_device": (C.c_int, [vp, vp, P(_Grid), P(_Grid), P(_Grid),
                                                                    P(_FaceChannel), f32, f32, i32]),
         "uaamg_poisson_solve_pcg": (C.c_int, [vp, vp, vp, P(i32)]),
@@ -247,19 +248,21 @@ class Grid:
         return g
 
 
-def calculate_face_weights(liquid_sdf: Grid, solid_sdf: Grid, device: int = 0) -> Grid:
+def calculate_face_weights(liquid_sdf: Grid, solid_sdf: Grid, device: int = 0, *, member: bool = False) -> Grid:
     """FLIP_Solver_OpenVDB::calculate_face_weights (FLIP_vdb.cpp:1990-2068) with the weights computed on
     HIP device `device`: a Vec3f grid on the liquid SDF's topology dilated twice (26 neighbours), leaves
-    in tree order, background (1, 1, 1)."""
+    in tree order, background (1, 1, 1). member=True: the member variant FLIP's substep calls
+    (FLIP_vdb.cpp:2669-2752), which sets every weight below 1e-2 to 0."""
     gl, gs = liquid_sdf._struct(), solid_sdf._struct()
+    variant = 1 if member else 0
     n = C.c_int64()
-    _check(lib().uaamg_face_weights(C.byref(gl), C.byref(gs), int(device), C.byref(n), None, None, None),
+    _check(lib().uaamg_face_weights_ex(C.byref(gl), C.byref(gs), int(device), variant, C.byref(n), None, None, None),
            "face_weights")
     org = np.zeros((n.value, 3), np.int32)
     act = np.zeros((n.value, 8), np.uint64)
     val = np.zeros((n.value, 512, 3), np.float32)
-    _check(lib().uaamg_face_weights(C.byref(gl), C.byref(gs), int(device), C.byref(n), _ptr(org), _ptr(act),
-                                    _ptr(val)), "face_weights")
+    _check(lib().uaamg_face_weights_ex(C.byref(gl), C.byref(gs), int(device), variant, C.byref(n), _ptr(org),
+                                       _ptr(act), _ptr(val)), "face_weights")
     return Grid(org, val, act, (1.0, 1.0, 1.0))
 
 
@@ -425,6 +428,7 @@ class LaplacianWithLevel:
                "uaamg_poisson_create")
         self._h = h
         self._device = device
+        self.precision = precision
         self.mDt = float(dt)
         self.mDxThisLevel = float(dx)
         self.mLevel = 0

@@ -42,6 +42,8 @@ SOURCES = [
     "face_weights.cpp",
     "finest_kernels.hip",
     "finest_device.cpp",
+    "visc_volumes_kernels.hip",
+    "visc_volumes.cpp",
 ]
 
 ARCH = os.environ.get("UAAMG_OFFLOAD_ARCH", "gfx950")

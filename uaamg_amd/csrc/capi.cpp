@@ -413,8 +413,16 @@ int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* 
 
 int uaamg_face_weights(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf, int32_t device, int64_t* nleaf,
                        int32_t* origins, uint64_t* active, float* values) {
+    return uaamg_face_weights_ex(liquid_sdf, solid_sdf, device, UAAMG_FACE_WEIGHTS_STATIC, nleaf, origins, active,
+                                 values);
+}
+
+int uaamg_face_weights_ex(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf, int32_t device, int32_t variant,
+                          int64_t* nleaf, int32_t* origins, uint64_t* active, float* values) {
     UAAMG_GUARD_BEGIN
     if (!nleaf) return fail(UAAMG_EINVAL, "null argument");
+    if (variant != UAAMG_FACE_WEIGHTS_STATIC && variant != UAAMG_FACE_WEIGHTS_MEMBER)
+        return fail(UAAMG_EINVAL, "unknown face-weight variant");
     HostGrid liquid;
     if (!to_host_grid(liquid_sdf, 1, liquid, "liquid_sdf")) return UAAMG_EINVAL;
     if (!solid_sdf || solid_sdf->ncomp != 1 || solid_sdf->nleaf < 0 ||
@@ -424,8 +432,9 @@ int uaamg_face_weights(const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf
     std::vector<Coord> org;
     std::vector<uint64_t> act;
     std::string err;
+    const float cut = variant == UAAMG_FACE_WEIGHTS_MEMBER ? 1e-2f : 0.f;
     const int rc = face_weights_device(liquid, solid_sdf->nleaf, solid_sdf->origins, solid_sdf->values,
-                                       solid_sdf->background[0], device, org, act, fill ? values : nullptr, err);
+                                       solid_sdf->background[0], cut, device, org, act, fill ? values : nullptr, err);
     if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
     *nleaf = int64_t(org.size());
     if (fill) {
@@ -570,7 +579,7 @@ int uaamg_poisson_coarsest_solve(uaamg_poisson* h, const float* b, float* x) {
 }
 
 int uaamg_poisson_algorithmic_bytes(const uaamg_poisson* h, int level, int mode, double* bytes) {
-    if (!h || !bytes || level < 0 || level >= h->dev.nlevels() || mode < 0 || mode > UAAMG_MODE_JACOBI_BF16)
+    if (!h || !bytes || level < 0 || level >= h->dev.nlevels() || mode < 0 || mode > UAAMG_MODE_COARSEST)
         return fail(UAAMG_EINVAL, "bad argument");
     *bytes = h->dev.algorithmic_bytes(level, mode);
     return 0;

@@ -91,8 +91,11 @@ int prepare_viscosity(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf,
     if (!to_host_grid(viscosity, 1, mu, "viscosity")) return UAAMG_EINVAL;
     if (!to_host_grid(liquid_sdf, 1, liq, "liquid_sdf")) return UAAMG_EINVAL;
     if (!to_host_grid(solid_sdf, 1, sol, "solid_sdf")) return UAAMG_EINVAL;
-    if (!build_visc_hierarchy(liq, sol, mu, dt, rho, dx, o.max_coarsest_dof, !(o.flags & 8), hh))
-        return fail(UAAMG_EINVAL, "hierarchy construction failed");
+    // the level-0 volume stage runs on the handle's device unless flags bit 8 keeps it on the host
+    const int volume_device = (o.device >= 0 && !(o.flags & 256)) ? o.device : -1;
+    std::string err;
+    if (!build_visc_hierarchy(liq, sol, mu, dt, rho, dx, o.max_coarsest_dof, !(o.flags & 8), hh, volume_device, &err))
+        return fail(UAAMG_EINVAL, "hierarchy construction failed: " + err);
     vo = ViscOptions{};
     vo.max_iterations = o.max_iterations;
     vo.device = o.device;

@@ -101,12 +101,17 @@ int apply_pressure_gradient_device(PoissonDevice& d, const float* pressure_dev, 
             t[kGradSolid] = in_lookup[2].find(o);
         }
         const size_t vbytes = n * kLeafSize * sizeof(float);
-        DevBuf tab_d, act_d, vel_d, upd_d, refl_d;
-        if (hipError_t e = hipMalloc(&tab_d.p, tab.size() * sizeof(int32_t))) return hip_fail(e, "hipMalloc");
-        if (hipError_t e = hipMalloc(&act_d.p, n * 8 * sizeof(uint64_t))) return hip_fail(e, "hipMalloc");
-        if (hipError_t e = hipMemcpyAsync(tab_d.p, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice, s))
+        // the leaf tables and the masks in the handle's grow-only buffer (a repeated call allocates
+        // nothing; the channels take turns, each synchronised before the next)
+        const size_t tab_bytes = (tab.size() * sizeof(int32_t) + 255) & ~size_t(255);
+        char* aux = static_cast<char*>(d.aux(tab_bytes + n * 8 * sizeof(uint64_t)));
+        if (!aux) return hip_fail(hipErrorOutOfMemory, "device scratch");
+        int32_t* tab_d = reinterpret_cast<int32_t*>(aux);
+        uint64_t* act_d = reinterpret_cast<uint64_t*>(aux + tab_bytes);
+        DevBuf vel_d, upd_d, refl_d;
+        if (hipError_t e = hipMemcpyAsync(tab_d, tab.data(), tab.size() * sizeof(int32_t), hipMemcpyHostToDevice, s))
             return hip_fail(e, "upload tables");
-        if (hipError_t e = hipMemcpyAsync(act_d.p, ch.active, n * 8 * sizeof(uint64_t), hipMemcpyHostToDevice, s))
+        if (hipError_t e = hipMemcpyAsync(act_d, ch.active, n * 8 * sizeof(uint64_t), hipMemcpyHostToDevice, s))
             return hip_fail(e, "upload active masks");
         float* vel = ch.values;
         float* upd = ch.update;
@@ -126,7 +131,7 @@ int apply_pressure_gradient_device(PoissonDevice& d, const float* pressure_dev, 
             }
         }
         GradArgs a{};
-        a.tab = static_cast<const int32_t*>(tab_d.p);
+        a.tab = tab_d;
         a.phi = in_dev[0];
         a.w = in_dev[1];
         a.sv = in_dev[2];
@@ -142,9 +147,9 @@ int apply_pressure_gradient_device(PoissonDevice& d, const float* pressure_dev, 
         a.dt = dt;
         a.dx = l0.dx;
         a.friction = friction;
-        if (hipError_t e = launch_pressure_gradient(ch.nleaf, a, static_cast<uint64_t*>(act_d.p), vel, upd, refl, s))
+        if (hipError_t e = launch_pressure_gradient(ch.nleaf, a, act_d, vel, upd, refl, s))
             return hip_fail(e, "pressure gradient kernel");
-        if (hipError_t e = hipMemcpyAsync(ch.active, act_d.p, n * 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s))
+        if (hipError_t e = hipMemcpyAsync(ch.active, act_d, n * 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s))
             return hip_fail(e, "download active masks");
         if (!values_on_device) {
             if (hipError_t e = hipMemcpyAsync(ch.values, vel, vbytes, hipMemcpyDeviceToHost, s))

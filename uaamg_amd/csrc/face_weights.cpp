@@ -98,8 +98,8 @@ void dilate_active_26(const std::vector<Coord>& origins, const std::vector<uint6
 }
 
 int face_weights_device(const HostGrid& liquid, int64_t nsolid, const int32_t* solid_origins, const float* solid_values,
-                        float solid_bg, int device, std::vector<Coord>& origins, std::vector<uint64_t>& active,
-                        float* values, std::string& err) {
+                        float solid_bg, float cut, int device, std::vector<Coord>& origins,
+                        std::vector<uint64_t>& active, float* values, std::string& err) {
     dilate_active_26(liquid.origin, liquid.active, 2, origins, active);
     if (!values) return 0;
     if (hipSetDevice(device) != hipSuccess) {
@@ -160,7 +160,7 @@ int face_weights_device(const HostGrid& liquid, int64_t nsolid, const int32_t* s
     if (hipError_t e =
             hipMemcpyAsync(act_d.p, active.data(), active.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s))
         return hip_fail(e, "upload masks");
-    FaceWeightArgs a{static_cast<const float*>(solid_d.p), static_cast<const int32_t*>(tab_d.p), solid_bg};
+    FaceWeightArgs a{static_cast<const float*>(solid_d.p), static_cast<const int32_t*>(tab_d.p), solid_bg, cut};
     if (hipError_t e = launch_face_weights(int64_t(n), static_cast<const uint64_t*>(act_d.p), a,
                                            static_cast<float*>(val_d.p), s))
         return hip_fail(e, "face weights kernel");

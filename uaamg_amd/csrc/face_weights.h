@@ -27,6 +27,7 @@ struct FaceWeightArgs {
     const float* solid;        // solid SDF values [nsolid][512] (device)
     const int32_t* tab;        // [nleaf][8] solid leaf at origin + 8 (i, j, k), i,j,k in {0,1} (k fastest)
     float solid_bg;
+    float cut;                 // a clamped weight below this becomes 0 (0: none; the member variant: 1e-2)
 };
 
 // values: [nleaf][512][3] (device)
@@ -37,7 +38,7 @@ hipError_t launch_face_weights(int64_t nleaf, const uint64_t* active, const Face
 // tree order, masks) and, when values is non-null, the weights [nleaf][512][3]
 // in host memory. Returns 0, or a negative uaamg status with err set.
 int face_weights_device(const HostGrid& liquid, int64_t nsolid, const int32_t* solid_origins, const float* solid_values,
-                        float solid_bg, int device, std::vector<Coord>& origins, std::vector<uint64_t>& active,
-                        float* values, std::string& err);
+                        float solid_bg, float cut, int device, std::vector<Coord>& origins,
+                        std::vector<uint64_t>& active, float* values, std::string& err);
 
 }  // namespace uaamg

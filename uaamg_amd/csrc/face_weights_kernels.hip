@@ -45,9 +45,12 @@ __global__ __launch_bounds__(kBlockThreads) void face_weights_kernel(int64_t nle
         const float fu = fraction_inside4(s[0][0][0], s[0][1][0], s[0][0][1], s[0][1][1]);
         const float fv = fraction_inside4(s[0][0][0], s[0][0][1], s[1][0][0], s[1][0][1]);
         const float fw = fraction_inside4(s[0][0][0], s[1][0][0], s[0][1][0], s[1][1][0]);
-        out[0] = clamp01(1.0f - fu);
-        out[1] = clamp01(1.0f - fv);
-        out[2] = clamp01(1.0f - fw);
+        float w[3] = {clamp01(1.0f - fu), clamp01(1.0f - fv), clamp01(1.0f - fw)};
+        for (int c = 0; c < 3; ++c)
+            if (w[c] < a.cut) w[c] = 0.f;
+        out[0] = w[0];
+        out[1] = w[1];
+        out[2] = w[2];
     }
 }
 

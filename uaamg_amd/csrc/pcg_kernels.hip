@@ -8,6 +8,7 @@
 // the partials visible once, and the fixed order makes every result
 // reproducible run to run. Dot products accumulate in FP64.
 #include <cfloat>
+#include <cstdlib>
 #include <cstring>
 
 #include "batch_kernels.h"
@@ -316,13 +317,22 @@ __global__ __launch_bounds__(kBlockThreads) void update_kernel(LevelView L, cons
 }
 
 // ---------------------------------------------------------- coarsest solve
-constexpr int kCgThreads = 512;
+// Workgroup size of the coarsest CG: 512 threads by default; UAAMG_CG_THREADS = 256 or
+// 1024 selects another (a tuning knob: the reductions' partial sums then group
+// differently, so single and batched solves must run with the same setting).
+int cg_threads() {
+    static const int t = [] {
+        const char* e = std::getenv("UAAMG_CG_THREADS");
+        const int v = e ? std::atoi(e) : 512;
+        return (v == 256 || v == 1024) ? v : 512;
+    }();
+    return t;
+}
 
-
-template <class V, int kMaxDof>
-__global__ __launch_bounds__(kCgThreads) void coarsest_cg_kernel(CoarsestView C, const V* B, V* X) {
-    __shared__ CgShared<kCgThreads> sh;
-    coarsest_cg_block<kCgThreads, V, kMaxDof>(C, B, X, sh);
+template <class V, int kMaxDof, int kT>
+__global__ __launch_bounds__(kT) void coarsest_cg_kernel(CoarsestView C, const V* B, V* X) {
+    __shared__ CgShared<kT> sh;
+    coarsest_cg_block<kT, V, kMaxDof>(C, B, X, sh);
 }
 
 // r = b - A x with FP64 products and sums (fma(x, diag, s) of the operator
@@ -621,11 +631,35 @@ __global__ __launch_bounds__(kFinThreads) void bstatus_kernel(int nprob, const B
     mb->word = (uint64_t(q) << 32) | uint32_t(total);
 }
 
-template <class V, int kMaxDof>
-__global__ __launch_bounds__(kCgThreads) void bcoarsest_kernel(const CoarsestView* views, const V* B, V* X) {
-    __shared__ CgShared<kCgThreads> sh;
+template <class V, int kMaxDof, int kT>
+__global__ __launch_bounds__(kT) void bcoarsest_kernel(const CoarsestView* views, const V* B, V* X) {
+    __shared__ CgShared<kT> sh;
     const CoarsestView C = views[blockIdx.x];
-    coarsest_cg_block<kCgThreads, V, kMaxDof>(C, B, X, sh);
+    coarsest_cg_block<kT, V, kMaxDof>(C, B, X, sh);
+}
+
+// one workgroup per system (grid n), the widest system ndof rows; the 4096-row
+// form keeps at least 512 threads (16 rows a thread would not fit the registers)
+template <class V>
+hipError_t launch_cg_variant(bool batched, const CoarsestView* views, const CoarsestView& C, unsigned n, int ndof,
+                             const V* b, V* x, hipStream_t s) {
+    const int t = cg_threads();
+    const bool small = ndof <= kCgSmallDof;
+#define UAAMG_CG_LAUNCH(MAXDOF, T)                                                                 \
+    do {                                                                                          \
+        if (batched) bcoarsest_kernel<V, MAXDOF, T><<<n, T, 0, s>>>(views, b, x);                  \
+        else coarsest_cg_kernel<V, MAXDOF, T><<<1, T, 0, s>>>(C, b, x);                            \
+    } while (0)
+    if (small) {
+        if (t == 256) UAAMG_CG_LAUNCH(kCgSmallDof, 256);
+        else if (t == 1024) UAAMG_CG_LAUNCH(kCgSmallDof, 1024);
+        else UAAMG_CG_LAUNCH(kCgSmallDof, 512);
+    } else {
+        if (t == 1024) UAAMG_CG_LAUNCH(kMaxCoarsestDof, 1024);
+        else UAAMG_CG_LAUNCH(kMaxCoarsestDof, 512);
+    }
+#undef UAAMG_CG_LAUNCH
+    return hipGetLastError();
 }
 
 hipError_t finalize(int op, int n, const double* partials, PcgScalars* sc, HostMailbox* mb, hipStream_t s) {
@@ -640,17 +674,13 @@ int reduction_blocks(const LevelView& L) { return int(grid_for(L.nleaf)) > 0 ? i
 hipError_t launch_coarsest_cg(const CoarsestView& C, const float* b, float* x, hipStream_t s) {
     if (C.ndof <= 0) return hipSuccess;
     if (C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
-    if (C.ndof <= kCgSmallDof) coarsest_cg_kernel<float, kCgSmallDof><<<1, kCgThreads, 0, s>>>(C, b, x);
-    else coarsest_cg_kernel<float, kMaxCoarsestDof><<<1, kCgThreads, 0, s>>>(C, b, x);
-    return hipGetLastError();
+    return launch_cg_variant<float>(false, nullptr, C, 1, C.ndof, b, x, s);
 }
 
 hipError_t launch_coarsest_cg(const CoarsestView& C, const bf16* b, bf16* x, hipStream_t s) {
     if (C.ndof <= 0) return hipSuccess;
     if (C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
-    if (C.ndof <= kCgSmallDof) coarsest_cg_kernel<bf16, kCgSmallDof><<<1, kCgThreads, 0, s>>>(C, b, x);
-    else coarsest_cg_kernel<bf16, kMaxCoarsestDof><<<1, kCgThreads, 0, s>>>(C, b, x);
-    return hipGetLastError();
+    return launch_cg_variant<bf16>(false, nullptr, C, 1, C.ndof, b, x, s);
 }
 
 hipError_t launch_residual64_absmax(const LevelView& L, const float* x, const float* b, float* r, PcgScalars* sc,
@@ -825,17 +855,13 @@ hipError_t launch_bstatus(int nprob, const BatchPcg* st, unsigned int* seq, Host
 hipError_t launch_bcoarsest(const CoarsestView* views, int n, int max_ndof, const float* b, float* x, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     if (max_ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
-    if (max_ndof <= kCgSmallDof) bcoarsest_kernel<float, kCgSmallDof><<<unsigned(n), kCgThreads, 0, s>>>(views, b, x);
-    else bcoarsest_kernel<float, kMaxCoarsestDof><<<unsigned(n), kCgThreads, 0, s>>>(views, b, x);
-    return hipGetLastError();
+    return launch_cg_variant<float>(true, views, CoarsestView{}, unsigned(n), max_ndof, b, x, s);
 }
 
 hipError_t launch_bcoarsest(const CoarsestView* views, int n, int max_ndof, const bf16* b, bf16* x, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     if (max_ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
-    if (max_ndof <= kCgSmallDof) bcoarsest_kernel<bf16, kCgSmallDof><<<unsigned(n), kCgThreads, 0, s>>>(views, b, x);
-    else bcoarsest_kernel<bf16, kMaxCoarsestDof><<<unsigned(n), kCgThreads, 0, s>>>(views, b, x);
-    return hipGetLastError();
+    return launch_cg_variant<bf16>(true, views, CoarsestView{}, unsigned(n), max_ndof, b, x, s);
 }
 
 }  // namespace uaamg

@@ -90,6 +90,8 @@ enum ApplyMode : int {
     kModeRedGS = 4,
     kModeBlackGS = 5,
     kModeJacobiBf16 = 6,   // measurement only: the weighted Jacobi sweep of the BF16-stored W-cycle
+    kModeBlockPost = 7,    // measurement only: a coarse level's post-visit block kernel (prolongation + 3 sweeps)
+    kModeCoarsest = 8,     // measurement only: the coarsest-level CG (level = the coarsest)
 };
 
 struct LevelView {

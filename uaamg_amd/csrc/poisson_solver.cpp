@@ -88,7 +88,26 @@ void PoissonDevice::release_device() {
     for (void* p : spai0_bufs_) (void)hipFree(p);
     spai0_bufs_.clear();
     spai0_on_device_ = false;
+    if (aux_) (void)hipFree(aux_);
+    aux_ = nullptr;
+    aux_bytes_ = 0;
     arena_.release();
+}
+
+void* PoissonDevice::aux(size_t bytes) {
+    if (bytes <= aux_bytes_) return aux_;
+    if (stream_) (void)hipStreamSynchronize(stream_);   // no queued work still reads the old buffer
+    if (aux_) (void)hipFree(aux_);
+    aux_ = nullptr;
+    aux_bytes_ = 0;
+    const size_t grown = (bytes + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);   // whole MiB
+    if (hipMalloc(&aux_, grown) != hipSuccess) {
+        (void)hipGetLastError();
+        aux_ = nullptr;
+        return nullptr;
+    }
+    aux_bytes_ = grown;
+    return aux_;
 }
 
 void PoissonDevice::destroy_graphs() {
@@ -1203,7 +1222,13 @@ int PoissonDevice::precondition_dev(const float* r_dof, float* z_dof) {
 double PoissonDevice::algorithmic_bytes(int l, int mode) const {
     if (l < 0 || l >= nlevels()) return 0.0;
     const PoissonLevelHost& h = host_.levels[l];
+    const double sv = (mixed_ && l > 0) ? 2.0 : 4.0;   // bytes of a W-cycle level vector entry
+    if (mode == kModeCoarsest) {
+        // b in, x out, and the system's rows (6 neighbour indices, 8 coefficients, 1 position) once
+        return double(h.ndof) * (2.0 * sv + 24.0 + 32.0 + 4.0);
+    }
     double total = 0.0;
+    if (mode == kModeBlockPost && l + 1 < nlevels()) total += sv * double(host_.levels[l + 1].ndof);   // coarse x
     for (size_t i = 0; i < h.nleaf(); ++i) {
         const double nd = mask_count(&h.mask[i * 8]);
         const int kd = h.keep[kCoefDiag][i], kx = h.keep[kCoefX][i], ky = h.keep[kCoefY][i], kz = h.keep[kCoefZ][i];
@@ -1216,6 +1241,8 @@ double PoissonDevice::algorithmic_bytes(int l, int mode) const {
             case kModeJacobi: per = 12 + c + d; break;
             case kModeSpai0: per = 12 + c + 4.0 * ks; break;
             case kModeJacobiBf16: per = (l == 0 ? 8 : 6) + c + d; break;   // x, y at 2 B; b at 4 B on level 0
+            // fused prolongation + three sweeps: x in, b, x out, the coefficients once
+            case kModeBlockPost: per = 3.0 * sv + c + d; break;
             default: per = 12 + 4.0 * (kx + ky + kz) + d; break;   // one GS colour pass
         }
         total += nd * per;
@@ -1224,16 +1251,30 @@ double PoissonDevice::algorithmic_bytes(int l, int mode) const {
 }
 
 int PoissonDevice::time_apply(int l, int mode, int reps, float* ms) {
-    if (!on_device() || l < 0 || l >= nlevels() || reps <= 0 || mode < 0 || mode > kModeJacobiBf16) return -1;
+    if (!on_device() || l < 0 || l >= nlevels() || reps <= 0 || mode < 0 || mode > kModeCoarsest) return -1;
     if (mode == kModeJacobiBf16 && !mixed_) return -1;
+    if (mode == kModeBlockPost && (l == 0 || l + 1 >= nlevels() || !lv_[l].v.nb27)) return -1;
+    if (mode == kModeCoarsest && l != nlevels() - 1) return -1;
     if (mode == kModeSpai0 && ensure_spai0()) return -3;
     const LevelView& v = lv_[l].v;
     const float* X = l == 0 ? x0_ : lv_[l].X;
     const float* B = l == 0 ? b0_ : lv_[l].B;
     float* Y = staging_[2];
     DevLevel& d = lv_[l];
-    // the BF16 sweep as the mixed W-cycle runs it: level 0 reads the FP32 residual
+    // the BF16 sweep as the mixed W-cycle runs it: level 0 reads the FP32 residual; the block post-visit
+    // and the coarsest CG on the level's own scratch vectors (every solve overwrites them before use)
     auto launch = [&]() {
+        if (mode == kModeBlockPost) {
+            DevLevel& c = lv_[l + 1];
+            const float w0 = scheduled_weight(kSweeps, kSweeps - 1), w1 = scheduled_weight(kSweeps, kSweeps - 2),
+                        w2 = scheduled_weight(kSweeps, kSweeps - 3);
+            if (mixed_) return launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, w0, w1, w2, stream_);
+            return launch_block_post(v, d.T, c.X, d.B, d.X, w0, w1, w2, stream_);
+        }
+        if (mode == kModeCoarsest) {
+            if (mixed_) return launch_coarsest_cg(cv_, d.Bh, d.Xh, stream_);
+            return launch_coarsest_cg(cv_, d.B, d.X, stream_);
+        }
         if (mode != kModeJacobiBf16) return launch_apply(v, mode, X, B, Y, 0.8571f, stream_);
         if (l == 0) return launch_jacobi(v, d.Th, b0_, d.Uh, 0.8571f, stream_);
         return launch_jacobi(v, d.Th, d.Bh, d.Uh, 0.8571f, stream_);

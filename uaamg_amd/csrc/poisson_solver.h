@@ -82,6 +82,10 @@ public:
     // orders the solver's stream after the caller's null-stream work (own stream only)
     int follow_caller();
     int device() const { return opt_.device; }
+    // device memory for the steps either side of the solve (leaf tables, masks): grown on demand and
+    // kept until the device resources are released, so repeated calls allocate nothing; the caller
+    // has made the handle's device current. Returns null when the allocation fails.
+    void* aux(size_t bytes);
     // device view of a level (on_device() only)
     const LevelView& level_view(int l) const { return lv_[size_t(l)].v; }
     const std::string& error() const { return err_; }
@@ -171,6 +175,8 @@ private:
     hipStream_t stream_ = nullptr;
     CallerOrder caller_;
     bool own_stream_ = false;
+    void* aux_ = nullptr;
+    size_t aux_bytes_ = 0;
     DeviceArena arena_;
     std::vector<DevLevel> lv_;
     std::vector<float> sor_w_;

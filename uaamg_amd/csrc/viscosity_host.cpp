@@ -6,6 +6,8 @@
 #include <unordered_set>
 
 #include "poisson_host.h"
+#include "visc_volumes.h"
+#include "volume_fraction.h"
 
 namespace uaamg {
 
@@ -142,27 +144,6 @@ LeafSet dilate(const HostGrid& g, int k, bool nn26) {
     return out;
 }
 
-// volume_fractions.cpp restated (tetrahedron / prism fractions, util.h sort)
-inline float tet_frac(float p0, float p1, float p2, float p3) {
-    return p0 * p0 * p0 / ((p0 - p1) * (p0 - p2) * (p0 - p3));
-}
-inline float prism_frac(float p0, float p1, float p2, float p3) {
-    const float a = p0 / (p0 - p2), b = p0 / (p0 - p3), c = p1 / (p1 - p3), d = p1 / (p1 - p2);
-    return a * b * (1 - d) + b * (1 - c) * d + c * d;
-}
-float tet_volume(float p0, float p1, float p2, float p3) {
-    if (p0 > p1) std::swap(p0, p1);
-    if (p2 > p3) std::swap(p2, p3);
-    if (p0 > p2) std::swap(p0, p2);
-    if (p1 > p3) std::swap(p1, p3);
-    if (p1 > p2) std::swap(p1, p2);
-    if (p3 <= 0) return 1;
-    if (p2 <= 0) return 1 - tet_frac(p3, p2, p1, p0);
-    if (p1 <= 0) return prism_frac(p0, p1, p2, p3);
-    if (p0 <= 0) return tet_frac(p0, p1, p2, p3);
-    return 0;
-}
-
 // subcell fields: each liquid leaf (dilated) owns the 16^3 subcells of its volume
 constexpr int kSub = 16, kSubSize = kSub * kSub * kSub;
 struct SubField {
@@ -271,7 +252,7 @@ LeafSet cross_candidates(const BuildLevel& L, int c1, int c2) {
 
 // --------------------------------------------------------------- finest
 bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid& mu, float dt, float rho, float dx,
-                  BuildLevel& L, ViscGeometry& G) {
+                  int volume_device, BuildLevel& L, ViscGeometry& G, std::string& err) {
     L.dx = dx;
     G.dx = dx;
     G.dt = dt;
@@ -279,8 +260,18 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     G.mu = mu;
     G.mu.lookup.build(G.mu.origin);
 
-    // subcell SDF: corner s of the subcell lattice sits at liquid index (s-1)/2
     const LeafSet sub_ls = dilate(liquid, 1, true);
+    const LeafSet vol_ls = dilate(liquid, 2, true);
+    G.vol_origin = vol_ls.org;
+    G.vol_lookup.build(G.vol_origin);
+    const size_t nv = vol_ls.size();
+    if (volume_device >= 0) {
+        // the subcell SDF, subcell volumes and voxel / face / edge volumes on the device
+        if (visc_volumes_device(volume_device, liquid, sub_ls.org, vol_ls.org, dx, G.voxel_vol, G.face_vol,
+                                G.edge_vol, err) != 0)
+            return false;
+    } else {
+    // subcell SDF: corner s of the subcell lattice sits at liquid index (s-1)/2
     SubField sdf{&sub_ls, std::vector<float>(sub_ls.size() * kSubSize), 1.5f * dx};
     parallel_for(int64_t(sub_ls.size()), [&](int64_t b, int64_t e) {
         GridAcc acc{&liquid};
@@ -333,10 +324,6 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     });
 
     // voxel / face / edge volumes on the liquid leaves dilated twice
-    const LeafSet vol_ls = dilate(liquid, 2, true);
-    G.vol_origin = vol_ls.org;
-    G.vol_lookup.build(G.vol_origin);
-    const size_t nv = vol_ls.size();
     G.voxel_vol.assign(nv * kLeafSize, 0.f);
     for (int c = 0; c < 3; ++c) {
         G.face_vol[c].assign(nv * kLeafSize, 0.f);
@@ -378,6 +365,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
             }
         }
     });
+    }
     LeafSet vls;
     vls.org = G.vol_origin;
     vls.lk.build(vls.org);
@@ -816,23 +804,24 @@ void link_levels(ViscLevelHost& F, ViscLevelHost& C) {
 
 float visc_volume_fraction(float p000, float p100, float p010, float p110, float p001, float p101, float p011,
                            float p111) {
-    if (p000 < 0.f && p100 < 0.f && p010 < 0.f && p110 < 0.f && p001 < 0.f && p101 < 0.f && p011 < 0.f &&
-        p111 < 0.f)
-        return 1.0f;
-    return (tet_volume(p000, p001, p101, p011) + tet_volume(p000, p101, p100, p110) +
-            tet_volume(p000, p010, p011, p110) + tet_volume(p101, p011, p111, p110) +
-            2 * tet_volume(p000, p011, p101, p110) + tet_volume(p100, p101, p001, p111) +
-            tet_volume(p100, p001, p000, p010) + tet_volume(p100, p110, p111, p010) +
-            tet_volume(p001, p111, p011, p010) + 2 * tet_volume(p100, p111, p001, p010)) /
-           12;
+    return volume_fraction8(p000, p100, p010, p110, p001, p101, p011, p111);
 }
 
 bool build_visc_hierarchy(const HostGrid& liquid, const HostGrid& solid, const HostGrid& mu, float dt, float rho,
-                          float dx, int64_t max_coarsest_dof, bool do_trim, ViscHierarchyHost& out) {
-    if (liquid.ncomp != 1 || solid.ncomp != 1 || mu.ncomp != 1) return false;
+                          float dx, int64_t max_coarsest_dof, bool do_trim, ViscHierarchyHost& out, int volume_device,
+                          std::string* err) {
+    std::string e;
+    if (!err) err = &e;
+    if (liquid.ncomp != 1 || solid.ncomp != 1 || mu.ncomp != 1) {
+        *err = "grids must have one component";
+        return false;
+    }
     out = ViscHierarchyHost();
     std::vector<BuildLevel> B(1);
-    if (!build_finest(liquid, solid, mu, dt, rho, dx, B[0], out.geo)) return false;
+    if (!build_finest(liquid, solid, mu, dt, rho, dx, volume_device, B[0], out.geo, *err)) {
+        if (err->empty()) *err = "level-0 construction failed";
+        return false;
+    }
     while (B.back().ndof > max_coarsest_dof) {
         B.emplace_back();
         build_coarse(B[B.size() - 2], B.back());

@@ -17,6 +17,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "leaf_topology.h"
@@ -88,8 +89,11 @@ struct ViscHierarchyHost {
 
 // Builds every level. trim = false keeps level 0 untrimmed (inspection).
 // Returns false on bad input.
+// volume_device >= 0 runs the level-0 volume stage (subcell SDF, subcell and voxel / face / edge volumes)
+// on that HIP device, bit-identical to the host stage (-1); err receives the reason of a failure.
 bool build_visc_hierarchy(const HostGrid& liquid, const HostGrid& solid, const HostGrid& mu, float dt, float rho,
-                          float dx, int64_t max_coarsest_dof, bool trim, ViscHierarchyHost& out);
+                          float dx, int64_t max_coarsest_dof, bool trim, ViscHierarchyHost& out,
+                          int volume_device = -1, std::string* err = nullptr);
 
 // SPAI0 of one level (build_SPAI0_matrix / SPAI0_collector, :1453-1698).
 void build_visc_spai0(ViscLevelHost& L);
