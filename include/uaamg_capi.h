@@ -204,6 +204,22 @@ int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weig
                                    const uaamg_grid* vy, const uaamg_grid* vz, const uaamg_grid* solid_velocity,
                                    int32_t values_on_device, float* rhs_dev);
 
+/* Level-0 DOF leaves uaamg_poisson_build_rhs_device_noalloc takes at most. */
+#define UAAMG_RHS_NOALLOC_MAX_LEAVES 131072
+
+/*
+ * uaamg_poisson_build_rhs_device with no device allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory - code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc. Every grid's values must already be device memory (as with
+ * values_on_device = 1); the leaf tables go into a fixed scratch array of the library (10 MiB per
+ * GPU, held until the process ends) instead of a per-call allocation. Returns UAAMG_EINVAL when
+ * level 0 has more than UAAMG_RHS_NOALLOC_MAX_LEAVES DOF leaves. All calls on one GPU share that
+ * array: calls that overlap on one GPU (several host threads) race on it and must be serialised by
+ * the caller. Same result as uaamg_poisson_build_rhs_device, bit for bit.
+ */
+int uaamg_poisson_build_rhs_device_noalloc(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                                           const uaamg_grid* vy, const uaamg_grid* vz,
+                                           const uaamg_grid* solid_velocity, float* rhs_dev);
 
 /*
  * One face-velocity channel of the caller's grid, updated in place: leaves in any order, active

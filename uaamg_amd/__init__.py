@@ -125,6 +125,8 @@ def lib() -> C.CDLL:
         "uaamg_poisson_level_coefficients": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
         "uaamg_poisson_build_rhs": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), vp]),
         "uaamg_poisson_build_rhs_device": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid), i32, vp]),
+        "uaamg_poisson_build_rhs_device_noalloc": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(_Grid),
+                                                           vp]),
         "uaamg_face_weights": (C.c_int, [P(_Grid), P(_Grid), i32, P(i64), vp, vp, vp]),
         "uaamg_face_weights_ex": (C.c_int, [P(_Grid), P(_Grid), i32, i32, P(i64), vp, vp, vp]),
         "uaamg_poisson_apply_pressure_gradient_device": (C.c_int, [vp, vp, P(_Grid), P(_Grid), P(_Grid),

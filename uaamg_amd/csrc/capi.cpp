@@ -366,6 +366,21 @@ int uaamg_poisson_build_rhs_device(uaamg_poisson* h, const uaamg_grid* face_weig
     UAAMG_GUARD_END
 }
 
+int uaamg_poisson_build_rhs_device_noalloc(uaamg_poisson* h, const uaamg_grid* face_weights, const uaamg_grid* vx,
+                                           const uaamg_grid* vy, const uaamg_grid* vz,
+                                           const uaamg_grid* solid_velocity, float* rhs_dev) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !rhs_dev) return fail(UAAMG_EINVAL, "null argument");
+    const uaamg_grid* in[kRhsGrids] = {face_weights, vx, vy, vz, solid_velocity};
+    RhsGridRef refs[kRhsGrids];
+    if (int rc = rhs_refs(in, true, refs)) return rc;
+    std::string err;
+    const int rc = build_poisson_rhs_device_noalloc(h->dev, refs, rhs_dev, err);
+    if (rc < 0) return fail(rc == -4 ? UAAMG_ENODEV : (rc == -1 ? UAAMG_EINVAL : rc), err);
+    return 0;
+    UAAMG_GUARD_END
+}
+
 int uaamg_poisson_apply_pressure_gradient_device(uaamg_poisson* h, const float* pressure_dev,
                                                  const uaamg_grid* liquid_phi, const uaamg_grid* face_weights,
                                                  const uaamg_grid* solid_velocity, uaamg_face_channel* channels,

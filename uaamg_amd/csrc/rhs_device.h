@@ -57,4 +57,14 @@ class PoissonDevice;
 // Returns 0, or a negative uaamg status with err set.
 int build_poisson_rhs_device(PoissonDevice& d, const RhsGridRef* grids, float* rhs_dof, std::string& err);
 
+// Level-0 DOF leaves the allocation-free build takes at most (its leaf tables
+// live in a fixed device array of 80 bytes per leaf, 10 MiB).
+constexpr int64_t kRhsNoAllocMaxLeaves = 131072;
+
+// The same build with no device allocation: every grid's values must already
+// be device memory, and the leaf tables go into the library's fixed per-device
+// scratch array, which all calls on one GPU share (calls that overlap on one
+// GPU race on it). Returns 0, or a negative uaamg status with err set.
+int build_poisson_rhs_device_noalloc(PoissonDevice& d, const RhsGridRef* grids, float* rhs_dof, std::string& err);
+
 }  // namespace uaamg
