@@ -881,6 +881,7 @@ def run_viscosity(args, dev) -> dict:
         lat_mx.append(time.perf_counter() - ts)
     ms_mx = 1e3 * (time.perf_counter() - t_start) / args.visc_steps
     x_mx = x.cpu().numpy()
+    map_mx = lw_mx.kernel_map()
     del solver_mx, lw_mx
     # handle creation with the level-0 volume stage on the device (the default) and on the host threads
     # (flags bit 8), best of 2 each
@@ -907,7 +908,8 @@ def run_viscosity(args, dev) -> dict:
         "mixed": {"ms_per_solve": round(ms_mx, 3), "iterations": it_mx,
                   "p50_ms": round(float(np.percentile(np.array(lat_mx) * 1e3, 50)), 3),
                   "p99_ms": round(float(np.percentile(np.array(lat_mx) * 1e3, 99)), 3),
-                  "storage": "bf16 W-cycle level vectors below level 0; f32 level 0 and PCG, f64 dot products"},
+                  "storage": "bf16 W-cycle level vectors below level 0; f32 level 0 and PCG, f64 dot products",
+                  "level_kernel_map": map_mx},
         "speedup_mixed_vs_fp32_plain": round(ms / ms_mx, 3),
         "handle_create_s": create_s,
         "level_kernel_map": lw.kernel_map(),

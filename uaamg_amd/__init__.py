@@ -416,10 +416,12 @@ class LaplacianWithLevel:
     """
 
     def __init__(self, liquid_phi: Grid, face_weights: Grid, dt: float, dx: float, *, device: int = 0,
-                 max_coarsest_dof: int = 4000, flags: int = 0, precision: int = 0):
+                 max_coarsest_dof: int = 4000, flags: int = 0, precision: int = 0, smoother: int | None = None):
         L = lib()
         o = _Options()
         L.uaamg_options_default(C.byref(o))
+        if smoother is not None:
+            o.smoother = int(smoother)   # the smoother the handle is created with (PoissonSolver may change it)
         o.device = device
         o.max_coarsest_dof = max_coarsest_dof
         o.flags = flags

@@ -42,6 +42,8 @@ SOURCES = [
     "face_weights.cpp",
     "finest_kernels.hip",
     "finest_device.cpp",
+    "hierarchy_kernels.hip",
+    "hierarchy_device.cpp",
     "visc_volumes_kernels.hip",
     "visc_volumes.cpp",
 ]

@@ -18,6 +18,8 @@
 #include "rhs_device.h"
 #include "face_update.h"
 #include "face_weights.h"
+#include "hierarchy_device.h"
+#include "build_timing.h"
 #include "capi_common.h"
 #include "scene.h"
 
@@ -55,7 +57,9 @@ bool to_host_grid(const uaamg_grid* g, int ncomp, HostGrid& out, const char* wha
         g_err = std::string(what) + ": wrong component count";
         return false;
     }
-    if (!make_host_grid(g->nleaf, g->origins, g->active, g->values, g->ncomp, g->background, out)) {
+    // the grids built here live only for the entry point's call, so tree-ordered
+    // caller values are read in place
+    if (!make_host_grid(g->nleaf, g->origins, g->active, g->values, g->ncomp, g->background, out, true)) {
         g_err = std::string(what) + ": malformed leaves (origin not a multiple of 8, duplicate, or null arrays)";
         return false;
     }
@@ -119,7 +123,7 @@ int uaamg_options_default(uaamg_options* o) {
 namespace uaamg {
 namespace capi {
 int prepare_poisson(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
-                    const uaamg_options* opts, PoissonHierarchyHost& hier, SolverOptions& so) {
+                    const uaamg_options* opts, PoissonHierarchyHost& hier, SolverOptions& so, bool device_resident) {
     if (!(dx > 0.f) || !(dt > 0.f)) return fail(UAAMG_EINVAL, "dt and dx must be positive");
     uaamg_options o;
     uaamg_options_default(&o);
@@ -130,15 +134,22 @@ int prepare_poisson(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights
         return fail(UAAMG_EINVAL, "unsupported precision");
     const bool host_only = o.device < 0;   // hierarchy inspection without a GPU; solves fail
     if (!host_only && uaamg_device_count() <= o.device) return fail(UAAMG_ENODEV, "no HIP device");
+    StageClock clock;
     HostGrid phi, w;
     if (!to_host_grid(liquid_phi, 1, phi, "liquid_phi")) return UAAMG_EINVAL;
     if (!to_host_grid(face_weights, 3, w, "face_weights")) return UAAMG_EINVAL;
+    clock.lap("input grids to tree order");
     // the per-voxel level-0 stage runs on the handle's device (bit-identical to the host stage)
     std::string err;
-    const bool host_stage = host_only || (o.flags & 256);   // flags bit 8: per-voxel stage on the host threads
-    if (!build_hierarchy(phi, w, dt, dx, o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000, hier,
-                         host_stage ? -1 : o.device, &err))
+    const bool host_stage = host_only || (o.flags & 256);   // flags bit 8: the whole build on the host threads
+    const int64_t max_coarsest = o.max_coarsest_dof > 0 ? o.max_coarsest_dof : 4000;
+    if (device_resident && !host_stage) {
+        if (!build_hierarchy_device(phi, w, dt, dx, max_coarsest, hier, o.device, err))
+            return fail(UAAMG_EHIP, err.empty() ? "device hierarchy build failed" : err);
+    } else if (!build_hierarchy(phi, w, dt, dx, max_coarsest, hier, host_stage ? -1 : o.device, &err)) {
         return fail(err.empty() ? UAAMG_EINVAL : UAAMG_EHIP, err.empty() ? "bad grids" : err);
+    }
+    clock.lap("hierarchy");
     if (hier.levels[0].ndof == 0) return fail(UAAMG_EINVAL, "no degrees of freedom");
     so = SolverOptions{};
     so.relative_tolerance = o.relative_tolerance;
@@ -163,7 +174,7 @@ int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_we
     *out = nullptr;
     PoissonHierarchyHost hier;
     SolverOptions so;
-    if (int rc = capi::prepare_poisson(liquid_phi, face_weights, dt, dx, opts, hier, so)) return rc;
+    if (int rc = capi::prepare_poisson(liquid_phi, face_weights, dt, dx, opts, hier, so, true)) return rc;
     const bool host_only = so.device < 0;
     std::unique_ptr<uaamg_poisson> h(new uaamg_poisson());
     if (host_only) {
@@ -203,6 +214,8 @@ int uaamg_poisson_level_triplets(const uaamg_poisson* h, int level, int64_t* cou
                                  float* vals) {
     UAAMG_GUARD_BEGIN
     if (!h || !count || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad argument");
+    // a query of the explicit matrix reads the host bricks (copied from the device on first use)
+    if (const_cast<PoissonDevice&>(h->dev).ensure_host_coefficients()) return fail(UAAMG_EHIP, h->dev.error());
     std::vector<int32_t> r, c;
     std::vector<float> v;
     level_triplets(h->dev.host().levels[size_t(level)], r, c, v);
@@ -295,6 +308,8 @@ int uaamg_poisson_level_coefficients(const uaamg_poisson* h, int level, int kind
     // the SPAI0 coefficients are built on first use; a query is a use (the handle's
     // logical state does not change)
     if (kind == kCoefSpai0 && const_cast<PoissonDevice&>(h->dev).ensure_spai0())
+        return fail(UAAMG_EHIP, h->dev.error());
+    if (bricks && const_cast<PoissonDevice&>(h->dev).ensure_host_coefficients())
         return fail(UAAMG_EHIP, h->dev.error());
     const auto& l = h->dev.host().levels[level];
     if (bricks && !l.coef[kind].empty())

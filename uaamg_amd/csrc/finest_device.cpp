@@ -1,6 +1,8 @@
 // Host side of the device per-voxel level-0 build (finest_device.h).
 #include "finest_device.h"
 
+#include "poisson_host.h"
+
 namespace uaamg {
 
 namespace {
@@ -14,13 +16,31 @@ struct DevBuf {
 
 }  // namespace
 
-int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, float term, int device,
-                               std::vector<uint64_t>& dmask, std::vector<float>& cd, std::vector<float>* cxyz,
-                               std::string& err) {
+void finest_tables(const HostGrid& phi, const HostGrid& weights, std::vector<int32_t>& pnb,
+                   std::vector<int32_t>& wtab) {
     const size_t n = phi.nleaf();
-    dmask.assign(n * 8, 0);
-    cd.resize(n * kLeafSize);
-    for (int c = 0; c < 3; ++c) cxyz[c].resize(n * kLeafSize);
+    pnb.resize(n * 6);
+    wtab.resize(n * 4);
+    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+        for (int64_t li = b; li < e; ++li) {
+            const size_t i = size_t(li);
+            const Coord o = phi.origin[i];
+            for (int c = 0; c < 3; ++c) {
+                Coord lo = o, hi = o;
+                lo[c] -= 8;
+                hi[c] += 8;
+                pnb[i * 6 + size_t(2 * c)] = phi.lookup.find(lo);
+                pnb[i * 6 + size_t(2 * c + 1)] = phi.lookup.find(hi);
+                wtab[i * 4 + size_t(1 + c)] = weights.lookup.find(hi);
+            }
+            wtab[i * 4] = weights.lookup.find(o);
+        }
+    });
+}
+
+int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, float term, int device, uint64_t* dmask,
+                               float* cd, float* const cxyz[3], std::string& err) {
+    const size_t n = phi.nleaf();
     if (n == 0) return 0;
     if (hipSetDevice(device) != hipSuccess) {
         err = "hipSetDevice failed";
@@ -42,20 +62,8 @@ int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, flo
         err = std::string(what) + ": " + hipGetErrorString(e);
         return -3;
     };
-    // leaf tables: phi face neighbours, weight leaves at the origin and its + neighbours
-    std::vector<int32_t> pnb(n * 6), wtab(n * 4);
-    for (size_t i = 0; i < n; ++i) {
-        const Coord o = phi.origin[i];
-        for (int c = 0; c < 3; ++c) {
-            Coord lo = o, hi = o;
-            lo[c] -= 8;
-            hi[c] += 8;
-            pnb[i * 6 + size_t(2 * c)] = phi.lookup.find(lo);
-            pnb[i * 6 + size_t(2 * c + 1)] = phi.lookup.find(hi);
-            wtab[i * 4 + size_t(1 + c)] = weights.lookup.find(hi);
-        }
-        wtab[i * 4] = weights.lookup.find(o);
-    }
+    std::vector<int32_t> pnb, wtab;
+    finest_tables(phi, weights, pnb, wtab);
     const size_t brick = n * kLeafSize * sizeof(float);
     const size_t wbytes = weights.nleaf() * kLeafSize * 3 * sizeof(float);
     DevBuf phi_d, act_d, pnb_d, w_d, wtab_d, mask_d, out_d;
@@ -71,11 +79,11 @@ int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, flo
         void* dst;
         const void* src;
         size_t bytes;
-    } up[] = {{phi_d.p, phi.values.data(), brick},
+    } up[] = {{phi_d.p, phi.values, brick},
               {act_d.p, phi.active.data(), n * 8 * sizeof(uint64_t)},
               {pnb_d.p, pnb.data(), pnb.size() * sizeof(int32_t)},
               {wtab_d.p, wtab.data(), wtab.size() * sizeof(int32_t)},
-              {w_d.p, weights.values.data(), wbytes}};
+              {w_d.p, weights.values, wbytes}};
     for (const auto& u : up)
         if (u.bytes)
             if (hipError_t e = hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, s))
@@ -96,9 +104,9 @@ int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, flo
     a.cy = out + 2 * n * kLeafSize;
     a.cz = out + 3 * n * kLeafSize;
     if (hipError_t e = launch_finest(int64_t(n), a, s)) return hip_fail(e, "finest kernel");
-    if (hipError_t e = hipMemcpyAsync(dmask.data(), mask_d.p, n * 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s))
+    if (hipError_t e = hipMemcpyAsync(dmask, mask_d.p, n * 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, s))
         return hip_fail(e, "download");
-    float* dst[4] = {cd.data(), cxyz[0].data(), cxyz[1].data(), cxyz[2].data()};
+    float* dst[4] = {cd, cxyz[0], cxyz[1], cxyz[2]};
     for (int k = 0; k < 4; ++k)
         if (hipError_t e = hipMemcpyAsync(dst[k], out + size_t(k) * n * kLeafSize, brick, hipMemcpyDeviceToHost, s))
             return hip_fail(e, "download");

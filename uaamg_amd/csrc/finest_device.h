@@ -35,10 +35,15 @@ struct FinestArgs {
 
 hipError_t launch_finest(int64_t nphi, const FinestArgs& a, hipStream_t s);
 
-// phi, weights: tree-ordered host grids. Outputs sized [nphi][8] and [nphi][512] (x4).
-// Returns 0, or a negative uaamg status with err set.
-int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, float term, int device,
-                               std::vector<uint64_t>& dmask, std::vector<float>& cd, std::vector<float>* cxyz,
-                               std::string& err);
+// Leaf tables of the stage: pnb [nphi][6] as FinestArgs::pnb, wtab [nphi][4]
+// as FinestArgs::wtab.
+void finest_tables(const HostGrid& phi, const HostGrid& weights, std::vector<int32_t>& pnb,
+                   std::vector<int32_t>& wtab);
+
+// phi, weights: tree-ordered host grids. Outputs (caller arrays, every element
+// written): dmask [nphi][8], cd and cxyz[0..2] [nphi][512]. Returns 0, or a
+// negative uaamg status with err set.
+int finest_coefficients_device(const HostGrid& phi, const HostGrid& weights, float term, int device, uint64_t* dmask,
+                               float* cd, float* const cxyz[3], std::string& err);
 
 }  // namespace uaamg

@@ -15,7 +15,9 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 namespace uaamg {
@@ -69,6 +71,29 @@ inline uint64_t pack_origin(const Coord& o) {
            (uint64_t(uint32_t(o.z >> 3)) & m);
 }
 
+// Allocator whose resize() leaves new elements uninitialised (value-less
+// construction), for large bricks that are written in full right afterwards:
+// it skips a single-threaded zero fill of every page.
+template <class T>
+struct DefaultInitAllocator : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAllocator<U>;
+    };
+    DefaultInitAllocator() = default;
+    template <class U>
+    DefaultInitAllocator(const DefaultInitAllocator<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+using BrickVec = std::vector<float, DefaultInitAllocator<float>>;
+
 // Origin -> position lookup for a leaf list.
 class LeafLookup {
 public:
@@ -102,11 +127,34 @@ struct HostGrid {
     float background[3] = {0.f, 0.f, 0.f};
     std::vector<Coord> origin;
     std::vector<uint64_t> active;   // [nleaf][8]
-    std::vector<float> values;      // [nleaf][512][ncomp] (Vec3f leaves are AoS)
+    // [nleaf][512][ncomp] (Vec3f leaves are AoS): the caller's array read in
+    // place when make_host_grid may borrow it and its leaves are already in
+    // tree order, otherwise the grid's own copy in `owned`
+    const float* values = nullptr;
+    std::unique_ptr<float[]> owned;
     LeafLookup lookup;
 
+    HostGrid() = default;
+    HostGrid(HostGrid&&) = default;   // a moved vector keeps its buffer, so values stays valid
+    HostGrid& operator=(HostGrid&&) = default;
+    // a copy always owns its values (it may outlive a borrowed array)
+    HostGrid(const HostGrid& o) { *this = o; }
+    HostGrid& operator=(const HostGrid& o) {
+        if (this == &o) return *this;
+        ncomp = o.ncomp;
+        for (int c = 0; c < 3; ++c) background[c] = o.background[c];
+        origin = o.origin;
+        active = o.active;
+        lookup = o.lookup;
+        const size_t n = o.values ? o.origin.size() * kLeafSize * size_t(o.ncomp) : 0;
+        owned.reset(n ? new float[n] : nullptr);
+        if (n) std::copy(o.values, o.values + n, owned.get());
+        values = owned.get();
+        return *this;
+    }
+
     size_t nleaf() const { return origin.size(); }
-    const float* leaf_values(size_t i) const { return values.data() + i * kLeafSize * ncomp; }
+    const float* leaf_values(size_t i) const { return values + i * kLeafSize * ncomp; }
 
     float value(const Coord& v, int comp = 0) const {
         int32_t li = lookup.find(leaf_origin_of(v));
@@ -116,9 +164,12 @@ struct HostGrid {
     }
 };
 
-// Build a tree-ordered HostGrid from caller arrays in any leaf order.
-// Returns false on malformed input (origin not a multiple of 8, duplicates).
+// Build a tree-ordered HostGrid from caller arrays in any leaf order. With
+// borrow set and the leaves already in tree order the values are read in
+// place (the caller keeps them alive while the grid is used); otherwise they
+// are copied. Returns false on malformed input (origin not a multiple of 8,
+// duplicates).
 bool make_host_grid(int64_t nleaf, const int32_t* origins, const uint64_t* active, const float* values,
-                    int ncomp, const float* background, HostGrid& out);
+                    int ncomp, const float* background, HostGrid& out, bool borrow = false);
 
 }  // namespace uaamg

@@ -1,11 +1,13 @@
 // Host construction of the UAAMG Poisson level hierarchy (see poisson_host.h).
 #include "poisson_host.h"
 
+#include "build_timing.h"
 #include "finest_device.h"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <memory>
 #include <thread>
 
 namespace uaamg {
@@ -21,28 +23,48 @@ int host_threads() {
 }
 
 bool make_host_grid(int64_t nleaf, const int32_t* origins, const uint64_t* active, const float* values, int ncomp,
-                    const float* background, HostGrid& out) {
+                    const float* background, HostGrid& out, bool borrow) {
     if (nleaf < 0 || (nleaf > 0 && (!origins || !values)) || (ncomp != 1 && ncomp != 3)) return false;
+    out = HostGrid();
     out.ncomp = ncomp;
     for (int c = 0; c < 3; ++c) out.background[c] = background ? background[std::min(c, ncomp - 1)] : 0.f;
-    std::vector<int64_t> order(nleaf);
+    auto org = [&](int64_t i) { return Coord{origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]}; };
+    bool sorted = true;
     for (int64_t i = 0; i < nleaf; ++i) {
-        order[i] = i;
         const int32_t* o = origins + 3 * i;
         if ((o[0] & 7) || (o[1] & 7) || (o[2] & 7)) return false;
+        if (i > 0 && !tree_order_less(org(i - 1), org(i))) sorted = false;
     }
-    auto org = [&](int64_t i) { return Coord{origins[3 * i], origins[3 * i + 1], origins[3 * i + 2]}; };
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return tree_order_less(org(a), org(b)); });
-    out.origin.resize(nleaf);
-    out.active.assign(size_t(nleaf) * 8, ~uint64_t(0));
-    out.values.resize(size_t(nleaf) * kLeafSize * ncomp);
+    std::vector<int64_t> order;
+    if (!sorted) {
+        order.resize(size_t(nleaf));
+        for (int64_t i = 0; i < nleaf; ++i) order[size_t(i)] = i;
+        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return tree_order_less(org(a), org(b)); });
+    }
+    auto src = [&](int64_t k) { return sorted ? k : order[size_t(k)]; };
+    out.origin.resize(size_t(nleaf));
     for (int64_t k = 0; k < nleaf; ++k) {
-        int64_t i = order[k];
-        out.origin[k] = org(i);
-        if (k > 0 && out.origin[k] == out.origin[k - 1]) return false;
-        if (active) std::copy(active + 8 * i, active + 8 * i + 8, out.active.begin() + 8 * k);
-        std::copy(values + size_t(i) * kLeafSize * ncomp, values + size_t(i + 1) * kLeafSize * ncomp,
-                  out.values.begin() + size_t(k) * kLeafSize * ncomp);
+        out.origin[size_t(k)] = org(src(k));
+        if (k > 0 && out.origin[size_t(k)] == out.origin[size_t(k - 1)]) return false;
+    }
+    out.active.assign(size_t(nleaf) * 8, ~uint64_t(0));
+    if (active) {
+        if (sorted) std::copy(active, active + 8 * nleaf, out.active.begin());
+        else
+            for (int64_t k = 0; k < nleaf; ++k)
+                std::copy(active + 8 * src(k), active + 8 * src(k) + 8, out.active.begin() + 8 * k);
+    }
+    const size_t per = size_t(kLeafSize) * size_t(ncomp);
+    if (sorted && borrow) {
+        out.values = values;
+    } else {
+        out.owned.reset(new float[size_t(nleaf) * per]);
+        float* dst = out.owned.get();
+        parallel_for(nleaf, [&](int64_t b, int64_t e) {
+            for (int64_t k = b; k < e; ++k)
+                std::copy(values + size_t(src(k)) * per, values + size_t(src(k) + 1) * per, dst + size_t(k) * per);
+        });
+        out.values = dst;
     }
     out.lookup.build(out.origin);
     return true;
@@ -103,6 +125,73 @@ inline float fraction_inside2(float phi_left, float phi_right) {
     return 0;
 }
 
+
+// trimDefaultNodes (simd_vdb_poisson.cpp:1773-1811): a leaf whose active
+// values all lie within |eps| of the default is removed; afterwards every
+// voxel of that leaf reads the default.
+bool trim_leaf(const uint64_t* m, float* v, float default_val, float eps) {
+    float max_err = 0;
+    for (int off = 0; off < kLeafSize; ++off) {
+        if (!mask_test(m, off)) continue;
+        float err = std::fabs(v[off] - default_val);
+        if (err > max_err) max_err = err;
+    }
+    if (max_err <= eps) {
+        std::fill(v, v + kLeafSize, default_val);
+        return true;
+    }
+    return false;
+}
+
+void trim_kind(PoissonLevelHost& l, int kind, float default_val, float eps) {
+    eps = std::fabs(eps);
+    const size_t n = l.origin.size();
+    l.keep[kind].assign(n, 1);
+    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i)
+            if (trim_leaf(&l.mask[i * 8], &l.coef[kind][size_t(i) * kLeafSize], default_val, eps)) l.keep[kind][i] = 0;
+    });
+}
+
+// the four stencil kinds trimmed, then 1/diag (0 where diag == 0, the
+// default where the diagonal leaf is trimmed or the voxel holds no DOF), in
+// one pass over each leaf
+void trim_level(PoissonLevelHost& l) {
+    const float term = -l.dt / (l.dx * l.dx);
+    const float diagonal_term = -6.0f * term;
+    const float epsilon = 1e-5f;
+    const float dflt[4] = {diagonal_term, term, term, term};
+    const float eps[4] = {std::fabs(diagonal_term * epsilon), std::fabs(term * epsilon), std::fabs(term * epsilon),
+                          std::fabs(term * epsilon)};
+    const size_t n = l.origin.size();
+    const int kinds[4] = {kCoefDiag, kCoefX, kCoefY, kCoefZ};
+    for (int k : kinds) l.keep[k].assign(n, 1);
+    l.invdiag.resize(n * kLeafSize);
+    const float inv_dflt = l.default_invdiag();
+    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            const uint64_t* m = &l.mask[i * 8];
+            for (int j = 0; j < 4; ++j)
+                if (trim_leaf(m, &l.coef[kinds[j]][size_t(i) * kLeafSize], dflt[j], eps[j])) l.keep[kinds[j]][i] = 0;
+            float* inv = &l.invdiag[size_t(i) * kLeafSize];
+            const float* d = &l.coef[kCoefDiag][size_t(i) * kLeafSize];
+            if (!l.keep[kCoefDiag][i]) {
+                std::fill(inv, inv + kLeafSize, inv_dflt);
+                continue;
+            }
+            for (int off = 0; off < kLeafSize; ++off) {
+                if (!mask_test(m, off)) {
+                    inv[off] = inv_dflt;
+                    continue;
+                }
+                inv[off] = d[off] == 0.f ? 0.f : 1.0f / d[off];
+            }
+        }
+    });
+}
+
+}  // namespace
+
 void finalize_topology(PoissonLevelHost& l) {
     const size_t n = l.origin.size();
     l.lookup.build(l.origin);
@@ -125,67 +214,6 @@ void finalize_topology(PoissonLevelHost& l) {
     });
 }
 
-// trimDefaultNodes (simd_vdb_poisson.cpp:1773-1811): a leaf whose active
-// values all lie within |eps| of the default is removed; afterwards every
-// voxel of that leaf reads the default.
-void trim_kind(PoissonLevelHost& l, int kind, float default_val, float eps) {
-    eps = std::fabs(eps);
-    const size_t n = l.origin.size();
-    l.keep[kind].assign(n, 1);
-    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
-            const uint64_t* m = &l.mask[i * 8];
-            float* v = &l.coef[kind][size_t(i) * kLeafSize];
-            float max_err = 0;
-            for (int off = 0; off < kLeafSize; ++off) {
-                if (!mask_test(m, off)) continue;
-                float err = std::fabs(v[off] - default_val);
-                if (err > max_err) max_err = err;
-            }
-            if (max_err <= eps) {
-                l.keep[kind][i] = 0;
-                std::fill(v, v + kLeafSize, default_val);
-            }
-        }
-    });
-}
-
-void trim_level(PoissonLevelHost& l) {
-    float term = -l.dt / (l.dx * l.dx);
-    float diagonal_term = -6.0f * term;
-    float epsilon = 1e-5f;
-    trim_kind(l, kCoefDiag, diagonal_term, diagonal_term * epsilon);
-    trim_kind(l, kCoefX, term, term * epsilon);
-    trim_kind(l, kCoefY, term, term * epsilon);
-    trim_kind(l, kCoefZ, term, term * epsilon);
-}
-
-void build_invdiag(PoissonLevelHost& l) {
-    const size_t n = l.origin.size();
-    l.invdiag.resize(n * kLeafSize);
-    const float dflt = l.default_invdiag();
-    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
-            float* inv = &l.invdiag[size_t(i) * kLeafSize];
-            const float* d = &l.coef[kCoefDiag][size_t(i) * kLeafSize];
-            if (!l.keep[kCoefDiag][i]) {
-                std::fill(inv, inv + kLeafSize, dflt);
-                continue;
-            }
-            const uint64_t* m = &l.mask[i * 8];
-            for (int off = 0; off < kLeafSize; ++off) {
-                if (!mask_test(m, off)) {
-                    inv[off] = dflt;
-                    continue;
-                }
-                inv[off] = d[off] == 0.f ? 0.f : 1.0f / d[off];
-            }
-        }
-    });
-}
-
-}  // namespace
-
 float PoissonLevelHost::default_coef(int kind) const {
     switch (kind) {
         case kCoefDiag: return 6.f * term;
@@ -207,14 +235,18 @@ bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, 
     l.full_origin = phi.origin;
 
     // per phi leaf: DOF mask and the four coefficient bricks
-    std::vector<uint64_t> dmask(nphi * 8, 0);
-    std::vector<float> cd(nphi * kLeafSize), cxyz[3];
-    for (int c = 0; c < 3; ++c) cxyz[c].resize(nphi * kLeafSize);
+    StageClock clock;
+    // every element is written by the per-voxel stage (uninitialised until then)
+    std::unique_ptr<uint64_t[]> dmask(new uint64_t[nphi * 8 + 1]);
+    std::unique_ptr<float[]> cd(new float[nphi * kLeafSize + 1]), cxyz[3];
+    for (int c = 0; c < 3; ++c) cxyz[c].reset(new float[nphi * kLeafSize + 1]);
+    clock.lap("level 0: allocate");
 
     // the per-voxel stage on the device (finest_device.h) or on the host threads
     if (device >= 0) {
         std::string e;
-        if (finest_coefficients_device(phi, weights, dt_over_dx2, device, dmask, cd, cxyz, e) != 0) {
+        float* const out[3] = {cxyz[0].get(), cxyz[1].get(), cxyz[2].get()};
+        if (finest_coefficients_device(phi, weights, dt_over_dx2, device, dmask.get(), cd.get(), out, e) != 0) {
             if (err) *err = e;
             return false;
         }
@@ -227,6 +259,7 @@ bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, 
             const Coord o = phi.origin[li];
             const float* pv = phi.leaf_values(li);
             const uint64_t* am = &phi.active[li * 8];
+            std::fill(&dmask[li * 8], &dmask[li * 8] + 8, uint64_t(0));
             float* dleaf = &cd[li * kLeafSize];
             float* tleaf[3] = {&cxyz[0][li * kLeafSize], &cxyz[1][li * kLeafSize], &cxyz[2][li * kLeafSize]};
             std::fill(dleaf, dleaf + kLeafSize, 6.f * dt_over_dx2);
@@ -273,6 +306,7 @@ bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, 
         }
     });
 
+    clock.lap("level 0: per-voxel stage");
     // compact: keep leaves holding DOFs, in tree order
     std::vector<int64_t> keep_idx;
     keep_idx.reserve(nphi);
@@ -283,21 +317,26 @@ bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, 
     l.mask.resize(n * 8);
     for (int k = 0; k < kNumCoef; ++k) l.coef[k].clear();
     for (int k = 0; k < 4; ++k) l.coef[k].resize(n * kLeafSize);
-    for (size_t k = 0; k < n; ++k) {
-        const int64_t i = keep_idx[k];
-        l.origin[k] = phi.origin[i];
-        std::copy(&dmask[i * 8], &dmask[i * 8] + 8, &l.mask[k * 8]);
-        std::copy(&cd[i * kLeafSize], &cd[i * kLeafSize] + kLeafSize, &l.coef[kCoefDiag][k * kLeafSize]);
-        for (int c = 0; c < 3; ++c)
-            std::copy(&cxyz[c][i * kLeafSize], &cxyz[c][i * kLeafSize] + kLeafSize, &l.coef[1 + c][k * kLeafSize]);
-    }
+    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+        for (int64_t k = b; k < e; ++k) {
+            const int64_t i = keep_idx[size_t(k)];
+            l.origin[size_t(k)] = phi.origin[size_t(i)];
+            std::copy(&dmask[i * 8], &dmask[i * 8] + 8, &l.mask[size_t(k) * 8]);
+            std::copy(&cd[i * kLeafSize], &cd[i * kLeafSize] + kLeafSize, &l.coef[kCoefDiag][size_t(k) * kLeafSize]);
+            for (int c = 0; c < 3; ++c)
+                std::copy(&cxyz[c][i * kLeafSize], &cxyz[c][i * kLeafSize] + kLeafSize,
+                          &l.coef[1 + c][size_t(k) * kLeafSize]);
+        }
+    });
+    clock.lap("level 0: compact");
     finalize_topology(l);
+    clock.lap("level 0: topology");
     trim_level(l);
-    build_invdiag(l);
+    clock.lap("level 0: trim + invdiag");
     return true;
 }
 
-void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& l) {
+void build_coarse_topology(const PoissonLevelHost& fine, PoissonLevelHost& l) {
     l = PoissonLevelHost();
     l.level = fine.level + 1;
     l.dt = fine.dt;
@@ -350,12 +389,24 @@ void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& l) {
         }
     });
     finalize_topology(l);
+}
 
-    // Galerkin coefficients (initializeFromFineLevel, simd_vdb_poisson.cpp:392-467)
-    for (int k = 0; k < 4; ++k) l.coef[k].assign(n * kLeafSize, 0.f);
+void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& l) {
+    StageClock clock;
+    build_coarse_topology(fine, l);
+    const size_t n = l.nleaf();
+    clock.lap("coarse level: topology");
+    // Galerkin coefficients (initializeFromFineLevel, simd_vdb_poisson.cpp:392-467);
+    // every element is written below
+    for (int k = 0; k < 4; ++k) l.coef[k].resize(n * kLeafSize);
     const float dt_over_dx2 = l.term;
     parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
-        MaskAccessor self_axr(fine), nbr_axr[3] = {MaskAccessor(fine), MaskAccessor(fine), MaskAccessor(fine)};
+        // the 2^3 fine voxels of a coarse voxel lie in one child leaf (octant of
+        // the voxel); a fine voxel's -x/-y/-z neighbour lies in that leaf or in
+        // its face neighbour, so every test and read is a direct array access
+        auto on_at = [&](int32_t leaf, int lx, int ly, int lz) {
+            return leaf >= 0 && mask_test(&fine.mask[size_t(leaf) * 8], voxel_offset(lx, ly, lz));
+        };
         for (int64_t i = b; i < e; ++i) {
             float* dl = &l.coef[kCoefDiag][size_t(i) * kLeafSize];
             float* xl = &l.coef[kCoefX][size_t(i) * kLeafSize];
@@ -366,29 +417,33 @@ void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& l) {
             std::fill(yl, yl + kLeafSize, -dt_over_dx2);
             std::fill(zl, zl + kLeafSize, -dt_over_dx2);
             const uint64_t* m = &l.mask[i * 8];
-            const Coord o = l.origin[i];
             for (int off = 0; off < kLeafSize; ++off) {
                 if (!mask_test(m, off)) continue;
-                const Coord cg{o.x + (off >> 6), o.y + ((off >> 3) & 7), o.z + (off & 7)};
-                const Coord fb{cg.x * 2, cg.y * 2, cg.z * 2};
+                const int cx = off >> 6, cy = (off >> 3) & 7, cz = off & 7;
+                const int32_t fl = l.child[size_t(i) * 8 + size_t(((cx >= 4) << 2) | ((cy >= 4) << 1) | (cz >= 4))];
+                const int32_t nx = fl >= 0 ? fine.nbr[size_t(fl) * 6 + 0] : -1;
+                const int32_t ny = fl >= 0 ? fine.nbr[size_t(fl) * 6 + 2] : -1;
+                const int32_t nz = fl >= 0 ? fine.nbr[size_t(fl) * 6 + 4] : -1;
+                const int fx = 2 * (cx & 3), fy = 2 * (cy & 3), fz = 2 * (cz & 3);
                 float diag = 0, x = 0, y = 0, z = 0;
                 for (int ii = 0; ii < 2; ++ii)
                     for (int jj = 0; jj < 2; ++jj)
                         for (int kk = 0; kk < 2; ++kk) {
-                            const Coord fc{fb.x + ii, fb.y + jj, fb.z + kk};
-                            if (!self_axr.on(fc)) continue;
-                            diag += self_axr.coef(kCoefDiag, fc);
-                            if (nbr_axr[0].on(Coord{fc.x - 1, fc.y, fc.z})) {
-                                if (ii == 0) x += self_axr.coef(kCoefX, fc);
-                                else diag += 2 * self_axr.coef(kCoefX, fc);
+                            const int lx = fx + ii, ly = fy + jj, lz = fz + kk;
+                            if (!on_at(fl, lx, ly, lz)) continue;
+                            const size_t at = size_t(fl) * kLeafSize + size_t(voxel_offset(lx, ly, lz));
+                            diag += fine.coef[kCoefDiag][at];
+                            if (lx > 0 ? on_at(fl, lx - 1, ly, lz) : on_at(nx, 7, ly, lz)) {
+                                if (ii == 0) x += fine.coef[kCoefX][at];
+                                else diag += 2 * fine.coef[kCoefX][at];
                             }
-                            if (nbr_axr[1].on(Coord{fc.x, fc.y - 1, fc.z})) {
-                                if (jj == 0) y += self_axr.coef(kCoefY, fc);
-                                else diag += 2 * self_axr.coef(kCoefY, fc);
+                            if (ly > 0 ? on_at(fl, lx, ly - 1, lz) : on_at(ny, lx, 7, lz)) {
+                                if (jj == 0) y += fine.coef[kCoefY][at];
+                                else diag += 2 * fine.coef[kCoefY][at];
                             }
-                            if (nbr_axr[2].on(Coord{fc.x, fc.y, fc.z - 1})) {
-                                if (kk == 0) z += self_axr.coef(kCoefZ, fc);
-                                else diag += 2 * self_axr.coef(kCoefZ, fc);
+                            if (lz > 0 ? on_at(fl, lx, ly, lz - 1) : on_at(nz, lx, ly, 7)) {
+                                if (kk == 0) z += fine.coef[kCoefZ][at];
+                                else diag += 2 * fine.coef[kCoefZ][at];
                             }
                         }
                 const float factor = 0.5f * (1.0f / 8.0f);
@@ -399,8 +454,9 @@ void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& l) {
             }
         }
     });
+    clock.lap("coarse level: Galerkin");
     trim_level(l);
-    build_invdiag(l);
+    clock.lap("coarse level: trim + invdiag");
 }
 
 bool build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, float dx, int64_t max_coarsest_dof,
@@ -409,10 +465,12 @@ bool build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, flo
     out.nleaf_phi = int64_t(phi.nleaf());
     out.levels.emplace_back();
     if (!build_finest_level(phi, weights, dt, dx, out.levels.back(), device, err)) return false;
+    StageClock clock;
     while (out.levels.back().ndof > max_coarsest_dof) {
         PoissonLevelHost next;
         build_coarse_level(out.levels.back(), next);
         out.levels.push_back(std::move(next));
+        clock.lap("coarse level");
     }
     return true;
 }

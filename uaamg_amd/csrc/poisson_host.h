@@ -10,6 +10,7 @@
 
 #include <cstdint>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "leaf_topology.h"
@@ -40,9 +41,13 @@ struct PoissonLevelHost {
 
     // effective coefficient bricks [nleaf][512] and whether the reference keeps
     // that coefficient leaf (1) or trimmed it to the default (0)
-    std::vector<float> coef[kNumCoef];
+    BrickVec coef[kNumCoef];
     std::vector<uint8_t> keep[kNumCoef];
-    std::vector<float> invdiag;      // [nleaf][512], 1/diag (0 where diag == 0); default where diag trimmed
+    BrickVec invdiag;                // [nleaf][512], 1/diag (0 where diag == 0); default where diag trimmed
+
+    // set when the bricks (coef[0..3], invdiag) live only in the hierarchy's
+    // device store (build_hierarchy_device); the host arrays are then empty
+    bool coef_on_device = false;
 
     // aggregation links to the finer level (empty on level 0):
     // child[nleaf][8]: finer-level leaf index of octant (bx<<2|by<<1|bz), -1 absent
@@ -56,10 +61,29 @@ struct PoissonLevelHost {
     float default_spai0_apply() const { return 1.0f / (7.f * term); }
 };
 
+// Device copies of a hierarchy's coefficient bricks (full [nleaf][512]
+// arrays laid out as the host arrays, trimmed leaves holding the default),
+// one entry per level; levels kept on the host have null pointers. The
+// memory is released with the last owner.
+struct DeviceCoefStore {
+    struct Level {
+        float* coef[4] = {nullptr, nullptr, nullptr, nullptr};   // diag, x, y, z
+        float* invdiag = nullptr;
+    };
+    int device = -1;
+    std::vector<Level> levels;
+    std::vector<void*> blocks;   // allocations owned by the store
+    DeviceCoefStore() = default;
+    DeviceCoefStore(const DeviceCoefStore&) = delete;
+    DeviceCoefStore& operator=(const DeviceCoefStore&) = delete;
+    ~DeviceCoefStore();
+};
+
 struct PoissonHierarchyHost {
     std::vector<PoissonLevelHost> levels;
     int64_t nleaf_phi = 0;
     bool spai0_ready = false;
+    std::shared_ptr<DeviceCoefStore> dcoef;   // levels with coef_on_device
 };
 
 // Level 0 from the liquid phi and face-weight grids. device >= 0 runs the
@@ -70,6 +94,13 @@ bool build_finest_level(const HostGrid& phi, const HostGrid& weights, float dt, 
 
 // Next coarser level from a finer one.
 void build_coarse_level(const PoissonLevelHost& fine, PoissonLevelHost& out);
+// Its topology only: origins (TouchCoarseLeafReducer), child table, DOF masks
+// (a coarse voxel is a DOF iff one of its 2^3 fine voxels is), numbering and
+// face neighbours; no coefficients.
+void build_coarse_topology(const PoissonLevelHost& fine, PoissonLevelHost& out);
+// DOF numbering, leaf lookup and face-neighbour table of a level whose
+// origins and masks are set.
+void finalize_topology(PoissonLevelHost& l);
 
 // Full hierarchy: coarsen while ndof > max_coarsest_dof (device: as build_finest_level).
 bool build_hierarchy(const HostGrid& phi, const HostGrid& weights, float dt, float dx, int64_t max_coarsest_dof,

@@ -1,6 +1,9 @@
 // Device-resident UAAMG-PCG Poisson solver (see poisson_solver.h).
 #include "poisson_solver.h"
 
+#include "build_timing.h"
+#include "hierarchy_device.h"
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -64,9 +67,10 @@ void* DeviceArena::take(size_t bytes) {
     return p;
 }
 
-PoissonDevice::~PoissonDevice() { release_device(); }
+PoissonDevice::~PoissonDevice() { release_device(false); }
 
-void PoissonDevice::release_device() {
+void PoissonDevice::release_device(bool keep_host) {
+    if (keep_host && on_device()) (void)ensure_host_coefficients();
     if (stream_) (void)hipStreamSynchronize(stream_);
     destroy_graphs();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -232,7 +236,10 @@ int PoissonDevice::init(PoissonHierarchyHost&& h, const SolverOptions& opt) {
     }
     // SPAI0 coefficients only when that smoother is selected (ensure_spai0
     // builds and uploads them later when it is selected afterwards)
-    if (opt_.smoother == kSpai0) build_spai0(host_);
+    if (opt_.smoother == kSpai0) {
+        if (int rc = ensure_host_coefficients()) return rc;
+        build_spai0(host_);
+    }
     return acquire_device();
 }
 
@@ -303,11 +310,19 @@ int PoissonDevice::acquire_device() {
     sor_w_.assign(L, 1.2f);
     const size_t n0 = host_.levels[0].nleaf();
     bool tune_pending = false;
+    StageClock clock;
     if (check(arena_.reserve(device_bytes()), "hipMalloc")) return -2;
+    clock.lap("device: arena");
 
     lv_.resize(L);
     for (int l = 0; l < L; ++l)
         if (int rc = upload_level(host_.levels[l], lv_[l], true)) return rc;
+    if (host_.dcoef) {
+        // the packed arena arrays now hold the bricks; the build's full arrays go
+        if (check(hipStreamSynchronize(stream_), "pack sync")) return -3;
+        host_.dcoef.reset();
+    }
+    clock.lap("device: level upload");
     spai0_on_device_ = host_.spai0_ready;   // packed into the arena with the other coefficients
     // parent table of each finer level from the child table of the coarser one
     for (int l = 0; l + 1 < L; ++l) {
@@ -341,6 +356,7 @@ int PoissonDevice::acquire_device() {
         // the host vectors die at the end of this iteration
         if (check(hipStreamSynchronize(stream_), "upload parent")) return -3;
     }
+    clock.lap("device: leaf records");
     auto vec = [&](size_t nleaf) { return static_cast<float*>(arena_.take(nleaf * kLeafSize * sizeof(float))); };
     x0_ = vec(n0);
     b0_ = vec(n0);
@@ -423,9 +439,13 @@ int PoissonDevice::acquire_device() {
             }
         }
     }
+    if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
+    clock.lap("device: vectors and options");
     if (int rc = build_coarsest()) return rc;
+    clock.lap("device: coarsest system");
     if (tune_pending) tune_levels();
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
+    clock.lap("device: level mapping timing");
     return first_error_ == hipSuccess ? 0 : -3;
 }
 
@@ -521,7 +541,10 @@ void PoissonDevice::init_host_only(PoissonHierarchyHost&& h, const SolverOptions
 // and the leaf records that point to them. Captured graphs are dropped (the
 // next solve recaptures them with the new level views).
 int PoissonDevice::ensure_spai0() {
-    if (!host_.spai0_ready) build_spai0(host_);
+    if (!host_.spai0_ready) {
+        if (int rc = ensure_host_coefficients()) return rc;
+        build_spai0(host_);
+    }
     if (!on_device() || spai0_on_device_) return 0;
     for (int l = 0; l < nlevels(); ++l) {
         const PoissonLevelHost& h = host_.levels[size_t(l)];
@@ -553,6 +576,55 @@ int PoissonDevice::ensure_spai0() {
     spai0_on_device_ = true;
     destroy_graphs();
     return first_error_ == hipSuccess ? 0 : -3;
+}
+
+int PoissonDevice::ensure_host_coefficients() {
+    bool any = false;
+    for (const auto& l : host_.levels) any = any || l.coef_on_device;
+    if (!any) return 0;
+    if (host_.dcoef) {
+        std::string e;
+        if (!download_device_coefficients(host_, e)) {
+            err_ = e;
+            return -3;
+        }
+        return 0;
+    }
+    if (!on_device() || lv_.size() != host_.levels.size()) {
+        err_ = "coefficient bricks are not available";
+        return -1;
+    }
+    // from the packed arena arrays: kept bricks in slot order, the default elsewhere
+    for (size_t l = 0; l < host_.levels.size(); ++l) {
+        PoissonLevelHost& h = host_.levels[l];
+        if (!h.coef_on_device) continue;
+        const size_t n = h.nleaf();
+        const LevelView& v = lv_[l].v;
+        const float* src[5] = {v.diag, v.cx, v.cy, v.cz, v.invdiag};
+        const int kinds[5] = {kCoefDiag, kCoefX, kCoefY, kCoefZ, kCoefDiag};
+        for (int k = 0; k < 5; ++k) {
+            const std::vector<uint8_t>& keep = h.keep[kinds[k]];
+            size_t kept = 0;
+            for (size_t i = 0; i < n; ++i) kept += keep[i] ? 1 : 0;
+            BrickVec packed(std::max<size_t>(kept, 1) * kLeafSize);
+            if (kept && check(hipMemcpyAsync(packed.data(), src[k], kept * kLeafSize * sizeof(float),
+                                             hipMemcpyDeviceToHost, stream_),
+                              "download bricks"))
+                return -3;
+            if (check(hipStreamSynchronize(stream_), "download bricks")) return -3;
+            BrickVec& dst = k < 4 ? h.coef[kinds[k]] : h.invdiag;
+            dst.resize(n * kLeafSize);
+            const float dflt = k < 4 ? h.default_coef(kinds[k]) : h.default_invdiag();
+            size_t j = 0;
+            for (size_t i = 0; i < n; ++i) {
+                float* out = &dst[i * kLeafSize];
+                if (keep[i]) std::memcpy(out, &packed[(j++) * kLeafSize], kLeafSize * sizeof(float));
+                else std::fill(out, out + kLeafSize, dflt);
+            }
+        }
+        h.coef_on_device = false;
+    }
+    return 0;
 }
 
 // parent of every leaf of level l (coarse leaf * 8 + octant), empty on the coarsest level
@@ -613,24 +685,83 @@ int PoissonDevice::upload_level(const PoissonLevelHost& h, DevLevel& d, bool spa
     // coefficient bricks of kept leaves only
     // at least one brick per array (zeros when no leaf keeps one), so a kernel
     // may read slot 0 unconditionally and select the default afterwards
-    auto pack = [&](const std::vector<float>& src, int kind, const std::vector<uint8_t>& keep) -> const float* {
-        std::vector<float> buf(std::max<size_t>(size_t(count[kind]), 1) * kLeafSize, 0.f);
-        size_t j = 0;
-        for (size_t i = 0; i < n; ++i)
-            if (keep[i]) {
-                std::memcpy(&buf[j * kLeafSize], &src[i * kLeafSize], kLeafSize * sizeof(float));
-                ++j;
-            }
+    // bricks built on the device: packed there from the build's store
+    const DeviceCoefStore::Level* dsrc =
+        (h.coef_on_device && host_.dcoef && size_t(h.level) < host_.dcoef->levels.size() &&
+         host_.dcoef->levels[size_t(h.level)].coef[0])
+            ? &host_.dcoef->levels[size_t(h.level)]
+            : nullptr;
+    if (h.coef_on_device && !dsrc) {
+        err_ = "level bricks are neither on the host nor in the device store";
+        return -1;
+    }
+    int32_t* slot_d = nullptr;
+    if (dsrc) {
+        slot_d = static_cast<int32_t*>(aux(kNumCoef * std::max<size_t>(n, 1) * sizeof(int32_t)));
+        if (!slot_d) {
+            err_ = "device scratch allocation failed";
+            return -2;
+        }
+        std::vector<int32_t> slots(kNumCoef * n, -1);
+        for (int k = 0; k < 4; ++k) {
+            int32_t j = 0;
+            for (size_t i = 0; i < n; ++i)
+                if (h.keep[k][i]) slots[size_t(k) * n + i] = j++;
+        }
+        if (n && check(hipMemcpyAsync(slot_d, slots.data(), 4 * n * sizeof(int32_t), hipMemcpyHostToDevice, stream_),
+                       "upload slots"))
+            return -3;
+        (void)check(hipStreamSynchronize(stream_), "upload sync");   // the host table dies here
+    }
+    auto dpack = [&](const float* src, int kind) -> const float* {
+        float* p = static_cast<float*>(arena_.take(std::max<size_t>(size_t(count[kind]), 1) * kLeafSize * sizeof(float)));
+        if (!p) return nullptr;
+        if (count[kind] == 0)
+            (void)check(hipMemsetAsync(p, 0, kLeafSize * sizeof(float), stream_), "memset");
+        else
+            (void)check(launch_pack_bricks(int32_t(n), slot_d + size_t(kind) * n, src, p, stream_), "pack bricks");
+        return p;
+    };
+    auto pack = [&](const BrickVec& src, int kind, const std::vector<uint8_t>& keep) -> const float* {
+        if (size_t(count[kind]) == n && n > 0) {
+            // every leaf keeps its brick: upload the array as it is
+            const float* p = static_cast<const float*>(up(src.data(), n * kLeafSize * sizeof(float)));
+            (void)check(hipStreamSynchronize(stream_), "upload sync");
+            return p;
+        }
+        BrickVec buf(std::max<size_t>(size_t(count[kind]), 1) * kLeafSize);
+        if (count[kind] == 0) std::fill(buf.begin(), buf.end(), 0.f);
+        std::vector<int64_t> dst(n);
+        int64_t j = 0;
+        for (size_t i = 0; i < n; ++i) dst[i] = keep[i] ? j++ : -1;
+        parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+            for (int64_t i = b; i < e; ++i)
+                if (dst[size_t(i)] >= 0)
+                    std::memcpy(&buf[size_t(dst[size_t(i)]) * kLeafSize], &src[size_t(i) * kLeafSize],
+                                kLeafSize * sizeof(float));
+        });
         const float* p = static_cast<const float*>(up(buf.data(), buf.size() * sizeof(float)));
         // the host buffer must outlive the async copy
         (void)check(hipStreamSynchronize(stream_), "upload sync");
         return p;
     };
-    v.diag = pack(h.coef[kCoefDiag], kCoefDiag, h.keep[kCoefDiag]);
-    v.invdiag = pack(h.invdiag, kCoefDiag, h.keep[kCoefDiag]);
-    v.cx = pack(h.coef[kCoefX], kCoefX, h.keep[kCoefX]);
-    v.cy = pack(h.coef[kCoefY], kCoefY, h.keep[kCoefY]);
-    v.cz = pack(h.coef[kCoefZ], kCoefZ, h.keep[kCoefZ]);
+    if (dsrc) {
+        v.diag = dpack(dsrc->coef[0], kCoefDiag);
+        v.invdiag = dpack(dsrc->invdiag, kCoefDiag);
+        v.cx = dpack(dsrc->coef[1], kCoefX);
+        v.cy = dpack(dsrc->coef[2], kCoefY);
+        v.cz = dpack(dsrc->coef[3], kCoefZ);
+        if (!v.diag || !v.invdiag || !v.cx || !v.cy || !v.cz) {
+            err_ = "arena too small";
+            return -2;
+        }
+    } else {
+        v.diag = pack(h.coef[kCoefDiag], kCoefDiag, h.keep[kCoefDiag]);
+        v.invdiag = pack(h.invdiag, kCoefDiag, h.keep[kCoefDiag]);
+        v.cx = pack(h.coef[kCoefX], kCoefX, h.keep[kCoefX]);
+        v.cy = pack(h.coef[kCoefY], kCoefY, h.keep[kCoefY]);
+        v.cz = pack(h.coef[kCoefZ], kCoefZ, h.keep[kCoefZ]);
+    }
     if (spai0 && host_.spai0_ready) v.spai0 = pack(h.coef[kCoefSpai0], kCoefSpai0, h.keep[kCoefSpai0]);
     d.X = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));
     d.B = static_cast<float*>(arena_.take(n * kLeafSize * sizeof(float)));

@@ -73,7 +73,9 @@ public:
     // freeing every device resource while keeping the host hierarchy
     size_t device_bytes() const;
     int acquire_device();
-    void release_device();
+    // keep_host: first copy device-held coefficient bricks to the host, so a
+    // later acquire_device() can upload them again
+    void release_device(bool keep_host = true);
 
     int nlevels() const { return int(host_.levels.size()); }
     const PoissonHierarchyHost& host() const { return host_; }
@@ -127,6 +129,10 @@ public:
     float last_true_residual() const { return true_nu_; }
     // SPAI0 coefficients, built (host) and uploaded (device) on first use
     int ensure_spai0();
+    // Copies every level's coefficient bricks to the host hierarchy where they
+    // live only on the device (from the build's device store, or from the
+    // packed arrays of the resident hierarchy); 0 or negative with error() set.
+    int ensure_host_coefficients();
     int last_replacements() const { return replacements_; }
 
 private:

@@ -70,7 +70,7 @@ int visc_volumes_device(int device, const HostGrid& liquid, const std::vector<Co
         if (hipError_t e = hipMalloc(&m.p, bytes > 0 ? bytes : 4)) return e;
         return bytes ? hipMemcpyAsync(m.p, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
     };
-    if (hipError_t e = upload(liq_d, liquid.values.data(), nliq * kLeafSize * sizeof(float))) return hip_fail(e, "liquid");
+    if (hipError_t e = upload(liq_d, liquid.values, nliq * kLeafSize * sizeof(float))) return hip_fail(e, "liquid");
     if (hipError_t e = upload(liq_nb_d, liq_nb.data(), liq_nb.size() * sizeof(int32_t))) return hip_fail(e, "tables");
     if (hipError_t e = upload(sub_up_d, sub_up.data(), sub_up.size() * sizeof(int32_t))) return hip_fail(e, "tables");
     if (hipError_t e = upload(vol_nb_d, vol_nb.data(), vol_nb.size() * sizeof(int32_t))) return hip_fail(e, "tables");

@@ -1,5 +1,7 @@
 #include "viscosity_host.h"
 
+#include "build_timing.h"
+
 #include <algorithm>
 #include <cmath>
 #include <unordered_map>
@@ -257,6 +259,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     G.dx = dx;
     G.dt = dt;
     G.rho = rho;
+    StageClock clock;
     G.mu = mu;
     G.mu.lookup.build(G.mu.origin);
 
@@ -366,6 +369,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
         }
     });
     }
+    clock.lap("viscosity level 0: volume stage");
     LeafSet vls;
     vls.org = G.vol_origin;
     vls.lk.build(vls.org);
@@ -428,6 +432,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
         }
     }
 
+    clock.lap("viscosity level 0: DOF marking + numbering");
     // coefficients (finest_level_matrix_setter, :539-908)
     const float factor = dt / (rho * dx * dx);
     L.default_off = factor * mu.background[0];
@@ -816,24 +821,30 @@ bool build_visc_hierarchy(const HostGrid& liquid, const HostGrid& solid, const H
         *err = "grids must have one component";
         return false;
     }
+    StageClock clock;
     out = ViscHierarchyHost();
     std::vector<BuildLevel> B(1);
     if (!build_finest(liquid, solid, mu, dt, rho, dx, volume_device, B[0], out.geo, *err)) {
         if (err->empty()) *err = "level-0 construction failed";
         return false;
     }
+    clock.lap("viscosity level 0");
     while (B.back().ndof > max_coarsest_dof) {
         B.emplace_back();
         build_coarse(B[B.size() - 2], B.back());
         if (B.size() > 40) return false;
+        clock.lap("viscosity coarse level");
     }
     if (do_trim) trim(B[0]);
     else drop_empty(B[0]);
     for (size_t l = 1; l < B.size(); ++l) drop_empty(B[l]);
+    clock.lap("viscosity trim");
     out.levels.resize(B.size());
     for (size_t l = 0; l < B.size(); ++l) finalize(B[l], int(l), dt, rho, out.levels[l]);
     for (size_t l = 0; l + 1 < B.size(); ++l) link_levels(out.levels[l], out.levels[l + 1]);
+    clock.lap("viscosity finalize + links");
     visc_explicit_matrix(out.levels.back(), out.cptr, out.ccol, out.cval);
+    clock.lap("viscosity coarsest matrix");
     return true;
 }
 

@@ -1,6 +1,8 @@
 // Device-resident variational viscosity solver (see viscosity_solver.h).
 #include "viscosity_solver.h"
 
+#include "build_timing.h"
+
 #include <algorithm>
 #include <cfloat>
 #include <cstdlib>
@@ -269,13 +271,16 @@ int ViscDevice::acquire_device() {
     const ViscLevelHost& l0 = host_.levels[0];
     const size_t vec0 = l0.nleaf_dof * 3 * 512 * 4;
     const size_t ndof0 = size_t(l0.ndof);
+    StageClock clock;
     if (hipError_t e = arena_.reserve(device_bytes())) {
         err_ = std::string("device allocation: ") + hipGetErrorString(e);
         return e == hipErrorOutOfMemory ? -2 : -3;
     }
+    clock.lap("viscosity device: arena");
     lv_.resize(L);
     for (int l = 0; l < L; ++l)
         if (int rc = upload_level(host_.levels[l], lv_[l])) return rc;
+    clock.lap("viscosity device: level upload");
     auto take = [&](size_t b) { return static_cast<float*>(arena_.take(std::max<size_t>(b, 4))); };
     x0_ = take(vec0);
     b0_ = take(vec0);
@@ -298,8 +303,11 @@ int ViscDevice::acquire_device() {
     mailbox_->word = 0;
     if (check(hipHostGetDevicePointer(reinterpret_cast<void**>(&mailbox_dev_), mailbox_, 0), "mailbox map")) return -3;
     if (int rc = build_coarsest()) return rc;
+    if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
+    clock.lap("viscosity device: vectors + coarsest");
     if (tune) tune_levels();
     if (check(hipStreamSynchronize(stream_), "init sync")) return -3;
+    clock.lap("viscosity device: level mapping timing");
     return first_error_ == hipSuccess ? 0 : -3;
 }
 
