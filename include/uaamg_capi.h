@@ -96,8 +96,9 @@ typedef struct uaamg_options {
                                  at FP32 precision; bit 7 = pure-multigrid fallback in the merged batch
                                  and the solve server: a problem whose PCG returns FAILED is zeroed and
                                  solved by solvePureMultigrid, as FLIP_vdb.cpp:3150-3155 does; bit 8 =
-                                 level-0 per-voxel build stage on the host threads instead of the
-                                 device (same result) */
+                                 the whole hierarchy build on the host threads instead of the device
+                                 (same result: by default uaamg_poisson_create builds every level on
+                                 the device and keeps the coefficient bricks there) */
     void* stream;             /* hipStream_t to run on; NULL = a stream owned by the handle, which each device
                                  entry point first orders after the work queued on the null stream */
 } uaamg_options;
@@ -422,7 +423,8 @@ typedef struct uaamg_viscosity_options {
     int32_t device;           /* HIP device ordinal; -1 = host-only handle (inspection, no solves) */
     int32_t max_coarsest_dof; /* coarsen while ndof > this (4000, :3638) */
     int32_t flags;            /* 0 normally; bit 1 = no graph replay (A/B checks); bit 3 = keep level 0 untrimmed;
-                                 bit 5 = small coarse levels as single-workgroup half-visits; bit 8 = level-0
+                                 bit 5 = small coarse levels (widest channel <= 16 leaves) as single-workgroup
+                                 half-visits instead of the per-pass launches (the default); bit 8 = level-0
                                  volume stage (subcell SDF, subcell / voxel / face / edge volumes) on the host
                                  threads instead of the device (same result) */
     int32_t precision;        /* UAAMG_PRECISION_FP32 (default, the reference's arithmetic) or

@@ -94,21 +94,120 @@ struct DefaultInitAllocator : std::allocator<T> {
 };
 using BrickVec = std::vector<float, DefaultInitAllocator<float>>;
 
-// Origin -> position lookup for a leaf list.
+// Origin -> position lookup for a leaf list: an open-addressing table
+// (linear probing, multiplicative hash of the packed origin, load <= 1/2).
+// The first of duplicate origins wins.
 class LeafLookup {
 public:
     void build(const std::vector<Coord>& origins) {
-        map_.clear();
-        map_.reserve(origins.size() * 2 + 1);
-        for (size_t i = 0; i < origins.size(); ++i) map_.emplace(pack_origin(origins[i]), int32_t(i));
+        int bits = 4;
+        while ((size_t(1) << bits) < origins.size() * 2) ++bits;
+        shift_ = 64 - bits;
+        mask_ = (size_t(1) << bits) - 1;
+        keys_.assign(mask_ + 1, kEmpty);
+        vals_.assign(mask_ + 1, -1);
+        for (size_t i = 0; i < origins.size(); ++i) {
+            const uint64_t k = pack_origin(origins[i]);
+            size_t h = slot(k);
+            while (keys_[h] != kEmpty && keys_[h] != k) h = (h + 1) & mask_;
+            if (keys_[h] == kEmpty) {
+                keys_[h] = k;
+                vals_[h] = int32_t(i);
+            }
+        }
     }
     int32_t find(const Coord& origin) const {
-        auto it = map_.find(pack_origin(origin));
-        return it == map_.end() ? -1 : it->second;
+        if (keys_.empty()) return -1;
+        const uint64_t k = pack_origin(origin);
+        for (size_t h = slot(k);; h = (h + 1) & mask_) {
+            const uint64_t q = keys_[h];
+            if (q == k) return vals_[h];
+            if (q == kEmpty) return -1;
+        }
     }
 
 private:
-    std::unordered_map<uint64_t, int32_t> map_;
+    // packed origins use 63 bits, so the all-ones word marks an empty slot
+    static constexpr uint64_t kEmpty = ~uint64_t(0);
+    size_t slot(uint64_t k) const { return size_t((k * 0x9E3779B97F4A7C15ull) >> shift_); }
+    std::vector<uint64_t> keys_;
+    std::vector<int32_t> vals_;
+    size_t mask_ = 0;
+    int shift_ = 60;
+};
+
+// Packed origin -> int32 map with insertion (open addressing, linear probing,
+// grows at load 1/2); emplace keeps the first value of a key.
+class OriginMap {
+public:
+    // (value, inserted)
+    std::pair<int32_t, bool> emplace(uint64_t k, int32_t v) {
+        if ((count_ + 1) * 2 > keys_.size()) grow();
+        size_t h = slot(k);
+        while (keys_[h] != kEmpty) {
+            if (keys_[h] == k) {
+                if (vals_[h] >= 0) return {vals_[h], false};
+                vals_[h] = v;   // a removed key comes back
+                return {v, true};
+            }
+            h = (h + 1) & mask_;
+        }
+        keys_[h] = k;
+        vals_[h] = v;
+        ++count_;
+        return {v, true};
+    }
+    int32_t find(uint64_t k) const {
+        if (keys_.empty()) return -1;
+        for (size_t h = slot(k);; h = (h + 1) & mask_) {
+            const uint64_t q = keys_[h];
+            if (q == k) return vals_[h];
+            if (q == kEmpty) return -1;
+        }
+    }
+    // values are >= 0; a removed key keeps its slot (so probes pass it) with value -1
+    void remove(uint64_t k) {
+        if (keys_.empty()) return;
+        for (size_t h = slot(k);; h = (h + 1) & mask_) {
+            if (keys_[h] == k) {
+                vals_[h] = -1;
+                return;
+            }
+            if (keys_[h] == kEmpty) return;
+        }
+    }
+    // f(key, value) for every present key, in slot order
+    template <class F>
+    void for_each(F&& f) const {
+        for (size_t h = 0; h < keys_.size(); ++h)
+            if (keys_[h] != kEmpty && vals_[h] >= 0) f(keys_[h], vals_[h]);
+    }
+
+private:
+    static constexpr uint64_t kEmpty = ~uint64_t(0);
+    size_t slot(uint64_t k) const { return size_t((k * 0x9E3779B97F4A7C15ull) >> shift_); }
+    void grow() {
+        std::vector<uint64_t> ok;
+        std::vector<int32_t> ov;
+        ok.swap(keys_);
+        ov.swap(vals_);
+        const int bits = ok.empty() ? 4 : 64 - shift_ + 1;
+        shift_ = 64 - bits;
+        mask_ = (size_t(1) << bits) - 1;
+        keys_.assign(mask_ + 1, kEmpty);
+        vals_.assign(mask_ + 1, -1);
+        for (size_t i = 0; i < ok.size(); ++i) {
+            if (ok[i] == kEmpty) continue;
+            size_t h = slot(ok[i]);
+            while (keys_[h] != kEmpty) h = (h + 1) & mask_;
+            keys_[h] = ok[i];
+            vals_[h] = ov[i];
+        }
+    }
+    std::vector<uint64_t> keys_;
+    std::vector<int32_t> vals_;
+    size_t count_ = 0, mask_ = 0;
+    int shift_ = 60;
 };
 
 inline bool mask_test(const uint64_t* mask8, int offset) { return (mask8[offset >> 6] >> (offset & 63)) & 1u; }

@@ -5,6 +5,9 @@
 #include "hierarchy_device.h"
 
 #include <algorithm>
+#include <tuple>
+#include <mutex>
+#include <map>
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
@@ -499,20 +502,52 @@ void PoissonDevice::visit_pair(int l, bool block) {
     }
 }
 
+namespace {
+// Timings of the two level mappings, kept per process and reused by handles
+// whose level has about the same size (FLIP rebuilds its hierarchy every
+// substep with nearly the same level sizes). Key: device, precision, level,
+// and the leaf count in buckets of 2^(1/8) (about 9%). Either mapping gives
+// bit-identical results, so a reused choice changes only the speed.
+struct TuneKey {
+    int device, mixed, level, bucket;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(device, mixed, level, bucket) < std::tie(o.device, o.mixed, o.level, o.bucket);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<float, float>> g_tune_cache;
+}  // namespace
+
 // Times both kernel mappings of every candidate coarse level with HIP events
 // on the solver's stream and keeps the faster. The scratch vectors hold zeros
-// before and after (zero in, zero out), so no solve state is touched.
+// before and after (zero in, zero out), so no solve state is touched. Timings
+// come from the process-wide cache when a level of about the same size was
+// timed before (UAAMG_TUNE_CACHE=0 always times).
 void PoissonDevice::tune_levels() {
     const int L = nlevels();
     tune_ms_[0].assign(size_t(L), -1.f);
     tune_ms_[1].assign(size_t(L), -1.f);
     const bool jacobi = opt_.smoother == kScheduledRelaxedJacobi || opt_.smoother == kWeightedJacobi;
     if (!fuse_ || !jacobi) return;
+    const char* env = std::getenv("UAAMG_TUNE_CACHE");
+    const bool cache = !(env && env[0] == '0');
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return;
     constexpr int kReps = 8;
     for (int l = 1; l + 1 < L; ++l) {
         if (!use_block_[size_t(l)] || l + 1 == tail_level_) continue;
+        const TuneKey key{opt_.device, mixed_ ? 1 : 0, l,
+                          int(std::floor(std::log2(double(std::max<size_t>(host_.levels[size_t(l)].nleaf(), 1))) * 8))};
+        if (cache) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                tune_ms_[0][size_t(l)] = it->second.first;
+                tune_ms_[1][size_t(l)] = it->second.second;
+                use_block_[size_t(l)] = it->second.first < it->second.second ? 1 : 0;
+                continue;
+            }
+        }
         float ms[2] = {0.f, 0.f};
         for (int variant = 0; variant < 2; ++variant) {
             visit_pair(l, variant == 0);   // warm-up
@@ -526,6 +561,10 @@ void PoissonDevice::tune_levels() {
         tune_ms_[0][size_t(l)] = ms[0];
         tune_ms_[1][size_t(l)] = ms[1];
         use_block_[size_t(l)] = ms[0] < ms[1] ? 1 : 0;
+        if (cache) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune_cache[key] = {ms[0], ms[1]};
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -163,21 +163,25 @@ struct SubField {
 
 // coefficient bricks of a level under construction
 struct CoefStore {
-    std::unordered_map<uint64_t, int32_t> map[kVNumGrids];
-    std::vector<float> pool;
+    OriginMap map[kVNumGrids];
+    BrickVec pool;
     std::vector<uint64_t> on;
+    int32_t nbricks = 0;
+    // registers a brick (its storage comes with allocate())
     int32_t add(int gid, const Coord& o) {
-        auto r = map[gid].emplace(pack_origin(o), int32_t(on.size() / 8));
-        if (r.second) {
-            pool.resize(pool.size() + kLeafSize, 0.f);
-            on.resize(on.size() + 8, 0);
-        }
-        return r.first->second;
+        auto r = map[gid].emplace(pack_origin(o), nbricks);
+        if (r.second) ++nbricks;
+        return r.first;
     }
-    int32_t find(int gid, const Coord& o) const {
-        auto it = map[gid].find(pack_origin(o));
-        return it == map[gid].end() ? -1 : it->second;
+    // zeroed storage for every registered brick, after the last add()
+    void allocate() {
+        pool.resize(size_t(nbricks) * kLeafSize);
+        on.assign(size_t(nbricks) * 8, 0);
+        parallel_for(int64_t(nbricks), [&](int64_t b, int64_t e) {
+            std::fill(pool.begin() + b * kLeafSize, pool.begin() + e * kLeafSize, 0.f);
+        });
     }
+    int32_t find(int gid, const Coord& o) const { return map[gid].find(pack_origin(o)); }
     void set(int32_t b, int off, float v) {
         pool[size_t(b) * kLeafSize + off] = v;
         on[size_t(b) * 8 + (off >> 6)] |= uint64_t(1) << (off & 63);
@@ -453,6 +457,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
         for (const Coord& o : cand[d0].org)
             for (int t = 0; t < 4; ++t) L.cs.add(vcross(d0, t >> 1, t & 1), o);
     }
+    L.cs.allocate();
     for (int c = 0; c < 3; ++c) {
         const int d0 = c, d1 = (c + 1) % 3, d2 = (c + 2) % 3;
         const LeafSet& ls = L.dls[c];
@@ -596,6 +601,7 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
         for (const Coord& o : cand[d0].org)
             for (int t = 0; t < 4; ++t) C.cs.add(vcross(d0, t >> 1, t & 1), o);
     }
+    C.cs.allocate();
     const float rap = 0.125f;
     for (int i = 0; i < 3; ++i) {
         const int d0 = i, d1 = (i + 1) % 3, d2 = (i + 2) % 3;
@@ -697,26 +703,35 @@ void trim(BuildLevel& L) {
         else if (gid < kVNeg) ep = 1.0f / L.default_diag * eps;
         else ep = dv * eps;
         ep = std::abs(ep);
-        for (auto it = L.cs.map[gid].begin(); it != L.cs.map[gid].end();) {
-            const float* v = &L.cs.pool[size_t(it->second) * kLeafSize];
-            const uint64_t* on = &L.cs.on[size_t(it->second) * 8];
-            float me = 0;
-            for (int off = 0; off < kLeafSize; ++off)
-                if (mask_test(on, off)) me = std::max(me, std::abs(v[off] - dv));
-            if (me <= ep) it = L.cs.map[gid].erase(it);
-            else ++it;
-        }
+        std::vector<std::pair<uint64_t, int32_t>> bricks;
+        L.cs.map[gid].for_each([&](uint64_t key, int32_t b) { bricks.emplace_back(key, b); });
+        std::vector<uint8_t> drop(bricks.size(), 0);
+        parallel_for(int64_t(bricks.size()), [&](int64_t lo, int64_t hi) {
+            for (int64_t t = lo; t < hi; ++t) {
+                const int32_t b = bricks[size_t(t)].second;
+                const float* v = &L.cs.pool[size_t(b) * kLeafSize];
+                const uint64_t* on = &L.cs.on[size_t(b) * 8];
+                float me = 0;
+                for (int off = 0; off < kLeafSize; ++off)
+                    if (mask_test(on, off)) me = std::max(me, std::abs(v[off] - dv));
+                drop[size_t(t)] = me <= ep;
+            }
+        });
+        for (size_t t = 0; t < bricks.size(); ++t)
+            if (drop[t]) L.cs.map[gid].remove(bricks[t].first);
     }
 }
 
 // drop bricks with no active value (coarse levels keep zeros there, which the
 // operator multiplies with zero velocities only)
 void drop_empty(BuildLevel& L) {
-    for (int gid = 0; gid < kVNumGrids; ++gid)
-        for (auto it = L.cs.map[gid].begin(); it != L.cs.map[gid].end();) {
-            if (!mask_count(&L.cs.on[size_t(it->second) * 8])) it = L.cs.map[gid].erase(it);
-            else ++it;
-        }
+    for (int gid = 0; gid < kVNumGrids; ++gid) {
+        std::vector<uint64_t> gone;
+        L.cs.map[gid].for_each([&](uint64_t key, int32_t b) {
+            if (!mask_count(&L.cs.on[size_t(b) * 8])) gone.push_back(key);
+        });
+        for (uint64_t key : gone) L.cs.map[gid].remove(key);
+    }
 }
 
 void finalize(const BuildLevel& B, int level, float dt, float rho, ViscLevelHost& L) {
@@ -737,16 +752,17 @@ void finalize(const BuildLevel& B, int level, float dt, float rho, ViscLevelHost
     std::sort(dl.begin(), dl.end(), tree_order_less);
     std::unordered_map<uint64_t, Coord> all;
     for (int gid = 0; gid < kVNumGrids; ++gid)
-        for (const auto& kv : B.cs.map[gid])
-            if (!seen.count(kv.first)) {
+        B.cs.map[gid].for_each([&](uint64_t key, int32_t) {
+            if (!seen.count(key)) {
                 const int64_t m = (int64_t(1) << 21);
                 auto unpack = [&](uint64_t v) {
                     int64_t t = int64_t(v & uint64_t(m - 1));
                     if (t >= m / 2) t -= m;
                     return int32_t(t * 8);
                 };
-                all.emplace(kv.first, Coord{unpack(kv.first >> 42), unpack(kv.first >> 21), unpack(kv.first)});
+                all.emplace(key, Coord{unpack(key >> 42), unpack(key >> 21), unpack(key)});
             }
+        });
     for (const auto& kv : all) extra.push_back(kv.second);
     std::sort(extra.begin(), extra.end(), tree_order_less);
     L.origin = dl;
@@ -778,17 +794,24 @@ void finalize(const BuildLevel& B, int level, float dt, float rho, ViscLevelHost
                                                             L.origin[i].z + 8 * (d % 3 - 1)});
     });
     L.slot.assign(n * kVSlots, -1);
-    L.pool.clear();
-    L.pool_on.clear();
-    int32_t ns = 0;
+    // slots: the bricks of each leaf in grid order, leaves in level order
+    std::vector<int32_t> src;
     for (size_t i = 0; i < n; ++i)
         for (int gid = 0; gid < kVNumGrids; ++gid) {
             const int32_t b = B.cs.find(gid, L.origin[i]);
             if (b < 0) continue;
-            L.slot[i * kVSlots + gid] = ns++;
-            L.pool.insert(L.pool.end(), &B.cs.pool[size_t(b) * kLeafSize], &B.cs.pool[size_t(b + 1) * kLeafSize]);
-            L.pool_on.insert(L.pool_on.end(), &B.cs.on[size_t(b) * 8], &B.cs.on[size_t(b + 1) * 8]);
+            L.slot[i * kVSlots + gid] = int32_t(src.size());
+            src.push_back(b);
         }
+    L.pool.resize(src.size() * kLeafSize);
+    L.pool_on.resize(src.size() * 8);
+    parallel_for(int64_t(src.size()), [&](int64_t lo, int64_t hi) {
+        for (int64_t t = lo; t < hi; ++t) {
+            const size_t b = size_t(src[size_t(t)]);
+            std::copy(&B.cs.pool[b * kLeafSize], &B.cs.pool[(b + 1) * kLeafSize], &L.pool[size_t(t) * kLeafSize]);
+            std::copy(&B.cs.on[b * 8], &B.cs.on[(b + 1) * 8], &L.pool_on[size_t(t) * 8]);
+        }
+    });
 }
 
 void link_levels(ViscLevelHost& F, ViscLevelHost& C) {

@@ -45,7 +45,7 @@ struct ViscLevelHost {
     std::vector<int64_t> dofbase;    // [nleaf_dof][3] first DOF index of the channel in the leaf (-1 none)
     std::vector<int32_t> nb27;       // [nleaf][27] leaf at offset (ax,ay,az) in {-1,0,1}^3, -1 none
     std::vector<int32_t> slot;       // [nleaf][kVSlots] brick in pool or -1
-    std::vector<float> pool;         // [nslot][512]
+    BrickVec pool;                   // [nslot][512]
     std::vector<uint64_t> pool_on;   // [nslot][8] positions the reference keeps active
     std::vector<int32_t> chan_leaves[3];   // leaves holding DOFs of channel c
     std::vector<int32_t> parent;     // [nleaf_dof] coarse leaf * 8 + octant (not on the coarsest level)

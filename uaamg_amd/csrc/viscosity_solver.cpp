@@ -251,13 +251,23 @@ int ViscDevice::acquire_device() {
     use_graphs_ = !(opt_.flags & 2);
     // level-to-kernel mapping of the coarse levels: each half-visit as one
     // single-workgroup launch or as the per-pass launches. By default the
-    // faster of the two, timed below for levels whose widest channel has at
-    // most kSmallTuneLeaves leaves; flags bit 5 fixes "widest channel <= 16",
-    // UAAMG_VSMALL_LEAVES=n "<= n"; flags bit 7 keeps the per-pass launches.
-    bool tune = !(opt_.flags & (32 | 128));
-    small_max_leaves_ = (opt_.flags & 128) ? 0 : ((opt_.flags & 32) ? 16 : kSmallTuneLeaves);
-    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) {
-        small_max_leaves_ = std::atoi(env);
+    // per-pass launches (measured faster on every level tried: at 128^3 one
+    // workgroup takes 1.8 ms per level-2 half-visit pair against 0.16 ms);
+    // flags bit 5 selects the single-workgroup form for levels whose widest
+    // channel has at most 16 leaves, UAAMG_VSMALL_LEAVES=n at most n leaves;
+    // UAAMG_VSMALL_TUNE=1 times both forms on levels of at most
+    // kSmallTuneLeaves leaves and keeps the faster.
+    bool tune = false;
+    small_max_leaves_ = (opt_.flags & 32) ? 16 : 0;
+    if (const char* env = std::getenv("UAAMG_VSMALL_LEAVES")) small_max_leaves_ = std::atoi(env);
+    if (const char* env = std::getenv("UAAMG_VSMALL_TUNE")) {
+        if (env[0] == '1') {
+            tune = true;
+            small_max_leaves_ = kSmallTuneLeaves;
+        }
+    }
+    if (opt_.flags & 128) {
+        small_max_leaves_ = 0;
         tune = false;
     }
     const int L = int(host_.levels.size());
