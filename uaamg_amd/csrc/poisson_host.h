@@ -122,9 +122,11 @@ void build_poisson_rhs(const PoissonLevelHost& l0, const HostGrid& weights, cons
 void level_triplets(const PoissonLevelHost& l, std::vector<int32_t>& rows, std::vector<int32_t>& cols,
                     std::vector<float>& vals);
 
-// Run fn(begin, end) over [0, n) on the host worker threads.
+// Run fn(begin, end) over [0, n) on the host worker threads, at least `grain`
+// items per thread (the default suits light per-item work; per-leaf loops over
+// 512 voxels pass 1, so that even a few leaves spread over the threads).
 template <class Fn>
-void parallel_for(int64_t n, Fn&& fn);
+void parallel_for(int64_t n, Fn&& fn, int64_t grain = 32);
 
 int host_threads();
 
@@ -135,13 +137,14 @@ int host_threads();
 
 namespace uaamg {
 template <class Fn>
-void parallel_for(int64_t n, Fn&& fn) {
+void parallel_for(int64_t n, Fn&& fn, int64_t grain) {
     int nt = host_threads();
-    if (n < 64 || nt <= 1) {
+    grain = std::max<int64_t>(grain, 1);
+    if (n < 2 * grain || nt <= 1) {
         fn(int64_t(0), n);
         return;
     }
-    nt = int(std::min<int64_t>(nt, (n + 31) / 32));
+    nt = int(std::min<int64_t>(nt, (n + grain - 1) / grain));
     std::vector<std::thread> th;
     th.reserve(nt);
     int64_t chunk = (n + nt - 1) / nt;

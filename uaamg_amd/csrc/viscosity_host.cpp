@@ -15,6 +15,13 @@ namespace uaamg {
 
 namespace {
 
+// per-leaf loops: a leaf is 512 voxels of work, so even a few leaves are
+// spread over the host threads
+template <class Fn>
+void leaf_for(int64_t n, Fn&& fn) {
+    parallel_for(n, std::forward<Fn>(fn), 1);
+}
+
 inline Coord unit(int a) {
     Coord c{0, 0, 0};
     c[a] = 1;
@@ -103,7 +110,7 @@ float box_sample(GridAcc& acc, double px, double py, double pz) {
 LeafSet dilate(const HostGrid& g, int k, bool nn26) {
     const size_t n = g.nleaf();
     std::vector<uint32_t> touched(n, 0);
-    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+    leaf_for(int64_t(n), [&](int64_t b, int64_t e) {
         for (int64_t li = b; li < e; ++li) {
             uint32_t t = 0;
             const uint64_t* m = &g.active[size_t(li) * 8];
@@ -190,6 +197,30 @@ struct CoefStore {
     float get0(int gid, const Coord& p) const {
         const int32_t b = find(gid, leaf_origin_of(p));
         return b < 0 ? 0.f : pool[size_t(b) * kLeafSize + off_of(p)];
+    }
+};
+
+// get0 with a one-leaf cache per grid (the reads of one coarse voxel's
+// children fall in one fine leaf)
+struct StoreAcc {
+    const CoefStore* cs;
+    uint64_t key[kVNumGrids];
+    const float* brick[kVNumGrids];
+    explicit StoreAcc(const CoefStore& s) : cs(&s) {
+        for (int g = 0; g < kVNumGrids; ++g) {
+            key[g] = ~uint64_t(0);
+            brick[g] = nullptr;
+        }
+    }
+    float get0(int gid, const Coord& p) {
+        const Coord o = leaf_origin_of(p);
+        const uint64_t k = pack_origin(o);
+        if (k != key[gid]) {
+            key[gid] = k;
+            const int32_t b = cs->map[gid].find(k);
+            brick[gid] = b < 0 ? nullptr : &cs->pool[size_t(b) * kLeafSize];
+        }
+        return brick[gid] ? brick[gid][off_of(p)] : 0.f;
     }
 };
 
@@ -280,7 +311,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     } else {
     // subcell SDF: corner s of the subcell lattice sits at liquid index (s-1)/2
     SubField sdf{&sub_ls, std::vector<float>(sub_ls.size() * kSubSize), 1.5f * dx};
-    parallel_for(int64_t(sub_ls.size()), [&](int64_t b, int64_t e) {
+    leaf_for(int64_t(sub_ls.size()), [&](int64_t b, int64_t e) {
         GridAcc acc{&liquid};
         float blk[10][10][10];
         for (int64_t li = b; li < e; ++li) {
@@ -307,7 +338,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     });
     // subcell liquid volumes from the 8 corners
     SubField svol{&sub_ls, std::vector<float>(sub_ls.size() * kSubSize, 0.f), 0.f};
-    parallel_for(int64_t(sub_ls.size()), [&](int64_t b, int64_t e) {
+    leaf_for(int64_t(sub_ls.size()), [&](int64_t b, int64_t e) {
         std::vector<float> c17(17 * 17 * 17);
         for (int64_t li = b; li < e; ++li) {
             const Coord o = sub_ls.org[li];
@@ -336,7 +367,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
         G.face_vol[c].assign(nv * kLeafSize, 0.f);
         G.edge_vol[c].assign(nv * kLeafSize, 0.f);
     }
-    parallel_for(int64_t(nv), [&](int64_t b, int64_t e) {
+    leaf_for(int64_t(nv), [&](int64_t b, int64_t e) {
         std::vector<float> s18(18 * 18 * 18);
         for (int64_t li = b; li < e; ++li) {
             const Coord o = vol_ls.org[li];
@@ -395,7 +426,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
         G.solid_on[i].assign(nd * 8, 0);
         dof_on[i].assign(nd * 8, 0);
     }
-    parallel_for(int64_t(nd), [&](int64_t b, int64_t e) {
+    leaf_for(int64_t(nd), [&](int64_t b, int64_t e) {
         GridAcc sacc{&solid};
         FieldAcc va = vacc(G.voxel_vol);
         FieldAcc fa[3] = {vacc(G.face_vol[0]), vacc(G.face_vol[1]), vacc(G.face_vol[2])};
@@ -461,7 +492,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     for (int c = 0; c < 3; ++c) {
         const int d0 = c, d1 = (c + 1) % 3, d2 = (c + 2) % 3;
         const LeafSet& ls = L.dls[c];
-        parallel_for(int64_t(ls.size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(ls.size()), [&](int64_t b, int64_t e) {
             FieldAcc va = vacc(G.voxel_vol);
             FieldAcc fa = vacc(G.face_vol[d0]);
             FieldAcc e1 = vacc(G.edge_vol[d1]), e2 = vacc(G.edge_vol[d2]);
@@ -523,7 +554,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
     for (int d0 = 0; d0 < 3; ++d0) {
         const int d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
         const LeafSet& ls = cand[d0];
-        parallel_for(int64_t(ls.size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(ls.size()), [&](int64_t b, int64_t e) {
             FieldAcc ea = vacc(G.edge_vol[d0]);
             DofAcc a1 = L.dof_acc(d1), a2 = L.dof_acc(d2);
             GridAcc ma{&G.mu};
@@ -557,6 +588,7 @@ bool build_finest(const HostGrid& liquid, const HostGrid& solid, const HostGrid&
 
 // ------------------------------------------------------------- coarse
 void build_coarse(const BuildLevel& F, BuildLevel& C) {
+    StageClock clock;
     C.dx = F.dx * 2.0f;
     C.default_diag = 1.0f;
     C.default_off = 1.0f;
@@ -570,7 +602,7 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
         }
         cl.finish();
         std::vector<uint64_t> on(cl.size() * 8, 0);
-        parallel_for(int64_t(cl.size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(cl.size()), [&](int64_t b, int64_t e) {
             DofAcc fa = F.dof_acc(i);
             for (int64_t li = b; li < e; ++li) {
                 const Coord o = cl.org[li];
@@ -589,6 +621,7 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
         number_channel(cl, on, C, i);
         C.nchan[i] = C.ndof - before;
     }
+    clock.lap("viscosity coarse: topology");
     for (int c = 0; c < 3; ++c)
         for (const Coord& o : C.dls[c].org) {
             C.cs.add(kVDiag + c, o);
@@ -602,13 +635,16 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
             for (int t = 0; t < 4; ++t) C.cs.add(vcross(d0, t >> 1, t & 1), o);
     }
     C.cs.allocate();
+    clock.lap("viscosity coarse: brick store");
     const float rap = 0.125f;
     for (int i = 0; i < 3; ++i) {
         const int d0 = i, d1 = (i + 1) % 3, d2 = (i + 2) % 3;
         const int dd[3] = {d0, d1, d2};
         const LeafSet& ls = C.dls[i];
-        parallel_for(int64_t(ls.size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(ls.size()), [&](int64_t b, int64_t e) {
             DofAcc fa = F.dof_acc(i), ca = C.dof_acc(i);
+            DofAcc fn = F.dof_acc(i);
+            StoreAcc fs(F.cs), fsn(F.cs);
             for (int64_t li = b; li < e; ++li) {
                 const Coord o = ls.org[li];
                 const int32_t bd = C.cs.find(kVDiag + i, o), bi = C.cs.find(kVInvDiag + i, o);
@@ -627,11 +663,11 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
                                 f[d1] += jj;
                                 f[d2] += kk;
                                 if (fa.get(f) == -1) continue;
-                                nd += F.cs.get0(kVDiag + i, f);
+                                nd += fs.get0(kVDiag + i, f);
                                 const int loc[3] = {ii, jj, kk};
                                 for (int k = 0; k < 3; ++k) {
-                                    const float t = F.cs.get0(vneg(i, dd[k]), f);
-                                    if (fa.get(shifted(f, dd[k], -1)) == -1) continue;
+                                    const float t = fs.get0(vneg(i, dd[k]), f);
+                                    if (fn.get(shifted(f, dd[k], -1)) == -1) continue;
                                     if (loc[k] == 0) nn[k] += t;
                                     else nd += t * 2.0f;
                                 }
@@ -645,8 +681,9 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
         });
         const LeafSet& cl = cand[i];
         const int fx[4] = {vcross(i, 0, 0), vcross(i, 0, 1), vcross(i, 1, 0), vcross(i, 1, 1)};
-        parallel_for(int64_t(cl.size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(cl.size()), [&](int64_t b, int64_t e) {
             DofAcc a1 = C.dof_acc(d1), a2 = C.dof_acc(d2);
+            StoreAcc fs(F.cs);
             for (int64_t li = b; li < e; ++li) {
                 const Coord o = cl.org[li];
                 int32_t bx[4];
@@ -668,21 +705,21 @@ void build_coarse(const BuildLevel& F, BuildLevel& C) {
                                     f[d2] += kk;
                                     switch (t) {
                                         case 0:
-                                            s += F.cs.get0(fx[0], f);
-                                            if (kk == 0) s += F.cs.get0(fx[1], f);
-                                            if (jj == 0) s += F.cs.get0(fx[2], f);
-                                            if (kk == 0 && jj == 0) s += F.cs.get0(fx[3], f);
+                                            s += fs.get0(fx[0], f);
+                                            if (kk == 0) s += fs.get0(fx[1], f);
+                                            if (jj == 0) s += fs.get0(fx[2], f);
+                                            if (kk == 0 && jj == 0) s += fs.get0(fx[3], f);
                                             break;
                                         case 1:
-                                            if (kk == 1) s += F.cs.get0(fx[1], f);
-                                            if (jj == 0 && kk == 1) s += F.cs.get0(fx[3], f);
+                                            if (kk == 1) s += fs.get0(fx[1], f);
+                                            if (jj == 0 && kk == 1) s += fs.get0(fx[3], f);
                                             break;
                                         case 2:
-                                            if (jj == 1) s += F.cs.get0(fx[2], f);
-                                            if (jj == 1 && kk == 0) s += F.cs.get0(fx[3], f);
+                                            if (jj == 1) s += fs.get0(fx[2], f);
+                                            if (jj == 1 && kk == 0) s += fs.get0(fx[3], f);
                                             break;
                                         default:
-                                            if (jj == 1 && kk == 1) s += F.cs.get0(fx[3], f);
+                                            if (jj == 1 && kk == 1) s += fs.get0(fx[3], f);
                                     }
                                 }
                         C.cs.set(bx[t], off, rap * s);
@@ -706,7 +743,7 @@ void trim(BuildLevel& L) {
         std::vector<std::pair<uint64_t, int32_t>> bricks;
         L.cs.map[gid].for_each([&](uint64_t key, int32_t b) { bricks.emplace_back(key, b); });
         std::vector<uint8_t> drop(bricks.size(), 0);
-        parallel_for(int64_t(bricks.size()), [&](int64_t lo, int64_t hi) {
+        leaf_for(int64_t(bricks.size()), [&](int64_t lo, int64_t hi) {
             for (int64_t t = lo; t < hi; ++t) {
                 const int32_t b = bricks[size_t(t)].second;
                 const float* v = &L.cs.pool[size_t(b) * kLeafSize];
@@ -786,7 +823,7 @@ void finalize(const BuildLevel& B, int level, float dt, float rho, ViscLevelHost
             L.chan_leaves[c].push_back(int32_t(i));
         }
     L.nb27.assign(n * 27, -1);
-    parallel_for(int64_t(n), [&](int64_t b, int64_t e) {
+    leaf_for(int64_t(n), [&](int64_t b, int64_t e) {
         for (int64_t i = b; i < e; ++i)
             for (int d = 0; d < 27; ++d)
                 L.nb27[size_t(i) * 27 + d] = L.lookup.find({L.origin[i].x + 8 * (d / 9 - 1),
@@ -897,7 +934,7 @@ void build_visc_spai0(ViscLevelHost& L) {
     std::vector<std::vector<float>> vals(3, std::vector<float>(L.nleaf_dof * kLeafSize, 0.f));
     for (int c = 0; c < 3; ++c) {
         const int d0 = c, d1 = (c + 1) % 3, d2 = (c + 2) % 3;
-        parallel_for(int64_t(L.chan_leaves[c].size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(L.chan_leaves[c].size()), [&](int64_t b, int64_t e) {
             for (int64_t t = b; t < e; ++t) {
                 const size_t i = size_t(L.chan_leaves[c][t]);
                 const Coord o = L.origin[i];
@@ -1037,7 +1074,7 @@ void build_visc_rhs_impl(const ViscHierarchyHost& h, const UGet& uget, const Hos
     };
     // first part: u * max(face volume, 0.1) on every DOF
     for (int c = 0; c < 3; ++c) {
-        parallel_for(int64_t(G.dof_origin.size()), [&](int64_t b, int64_t e) {
+        leaf_for(int64_t(G.dof_origin.size()), [&](int64_t b, int64_t e) {
             FieldAcc fa = fld(G.face_vol[c]);
             for (int64_t li = b; li < e; ++li) {
                 const Coord o = G.dof_origin[li];
