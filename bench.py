@@ -616,7 +616,7 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
     base = srv.stats()
 
     tickets, lock, done = [], threading.Lock(), threading.Event()
-    results = []
+    results, req_pool = [], []
 
     def collector():
         k = 0
@@ -646,10 +646,12 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
         now = time.perf_counter()
         if t_next > now:
             time.sleep(t_next - now)
-        p = pool[int(rng.integers(n_pool))]
+        k_pool = int(rng.integers(n_pool))
+        p = pool[k_pool]
         t = srv.submit(p["pid"], p["rhs"][int(rng.integers(n_rhs))].data_ptr(), tol=p["tol"])
         with lock:
             tickets.append(t)
+            req_pool.append(k_pool)
         sent += 1
     done.set()
     col.join()
@@ -658,9 +660,16 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
     srv.close()
     lat_ms = np.array([r[3] for r in results])
     fails = sum(1 for r in results if r[0] != 0)
+    # non-SUCCESS results by status and problem (status 1 = FAILED, the reference's
+    # PCG failure rule; negative = a server or device error)
+    fail_detail = {}
+    for r, k in zip(results, req_pool):
+        if r[0] != 0:
+            key = f"status {r[0]} / {'viscosity' if pool[k]['visc'] else 'poisson'} n={pool[k]['n']}"
+            fail_detail[key] = fail_detail.get(key, 0) + 1
     return {"metric": "p50 / p99 latency (submission to completion) and solves/s, Poisson-process arrivals",
             "engine": "C++ solve server (uaamg_server_*), continuous admission",
-            "requests": sent, "completed": len(results), "failed": fails,
+            "requests": sent, "completed": len(results), "failed": fails, "failed_by_status": fail_detail,
             "solves_per_s": round(len(results) / wall, 2),
             "arrival_rate_per_s": round(rate, 2), "slots": args.serve_workers,
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 2) if len(results) else None,
