@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -901,6 +903,10 @@ bool build_visc_hierarchy(const HostGrid& liquid, const HostGrid& solid, const H
     clock.lap("viscosity trim");
     out.levels.resize(B.size());
     for (size_t l = 0; l < B.size(); ++l) finalize(B[l], int(l), dt, rho, out.levels[l]);
+    if (std::getenv("UAAMG_BUILD_TIMING"))
+        for (size_t l = 0; l < B.size(); ++l)
+            std::fprintf(stderr, "[uaamg build] viscosity level %zu: %zu leaves (%zu with DOFs), %zu bricks\n", l,
+                         out.levels[l].nleaf(), out.levels[l].nleaf_dof, out.levels[l].pool.size() / kLeafSize);
     for (size_t l = 0; l + 1 < B.size(); ++l) link_levels(out.levels[l], out.levels[l + 1]);
     clock.lap("viscosity finalize + links");
     visc_explicit_matrix(out.levels.back(), out.cptr, out.ccol, out.cval);
