@@ -140,7 +140,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=200)
-    ap.add_argument("--workload", choices=["poisson", "viscosity", "batch", "serve"], default="poisson",
+    ap.add_argument("--workload", choices=["poisson", "viscosity", "batch", "visc-batch", "serve"],
+                    default="poisson",
                     help="poisson: the headline line (plus viscosity / batch / serving objects unless --no-*); "
                          "viscosity, batch, serve: only that measurement (profiling runs)")
     ap.add_argument("--no-serve", action="store_true")
@@ -157,6 +158,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-concurrency", type=int, default=4)
+    ap.add_argument("--visc-batch-size", type=int, default=64)
     ap.add_argument("--visc-n", type=int, default=128)
     ap.add_argument("--visc-steps", type=int, default=10)
     ap.add_argument("--kernel-only", action="store_true",
@@ -180,7 +182,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
 
     if args.workload != "poisson":
-        fn = {"viscosity": run_viscosity, "batch": run_batch, "serve": run_serving}[args.workload]
+        fn = {"viscosity": run_viscosity, "batch": run_batch, "visc-batch": run_visc_batch,
+              "serve": run_serving}[args.workload]
         print(json.dumps(fn(args, dev)), flush=True)
         return 0
 
@@ -416,6 +419,8 @@ def main() -> int:
         out["viscosity"] = run_viscosity(args, dev)
     if not args.no_batch:
         out["batch"] = run_batch(args, dev)
+        if not args.no_viscosity:
+            out["viscosity_batch"] = run_visc_batch(args, dev)
     if not args.no_serve:
         out["serving"] = run_serving(args, dev)
         # the same stream under a memory budget of half the pool's resident footprint: admission
@@ -826,6 +831,79 @@ def run_batch(args, dev) -> dict:
             "merged_fp32_identical_to_sequential": bool(same_mrg),
             "streams_identical_to_sequential": bool(same_str),
             "bf16_max_rel_l2_vs_fp32": dev_mx,
+            "build_s": round(t_build, 2), "merge_build_s": round(t_build_merged, 2)}
+
+
+def run_visc_batch(args, dev) -> dict:
+    """The viscosity batch: `--visc-batch-size` independent viscosity problems
+    (the batch scenes: edge log-uniform in [32, 192], 1-6 spheres over a
+    10-50% pool, solid sphere with probability 1/2; seeds 0..n-1; mu = 1,
+    dt = 1/24, rho = 1000), tol 1e-6, b resident in HBM. Reports solves/s for
+    the problems one after another (one handle each) and for the merged batch
+    (one leaf table per level, one launch per colour pass for all problems),
+    whose results must equal the sequential ones bit for bit."""
+    import numpy as np
+    import torch
+
+    import uaamg_amd as U
+    from uaamg_amd.viscosity import ViscBatch
+
+    tol = 1e-6
+    t0 = time.perf_counter()
+    scenes, solvers, xs, bs, ndof = [], [], [], [], 0
+    for seed in range(args.visc_batch_size):
+        sc = U.ViscScene(3, 0, seed=seed, mu=1.0, dt=1.0 / 24.0, rho=1000.0)
+        lw = U.L_with_level(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx, device=0)
+        sv = U.simd_viscosity3d(lw, sc.velocity)
+        b = torch.from_numpy(sv.m_rhs.data).to(dev)
+        scenes.append(sc)
+        solvers.append(sv)
+        bs.append(b)
+        xs.append(torch.zeros_like(b))
+        ndof += lw.m_ndof
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    merged = ViscBatch([(sc.viscosity, sc.liquid, sc.solid, sc.dt, sc.rho, sc.dx) for sc in scenes])
+    t_build_merged = time.perf_counter() - t0
+
+    def sequential():
+        its = []
+        for sv, x, b in zip(solvers, xs, bs):
+            x.zero_()
+            its.append(sv.solve_pcg_device(tol, x.data_ptr(), b.data_ptr()))
+        torch.cuda.synchronize()
+        return its
+
+    def merged_solve():
+        for x in xs:
+            x.zero_()
+        torch.cuda.synchronize()
+        its = merged.solve_device(tol, [x.data_ptr() for x in xs], [b.data_ptr() for b in bs])
+        torch.cuda.synchronize()
+        return its
+
+    def timed(fn, reps=3):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            its = fn()
+        return (time.perf_counter() - t) / reps, its
+
+    t_seq, its_seq = timed(sequential)
+    x_seq = [x.cpu().numpy() for x in xs]
+    t_mrg, its_mrg = timed(merged_solve)
+    same = all(np.array_equal(a, x.cpu().numpy()) for a, x in zip(x_seq, xs)) and its_seq == its_mrg
+    n = args.visc_batch_size
+    info = merged.info()
+    return {"metric": f"solves/s over {n} independent viscosity problems (edge log-uniform [32,192]), tol 1e-6",
+            "value": round(n / t_mrg, 1), "unit": "solves/s", "form": "merged batch, FP32",
+            "problems": n, "total_ndof": ndof, "ms_per_batch": round(1e3 * t_mrg, 2),
+            "sequential_ms_per_batch": round(1e3 * t_seq, 2), "sequential_solves_per_s": round(n / t_seq, 1),
+            "speedup_vs_sequential": round(t_seq / t_mrg, 2),
+            "merged_levels": info["nlevels"], "merged_leaves0": info["leaves0"],
+            "merged_device_bytes": info["device_bytes"],
+            "mean_iterations": round(float(np.mean(its_mrg)), 2),
+            "merged_identical_to_sequential": bool(same),
             "build_s": round(t_build, 2), "merge_build_s": round(t_build_merged, 2)}
 
 

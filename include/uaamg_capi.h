@@ -553,6 +553,30 @@ int uaamg_viscosity_solve_pcg(uaamg_viscosity* h, float tolerance, float* x, con
 int uaamg_viscosity_solve_pcg_device(uaamg_viscosity* h, float tolerance, float* x_dev, const float* b_dev,
                                      int32_t* iterations);
 
+/*
+ * Merged viscosity batch: many simd_viscosity3d problems (uneven sizes, their own dt, rho, dx and
+ * viscosity) merged into one leaf table per level, so every Gauss-Seidel colour pass, transfer and
+ * PCG vector operation is one launch for the whole batch and one hipGraph per PCG iteration covers
+ * every problem (no counterpart in the reference, which solves one problem per simd_viscosity3d;
+ * each problem's result and m_iteration equal its pcg_solve alone, simd_viscosity3d.cpp:3694-3740).
+ * FP32 form (opts->precision must be UAAMG_PRECISION_FP32); opts as uaamg_viscosity_create.
+ */
+typedef struct uaamg_viscosity_batch uaamg_viscosity_batch;
+int uaamg_viscosity_batch_create(int32_t n, const uaamg_grid* const* viscosity, const uaamg_grid* const* liquid_sdf,
+                                 const uaamg_grid* const* solid_sdf, const float* dt, const float* rho,
+                                 const float* dx, const uaamg_viscosity_options* opts,
+                                 uaamg_viscosity_batch** out);
+void uaamg_viscosity_batch_destroy(uaamg_viscosity_batch* h);
+/* problems, merged levels, merged level-0 leaves (with alignment padding), device bytes */
+int uaamg_viscosity_batch_info(const uaamg_viscosity_batch* h, int32_t* nproblems, int32_t* nlevels,
+                               int64_t* leaves0, int64_t* device_bytes);
+/* m_ndof of problem i (-1 for a bad index) */
+int64_t uaamg_viscosity_batch_ndof(const uaamg_viscosity_batch* h, int32_t i);
+/* pcg_solve(tolerance) on every problem: x_dev[i] / b_dev[i] are DOF-ordered device vectors of
+ * problem i (x: initial guess in, result out); iterations[i] its m_iteration (may be NULL). */
+int uaamg_viscosity_batch_solve_device(uaamg_viscosity_batch* h, float tolerance, float* const* x_dev,
+                                       const float* const* b_dev, int32_t* iterations);
+
 /* Many independent viscosity solves at once, as uaamg_poisson_solve_batch_device. */
 int uaamg_viscosity_solve_batch_device(uaamg_viscosity* const* handles, int32_t n, float tolerance,
                                        float* const* x_dev, const float* const* b_dev, int32_t* iterations,

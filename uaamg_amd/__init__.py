@@ -22,7 +22,7 @@ import numpy as np
 __all__ = [
     "Grid", "Scene", "LaplacianWithLevel", "PoissonSolver", "SmootherOption", "library_path", "lib",
     "UaamgError", "SUCCESS", "FAILED", "device_count", "L_with_level", "simd_viscosity3d", "ViscScene",
-    "packed_FloatGrid3", "solve_batch_device",
+    "packed_FloatGrid3", "solve_batch_device", "PoissonBatch", "ViscBatch",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -187,6 +187,11 @@ def lib() -> C.CDLL:
         "uaamg_viscosity_algorithmic_bytes": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, P(C.c_double)]),
         "uaamg_viscosity_time_apply": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, P(f32)]),
         "uaamg_viscosity_kernel_map": (C.c_int, [vp, C.c_int, P(i32), P(f32), P(f32)]),
+        "uaamg_viscosity_batch_create": (C.c_int, [i32, vp, vp, vp, vp, vp, vp, P(_ViscOptions), P(vp)]),
+        "uaamg_viscosity_batch_destroy": (None, [vp]),
+        "uaamg_viscosity_batch_info": (C.c_int, [vp, P(i32), P(i32), P(i64), P(i64)]),
+        "uaamg_viscosity_batch_ndof": (i64, [vp, i32]),
+        "uaamg_viscosity_batch_solve_device": (C.c_int, [vp, f32, vp, vp, vp]),
         "uaamg_visc_scene_create": (C.c_int, [C.c_int, i32, C.c_uint64, f32, f32, f32, P(vp)]),
         "uaamg_visc_scene_grids": (C.c_int, [vp, P(_Grid), P(_Grid), P(_Grid), P(_Grid), P(f32), P(f32), P(f32)]),
         "uaamg_visc_scene_destroy": (None, [vp]),
@@ -698,4 +703,4 @@ class PoissonSolver:
         return z
 
 
-from .viscosity import L_with_level, ViscScene, packed_FloatGrid3, simd_viscosity3d  # noqa: E402
+from .viscosity import L_with_level, ViscBatch, ViscScene, packed_FloatGrid3, simd_viscosity3d  # noqa: E402

@@ -31,6 +31,7 @@ SOURCES = [
     "viscosity_solver.cpp",
     "capi_viscosity.cpp",
     "batch_solver.cpp",
+    "visc_batch.cpp",
     "server.cpp",
     "capi_server.cpp",
     "capi_packed3.cpp",

@@ -49,6 +49,10 @@ enum BatchFinOp : int {
     kBFinNuFirst = 4,
     kBFinNuIter = 5,
     kBFinReplace = 6,   // FP64 true residual: restart problems whose true residual misses the tolerance
+    // the viscosity pcg_solve rule (simd_viscosity3d.cpp:3694-3740): stop when ||r||_inf <= tol ||r0||_inf
+    // or at the last allowed iteration, no failure test; the iteration form counts the step first
+    kBFinNuViscFirst = 7,
+    kBFinNuViscIter = 8,
 };
 constexpr int kBMaxReplacements = 4;
 
@@ -78,6 +82,9 @@ hipError_t launch_baxpy_absmax(const LevelView& L, const int32_t* leafprob, cons
                                hipStream_t s);
 hipError_t launch_bupdate_x(const LevelView& L, const int32_t* leafprob, BatchPcg* st, const float* p, float* x,
                             hipStream_t s);
+// x += alpha p; p = z + beta p for the problems in kBActive (update_kernel<true>'s arithmetic)
+hipError_t launch_bupdate_xp(const LevelView& L, const int32_t* leafprob, const BatchPcg* st, const float* z,
+                             float* x, float* p, hipStream_t s);
 // One workgroup per problem: sums its blocks' partials in the single-problem
 // order and applies `op` (scalars, then the convergence / failure rules);
 // op < 0 moves problems whose last x update ran from kBConverging to kBSuccess.

@@ -11,6 +11,7 @@
 
 #include "capi_common.h"
 #include "scene.h"
+#include "visc_batch.h"
 #include "viscosity_solver.h"
 
 using namespace uaamg;
@@ -458,6 +459,75 @@ int uaamg_visc_scene_grids(const uaamg_visc_scene* sc, uaamg_grid* liquid, uaamg
 }
 
 void uaamg_visc_scene_destroy(uaamg_visc_scene* s) { delete s; }
+
+}  // extern "C"
+
+struct uaamg_viscosity_batch {
+    ViscBatch b;
+};
+
+extern "C" {
+
+int uaamg_viscosity_batch_create(int32_t n, const uaamg_grid* const* viscosity, const uaamg_grid* const* liquid_sdf,
+                                 const uaamg_grid* const* solid_sdf, const float* dt, const float* rho,
+                                 const float* dx, const uaamg_viscosity_options* opts,
+                                 uaamg_viscosity_batch** out) {
+    UAAMG_GUARD_BEGIN
+    if (!out) return fail(UAAMG_EINVAL, "null output handle");
+    *out = nullptr;
+    if (n <= 0 || !viscosity || !liquid_sdf || !solid_sdf || !dt || !rho || !dx)
+        return fail(UAAMG_EINVAL, "empty or null batch");
+    std::vector<ViscHierarchyHost> probs(static_cast<size_t>(n));
+    ViscOptions vo;
+    for (int32_t i = 0; i < n; ++i) {
+        ViscOptions oi;
+        if (int rc = capi::prepare_viscosity(viscosity[i], liquid_sdf[i], solid_sdf[i], dt[i], rho[i], dx[i], opts,
+                                             probs[size_t(i)], oi))
+            return rc;
+        if (i == 0) vo = oi;
+    }
+    if (vo.device < 0) return fail(UAAMG_ENODEV, "the merged batch needs a device");
+    std::unique_ptr<uaamg_viscosity_batch> h(new uaamg_viscosity_batch());
+    int rc = h->b.init(std::move(probs), vo);
+    if (rc < 0) {
+        const int code = rc == -1 ? UAAMG_EINVAL : rc == -2 ? UAAMG_ENOMEM : rc == -4 ? UAAMG_ENODEV : UAAMG_EHIP;
+        return fail(code, h->b.error().empty() ? "batch initialisation failed" : h->b.error());
+    }
+    *out = h.release();
+    return 0;
+    UAAMG_GUARD_END
+}
+
+void uaamg_viscosity_batch_destroy(uaamg_viscosity_batch* h) { delete h; }
+
+int uaamg_viscosity_batch_info(const uaamg_viscosity_batch* h, int32_t* nproblems, int32_t* nlevels,
+                               int64_t* leaves0, int64_t* device_bytes) {
+    if (!h) return fail(UAAMG_EINVAL, "null handle");
+    if (nproblems) *nproblems = h->b.nproblems();
+    if (nlevels) *nlevels = h->b.nlevels();
+    if (leaves0) *leaves0 = h->b.level_leaves(0);
+    if (device_bytes) *device_bytes = int64_t(h->b.device_bytes());
+    return 0;
+}
+
+int64_t uaamg_viscosity_batch_ndof(const uaamg_viscosity_batch* h, int32_t i) { return h ? h->b.ndof(i) : -1; }
+
+int uaamg_viscosity_batch_solve_device(uaamg_viscosity_batch* h, float tolerance, float* const* x_dev,
+                                       const float* const* b_dev, int32_t* iterations) {
+    UAAMG_GUARD_BEGIN
+    if (!h || !x_dev || !b_dev) return fail(UAAMG_EINVAL, "null argument");
+    const int n = h->b.nproblems();
+    std::vector<int> it(static_cast<size_t>(n));
+    int rc = h->b.solve(tolerance, x_dev, b_dev, it.data());
+    if (rc < 0) {
+        const int code = rc == -1 ? UAAMG_EINVAL : rc == -4 ? UAAMG_ENODEV : UAAMG_EHIP;
+        return fail(code, h->b.error().empty() ? "batch solve failed" : h->b.error());
+    }
+    if (iterations)
+        for (int i = 0; i < n; ++i) iterations[i] = it[size_t(i)];
+    return 0;
+    UAAMG_GUARD_END
+}
 
 float uaamg_volume_fraction8(const float* p) {
     return p ? visc_volume_fraction(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]) : 0.f;

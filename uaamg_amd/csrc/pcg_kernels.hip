@@ -492,6 +492,37 @@ __global__ __launch_bounds__(kBlockThreads) void bupdate_x_kernel(LevelView L, c
     st8(X + row, x);
 }
 
+__global__ __launch_bounds__(kBlockThreads) void bupdate_xp_kernel(LevelView L, const int32_t* leafprob,
+                                                                   const BatchPcg* st, const float* Zv, float* X,
+                                                                   float* P) {
+    const int leaf = wave_leaf();
+    if (leaf >= L.nleaf) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t rm = row_mask(L, leaf, lane);
+    const BatchPcg& sp = st[leaf_problem(leafprob, leaf)];
+    if (!rm || sp.state != kBActive) return;
+    const float alpha = sp.alpha, beta = sp.beta;
+    const size_t row = size_t(leaf) * 512 + lane * 8;
+    float x[8], p[8], z[8];
+    ld8(X + row, x);
+    ld8(P + row, p);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((rm >> k) & 1u) {
+            const float t = alpha * p[k];
+            x[k] = x[k] + t;
+        }
+    st8(X + row, x);
+    ld8(Zv + row, z);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((rm >> k) & 1u) {
+            const float t = beta * p[k];
+            p[k] = z[k] + t;
+        }
+    st8(P + row, p);
+}
+
 // One workgroup per problem. The sums and the scalar updates repeat
 // finalize_kernel's on the problem's own block range; the state rules repeat
 // the host loop of solve_pcg_dev / pcg_iterate.
@@ -501,7 +532,8 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
     BatchPcg& sp = st[prob];
     const int b0 = blk_begin[prob], n = blk_begin[prob + 1] - b0;
     const double* part = partials + b0;
-    if (op == kBFinNu0 || op == kBFinNuFirst || op == kBFinNuIter || op == kBFinReplace) {
+    if (op == kBFinNu0 || op == kBFinNuFirst || op == kBFinNuIter || op == kBFinReplace || op == kBFinNuViscFirst ||
+        op == kBFinNuViscIter) {
         __shared__ float tv[kFinThreads];
         __shared__ int tb[kFinThreads];
         float v = 0.f;
@@ -549,6 +581,13 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
             return;
         }
         if (sp.state != kBActive) return;
+        if (op == kBFinNuViscFirst || op == kBFinNuViscIter) {
+            // the host loop of ViscDevice::solve_pcg_dev
+            if (op == kBFinNuViscIter) sp.iters += 1;
+            sp.nu = nu;
+            if (nu <= sp.numax || sp.iters == sp.max_iters - 1) sp.state = kBConverging;
+            return;
+        }
         sp.nu = nu;
         if (op == kBFinNuIter) {
             sp.iters += 1;
@@ -834,6 +873,13 @@ hipError_t launch_bupdate_x(const LevelView& L, const int32_t* leafprob, BatchPc
     bupdate_x_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, p, x);
     if (hipError_t e = hipGetLastError()) return e;
     return hipSuccess;
+}
+
+hipError_t launch_bupdate_xp(const LevelView& L, const int32_t* leafprob, const BatchPcg* st, const float* z,
+                             float* x, float* p, hipStream_t s) {
+    if (L.nleaf <= 0) return hipSuccess;
+    bupdate_xp_kernel<<<grid_for(L.nleaf), kBlockThreads, 0, s>>>(L, leafprob, st, z, x, p);
+    return hipGetLastError();
 }
 
 hipError_t launch_bfinalize(int op, int nprob, const int32_t* blk_begin, const double* partials, BatchPcg* st,

@@ -105,6 +105,9 @@ hipError_t launch_visc_prolong(const ViscLevelView& fine, const ViscLevelView& c
 // x = coarsest solve of b (vectors in level layout), one workgroup
 hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float* x, hipStream_t s);
 hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const bf16* b, bf16* x, hipStream_t s);
+// `count` systems in one launch, one workgroup each (views: device array); max_n its largest row count
+hipError_t launch_visc_bcoarsest(const ViscCoarsestView* views, int count, int max_n, const float* b, float* x,
+                                 hipStream_t s);
 // zero every brick of a level vector (set_grid_to_zero, :2304-2315)
 hipError_t launch_visc_zero(const ViscLevelView& L, float* v, hipStream_t s);
 hipError_t launch_visc_zero(const ViscLevelView& L, bf16* v, hipStream_t s);

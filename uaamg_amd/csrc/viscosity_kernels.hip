@@ -522,8 +522,12 @@ __device__ __forceinline__ float2 bsum2(float a, float b, float (*red)[kCgT / 64
 // ||r||^2 and <r, z>, whose barrier also publishes the new z; the new
 // direction's product is formed as A p = A z + beta A p_old (the same vector
 // as Eigen's A p, rounded differently), so no third barrier publishes p.
+// views != null: a batch of systems, workgroup i solving views[i] (one launch
+// for the problems of a merged batch whose coarsest level is the same)
 template <class V>
-__global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C, const V* B, V* X) {
+__global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C0, const ViscCoarsestView* views,
+                                                             const V* B, V* X) {
+    const ViscCoarsestView C = views ? views[blockIdx.x] : C0;
     __shared__ float sz[kViscMaxCoarsest];
     __shared__ float sq[kViscMaxCoarsest];   // A z of the thread's own rows
     __shared__ float red[2][kCgT / 64][2];
@@ -616,7 +620,9 @@ __global__ __launch_bounds__(kCgT) void visc_coarsest_kernel(ViscCoarsestView C,
 constexpr int kCgSmallN = 1024;
 
 template <class V>
-__global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestView C, const V* B, V* X) {
+__global__ __launch_bounds__(kCgT) void visc_coarsest_small_kernel(ViscCoarsestView C0, const ViscCoarsestView* views,
+                                                                   const V* B, V* X) {
+    const ViscCoarsestView C = views ? views[blockIdx.x] : C0;
     constexpr int kPer = kCgSmallN / kCgT;
     __shared__ float sz[kCgSmallN];
     __shared__ float red[2][kCgT / 64][2];
@@ -769,8 +775,18 @@ template <class V>
 hipError_t coarsest(const ViscCoarsestView& C, const V* b, V* x, hipStream_t s) {
     if (C.n <= 0) return hipSuccess;
     if (C.n > kViscMaxCoarsest) return hipErrorInvalidValue;
-    if (C.n <= kCgSmallN) visc_coarsest_small_kernel<V><<<1, kCgT, 0, s>>>(C, b, x);
-    else visc_coarsest_kernel<V><<<1, kCgT, 0, s>>>(C, b, x);
+    if (C.n <= kCgSmallN) visc_coarsest_small_kernel<V><<<1, kCgT, 0, s>>>(C, nullptr, b, x);
+    else visc_coarsest_kernel<V><<<1, kCgT, 0, s>>>(C, nullptr, b, x);
+    return hipGetLastError();
+}
+
+template <class V>
+hipError_t bcoarsest(const ViscCoarsestView* views, int count, int max_n, const V* b, V* x, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (max_n > kViscMaxCoarsest) return hipErrorInvalidValue;
+    // the register form when every system fits it, as each problem alone would run
+    if (max_n <= kCgSmallN) visc_coarsest_small_kernel<V><<<unsigned(count), kCgT, 0, s>>>({}, views, b, x);
+    else visc_coarsest_kernel<V><<<unsigned(count), kCgT, 0, s>>>({}, views, b, x);
     return hipGetLastError();
 }
 
@@ -834,6 +850,10 @@ hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const float* b, float
 }
 hipError_t launch_visc_coarsest(const ViscCoarsestView& C, const bf16* b, bf16* x, hipStream_t s) {
     return coarsest(C, b, x, s);
+}
+hipError_t launch_visc_bcoarsest(const ViscCoarsestView* views, int count, int max_n, const float* b, float* x,
+                                 hipStream_t s) {
+    return bcoarsest(views, count, max_n, b, x, s);
 }
 
 hipError_t launch_visc_small_pre(const ViscLevelView& L, const ViscLevelView& coarse, float* x, const float* b,

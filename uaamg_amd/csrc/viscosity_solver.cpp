@@ -56,8 +56,7 @@ void ViscDevice::init_host_only(ViscHierarchyHost&& h, const ViscOptions& opt) {
     opt_ = opt;
 }
 
-// Wave records of a level (layout at kVRecLeaf), chan_leaves order.
-static std::vector<int32_t> wave_records(const ViscLevelHost& h) {
+std::vector<int32_t> visc_wave_records(const ViscLevelHost& h) {
     std::vector<int32_t> rec;
     rec.reserve((h.chan_leaves[0].size() + h.chan_leaves[1].size() + h.chan_leaves[2].size()) * kVRec);
     auto slot_of = [&](int32_t lf, int gid) { return lf < 0 ? -1 : h.slot[size_t(lf) * kVSlots + size_t(gid)]; };
@@ -112,7 +111,7 @@ int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
         v.chan_off[c + 1] = int32_t(cl.size());
     }
     v.chan_leaves = static_cast<const int32_t*>(put(cl.data(), cl.size() * 4));
-    const std::vector<int32_t> wrec = wave_records(h);
+    const std::vector<int32_t> wrec = visc_wave_records(h);
     v.wrec = static_cast<const int32_t*>(put(wrec.data(), wrec.size() * 4));
     v.parent = h.parent.empty() ? nullptr : static_cast<const int32_t*>(put(h.parent.data(), h.parent.size() * 4));
     v.child = h.child.empty() ? nullptr : static_cast<const int32_t*>(put(h.child.data(), h.child.size() * 4));
@@ -138,65 +137,79 @@ int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
     return first_error_ == hipSuccess ? 0 : -3;
 }
 
-int ViscDevice::build_coarsest() {
-    const ViscLevelHost& L = host_.levels.back();
+bool visc_coarsest_host(const ViscHierarchyHost& H, const int32_t* leaf_map, ViscCoarsestHost& out,
+                        std::string& err) {
+    const ViscLevelHost& L = H.levels.back();
     const int64_t n = L.ndof;
     if (n > kViscMaxCoarsest) {
-        err_ = "coarsest level larger than the single-workgroup solve supports";
-        return -1;
+        err = "coarsest level larger than the single-workgroup solve supports";
+        return false;
     }
-    std::vector<float> inv(size_t(n), 1.f);
+    out = ViscCoarsestHost{};
+    out.n = n;
+    out.inv.assign(size_t(n), 1.f);
     for (int64_t i = 0; i < n; ++i)
-        for (int64_t k = host_.cptr[i]; k < host_.cptr[i + 1]; ++k)
-            if (host_.ccol[k] == i && host_.cval[k] != 0.f) inv[i] = 1.0f / host_.cval[k];
-    std::vector<int32_t> pos(size_t(n), 0);
-    for (size_t i = 0; i < L.nleaf_dof; ++i)
+        for (int64_t k = H.cptr[i]; k < H.cptr[i + 1]; ++k)
+            if (H.ccol[k] == i && H.cval[k] != 0.f) out.inv[i] = 1.0f / H.cval[k];
+    out.pos.assign(size_t(n), 0);
+    for (size_t i = 0; i < L.nleaf_dof; ++i) {
+        const int64_t li = leaf_map ? leaf_map[i] : int64_t(i);
         for (int c = 0; c < 3; ++c) {
             int64_t d = L.dofbase[i * 3 + c];
             const uint64_t* m = &L.mask[(i * 3 + c) * 8];
             for (int off = 0; off < 512; ++off)
-                if (mask_test(m, off)) pos[size_t(d++)] = int32_t((i * 3 + c) * 512 + off);
+                if (mask_test(m, off)) out.pos[size_t(d++)] = int32_t((li * 3 + c) * 512 + off);
         }
+    }
+    int32_t width = 0;
+    for (int64_t i = 0; i < n; ++i) width = std::max<int32_t>(width, int32_t(H.cptr[i + 1] - H.cptr[i]));
+    if (width > kViscMaxRow) {
+        err = "coarsest row longer than the stencil allows";
+        return false;
+    }
+    out.width = width;
+    // stored at the full kViscMaxRow width, so the kernel's loads need no bound
+    out.ecol.assign(static_cast<size_t>(n) * kViscMaxRow, 0);
+    out.elen.assign(static_cast<size_t>(n), 0);
+    out.eval.assign(size_t(n) * kViscMaxRow, 0.f);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t b = H.cptr[i], len = H.cptr[i + 1] - b;
+        out.elen[size_t(i)] = int32_t(len);
+        for (int32_t k = 0; k < kViscMaxRow; ++k) {
+            const size_t at = size_t(k) * size_t(n) + size_t(i);
+            out.ecol[at] = k < len ? H.ccol[size_t(b + k)] : int32_t(i);
+            out.eval[at] = k < len ? H.cval[size_t(b + k)] : 0.f;
+        }
+    }
+    return true;
+}
+
+int ViscDevice::build_coarsest() {
+    ViscCoarsestHost ch;
+    if (!visc_coarsest_host(host_, nullptr, ch, err_)) return -1;
     auto put = [&](const void* src, size_t bytes) -> void* {
         void* p = arena_.take(std::max<size_t>(bytes, 4));
         if (p && bytes) check(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream_), "upload");
         return p;
     };
-    cv_.n = int32_t(n);
+    cv_.n = int32_t(ch.n);
     cv_.ptr = static_cast<const int64_t*>(put(host_.cptr.data(), host_.cptr.size() * 8));
     cv_.col = static_cast<const int32_t*>(put(host_.ccol.data(), host_.ccol.size() * 4));
     cv_.val = static_cast<const float*>(put(host_.cval.data(), host_.cval.size() * 4));
-    cv_.invdiag = static_cast<const float*>(put(inv.data(), inv.size() * 4));
-    int32_t width = 0;
-    for (int64_t i = 0; i < n; ++i) width = std::max<int32_t>(width, int32_t(host_.cptr[i + 1] - host_.cptr[i]));
-    if (width > kViscMaxRow) {
-        err_ = "coarsest row longer than the stencil allows";
-        return -1;
-    }
-    // stored at the full kViscMaxRow width, so the kernel's loads need no bound
-    std::vector<int32_t> ecol(static_cast<size_t>(n) * kViscMaxRow);
-    std::vector<int32_t> elen(static_cast<size_t>(n));
-    std::vector<float> eval(size_t(n) * kViscMaxRow, 0.f);
-    for (int64_t i = 0; i < n; ++i) {
-        const int64_t b = host_.cptr[i], len = host_.cptr[i + 1] - b;
-        elen[size_t(i)] = int32_t(len);
-        for (int32_t k = 0; k < kViscMaxRow; ++k) {
-            const size_t at = size_t(k) * size_t(n) + size_t(i);
-            ecol[at] = k < len ? host_.ccol[size_t(b + k)] : int32_t(i);
-            eval[at] = k < len ? host_.cval[size_t(b + k)] : 0.f;
-        }
-    }
-    cv_.ewidth = width;
-    cv_.ecol = static_cast<const int32_t*>(put(ecol.data(), ecol.size() * 4));
-    cv_.eval = static_cast<const float*>(put(eval.data(), eval.size() * 4));
-    cv_.elen = static_cast<const int32_t*>(put(elen.data(), elen.size() * 4));
-    cv_.pos = static_cast<const int32_t*>(put(pos.data(), pos.size() * 4));
+    cv_.invdiag = static_cast<const float*>(put(ch.inv.data(), ch.inv.size() * 4));
+    cv_.ewidth = ch.width;
+    cv_.ecol = static_cast<const int32_t*>(put(ch.ecol.data(), ch.ecol.size() * 4));
+    cv_.eval = static_cast<const float*>(put(ch.eval.data(), ch.eval.size() * 4));
+    cv_.elen = static_cast<const int32_t*>(put(ch.elen.data(), ch.elen.size() * 4));
+    cv_.pos = static_cast<const int32_t*>(put(ch.pos.data(), ch.pos.size() * 4));
     cv_.max_iters = 10;
     cv_.tol = FLT_EPSILON;
     if (!cv_.ptr || !cv_.col || !cv_.val || !cv_.invdiag || !cv_.pos || !cv_.ecol || !cv_.eval || !cv_.elen) {
         err_ = "device arena too small";
         return -2;
     }
+    // the host arrays die here
+    if (check(hipStreamSynchronize(stream_), "coarsest upload")) return -3;
     return 0;
 }
 
@@ -393,7 +406,7 @@ int ViscDevice::upload_spai0(int level) {
     check(hipMemcpyAsync(pool, h.pool.data(), h.pool.size() * 4, hipMemcpyHostToDevice, stream_), "spai0 upload");
     check(hipMemcpyAsync(slot, h.slot.data(), h.slot.size() * 4, hipMemcpyHostToDevice, stream_), "spai0 upload");
     // the wave records carry the new SPAI0 slots
-    const std::vector<int32_t> rec = wave_records(h);
+    const std::vector<int32_t> rec = visc_wave_records(h);
     int32_t* wrec = nullptr;
     if (check(hipMalloc(reinterpret_cast<void**>(&wrec), std::max<size_t>(rec.size() * 4, 4)), "spai0 alloc"))
         return -3;

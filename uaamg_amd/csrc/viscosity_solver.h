@@ -28,6 +28,24 @@ struct ViscOptions {
     hipStream_t stream = nullptr;
 };
 
+// Wave records of a level (layout at kVRecLeaf), chan_leaves order: per
+// (channel, DOF leaf) the leaf, its 27-leaf neighbourhood and the coefficient
+// slots its 15-point row reads (-1: absent, the level default).
+std::vector<int32_t> visc_wave_records(const ViscLevelHost& h);
+
+// Host arrays of the coarsest level's explicit system (ViscCoarsestView):
+// Eigen's diagonal preconditioner, the rows as padded ELL, and each DOF's
+// position in the level vector, level leaf i standing at leaf_map[i] (null:
+// at i). False with err set when the system exceeds the kernel's limits.
+struct ViscCoarsestHost {
+    int64_t n = 0;
+    int32_t width = 0;
+    std::vector<float> inv, eval;
+    std::vector<int32_t> pos, ecol, elen;
+};
+bool visc_coarsest_host(const ViscHierarchyHost& h, const int32_t* leaf_map, ViscCoarsestHost& out,
+                        std::string& err);
+
 class ViscDevice {
 public:
     ~ViscDevice();

@@ -291,3 +291,57 @@ class simd_viscosity3d:
         z = np.zeros_like(r)
         _check(lib().uaamg_viscosity_precondition(self.m_matrix_levels._h, _ptr(r), _ptr(z)), "precondition")
         return z
+
+
+class ViscBatch:
+    """Many viscosity problems merged into one leaf table per level and solved
+    together (uaamg_viscosity_batch_*): one launch per colour pass, transfer and
+    PCG vector operation for the whole batch, per-problem PCG scalars and
+    stopping rule on the device; each problem's result and m_iteration equal
+    its pcg_solve alone (simd_viscosity3d.cpp:3694-3740). `problems` is a list
+    of (viscosity, liquid_sdf, solid_sdf, dt, rho, dx)."""
+
+    def __init__(self, problems, *, max_iterations: int = 100, device: int = 0, max_coarsest_dof: int = 4000,
+                 flags: int = 0):
+        L = lib()
+        n = len(problems)
+        o = _ViscOptions()
+        L.uaamg_viscosity_options_default(C.byref(o))
+        o.device, o.max_coarsest_dof, o.flags = device, max_coarsest_dof, flags
+        o.max_iterations = int(max_iterations)
+        self._grids = [(p[0]._struct(), p[1]._struct(), p[2]._struct()) for p in problems]
+        arrs = [(C.c_void_p * n)(*[C.addressof(g[k]) for g in self._grids]) for k in range(3)]
+        dts = np.array([p[3] for p in problems], np.float32)
+        rhos = np.array([p[4] for p in problems], np.float32)
+        dxs = np.array([p[5] for p in problems], np.float32)
+        h = C.c_void_p()
+        _check(L.uaamg_viscosity_batch_create(n, arrs[0], arrs[1], arrs[2], _ptr(dts), _ptr(rhos), _ptr(dxs),
+                                              C.byref(o), C.byref(h)), "uaamg_viscosity_batch_create")
+        self._h = h
+        self.n = n
+
+    def info(self) -> dict:
+        n, nl = C.c_int32(), C.c_int32()
+        l0, nb = C.c_int64(), C.c_int64()
+        _check(lib().uaamg_viscosity_batch_info(self._h, C.byref(n), C.byref(nl), C.byref(l0), C.byref(nb)),
+               "info")
+        return {"nproblems": n.value, "nlevels": nl.value, "leaves0": l0.value, "device_bytes": nb.value}
+
+    def ndof(self, i: int) -> int:
+        return int(lib().uaamg_viscosity_batch_ndof(self._h, i))
+
+    def solve_device(self, tolerance: float, x_ptrs, b_ptrs) -> list[int]:
+        """x_ptrs / b_ptrs: device addresses of each problem's DOF vectors (x: initial guess in, result
+        out); returns each problem's m_iteration."""
+        xs = (C.c_void_p * self.n)(*x_ptrs)
+        bs = (C.c_void_p * self.n)(*b_ptrs)
+        it = np.zeros(self.n, np.int32)
+        _check(lib().uaamg_viscosity_batch_solve_device(self._h, float(tolerance), xs, bs, _ptr(it)),
+               "viscosity batch solve")
+        return it.tolist()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().uaamg_viscosity_batch_destroy(h)
+            self._h = None
