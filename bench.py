@@ -181,9 +181,13 @@ def main() -> int:
         return 3
     dev = torch.device("cuda:0")
 
+    if args.workload == "serve":
+        out = {"serving": run_serving(args, dev)}
+        out["serving_tight_budget"] = run_tight_serving(args, dev, out["serving"])
+        print(json.dumps(out), flush=True)
+        return 0
     if args.workload != "poisson":
-        fn = {"viscosity": run_viscosity, "batch": run_batch, "visc-batch": run_visc_batch,
-              "serve": run_serving}[args.workload]
+        fn = {"viscosity": run_viscosity, "batch": run_batch, "visc-batch": run_visc_batch}[args.workload]
         print(json.dumps(fn(args, dev)), flush=True)
         return 0
 
@@ -423,10 +427,7 @@ def main() -> int:
             out["viscosity_batch"] = run_visc_batch(args, dev)
     if not args.no_serve:
         out["serving"] = run_serving(args, dev)
-        # the same stream under a memory budget of half the pool's resident footprint: admission
-        # waits for memory and idle problems are evicted (least recently used first) and restored
-        tight = max(int(0.5 * out["serving"]["resident_bytes"]), int(1.05 * out["serving"]["largest_problem_bytes"]))
-        out["serving_tight_budget"] = run_serving(args, dev, budget_bytes=tight, seconds=0.5 * args.serve_seconds)
+        out["serving_tight_budget"] = run_tight_serving(args, dev, out["serving"])
     print(json.dumps(out), flush=True)
     return 0
 
@@ -621,7 +622,7 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
     base = srv.stats()
 
     tickets, lock, done = [], threading.Lock(), threading.Event()
-    results, req_pool = [], []
+    results, req_pool, errors = [], [], []
 
     def collector():
         k = 0
@@ -636,7 +637,10 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
                     continue
                 time.sleep(1e-4)
                 continue
-            results.append(srv.wait(t))
+            r = srv.wait(t)
+            results.append(r)
+            if r[0] < 0:
+                errors.append(srv.last_error())
             k += 1
 
     col = threading.Thread(target=collector, daemon=True)
@@ -675,6 +679,7 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
     return {"metric": "p50 / p99 latency (submission to completion) and solves/s, Poisson-process arrivals",
             "engine": "C++ solve server (uaamg_server_*), continuous admission",
             "requests": sent, "completed": len(results), "failed": fails, "failed_by_status": fail_detail,
+            "errors": sorted(set(errors))[:8],
             "solves_per_s": round(len(results) / wall, 2),
             "arrival_rate_per_s": round(rate, 2), "slots": args.serve_workers,
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 2) if len(results) else None,
@@ -688,6 +693,13 @@ def run_serving(args, dev, budget_bytes: int | None = None, seconds: float | Non
             "pool": [{"n": p["n"], "kind": "viscosity" if p["visc"] else "poisson", "ndof": p["ndof"]}
                      for p in pool],
             "build_s": round(t_build, 2)}
+
+
+def run_tight_serving(args, dev, first: dict) -> dict:
+    """The same stream under a memory budget of half the pool's resident footprint: admission waits
+    for memory and idle problems are evicted (least recently used first) and restored."""
+    tight = max(int(0.5 * first["resident_bytes"]), int(1.05 * first["largest_problem_bytes"]))
+    return run_serving(args, dev, budget_bytes=tight, seconds=0.5 * args.serve_seconds)
 
 
 def run_poisson_512(args, dev) -> dict:

@@ -672,7 +672,9 @@ int uaamg_server_problem_info(uaamg_server* s, int64_t problem, int64_t* ndof, i
 int uaamg_server_submit(uaamg_server* s, int64_t problem, const float* b_dev, float* x_out_dev, const float* x0_dev,
                         float tolerance, int64_t* ticket);
 /* Blocks until the ticket's solve completes. *status receives the solver's status (UAAMG_SUCCESS,
- * UAAMG_FAILED or a negative error), the times are in ms from submission (queueing, and total). */
+ * UAAMG_FAILED or a negative error), the times are in ms from submission (queueing, and total).
+ * After a negative *status, uaamg_last_error() on the same thread names the failed step and its
+ * HIP error. */
 int uaamg_server_wait(uaamg_server* s, int64_t ticket, int32_t* status, int32_t* iterations, double* queue_ms,
                       double* total_ms);
 /* uaamg_server_wait plus the pure-multigrid fallback of a Poisson problem registered with flags bit 7:

@@ -2,7 +2,8 @@
 (Poisson: solveMultigridPCG's status, iterations within 1 and 1e-5 relative L2
 against the FP32 oracle and its FP64 twin; viscosity: pcg_solve's iterations
 within 1 and 1e-5 against both); each result is the same alone and in the mix,
-under a memory budget that forces evictions too; warm starts come from the
+under a memory budget that forces evictions too, also with four workers
+evicting and re-acquiring problems all the time; warm starts come from the
 resident (or evicted and restored) solution; the server runs the same solver
 as a direct handle (bit-identical to uaamg_poisson_solve_pcg_device); a
 problem registered with FLIP's fallback (flags bit 7) is solved again by pure
@@ -114,6 +115,31 @@ def test_eviction_under_a_tight_budget(pool):
     for key, (s, it, x) in alone.items():
         s2, it2, x2 = tight[key]
         assert s == s2 == 0 and it == it2 and np.array_equal(x, x2), key
+
+
+def test_eviction_churn_on_four_slots(pool):
+    """Four workers and a budget of about two problems: problems are evicted and made resident
+    again all the time, so graph captures (a re-acquired problem's first solve) run while other
+    workers free memory, copy an evicted solution out or order their streams after the null
+    stream. No request may end in an error, and each result is the one the problem gives alone."""
+    from uaamg_amd.server import SolveServer
+    with SolveServer(slots=1) as srv:
+        pids = [_register(srv, p) for p in pool]
+        sizes = sorted(srv.problem_info(pid)["device_bytes"] for pid in pids)
+        alone = _solve_all(srv, pool, pids, mix=False, reps=1)
+    budget = sizes[-1] + sizes[-2]
+    with SolveServer(slots=4, memory_budget=budget) as srv:
+        pids = [_register(srv, p) for p in pool]
+        churn = _solve_all(srv, pool, pids, mix=True, reps=6)
+        st = srv.stats()
+    _say(f"  budget {budget} B: evictions {st['evictions']}, acquisitions {st['acquisitions']}, "
+         f"peak in flight {st['peak_inflight']}")
+    errors = {key: r[0] for key, r in churn.items() if r[0] != 0}
+    assert not errors and st["failed"] == 0, errors
+    assert st["evictions"] > 0
+    for (i, r, k), (s, it, x) in churn.items():
+        s1, it1, x1 = alone[(i, r, 0)]
+        assert s == s1 and it == it1 and np.array_equal(x, x1), (i, r, k)
 
 
 def test_warm_start_survives_eviction(pool):

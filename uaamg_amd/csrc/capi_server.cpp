@@ -104,7 +104,10 @@ int uaamg_server_wait(uaamg_server* h, int64_t ticket, int32_t* status, int32_t*
     UAAMG_GUARD_BEGIN
     if (!h) return fail(UAAMG_EINVAL, "null handle");
     int st = 0, it = 0;
-    if (int rc = h->s->wait(ticket, &st, &it, queue_ms, total_ms)) return server_status(rc, *h->s);
+    std::string msg;
+    if (int rc = h->s->wait(ticket, &st, &it, queue_ms, total_ms, nullptr, nullptr, &msg))
+        return server_status(rc, *h->s);
+    if (st < 0) (void)fail(st, msg);   // uaamg_last_error() names the failed step
     if (status) *status = st;
     if (iterations) *iterations = it;
     return 0;
@@ -116,7 +119,9 @@ int uaamg_server_wait_fallback(uaamg_server* h, int64_t ticket, int32_t* status,
     UAAMG_GUARD_BEGIN
     if (!h) return fail(UAAMG_EINVAL, "null handle");
     int st = 0, it = 0, ms = -1, mi = -1;
-    if (int rc = h->s->wait(ticket, &st, &it, queue_ms, total_ms, &ms, &mi)) return server_status(rc, *h->s);
+    std::string msg;
+    if (int rc = h->s->wait(ticket, &st, &it, queue_ms, total_ms, &ms, &mi, &msg)) return server_status(rc, *h->s);
+    if (st < 0) (void)fail(st, msg);
     if (status) *status = st;
     if (iterations) *iterations = it;
     if (mg_status) *mg_status = ms;

@@ -44,14 +44,20 @@ float scheduled_weight_iterative(int n, int i) {
 
 DeviceArena::~DeviceArena() { release(); }
 
+// The arena's hipMalloc / hipFree (device-wide synchronisation) stay apart
+// from graph captures on other threads.
 void DeviceArena::release() {
-    if (base_) (void)hipFree(base_);
+    if (base_) {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        (void)hipFree(base_);
+    }
     base_ = nullptr;
     cap_ = used_ = 0;
 }
 
 hipError_t DeviceArena::reserve(size_t bytes) {
     if (base_) return hipErrorInvalidValue;
+    std::lock_guard<std::recursive_mutex> lk(capture_lock());
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&base_), bytes);
     if (e != hipSuccess) {
         base_ = nullptr;
@@ -74,6 +80,8 @@ PoissonDevice::~PoissonDevice() { release_device(false); }
 
 void PoissonDevice::release_device(bool keep_host) {
     if (keep_host && on_device()) (void)ensure_host_coefficients();
+    // frees synchronise the device: kept apart from graph captures on other threads
+    std::lock_guard<std::recursive_mutex> lk(capture_lock());
     if (stream_) (void)hipStreamSynchronize(stream_);
     destroy_graphs();
     if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
@@ -104,6 +112,7 @@ void PoissonDevice::release_device(bool keep_host) {
 void* PoissonDevice::aux(size_t bytes) {
     if (bytes <= aux_bytes_) return aux_;
     if (stream_) (void)hipStreamSynchronize(stream_);   // no queued work still reads the old buffer
+    std::lock_guard<std::recursive_mutex> lk(capture_lock());
     if (aux_) (void)hipFree(aux_);
     aux_ = nullptr;
     aux_bytes_ = 0;
@@ -128,11 +137,13 @@ void PoissonDevice::destroy_graphs() {
 }
 
 int PoissonDevice::capture(Graph& G, const std::function<void()>& body) {
-    if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
-    body();
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(stream_, &g);
-    if (check(e, "end capture")) return -3;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
+        body();
+        if (check(hipStreamEndCapture(stream_, &g), "end capture")) return -3;
+    }
     G.g = g;
     if (check(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0), "instantiate")) return -3;
     return 0;
@@ -306,6 +317,7 @@ int PoissonDevice::acquire_device() {
     if (opt_.stream) {
         stream_ = opt_.stream;
     } else {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
         if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate")) return -3;
         own_stream_ = true;
     }
@@ -377,13 +389,16 @@ int PoissonDevice::acquire_device() {
     }
     if (check(hipMemsetAsync(x0_, 0, 10 * n0 * kLeafSize * sizeof(float), stream_), "memset")) return -3;
     if (check(hipMemsetAsync(sc_, 0, sizeof(PcgScalars), stream_), "memset")) return -3;
-    if (check(hipHostMalloc(reinterpret_cast<void**>(&sc_host_), sizeof(PcgScalars), hipHostMallocDefault),
-              "hipHostMalloc"))
-        return -3;
-    if (check(hipHostMalloc(reinterpret_cast<void**>(&mailbox_), sizeof(HostMailbox),
-                            hipHostMallocMapped | hipHostMallocCoherent),
-              "hipHostMalloc mailbox"))
-        return -3;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        if (check(hipHostMalloc(reinterpret_cast<void**>(&sc_host_), sizeof(PcgScalars), hipHostMallocDefault),
+                  "hipHostMalloc"))
+            return -3;
+        if (check(hipHostMalloc(reinterpret_cast<void**>(&mailbox_), sizeof(HostMailbox),
+                                hipHostMallocMapped | hipHostMallocCoherent),
+                  "hipHostMalloc mailbox"))
+            return -3;
+    }
     std::memset(const_cast<HostMailbox*>(mailbox_), 0, sizeof(HostMailbox));
     if (check(hipHostGetDevicePointer(reinterpret_cast<void**>(&mailbox_dev_), mailbox_, 0), "mailbox alias"))
         return -3;

@@ -122,7 +122,7 @@ int Server::submit(int64_t pid, const float* b, float* x_out, const float* x0, f
 }
 
 int Server::wait(int64_t ticket, int* status, int* iterations, double* queue_ms, double* total_ms, int* mg_status,
-                 int* mg_iterations) {
+                 int* mg_iterations, std::string* error) {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = results_.find(ticket);
     if (it == results_.end()) {
@@ -138,6 +138,7 @@ int Server::wait(int64_t ticket, int* status, int* iterations, double* queue_ms,
     if (total_ms) *total_ms = r.total_ms;
     if (mg_status) *mg_status = r.mg_status;
     if (mg_iterations) *mg_iterations = r.mg_iterations;
+    if (error) *error = r.error;
     return 0;
 }
 
@@ -171,10 +172,17 @@ bool Server::plan_room(size_t need, std::vector<Problem*>& victims) {
 }
 
 void Server::evict(Problem& p) {
+    // frees synchronise the device: kept apart from other workers' graph captures
+    std::lock_guard<std::recursive_mutex> lk(capture_lock());
     if (p.x) {
         if (p.has_solution) {
+            // on the problem's own stream (the null stream must stay clear of captures)
+            hipStream_t s = p.visc ? p.vd->stream() : p.pd->stream();
             p.x_host.resize(size_t(p.ndof));
-            (void)hipMemcpy(p.x_host.data(), p.x, size_t(p.ndof) * sizeof(float), hipMemcpyDeviceToHost);
+            if (hipMemcpyAsync(p.x_host.data(), p.x, size_t(p.ndof) * sizeof(float), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                p.has_solution = false;   // restart from zero rather than from a partial copy
         }
         (void)hipFree(p.x);
         p.x = nullptr;
@@ -183,28 +191,52 @@ void Server::evict(Problem& p) {
     else p.pd->release_device();
 }
 
-int Server::make_resident(Problem& p) {
+namespace {
+int hip_step(hipError_t e, const char* what, std::string& msg) {
+    if (e == hipSuccess) return 0;
+    msg = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? -2 : -3;
+}
+}  // namespace
+
+int Server::make_resident(Problem& p, std::string& msg) {
     int rc = p.visc ? p.vd->acquire_device() : p.pd->acquire_device();
-    if (rc < 0) return rc;
+    if (rc < 0) {
+        msg = "make resident: " + (p.visc ? p.vd->error() : p.pd->error());
+        return rc;
+    }
     const size_t nb = std::max<size_t>(size_t(p.ndof), 1) * sizeof(float);
-    if (hipMalloc(reinterpret_cast<void**>(&p.x), nb) != hipSuccess) {
+    hipError_t me;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        me = hipMalloc(reinterpret_cast<void**>(&p.x), nb);
+    }
+    if (int e = hip_step(me, "make resident: solution hipMalloc", msg)) {
         p.x = nullptr;
-        return -2;
+        return e;
     }
     hipStream_t s = p.visc ? p.vd->stream() : p.pd->stream();
     hipError_t e = p.has_solution ? hipMemcpyAsync(p.x, p.x_host.data(), nb, hipMemcpyHostToDevice, s)
                                   : hipMemsetAsync(p.x, 0, nb, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    return e == hipSuccess ? 0 : -3;
+    return hip_step(e, "make resident: restore the solution", msg);
 }
 
-int Server::run(Problem& p, const Request& r, int* iterations, int* mg_status, int* mg_iterations) {
+int Server::run(Problem& p, const Request& r, int* iterations, int* mg_status, int* mg_iterations,
+                std::string& msg) {
     const size_t nb = size_t(p.ndof) * sizeof(float);
     hipStream_t s = p.visc ? p.vd->stream() : p.pd->stream();
+    auto solver_error = [&](const char* what) {
+        msg = std::string(what) + ": " + (p.visc ? p.vd->error() : p.pd->error());
+    };
     // the caller's x0 and b may still be in production on its null stream:
     // the copy of x0 below is the first read of either
-    if ((p.visc ? p.vd->follow_caller() : p.pd->follow_caller()) != 0) return -3;
-    if (r.x0 && nb && hipMemcpyAsync(p.x, r.x0, nb, hipMemcpyDeviceToDevice, s) != hipSuccess) return -3;
+    if ((p.visc ? p.vd->follow_caller() : p.pd->follow_caller()) != 0) {
+        solver_error("order after the caller's stream");
+        return -3;
+    }
+    if (r.x0 && nb)
+        if (int e = hip_step(hipMemcpyAsync(p.x, r.x0, nb, hipMemcpyDeviceToDevice, s), "copy x0", msg)) return e;
     int rc;
     if (p.visc) {
         p.vd->options().tolerance = r.tol;
@@ -215,17 +247,24 @@ int Server::run(Problem& p, const Request& r, int* iterations, int* mg_status, i
         if (rc == PoissonDevice::kFailed && (p.pd->options().flags & 128)) {
             // FLIP_vdb.cpp:3150-3155: zero the pressure and solve by pure
             // multigrid; the request keeps the PCG's FAILED status
-            if (hipMemsetAsync(p.x, 0, nb, s) != hipSuccess) return -3;
+            if (int e = hip_step(hipMemsetAsync(p.x, 0, nb, s), "zero the pressure", msg)) return e;
             const int mg = p.pd->solve_mg_dev(p.x, r.b, mg_iterations);
-            if (mg < 0) return mg;
+            if (mg < 0) {
+                solver_error("pure multigrid");
+                return mg;
+            }
             *mg_status = mg;
         }
     }
-    if (rc < 0) return rc;
+    if (rc < 0) {
+        solver_error("solve");
+        return rc;
+    }
     p.has_solution = true;
     if (r.x_out && nb) {
-        if (hipMemcpyAsync(r.x_out, p.x, nb, hipMemcpyDeviceToDevice, s) != hipSuccess) return -3;
-        if (hipStreamSynchronize(s) != hipSuccess) return -3;
+        if (int e = hip_step(hipMemcpyAsync(r.x_out, p.x, nb, hipMemcpyDeviceToDevice, s), "copy x out", msg))
+            return e;
+        if (int e = hip_step(hipStreamSynchronize(s), "synchronize", msg)) return e;
     }
     return rc;
 }
@@ -269,11 +308,12 @@ void Server::worker() {
         lk.unlock();
 
         for (Problem* v : victims) evict(*v);
-        int rc = acquire ? make_resident(*p) : 0;
+        std::string msg;
+        int rc = acquire ? make_resident(*p, msg) : 0;
         const bool acquire_failed = acquire && rc != 0;
         if (acquire_failed) evict(*p);   // back to host-only
         int its = 0, mg_st = -1, mg_its = -1;
-        if (rc == 0) rc = run(*p, r, &its, &mg_st, &mg_its);
+        if (rc == 0) rc = run(*p, r, &its, &mg_st, &mg_its, msg);
         const auto t_end = clock::now();
 
         lk.lock();
@@ -294,6 +334,7 @@ void Server::worker() {
         res.iterations = its;
         res.mg_status = mg_st;
         res.mg_iterations = mg_st < 0 ? -1 : mg_its;
+        if (rc < 0) res.error = msg.empty() ? "server error" : msg;
         res.queue_ms = ms_between(r.submitted, t_start);
         res.total_ms = ms_between(r.submitted, t_end);
         ++st_.completed;

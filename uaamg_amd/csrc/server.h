@@ -67,8 +67,9 @@ public:
     int submit(int64_t pid, const float* b, float* x_out, const float* x0, float tolerance, int64_t* ticket);
     // Blocks until the ticket completes; status is the solver's (0 SUCCESS,
     // 1 FAILED, < 0 error); times in milliseconds from submission.
+    // A negative status comes with the failed step and its HIP error in *error.
     int wait(int64_t ticket, int* status, int* iterations, double* queue_ms, double* total_ms,
-             int* mg_status = nullptr, int* mg_iterations = nullptr);
+             int* mg_status = nullptr, int* mg_iterations = nullptr, std::string* error = nullptr);
     ServerStats stats();
     const std::string& error() const { return err_; }
 
@@ -100,11 +101,12 @@ private:
         int status = 0, iterations = 0;
         int mg_status = -1, mg_iterations = -1;   // the pure-multigrid fallback's (-1: not run)
         double queue_ms = 0, total_ms = 0;
+        std::string error;                        // the failed step, for a negative status
     };
 
     void worker();
-    int run(Problem& p, const Request& r, int* iterations, int* mg_status, int* mg_iterations);
-    int make_resident(Problem& p);
+    int run(Problem& p, const Request& r, int* iterations, int* mg_status, int* mg_iterations, std::string& msg);
+    int make_resident(Problem& p, std::string& msg);
     void evict(Problem& p);
     // under the lock: picks victims so that `need` more bytes fit; false if impossible now
     bool plan_room(size_t need, std::vector<Problem*>& victims);

@@ -459,10 +459,13 @@ void ViscBatch::issue_final() {
 }
 
 int ViscBatch::capture(Graph& G, const std::function<void()>& body) {
-    if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
-    body();
     hipGraph_t g = nullptr;
-    if (check(hipStreamEndCapture(stream_, &g), "end capture")) return -3;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
+        body();
+        if (check(hipStreamEndCapture(stream_, &g), "end capture")) return -3;
+    }
     G.g = g;
     if (check(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0), "instantiate")) return -3;
     return 0;

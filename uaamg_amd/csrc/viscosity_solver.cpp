@@ -20,6 +20,8 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 ViscDevice::~ViscDevice() { release_device(); }
 
 void ViscDevice::release_device() {
+    // frees synchronise the device: kept apart from graph captures on other threads
+    std::lock_guard<std::recursive_mutex> lk(capture_lock());
     if (stream_) (void)hipStreamSynchronize(stream_);
     destroy_graphs();
     for (float* p : spai0_bufs_) (void)hipFree(p);
@@ -258,6 +260,7 @@ int ViscDevice::acquire_device() {
     if (check(hipSetDevice(opt_.device), "hipSetDevice")) return -3;
     if (opt_.stream) stream_ = opt_.stream;
     else {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
         if (check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "stream")) return -3;
         own_stream_ = true;
     }
@@ -319,10 +322,13 @@ int ViscDevice::acquire_device() {
     }
     for (float* p : {x0_, b0_, r0_, p0_, z0_}) check(hipMemsetAsync(p, 0, std::max<size_t>(vec0, 4), stream_), "memset");
     check(hipMemsetAsync(sc_, 0, sizeof(PcgScalars), stream_), "memset");
-    if (check(hipHostMalloc(reinterpret_cast<void**>(&mailbox_), sizeof(HostMailbox),
-                            hipHostMallocMapped | hipHostMallocCoherent),
-              "mailbox"))
-        return -3;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        if (check(hipHostMalloc(reinterpret_cast<void**>(&mailbox_), sizeof(HostMailbox),
+                                hipHostMallocMapped | hipHostMallocCoherent),
+                  "mailbox"))
+            return -3;
+    }
     mailbox_->word = 0;
     if (check(hipHostGetDevicePointer(reinterpret_cast<void**>(&mailbox_dev_), mailbox_, 0), "mailbox map")) return -3;
     if (int rc = build_coarsest()) return rc;
@@ -530,10 +536,13 @@ void ViscDevice::destroy_graphs() {
 }
 
 int ViscDevice::capture(Graph& G, const std::function<void()>& body) {
-    if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
-    body();
     hipGraph_t g = nullptr;
-    if (check(hipStreamEndCapture(stream_, &g), "end capture")) return -3;
+    {
+        std::lock_guard<std::recursive_mutex> lk(capture_lock());
+        if (check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return -3;
+        body();
+        if (check(hipStreamEndCapture(stream_, &g), "end capture")) return -3;
+    }
     G.g = g;
     if (check(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0), "instantiate")) return -3;
     return 0;

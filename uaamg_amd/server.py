@@ -114,6 +114,11 @@ class SolveServer:
                "wait")
         return int(st.value), int(it.value), float(q.value), float(tot.value)
 
+    @staticmethod
+    def last_error() -> str:
+        """After wait() returned a negative status on this thread: the failed step and its HIP error."""
+        return _bind().uaamg_last_error().decode(errors="replace")
+
     def wait_fallback(self, ticket: int) -> tuple[int, int, int, int, float, float]:
         """(status, iterations, fallback status, fallback iterations, queue ms, total ms): wait() plus
         the pure-multigrid fallback of a problem registered with flags bit 7 (-1s when it did not run)."""
