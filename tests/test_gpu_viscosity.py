@@ -140,3 +140,27 @@ def test_graph_replay_bit_identical(problem):
     for x, it in out[1:]:
         assert it == out[0][1]
         assert np.array_equal(x, out[0][0])
+
+
+def test_wide_kernels_bit_identical(problem, monkeypatch):
+    """UAAMG_VWIDE=1 runs every level's passes on the two-waves-per-SIMD
+    kernels (chosen by default only for levels wider than one wave per SIMD):
+    a register budget, not arithmetic, so the operator applications and the
+    solve are bit-identical."""
+    s, lw, orc, b = problem
+    out = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("UAAMG_VWIDE", wide)
+        l2 = U.L_with_level(s.viscosity, s.liquid, s.solid, s.dt, s.rho, s.dx)
+        solver = U.simd_viscosity3d(l2, s.velocity)
+        x = l2.get_zero_vec()
+        solver.pcg_solve(x, 1e-6)
+        rng = np.random.default_rng(3)
+        xr = rng.standard_normal(l2.level_ndof(0)).astype(np.float32)
+        ap = [l2.apply(0, m, xr, b) for m in (0, 1, 2, 4, 5)]
+        out.append((x.data.copy(), solver.m_iteration, ap))
+    monkeypatch.delenv("UAAMG_VWIDE")
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
+    for a0, a1 in zip(out[0][2], out[1][2]):
+        assert np.array_equal(a0, a1)

@@ -12,6 +12,9 @@
 // leaves (simd_viscosity3d.cpp:2991-3000, vdb_SIMD_IO.h:5-191).
 #pragma once
 
+#include <algorithm>
+#include <cstdlib>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -47,7 +50,23 @@ struct ViscLevelView {
     const int32_t* parent;     // [nleaf_dof] coarse leaf * 8 + octant
     const int32_t* child;      // [nleaf_dof][8]
     float dflt[32];
+    // 1: the colour passes and appliers launch more waves than one wave per
+    // SIMD holds at once; their kernels then keep two waves per SIMD
+    // resident (at most 256 registers each) rather than all of one row's
+    // loads in flight in one wave
+    int32_t wide;
 };
+
+// Whether a level's passes take the two-waves-per-SIMD kernels: when a
+// channel's pass launches more waves (one per leaf) than one wave per SIMD
+// of the device holds at once (256 CUs x 4 SIMDs). UAAMG_VWIDE=0 / =1 forces
+// the choice for every level (A/B timing; results are the same either way).
+inline int32_t visc_wide(const ViscLevelView& v) {
+    if (const char* env = std::getenv("UAAMG_VWIDE")) return env[0] == '1' ? 1 : 0;
+    int32_t widest = 0;
+    for (int c = 0; c < 3; ++c) widest = std::max(widest, v.chan_off[c + 1] - v.chan_off[c]);
+    return widest > 1024 ? 1 : 0;
+}
 
 // applier modes (light_weight_applier working_mode, simd_viscosity3d.h:13)
 enum ViscMode : int { kVNormal = 0, kVResidual = 1, kVJacobi = 2, kVSpai0Mode = 3, kVRedGS = 4, kVBlackGS = 5 };

@@ -328,6 +328,28 @@ __global__ __launch_bounds__(kVThreads) void visc_apply_kernel(ViscLevelView L, 
     visc_row<MODE, C>(L, L.chan_off[C] + w, threadIdx.x & 63, Xv, Bv, Yv);
 }
 
+// The same two kernels for levels with many leaves (L.wide): two waves per
+// SIMD (the register budget halves; the compiler keeps fewer of a row's loads
+// in flight per wave, the second wave hides their latency). Same arithmetic,
+// same results.
+template <int MODE, class V>
+__global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(2))) void visc_apply_all_wide_kernel(
+    ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
+    const int e = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (e < L.chan_off[1]) visc_row<MODE, 0>(L, e, lane, Xv, Bv, Yv);
+    else if (e < L.chan_off[2]) visc_row<MODE, 1>(L, e, lane, Xv, Bv, Yv);
+    else if (e < L.chan_off[3]) visc_row<MODE, 2>(L, e, lane, Xv, Bv, Yv);
+}
+
+template <int MODE, int C, class V>
+__global__ __launch_bounds__(kVThreads) __attribute__((amdgpu_waves_per_eu(2))) void visc_apply_wide_kernel(
+    ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
+    const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
+    if (w >= L.chan_off[C + 1] - L.chan_off[C]) return;
+    visc_row<MODE, C>(L, L.chan_off[C] + w, threadIdx.x & 63, Xv, Bv, Yv);
+}
+
 // restriction of one (coarse leaf, channel) row: 1/8 of the on fine children.
 // The row's two children (z < 4 and z >= 4) and their rows are loaded
 // unconditionally (an absent child reads leaf 0 and is dropped), so the
@@ -406,7 +428,7 @@ __device__ __forceinline__ void visc_prolong_row(const ViscLevelView& F, int lea
 // channel of a level has at most a few dozen leaves, one workgroup runs the
 // whole half-visit with a workgroup barrier between passes: the same row
 // routine, the same pass order, the same values.
-constexpr int kSmallThreads = 512;
+constexpr int kSmallThreads = 256;
 constexpr int kSmallWaves = kSmallThreads / 64;
 
 template <int MODE, int C, class V>
@@ -716,6 +738,14 @@ hipError_t apply_mode(const ViscLevelView& L, int c, const V* x, const V* b, V* 
     const int n = L.chan_off[c + 1] - L.chan_off[c];
     if (n <= 0) return hipSuccess;
     const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
+    if (L.wide) {
+        switch (c) {
+            case 0: visc_apply_wide_kernel<MODE, 0, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+            case 1: visc_apply_wide_kernel<MODE, 1, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+            default: visc_apply_wide_kernel<MODE, 2, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+        }
+        return hipGetLastError();
+    }
     switch (c) {
         case 0: visc_apply_kernel<MODE, 0, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
         case 1: visc_apply_kernel<MODE, 1, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
@@ -743,6 +773,16 @@ hipError_t apply_all(const ViscLevelView& L, int mode, const V* x, const V* b, V
     const int n = L.chan_off[3];
     if (n <= 0) return hipSuccess;
     const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
+    if (L.wide) {
+        switch (mode) {
+            case kVNormal: visc_apply_all_wide_kernel<kVNormal, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVResidual: visc_apply_all_wide_kernel<kVResidual, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVJacobi: visc_apply_all_wide_kernel<kVJacobi, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVSpai0Mode: visc_apply_all_wide_kernel<kVSpai0Mode, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (mode) {
         case kVNormal: visc_apply_all_kernel<kVNormal, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
         case kVResidual: visc_apply_all_kernel<kVResidual, V><<<g, kVThreads, 0, s>>>(L, x, b, y); break;
