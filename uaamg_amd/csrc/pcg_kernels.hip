@@ -224,6 +224,46 @@ __global__ __launch_bounds__(kBlockThreads) void absmax_kernel(LevelView L, cons
 
 constexpr int kFinThreads = 256;
 
+// Thread t's share of the partials: t, t + kFinThreads, ... For a sum the
+// additions run in that order (the fixed order every reduction here uses);
+// the loads are issued kFinAhead at a time so that they overlap, which does
+// not change any sum. Level 0 at 512^3 has about 24500 partials: one load at
+// a time took 35 us per reduction.
+constexpr int kFinAhead = 8;
+
+__device__ __forceinline__ double fin_sum(const double* part, int n) {
+    double t = 0;
+    int i = threadIdx.x;
+    for (; i + (kFinAhead - 1) * kFinThreads < n; i += kFinAhead * kFinThreads) {
+        double p[kFinAhead];
+#pragma unroll
+        for (int k = 0; k < kFinAhead; ++k) p[k] = part[i + k * kFinThreads];
+#pragma unroll
+        for (int k = 0; k < kFinAhead; ++k) t += p[k];
+    }
+    for (; i < n; i += kFinThreads) t += part[i];
+    return t;
+}
+
+// The largest partial maximum (v) and whether any partial flags a NaN / Inf (b).
+__device__ __forceinline__ void fin_max(const double* part, int n, float& v, int& b) {
+    v = 0.f;
+    b = 0;
+    auto take = [&](double p) {
+        if (p < 0) b = 1;
+        else v = fmaxf(v, float(p));
+    };
+    int i = threadIdx.x;
+    for (; i + (kFinAhead - 1) * kFinThreads < n; i += kFinAhead * kFinThreads) {
+        double p[kFinAhead];
+#pragma unroll
+        for (int k = 0; k < kFinAhead; ++k) p[k] = part[i + k * kFinThreads];
+#pragma unroll
+        for (int k = 0; k < kFinAhead; ++k) take(p[k]);
+    }
+    for (; i < n; i += kFinThreads) take(part[i]);
+}
+
 // Fixed-order reduction of n partials and the scalar update of `op`:
 // rho / (rho_new, beta = rho_new / rho, rho) / (sigma, alpha = rho / sigma) /
 // nu, the last also published to the host mailbox as one 64-bit store.
@@ -232,13 +272,9 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int n, co
     if (op == kFinNu) {
         __shared__ float tv[kFinThreads];
         __shared__ int tb[kFinThreads];
-        float v = 0.f;
-        int b = 0;
-        for (int i = threadIdx.x; i < n; i += kFinThreads) {
-            const double p = partials[i];
-            if (p < 0) b = 1;
-            else v = fmaxf(v, float(p));
-        }
+        float v;
+        int b;
+        fin_max(partials, n, v, b);
         tv[threadIdx.x] = v;
         tb[threadIdx.x] = b;
         __syncthreads();
@@ -261,9 +297,7 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(int op, int n, co
         }
         return;
     }
-    double t = 0;
-    for (int i = threadIdx.x; i < n; i += kFinThreads) t += partials[i];
-    const double total = block_sum<kFinThreads>(t);
+    const double total = block_sum<kFinThreads>(fin_sum(partials, n));
     if (threadIdx.x != 0) return;
     switch (op) {
         case kFinRho: sc->rho = total; break;
@@ -536,13 +570,9 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
         op == kBFinNuViscIter) {
         __shared__ float tv[kFinThreads];
         __shared__ int tb[kFinThreads];
-        float v = 0.f;
-        int b = 0;
-        for (int i = threadIdx.x; i < n; i += kFinThreads) {
-            const double p = part[i];
-            if (p < 0) b = 1;
-            else v = fmaxf(v, float(p));
-        }
+        float v;
+        int b;
+        fin_max(part, n, v, b);
         tv[threadIdx.x] = v;
         tb[threadIdx.x] = b;
         __syncthreads();
@@ -628,9 +658,7 @@ __global__ __launch_bounds__(kFinThreads) void bfinalize_kernel(int op, const in
         return;
     }
     if (sp.state != kBActive) return;
-    double t = 0;
-    for (int i = threadIdx.x; i < n; i += kFinThreads) t += part[i];
-    const double total = block_sum<kFinThreads>(t);
+    const double total = block_sum<kFinThreads>(fin_sum(part, n));
     if (threadIdx.x != 0) return;
     switch (op) {
         case kBFinRho: sp.rho = total; break;
