@@ -624,6 +624,53 @@ float uaamg_volume_fraction8(const float* phi8);
 /* Number of HIP devices visible (0 when none; never fails). */
 int uaamg_device_count(void);
 
+/* ================================================================ OpenVDB files
+ * Reads an OpenVDB file (file format versions 222-224: OpenVDB 6-10, Houdini 18-20) without
+ * OpenVDB, for the grids FLIP reads with openvdb::io::File (the liquid source and collision level
+ * sets and their velocity grids, FLIP_vdb/main.cpp:108-131,161-185,192-213) and writes
+ * (sdfvel<n>.vdb: liquid SDF and the three velocity channels; flip_particle<n>.vdb: the interior
+ * SDF beside a point grid, FLIP_vdb/main.cpp:266-284). Float and Vec3f grids (full or half float;
+ * uncompressed, zlib or Blosc with the LZ4 or zlib codec, active-mask compression) are decoded;
+ * other grid types (point data, integer and mask grids) are listed with decoded = 0.
+ */
+typedef struct uaamg_vdb_file uaamg_vdb_file;
+typedef struct uaamg_vdb_grid_desc {
+    const char* name;        /* grid name (valid until uaamg_vdb_close) */
+    const char* type;        /* tree type, e.g. "Tree_float_5_4_3" */
+    const char* grid_class;  /* metadata "class": "level set", "fog volume", ... or "" */
+    int32_t decoded;         /* 1: leaves available through uaamg_vdb_grid */
+    int32_t ncomp;           /* 1 (float) or 3 (Vec3f); 0 for other types */
+    int32_t half_float;      /* stored as half floats (read widened) */
+    int64_t nleaf;           /* leaf nodes */
+    int64_t ntiles;          /* root / internal tiles that are active or differ from the background */
+    double voxel_size[3];    /* from the grid's transform */
+    double translation[3];
+    float background[3];
+    int64_t meta_voxel_count; /* metadata "file_voxel_count" (-1 when absent) */
+    int32_t meta_bbox[6];     /* metadata "file_bbox_min", "file_bbox_max" (index space) */
+    int32_t has_meta_bbox;
+} uaamg_vdb_grid_desc;
+int uaamg_vdb_open(const char* path, uaamg_vdb_file** out);
+void uaamg_vdb_close(uaamg_vdb_file* f);
+int32_t uaamg_vdb_grid_count(const uaamg_vdb_file* f);
+int uaamg_vdb_grid_info(const uaamg_vdb_file* f, int32_t index, uaamg_vdb_grid_desc* info);
+/* The grid's leaves, in OpenVDB tree order, as a uaamg_grid whose arrays belong to the file handle
+ * (valid until uaamg_vdb_close). Tiles are not expanded into leaves: see uaamg_vdb_grid_tiles. */
+int uaamg_vdb_grid(const uaamg_vdb_file* f, int32_t index, uaamg_grid* out);
+/* The tiles: origins [ntiles][3], edge in voxels [ntiles] (8, 128 or 4096), values [ntiles][3],
+ * active [ntiles]; any array may be NULL. */
+int uaamg_vdb_grid_tiles(const uaamg_vdb_file* f, int32_t index, int32_t* origins, int32_t* dims, float* values,
+                         int32_t* active);
+/* Turns the grid's tiles of edge <= max_dim (8, 128 or 4096) into leaves holding the tile value
+ * (all voxels active when the tile is), in tree order, so that the solvers' lookups outside the
+ * leaves see what an OpenVDB accessor returns there (a level set's interior tiles: -background);
+ * *added receives the number of leaves added. UAAMG_ENOMEM (nothing changed) when the grid would
+ * exceed max_leaves. Earlier uaamg_vdb_grid views of the grid are invalidated. */
+int uaamg_vdb_expand_tiles(uaamg_vdb_file* f, int32_t index, int32_t max_dim, int64_t max_leaves, int64_t* added);
+/* The Blosc 1 decoder the reader uses (LZ4 or zlib codec, byte shuffle): decompresses the Blosc
+ * buffer src[srclen] into exactly dstlen bytes. 0 or UAAMG_EINVAL (malformed or unsupported). */
+int uaamg_blosc_decompress(const void* src, int64_t srclen, void* dst, int64_t dstlen);
+
 /* ================================================================ solve server
  * A request loop over many resident problems on one GPU (no reference counterpart: FLIP_vdb
  * builds a solver per time step and calls it synchronously, FLIP_vdb.cpp:3126-3208). Problems
