@@ -13,7 +13,12 @@ decode path against what the file itself states and what its geometry fixes:
 The Blosc and LZ4 decoders are also checked on buffers made here from their
 published formats (a greedy LZ4 encoder, the Blosc 1 container with and
 without shuffle, splits and short last blocks, the zlib codec, buffers
-stored uncompressed), and malformed inputs fail cleanly.
+stored uncompressed), and malformed inputs fail cleanly. Files written here
+from the published layout cover what the fixture does not: Vec3f grids, half
+floats, zlib and no compression, every inactive-value code of the
+active-mask compression, tiles at the root and both internal levels, several
+grids in one file and a grid type the reader skips; every leaf, mask, value
+and tile reads back exactly.
 """
 import struct
 import zlib
@@ -257,3 +262,213 @@ def test_malformed_buffers_fail_cleanly(tmp_path):
     p.write_bytes(b"\0" * 64)
     with pytest.raises(U.UaamgError):
         read_vdb(str(p))
+
+
+# ---- files written here from the published layout: the paths the fixture leaves open
+# (Vec3f grids, half floats, zlib, no compression, every inactive-value code, tiles at every level,
+# several grids, a grid type the reader skips)
+
+def _vdb_str(s: str) -> bytes:
+    b = s.encode()
+    return struct.pack("<I", len(b)) + b
+
+
+def _codec(raw: bytes, comp: int) -> bytes:
+    if comp & 4:
+        buf = _blosc_pack(raw, 2 if len(raw) % 4 else 4, 4096) if raw else b""
+        return struct.pack("<q", len(buf)) + buf if raw else struct.pack("<q", 0)
+    if comp & 1:
+        z = zlib.compress(raw)
+        return struct.pack("<q", len(z)) + z if raw else struct.pack("<q", 0)
+    return raw
+
+
+def _compressed_values(vals: np.ndarray, vmask: np.ndarray, bg: np.ndarray, comp: int, half: bool) -> bytes:
+    """writeCompressedValues: the metadata byte, inactive values and selection mask, then the
+    (active, with mask compression) values through the codec. vals [count, nc]"""
+    nc = vals.shape[1]
+    out = bytearray()
+    meta, inact = 6, []
+    sel = None
+    if comp & 2:
+        ina = vals[~vmask]
+        uniq = [] if len(ina) == 0 else [tuple(v) for v in np.unique(ina, axis=0)]
+        bgt, mbg = tuple(bg), tuple(-bg)
+        if not uniq or uniq == [bgt]:
+            meta = 0
+        elif uniq == [mbg]:
+            meta = 1
+        elif len(uniq) == 1:
+            meta, inact = 2, [uniq[0]]
+        elif len(uniq) == 2 and set(uniq) == {bgt, mbg}:
+            meta, sel = 3, np.all(vals == bg, axis=1)
+        elif len(uniq) == 2 and bgt in uniq:
+            other = uniq[0] if uniq[1] == bgt else uniq[1]
+            meta, inact, sel = 4, [other], np.all(vals == bg, axis=1)
+        elif len(uniq) == 2:
+            v0, v1 = uniq
+            meta, inact, sel = 5, [v0, v1], np.all(vals == np.array(v1, np.float32), axis=1)
+    out.append(meta)
+    for v in inact:
+        out += np.array(v, np.float32).tobytes()
+    if sel is not None:
+        out += np.packbits(sel & ~vmask, bitorder="little").tobytes()
+    data = vals[vmask] if (comp & 2) and meta != 6 else vals
+    raw = data.astype(np.float16 if half else np.float32).tobytes()
+    out += _codec(raw, comp)
+    return bytes(out)
+
+
+def _write_vdb(grids) -> bytes:
+    """grids: dicts with name, type, comp, half, bg, voxel, leaves {origin: (vals [512,nc], mask [512])},
+    i4_tiles [(origin, value, active)], i5_tiles, root_tiles; or {name, type: other} (skipped type)."""
+    head = struct.pack("<qIII", 0x56444220, 224, 10, 0) + b"\x01" + b"0" * 36 + struct.pack("<I", 0)
+    head += struct.pack("<I", len(grids))
+    # lay out: header, then per grid descriptor + body (offsets absolute)
+    out = bytearray(head)
+    for g in grids:
+        if g["type"] not in ("Tree_float_5_4_3", "Tree_vec3s_5_4_3"):
+            body = b"\0" * 40
+            tname = g["type"]
+            desc = _vdb_str(g["name"]) + _vdb_str(tname) + _vdb_str("")
+            gpos = len(out) + len(desc) + 24
+            out += desc + struct.pack("<qqq", gpos, gpos, gpos + len(body)) + body
+            continue
+        nc = 1 if g["type"] == "Tree_float_5_4_3" else 3
+        comp, half = g["comp"], g["half"]
+        bg = np.array(g["bg"], np.float32).reshape(nc)
+        meta = _vdb_str("class") + _vdb_str("string") + _vdb_str("fog volume")
+        body = bytearray(struct.pack("<I", comp) + struct.pack("<I", 1) + meta)
+        body += _vdb_str("UniformScaleMap") + struct.pack("<15d", *([g["voxel"]] * 3 + [g["voxel"]] * 3 +
+                                                                     [1 / g["voxel"]] * 3 + [1.0] * 3 + [0.5] * 3))
+        body += struct.pack("<i", 1) + bg.tobytes()
+        roots = sorted({tuple(np.array(o) >> 12 << 12) for o in g["leaves"]} |
+                       {tuple(np.array(o) >> 12 << 12) for o, _, _ in g["i4_tiles"] + g["i5_tiles"]})
+        body += struct.pack("<II", len(g["root_tiles"]), len(roots))
+        for o, v, a in g["root_tiles"]:
+            body += struct.pack("<3i", *o) + np.array(v, np.float32).reshape(nc).tobytes() + bytes([a])
+        leaf_order = []
+        for r in roots:
+            body += struct.pack("<3i", *r)
+            # internal 5
+            c5 = np.zeros(32768, bool)
+            v5 = np.zeros(32768, bool)
+            vals5 = np.tile(bg, (32768, 1))
+            i4s = {}
+            for o in list(g["leaves"]) + [t[0] for t in g["i4_tiles"]]:
+                if tuple(np.array(o) >> 12 << 12) != r:
+                    continue
+                i = ((o[0] >> 7) & 31) << 10 | ((o[1] >> 7) & 31) << 5 | ((o[2] >> 7) & 31)
+                c5[i] = True
+                i4s.setdefault(i, tuple(np.array(o) >> 7 << 7))
+            for o, v, a in g["i5_tiles"]:
+                if tuple(np.array(o) >> 12 << 12) != r:
+                    continue
+                i = ((o[0] >> 7) & 31) << 10 | ((o[1] >> 7) & 31) << 5 | ((o[2] >> 7) & 31)
+                vals5[i] = v
+                v5[i] = a
+            body += np.packbits(c5, bitorder="little").tobytes() + np.packbits(v5, bitorder="little").tobytes()
+            body += _compressed_values(vals5, v5, bg, comp, half)
+            for i in sorted(i4s):
+                o4 = i4s[i]
+                c4 = np.zeros(4096, bool)
+                v4 = np.zeros(4096, bool)
+                vals4 = np.tile(bg, (4096, 1))
+                for o in g["leaves"]:
+                    if tuple(np.array(o) >> 7 << 7) == o4:
+                        c4[((o[0] >> 3) & 15) << 8 | ((o[1] >> 3) & 15) << 4 | ((o[2] >> 3) & 15)] = True
+                for o, v, a in g["i4_tiles"]:
+                    if tuple(np.array(o) >> 7 << 7) == o4:
+                        j = ((o[0] >> 3) & 15) << 8 | ((o[1] >> 3) & 15) << 4 | ((o[2] >> 3) & 15)
+                        vals4[j] = v
+                        v4[j] = a
+                body += np.packbits(c4, bitorder="little").tobytes() + np.packbits(v4, bitorder="little").tobytes()
+                body += _compressed_values(vals4, v4, bg, comp, half)
+                for j in np.nonzero(c4)[0]:
+                    o = (o4[0] + ((j >> 8) & 15) * 8, o4[1] + ((j >> 4) & 15) * 8, o4[2] + (j & 15) * 8)
+                    body += np.packbits(g["leaves"][o][1], bitorder="little").tobytes()
+                    leaf_order.append(o)
+        bpos_rel = len(body)
+        for o in leaf_order:
+            vals, m = g["leaves"][o]
+            body += np.packbits(m, bitorder="little").tobytes() + _compressed_values(vals, m, bg, comp, half)
+        tname = g["type"] + ("_HalfFloat" if half else "")
+        desc = _vdb_str(g["name"]) + _vdb_str(tname) + _vdb_str("")
+        gpos = len(out) + len(desc) + 24
+        out += desc + struct.pack("<qqq", gpos, gpos + bpos_rel, gpos + len(body)) + bytes(body)
+    return bytes(out)
+
+
+def _make_grid(rng, nc, comp, half, name):
+    bg = np.full(nc, 0.5, np.float32)
+    leaves = {}
+    origins = [(0, 0, 0), (8, 0, 0), (0, 8, 16), (120, 64, 8), (128, 0, 0), (-8, -8, -8), (4096, 16, 24),
+               (-4096, 0, 4088), (256, 256, 256)]
+    patterns = ["bg", "minus_bg", "one", "pm_bg", "bg_other", "two", "many", "all_active", "none_active"]
+    for o, pat in zip(origins, patterns):
+        vals = rng.standard_normal((512, nc)).astype(np.float32)
+        if half:
+            vals = vals.astype(np.float16).astype(np.float32)
+        m = rng.random(512) < 0.4
+        if pat == "all_active":
+            m[:] = True
+        if pat == "none_active":
+            m[:] = False
+        ina = ~m
+        if pat == "bg":
+            vals[ina] = bg
+        elif pat == "minus_bg":
+            vals[ina] = -bg
+        elif pat == "one":
+            vals[ina] = 0.25
+        elif pat == "pm_bg":
+            vals[ina] = np.where(rng.random((ina.sum(), 1)) < 0.5, bg, -bg)
+        elif pat == "bg_other":
+            vals[ina] = np.where(rng.random((ina.sum(), 1)) < 0.5, bg, np.float32(-2.0))
+        elif pat == "two":
+            vals[ina] = np.where(rng.random((ina.sum(), 1)) < 0.5, np.float32(3.0), np.float32(-4.0))
+        elif pat == "none_active":
+            vals[:] = -bg
+        leaves[o] = (vals, m)
+    i4_tiles = [((16, 0, 0), np.full(nc, -0.5, np.float32), True), ((0, 16, 0), np.full(nc, 7.0, np.float32), False)]
+    i5_tiles = [((0, 128, 0), np.full(nc, -0.5, np.float32), False), ((256, 0, 0), np.full(nc, 1.5, np.float32),
+                                                                        True)]
+    root_tiles = [((8192, 0, 0), np.full(nc, -0.5, np.float32), False)]
+    return {"name": name, "type": "Tree_float_5_4_3" if nc == 1 else "Tree_vec3s_5_4_3", "comp": comp,
+            "half": half, "bg": bg, "voxel": 0.01, "leaves": leaves, "i4_tiles": i4_tiles, "i5_tiles": i5_tiles,
+            "root_tiles": root_tiles}
+
+
+@pytest.mark.parametrize("nc", [1, 3])
+@pytest.mark.parametrize("comp", [0, 1, 2, 3, 6])
+@pytest.mark.parametrize("half", [False, True])
+def test_written_files_read_back(tmp_path, nc, comp, half):
+    rng = np.random.default_rng(nc * 100 + comp * 10 + int(half))
+    g = _make_grid(rng, nc, comp, half, "main")
+    second = _make_grid(np.random.default_rng(7), 1, comp, False, "other")
+    p = tmp_path / "t.vdb"
+    p.write_bytes(_write_vdb([g, {"name": "points", "type": "Tree_ptdataidx32_5_4_3"}, second]))
+    out = read_vdb(str(p))
+    assert [x.name for x in out] == ["main", "points", "other"]
+    assert out[1].grid is None and out[1].type == "Tree_ptdataidx32_5_4_3"
+    for want, got in ((g, out[0]), (second, out[2])):
+        assert got.ncomp == (1 if want["type"] == "Tree_float_5_4_3" else 3)
+        assert got.half_float == want["half"] and got.grid_class == "fog volume"
+        assert got.voxel_size == (0.01, 0.01, 0.01)
+        G = got.grid
+        o = G.origins.astype(np.int64)
+        key = [tuple(r) + (int(a[0] << 10 | a[1] << 5 | a[2]), int(b[0] << 8 | b[1] << 4 | b[2]))
+               for r, a, b in zip(o >> 12, (o >> 7) & 31, (o >> 3) & 15)]
+        assert key == sorted(key)
+        assert sorted(map(tuple, G.origins)) == sorted(want["leaves"])
+        for i, org in enumerate(map(tuple, G.origins)):
+            vals, m = want["leaves"][org]
+            bits = np.unpackbits(G.active[i].view(np.uint8), bitorder="little").astype(bool)
+            assert np.array_equal(bits, m), org
+            v = G.values[i].reshape(512, -1)
+            assert np.array_equal(v, vals), org
+        tiles = {(tuple(o), int(d)): (tuple(v[:got.ncomp]), bool(a)) for o, d, v, a in zip(*got.tiles)}
+        want_tiles = {(tuple(o), 8): (tuple(v), a) for o, v, a in want["i4_tiles"]}
+        want_tiles.update({(tuple(o), 128): (tuple(v), a) for o, v, a in want["i5_tiles"]})
+        want_tiles.update({(tuple(o), 4096): (tuple(v), a) for o, v, a in want["root_tiles"]})
+        assert tiles == want_tiles
