@@ -118,7 +118,9 @@ const char* uaamg_version(void);
  * Builds the level-0 Laplacian and the aggregation hierarchy and uploads it.
  * Replaces LaplacianWithLevel::createPressurePoissonLaplacian (simd_vdb_poisson.h:54-57,
  * .cpp:95-99) followed by PoissonSolver(LaplacianWithLevel::Ptr) (simd_vdb_poisson.h:153-160).
- * dx is liquid_phi->voxelSize()[0].
+ * dx is liquid_phi->voxelSize()[0]. A liquid phi without degrees of freedom (no active voxel with
+ * phi < 0) is rejected with UAAMG_EINVAL: the reference's solve of that empty system returns
+ * SUCCESS after 0 iterations and changes nothing, so a caller with no liquid skips the solve.
  */
 int uaamg_poisson_create(const uaamg_grid* liquid_phi, const uaamg_grid* face_weights, float dt, float dx,
                          const uaamg_options* opts, uaamg_poisson** out);
@@ -457,7 +459,9 @@ int uaamg_viscosity_options_default(uaamg_viscosity_options* opts);
  * Builds level 0 from the viscosity, liquid and solid grids, the coarse levels and the coarsest
  * explicit solver, and uploads them. Replaces L_with_level::create_viscosity_matrix
  * (simd_viscosity3d.h:95-98, .cpp:14-20) followed by the simd_viscosity3d constructor's hierarchy
- * loop (simd_viscosity3d.h:292-294, .cpp:3659-3690). dx is liquid_sdf->voxelSize()[0].
+ * loop (simd_viscosity3d.h:292-294, .cpp:3659-3690). dx is liquid_sdf->voxelSize()[0]. A problem
+ * without degrees of freedom (no face-velocity DOF in the liquid) is rejected with UAAMG_EINVAL, as
+ * uaamg_poisson_create rejects an empty liquid.
  */
 int uaamg_viscosity_create(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf, const uaamg_grid* solid_sdf,
                            float dt, float rho, float dx, const uaamg_viscosity_options* opts,

@@ -97,6 +97,19 @@ def test_bad_inputs_rejected():
         U.LaplacianWithLevel(s.weights, s.weights, dt=1.0, dx=s.dx, device=-1)
 
 
+def test_problems_without_dofs_rejected():
+    """No liquid voxel (all phi >= 0, or no leaves at all): both creates report UAAMG_EINVAL
+    ("no degrees of freedom") rather than building an empty system."""
+    w = U.Grid(np.zeros((0, 3), np.int32), np.zeros((0, 512, 3), np.float32), None, (1.0, 1.0, 1.0))
+    dry = U.Grid(np.zeros((1, 3), np.int32), np.ones((1, 512), np.float32), None, 1.0)
+    empty = U.Grid(np.zeros((0, 3), np.int32), np.zeros((0, 512), np.float32), None, 1.0)
+    for phi in (dry, empty):
+        with pytest.raises(U.UaamgError, match="no degrees of freedom"):
+            U.LaplacianWithLevel(phi, w, dt=1.0, dx=0.1, device=-1)
+        with pytest.raises(U.UaamgError, match="no degrees of freedom"):
+            U.L_with_level(empty, phi, empty, 1.0 / 24.0, 1000.0, 0.1, device=-1)
+
+
 def test_fraction_inside_product_equals_oracle():
     lib = U.lib()
     rng = np.random.default_rng(1)

@@ -97,6 +97,7 @@ int prepare_viscosity(const uaamg_grid* viscosity, const uaamg_grid* liquid_sdf,
     std::string err;
     if (!build_visc_hierarchy(liq, sol, mu, dt, rho, dx, o.max_coarsest_dof, !(o.flags & 8), hh, volume_device, &err))
         return fail(UAAMG_EINVAL, "hierarchy construction failed: " + err);
+    if (hh.levels.empty() || hh.levels[0].ndof == 0) return fail(UAAMG_EINVAL, "no degrees of freedom");
     vo = ViscOptions{};
     vo.max_iterations = o.max_iterations;
     vo.device = o.device;
