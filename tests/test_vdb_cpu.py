@@ -472,3 +472,21 @@ def test_written_files_read_back(tmp_path, nc, comp, half):
         want_tiles.update({(tuple(o), 128): (tuple(v), a) for o, v, a in want["i5_tiles"]})
         want_tiles.update({(tuple(o), 4096): (tuple(v), a) for o, v, a in want["root_tiles"]})
         assert tiles == want_tiles
+
+
+def test_half_float_widening_exact(tmp_path):
+    """Every one of the 65536 half-float bit patterns (zeros, subnormals, normals, infinities, NaNs)
+    read back from half-float grids equals numpy's exact widening to float32, bit for bit."""
+    halves = np.arange(65536, dtype=np.uint32).astype(np.uint16).view(np.float16)
+    leaves = {}
+    for i in range(128):
+        vals = halves[i * 512:(i + 1) * 512].astype(np.float32).reshape(512, 1)
+        leaves[(8 * (i % 16), 8 * (i // 16), 0)] = (vals, np.ones(512, bool))
+    g = {"name": "h", "type": "Tree_float_5_4_3", "comp": 0, "half": True, "bg": np.float32([0.5]), "voxel": 1.0,
+         "leaves": leaves, "i4_tiles": [], "i5_tiles": [], "root_tiles": []}
+    p = tmp_path / "h.vdb"
+    p.write_bytes(_write_vdb([g]))
+    G = read_vdb(str(p))[0].grid
+    for i, org in enumerate(map(tuple, G.origins)):
+        want = leaves[org][0].reshape(512)
+        assert np.array_equal(G.values[i].view(np.uint32), want.view(np.uint32)), org
