@@ -86,12 +86,13 @@ def test_visc_batch_equals_each_problem_alone(visc_inputs, flags):
         assert np.array_equal(x1, x2), f"problem {i}"
 
 
-def test_visc_batch_iteration_limit(visc_inputs):
-    """m_max_iter = 3: every problem stops at it == m_max_iter - 1 (the reference's rule has no
-    failure status)"""
+@pytest.mark.parametrize("max_iterations", [0, 1, 3])
+def test_visc_batch_iteration_limit(visc_inputs, max_iterations):
+    """every problem stops at it == m_max_iter - 1 (the reference's rule has no failure status);
+    m_max_iter = 0 leaves x as it was"""
     scenes, rhs = visc_inputs
-    alone = _alone(scenes, rhs, max_iterations=3)
-    batched = _solve(_batch(scenes, max_iterations=3), rhs)
+    alone = _alone(scenes, rhs, max_iterations=max_iterations)
+    batched = _solve(_batch(scenes, max_iterations=max_iterations), rhs)
     for i, ((i1, x1, _), (i2, x2)) in enumerate(zip(alone, batched)):
         assert i1 == i2, f"problem {i}"
         assert np.array_equal(x1, x2), f"problem {i}"
