@@ -142,15 +142,17 @@ def test_graph_replay_bit_identical(problem):
         assert np.array_equal(x, out[0][0])
 
 
-def test_wide_kernels_bit_identical(problem, monkeypatch):
+@pytest.mark.parametrize("env", ["UAAMG_VWIDE", "UAAMG_VHALF"])
+def test_wide_and_half_row_kernels_bit_identical(problem, monkeypatch, env):
     """UAAMG_VWIDE=1 runs every level's passes on the two-waves-per-SIMD
-    kernels (chosen by default only for levels wider than one wave per SIMD):
-    a register budget, not arithmetic, so the operator applications and the
-    solve are bit-identical."""
+    kernels (by default only levels wider than one wave per SIMD), UAAMG_VHALF=1
+    on half rows (by default only the other levels), =0 on neither: a register
+    budget and a lane mapping, not arithmetic, so the operator applications and
+    the solve are bit-identical."""
     s, lw, orc, b = problem
     out = []
     for wide in ("0", "1"):
-        monkeypatch.setenv("UAAMG_VWIDE", wide)
+        monkeypatch.setenv(env, wide)
         l2 = U.L_with_level(s.viscosity, s.liquid, s.solid, s.dt, s.rho, s.dx)
         solver = U.simd_viscosity3d(l2, s.velocity)
         x = l2.get_zero_vec()
@@ -159,7 +161,7 @@ def test_wide_kernels_bit_identical(problem, monkeypatch):
         xr = rng.standard_normal(l2.level_ndof(0)).astype(np.float32)
         ap = [l2.apply(0, m, xr, b) for m in (0, 1, 2, 4, 5)]
         out.append((x.data.copy(), solver.m_iteration, ap))
-    monkeypatch.delenv("UAAMG_VWIDE")
+    monkeypatch.delenv(env)
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][0], out[1][0])
     for a0, a1 in zip(out[0][2], out[1][2]):

@@ -291,6 +291,7 @@ int ViscBatch::init(std::vector<ViscHierarchyHost>&& probs, const ViscOptions& o
             t.wrec = static_cast<const int32_t*>(up(h.rec[view].data(), h.rec[view].size() * 4));
             std::copy(h.chan_off[view], h.chan_off[view] + 4, t.chan_off);
             t.wide = visc_wide(t);
+            t.half = visc_half(t);
         }
         // the smoothing prefix's DOF leaves come first: its vectors are the prefix of the level's
         sv.nleaf_dof = int32_t(nsm[size_t(l)]);

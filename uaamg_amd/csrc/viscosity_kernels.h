@@ -55,6 +55,10 @@ struct ViscLevelView {
     // resident (at most 256 registers each) rather than all of one row's
     // loads in flight in one wave
     int32_t wide;
+    // 1: the colour passes and appliers run on half rows (two wavefronts per
+    // wave record, z 0-3 and 4-7), each wavefront with half the operands in
+    // flight; the same values
+    int32_t half;
 };
 
 // Whether a level's passes take the two-waves-per-SIMD kernels: when a
@@ -66,6 +70,19 @@ inline int32_t visc_wide(const ViscLevelView& v) {
     int32_t widest = 0;
     for (int c = 0; c < 3; ++c) widest = std::max(widest, v.chan_off[c + 1] - v.chan_off[c]);
     return widest > 1024 ? 1 : 0;
+}
+
+// Whether a level's passes run on half rows: the levels whose passes fit in
+// one wave per SIMD (the ones visc_wide leaves alone). There a pass is bound
+// by each wavefront's chain of memory round trips, which half rows shorten
+// (128^3: 5.9 -> 5.1 us per level-1 or level-2 pass); on wider levels the
+// doubled wavefront count costs more (level 0: 9.8 -> 12.6 us). UAAMG_VHALF=1
+// / =0 forces the choice for every level (results are the same either way).
+inline int32_t visc_half(const ViscLevelView& v) {
+    if (const char* env = std::getenv("UAAMG_VHALF")) return env[0] == '1' ? 1 : 0;
+    int32_t widest = 0;
+    for (int c = 0; c < 3; ++c) widest = std::max(widest, v.chan_off[c + 1] - v.chan_off[c]);
+    return widest <= 1024 ? 1 : 0;
 }
 
 // applier modes (light_weight_applier working_mode, simd_viscosity3d.h:13)

@@ -310,6 +310,274 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane
     st8v(Yv + row, y);
 }
 
+// ---------------------------------------------------------- half rows
+// The same row arithmetic on half a z-row per lane: two wavefronts share a
+// (leaf, channel) record, the first taking z 0-3 and the second z 4-7 of
+// every row. A lane then needs half the operands of a full row, so a
+// wavefront's loads fit in flight at once instead of in batches; per voxel
+// the terms, their order and the stored values are those of visc_row.
+__device__ __forceinline__ void ld4v(const float* p, float v[4]) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void st4v(float* p, const float v[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void ld4v(const bf16* p, float v[4]) {
+    const float4 a = dev::ld4(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+__device__ __forceinline__ void st4v(bf16* p, const float v[4]) {
+    *reinterpret_cast<uint2*>(p) = make_uint2(dev::pack_bf16(v[0], v[1]), dev::pack_bf16(v[2], v[3]));
+}
+
+// vfetch on half a row: z = z0 .. z0 + 3 (z0 = 0 or 4, the same for the
+// whole wavefront); the value a z shift brings in comes from the own row
+// (the other half) or, past the brick, from the +-z neighbour leaf
+template <int OX, int OY, int OZ, class V>
+__device__ __forceinline__ void vfetch4(const ViscLevelView& L, const V* v, int ch, const VRec& R, int lx, int ly,
+                                        int z0, float out[4]) {
+    const int X = lx + OX, Y = ly + OY;
+    const int lf = vleaf<OX, OY, 0>(R, lx, ly);
+    const bool have = lf >= 0 && lf < L.nleaf_dof;
+    const V* base = v + (size_t(have ? lf : 0) * 3 + ch) * 512 + row_off(X, Y);
+    float row[4];
+    if (have) ld4v(base + z0, row);
+    else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) row[k] = 0.f;
+    }
+    if (OZ == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = row[k];
+    } else if (OZ > 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = row[k + 1];
+        if (z0 == 0) {
+            out[3] = have ? ld1v(base + 4) : 0.f;
+        } else {
+            const int lz = vleaf<OX, OY, 1>(R, lx, ly);
+            out[3] = (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y)) : 0.f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 3; k > 0; --k) out[k] = row[k - 1];
+        if (z0 == 4) {
+            out[0] = have ? ld1v(base + 3) : 0.f;
+        } else {
+            const int lz = vleaf<OX, OY, -1>(R, lx, ly);
+            out[0] = (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y) + 7) : 0.f;
+        }
+    }
+}
+
+// cfetch on half a row (same rules as cfetch)
+template <int GID, int OX, int OY, int OZ>
+__device__ __forceinline__ void cfetch4(const ViscLevelView& L, int s_own, int s_nb, int lx, int ly, int z0,
+                                        float out[4]) {
+    static_assert((OX != 0) + (OY != 0) + (OZ != 0) <= 1, "single-axis coefficient offsets");
+    const int X = lx + OX, Y = ly + OY;
+    const bool beyond = X < 0 || X > 7 || Y < 0 || Y > 7;
+    const int s = beyond ? s_nb : s_own;
+    const float m = L.dflt[GID];
+    float row[4];
+    if (s >= 0) ld4v(L.pool + size_t(s) * 512 + row_off(X, Y) + z0, row);
+    else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) row[k] = m;
+    }
+    if (OZ == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = row[k];
+    } else if (OZ > 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k] = row[k + 1];
+        if (z0 == 0) out[3] = s >= 0 ? L.pool[size_t(s) * 512 + row_off(X, Y) + 4] : m;
+        else out[3] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y)] : m;
+    } else {
+#pragma unroll
+        for (int k = 3; k > 0; --k) out[k] = row[k - 1];
+        if (z0 == 4) out[0] = s >= 0 ? L.pool[size_t(s) * 512 + row_off(X, Y) + 3] : m;
+        else out[0] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y) + 7] : m;
+    }
+}
+
+template <int MODE, int C, class V>
+__device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lane, int z0, const V* Xv, const V* Bv,
+                                          V* Yv) {
+    const int lx = lane >> 3, ly = lane & 7;
+    const VRec R = load_vrec(L, e);
+    const int leaf = R.leaf;
+    const uint32_t rm = uint32_t(L.mask[(size_t(leaf) * 3 + C) * 8 + lx] >> (ly * 8)) & 0xffu;
+    constexpr int d0 = C, d1 = (C + 1) % 3, d2 = (C + 2) % 3;
+    using E0 = Unit<d0>;
+    using E1 = Unit<d1>;
+    using E2 = Unit<d2>;
+    constexpr int kN0 = kVNeg + 3 * C, kX2 = kVCross + 4 * d2, kX1 = kVCross + 4 * d1;
+
+    float s[4];
+    {
+        float a[4], wa[4], b[4], wb[4];
+        float f12[4], f34[4];
+        vfetch4<1, 0, 0>(L, Xv, C, R, lx, ly, z0, a);
+        cfetch4<kN0 + 0, 1, 0, 0>(L, R.own[0], R.nbs[0], lx, ly, z0, wa);
+        vfetch4<-1, 0, 0>(L, Xv, C, R, lx, ly, z0, b);
+        cfetch4<kN0 + 0, 0, 0, 0>(L, R.own[0], -1, lx, ly, z0, wb);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = a[k] * wa[k], q = b[k] * wb[k];
+            f12[k] = p + q;
+        }
+        vfetch4<0, 1, 0>(L, Xv, C, R, lx, ly, z0, a);
+        cfetch4<kN0 + 1, 0, 1, 0>(L, R.own[1], R.nbs[1], lx, ly, z0, wa);
+        vfetch4<0, -1, 0>(L, Xv, C, R, lx, ly, z0, b);
+        cfetch4<kN0 + 1, 0, 0, 0>(L, R.own[1], -1, lx, ly, z0, wb);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = a[k] * wa[k], q = b[k] * wb[k];
+            f34[k] = p + q;
+        }
+        vfetch4<0, 0, 1>(L, Xv, C, R, lx, ly, z0, a);
+        cfetch4<kN0 + 2, 0, 0, 1>(L, R.own[2], R.nbs[2], lx, ly, z0, wa);
+        vfetch4<0, 0, -1>(L, Xv, C, R, lx, ly, z0, b);
+        cfetch4<kN0 + 2, 0, 0, 0>(L, R.own[2], -1, lx, ly, z0, wb);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = a[k] * wa[k], q = b[k] * wb[k];
+            const float f56 = p + q;
+            const float t = f12[k] + f34[k];
+            s[k] = t + f56;
+        }
+    }
+    {
+        // normal d2: channel d1 at v, v + e1, v - e0, v - e0 + e1
+        float u[4], c[4], t01[4];
+        vfetch4<0, 0, 0>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetch4<kX2 + 0, 0, 0, 0>(L, R.own[3], -1, lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t01[k] = u[k] * c[k];
+        vfetch4<E1::x, E1::y, E1::z>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetch4<kX2 + 1, 0, 0, 0>(L, R.own[4], -1, lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = c[k] * u[k];
+            t01[k] = t01[k] + p;
+        }
+        float t23[4];
+        vfetch4<-E0::x, -E0::y, -E0::z>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetch4<kX2 + 2, -E0::x, -E0::y, -E0::z>(L, R.own[5], R.nbs[3], lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t23[k] = c[k] * u[k];
+        vfetch4<E1::x - E0::x, E1::y - E0::y, E1::z - E0::z>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetch4<kX2 + 3, -E0::x, -E0::y, -E0::z>(L, R.own[6], R.nbs[4], lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = c[k] * u[k];
+            t23[k] = t23[k] + p;
+            const float T2 = t01[k] + t23[k];
+            s[k] = s[k] + T2;
+        }
+    }
+    {
+        // normal d1: channel d2 at v, v - e0, v + e2, v - e0 + e2
+        float u[4], c[4], t01[4];
+        vfetch4<0, 0, 0>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetch4<kX1 + 0, 0, 0, 0>(L, R.own[7], -1, lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t01[k] = c[k] * u[k];
+        vfetch4<-E0::x, -E0::y, -E0::z>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetch4<kX1 + 1, -E0::x, -E0::y, -E0::z>(L, R.own[8], R.nbs[5], lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = c[k] * u[k];
+            t01[k] = t01[k] + p;
+        }
+        float t23[4];
+        vfetch4<E2::x, E2::y, E2::z>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetch4<kX1 + 2, 0, 0, 0>(L, R.own[9], -1, lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t23[k] = c[k] * u[k];
+        vfetch4<E2::x - E0::x, E2::y - E0::y, E2::z - E0::z>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetch4<kX1 + 3, -E0::x, -E0::y, -E0::z>(L, R.own[10], R.nbs[6], lx, ly, z0, c);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float p = c[k] * u[k];
+            t23[k] = t23[k] + p;
+            const float T1 = t01[k] + t23[k];
+            s[k] = s[k] + T1;
+        }
+    }
+
+    const size_t row = (size_t(leaf) * 3 + C) * 512 + size_t(lx * 8 + ly) * 8 + size_t(z0);
+    float lhs[4], y[4];
+    ld4v(Xv + row, lhs);
+    if (MODE == kVRedGS || MODE == kVBlackGS) {
+        float b[4], inv[4];
+        ld4v(Bv + row, b);
+        cfetch4<kVInvDiag + C, 0, 0, 0>(L, R.own[12], -1, lx, ly, z0, inv);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float t = b[k] - s[k];
+            t = t * inv[k];
+            t = t * 1.0f;
+            const float r = fmaf(lhs[k], 0.0f, t);
+            const bool red = ((lx + ly + z0 + k) & 1) == 0;
+            y[k] = (red == (MODE == kVRedGS)) ? r : lhs[k];
+        }
+    } else {
+        float dg[4];
+        cfetch4<kVDiag + C, 0, 0, 0>(L, R.own[11], -1, lx, ly, z0, dg);
+        if (MODE == kVNormal) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) y[k] = fmaf(lhs[k], dg[k], s[k]);
+        } else {
+            float b[4];
+            ld4v(Bv + row, b);
+            if (MODE == kVResidual) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) y[k] = b[k] - fmaf(lhs[k], dg[k], s[k]);
+            } else {
+                float sc[4];
+                if (MODE == kVJacobi) cfetch4<kVInvDiag + C, 0, 0, 0>(L, R.own[12], -1, lx, ly, z0, sc);
+                else cfetch4<kVSpai0 + C, 0, 0, 0>(L, R.own[13], -1, lx, ly, z0, sc);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const float a = fmaf(lhs[k], dg[k], s[k]);
+                    float t = b[k] - a;
+                    t = t * sc[k];
+                    y[k] = fmaf(1.0f, t, lhs[k]);
+                }
+            }
+        }
+    }
+    const uint32_t rmh = (rm >> z0) & 0xfu;
+    if (!rmh) return;   // half rows without a DOF are not written
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (!((rmh >> k) & 1u)) y[k] = 0.f;
+    st4v(Yv + row, y);
+}
+
+
+template <int MODE, class V>
+__global__ __launch_bounds__(kVThreads) void visc_apply_all_half_kernel(ViscLevelView L, const V* Xv, const V* Bv,
+                                                                        V* Yv) {
+    const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
+    const int e = w >> 1, z0 = (w & 1) * 4;
+    const int lane = threadIdx.x & 63;
+    if (e < L.chan_off[1]) visc_row4<MODE, 0>(L, e, lane, z0, Xv, Bv, Yv);
+    else if (e < L.chan_off[2]) visc_row4<MODE, 1>(L, e, lane, z0, Xv, Bv, Yv);
+    else if (e < L.chan_off[3]) visc_row4<MODE, 2>(L, e, lane, z0, Xv, Bv, Yv);
+}
+
+template <int MODE, int C, class V>
+__global__ __launch_bounds__(kVThreads) void visc_apply_half_kernel(ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
+    const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
+    const int e = w >> 1;
+    if (e >= L.chan_off[C + 1] - L.chan_off[C]) return;
+    visc_row4<MODE, C>(L, L.chan_off[C] + e, threadIdx.x & 63, (w & 1) * 4, Xv, Bv, Yv);
+}
+
 // All three channels in one launch (modes that write y != x: the channels
 // are independent); the wave's record index selects the channel.
 template <int MODE, class V>
@@ -737,6 +1005,15 @@ template <int MODE, class V>
 hipError_t apply_mode(const ViscLevelView& L, int c, const V* x, const V* b, V* y, hipStream_t s) {
     const int n = L.chan_off[c + 1] - L.chan_off[c];
     if (n <= 0) return hipSuccess;
+    if (L.half) {
+        const unsigned gh = unsigned((2 * n + kVWaves - 1) / kVWaves);
+        switch (c) {
+            case 0: visc_apply_half_kernel<MODE, 0, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case 1: visc_apply_half_kernel<MODE, 1, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            default: visc_apply_half_kernel<MODE, 2, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+        }
+        return hipGetLastError();
+    }
     const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
     if (L.wide) {
         switch (c) {
@@ -772,6 +1049,17 @@ template <class V>
 hipError_t apply_all(const ViscLevelView& L, int mode, const V* x, const V* b, V* y, hipStream_t s) {
     const int n = L.chan_off[3];
     if (n <= 0) return hipSuccess;
+    if (L.half) {
+        const unsigned gh = unsigned((2 * n + kVWaves - 1) / kVWaves);
+        switch (mode) {
+            case kVNormal: visc_apply_all_half_kernel<kVNormal, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVResidual: visc_apply_all_half_kernel<kVResidual, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVJacobi: visc_apply_all_half_kernel<kVJacobi, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVSpai0Mode: visc_apply_all_half_kernel<kVSpai0Mode, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const unsigned g = unsigned((n + kVWaves - 1) / kVWaves);
     if (L.wide) {
         switch (mode) {

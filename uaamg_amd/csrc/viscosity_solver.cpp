@@ -114,6 +114,7 @@ int ViscDevice::upload_level(const ViscLevelHost& h, DevLevel& d) {
     }
     v.chan_leaves = static_cast<const int32_t*>(put(cl.data(), cl.size() * 4));
     v.wide = visc_wide(v);
+    v.half = visc_half(v);
     const std::vector<int32_t> wrec = visc_wave_records(h);
     v.wrec = static_cast<const int32_t*>(put(wrec.data(), wrec.size() * 4));
     v.parent = h.parent.empty() ? nullptr : static_cast<const int32_t*>(put(h.parent.data(), h.parent.size() * 4));
