@@ -142,16 +142,16 @@ def test_graph_replay_bit_identical(problem):
         assert np.array_equal(x, out[0][0])
 
 
-@pytest.mark.parametrize("env", ["UAAMG_VWIDE", "UAAMG_VHALF"])
-def test_wide_and_half_row_kernels_bit_identical(problem, monkeypatch, env):
+@pytest.mark.parametrize("env,values", [("UAAMG_VWIDE", ("0", "1")), ("UAAMG_VHALF", ("0", "1", "2"))])
+def test_wide_and_part_row_kernels_bit_identical(problem, monkeypatch, env, values):
     """UAAMG_VWIDE=1 runs every level's passes on the two-waves-per-SIMD
     kernels (by default only levels wider than one wave per SIMD), UAAMG_VHALF=1
-    on half rows (by default only the other levels), =0 on neither: a register
-    budget and a lane mapping, not arithmetic, so the operator applications and
-    the solve are bit-identical."""
+    on half rows (by default only the other levels), =2 on quarter rows, =0 on
+    full rows: a register budget and a lane mapping, not arithmetic, so the
+    operator applications and the solve are bit-identical."""
     s, lw, orc, b = problem
     out = []
-    for wide in ("0", "1"):
+    for wide in values:
         monkeypatch.setenv(env, wide)
         l2 = U.L_with_level(s.viscosity, s.liquid, s.solid, s.dt, s.rho, s.dx)
         solver = U.simd_viscosity3d(l2, s.velocity)
@@ -162,7 +162,8 @@ def test_wide_and_half_row_kernels_bit_identical(problem, monkeypatch, env):
         ap = [l2.apply(0, m, xr, b) for m in (0, 1, 2, 4, 5)]
         out.append((x.data.copy(), solver.m_iteration, ap))
     monkeypatch.delenv(env)
-    assert out[0][1] == out[1][1]
-    assert np.array_equal(out[0][0], out[1][0])
-    for a0, a1 in zip(out[0][2], out[1][2]):
-        assert np.array_equal(a0, a1)
+    for other in out[1:]:
+        assert out[0][1] == other[1]
+        assert np.array_equal(out[0][0], other[0])
+        for a0, a1 in zip(out[0][2], other[2]):
+            assert np.array_equal(a0, a1)

@@ -57,7 +57,8 @@ struct ViscLevelView {
     int32_t wide;
     // 1: the colour passes and appliers run on half rows (two wavefronts per
     // wave record, z 0-3 and 4-7), each wavefront with half the operands in
-    // flight; the same values
+    // flight; 2: the colour passes on quarter rows (four wavefronts); the
+    // same values
     int32_t half;
 };
 
@@ -72,14 +73,17 @@ inline int32_t visc_wide(const ViscLevelView& v) {
     return widest > 1024 ? 1 : 0;
 }
 
-// Whether a level's passes run on half rows: the levels whose passes fit in
-// one wave per SIMD (the ones visc_wide leaves alone). There a pass is bound
-// by each wavefront's chain of memory round trips, which half rows shorten
-// (128^3: 5.9 -> 5.1 us per level-1 or level-2 pass); on wider levels the
-// doubled wavefront count costs more (level 0: 9.8 -> 12.6 us). UAAMG_VHALF=1
-// / =0 forces the choice for every level (results are the same either way).
+// Whether a level's passes run on part rows: half rows on the levels whose
+// passes fit in one wave per SIMD (the ones visc_wide leaves alone). There a
+// pass is bound by each wavefront's chain of memory round trips, which
+// shorter rows shorten (128^3, us per pass back to back, full / half /
+// quarter rows: level 1 (356 leaves) 5.9 / 5.1 / 6.2, level 2 (56) 5.8 / 5.1
+// / 4.8, level 3 (12) 5.1 / 5.0 / 4.9; the solve: 13.1 / 12.5 ms, and 12.6 ms
+// with quarter rows on levels 2 and 3); on wider levels the extra wavefronts
+// cost more (level 0: 9.9 / 12.6 / 16.9). UAAMG_VHALF=0 / 1 / 2 forces full
+// / half / quarter rows on every level (results are the same either way).
 inline int32_t visc_half(const ViscLevelView& v) {
-    if (const char* env = std::getenv("UAAMG_VHALF")) return env[0] == '1' ? 1 : 0;
+    if (const char* env = std::getenv("UAAMG_VHALF")) return env[0] == '2' ? 2 : env[0] == '1' ? 1 : 0;
     int32_t widest = 0;
     for (int c = 0; c < 3; ++c) widest = std::max(widest, v.chan_off[c + 1] - v.chan_off[c]);
     return widest <= 1024 ? 1 : 0;

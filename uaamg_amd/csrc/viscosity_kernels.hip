@@ -310,60 +310,79 @@ __device__ __forceinline__ void visc_row(const ViscLevelView& L, int e, int lane
     st8v(Yv + row, y);
 }
 
-// ---------------------------------------------------------- half rows
-// The same row arithmetic on half a z-row per lane: two wavefronts share a
-// (leaf, channel) record, the first taking z 0-3 and the second z 4-7 of
-// every row. A lane then needs half the operands of a full row, so a
-// wavefront's loads fit in flight at once instead of in batches; per voxel
-// the terms, their order and the stored values are those of visc_row.
-__device__ __forceinline__ void ld4v(const float* p, float v[4]) {
-    const float4 a = *reinterpret_cast<const float4*>(p);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+// ---------------------------------------------------------- part rows
+// The same row arithmetic on part of a z-row per lane: 8 / NZ wavefronts
+// share a (leaf, channel) record, wavefront q taking z = NZ*q .. NZ*q + NZ-1
+// of every row (NZ = 4: half rows, 2: quarter rows). A lane then needs
+// NZ / 8 of a full row's operands, so a wavefront's loads fit in flight at
+// once instead of in batches; per voxel the terms, their order and the
+// stored values are those of visc_row.
+template <int NZ>
+__device__ __forceinline__ void ldnv(const float* p, float v[NZ]) {
+    if (NZ == 4) {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+        const float2 a = *reinterpret_cast<const float2*>(p);
+        v[0] = a.x; v[1] = a.y;
+    }
 }
-__device__ __forceinline__ void st4v(float* p, const float v[4]) {
-    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+template <int NZ>
+__device__ __forceinline__ void stnv(float* p, const float v[NZ]) {
+    if (NZ == 4) *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    else *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
 }
-__device__ __forceinline__ void ld4v(const bf16* p, float v[4]) {
-    const float4 a = dev::ld4(p);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+template <int NZ>
+__device__ __forceinline__ void ldnv(const bf16* p, float v[NZ]) {
+    if (NZ == 4) {
+        const float4 a = dev::ld4(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+        const uint32_t q = *reinterpret_cast<const uint32_t*>(p);
+        v[0] = __uint_as_float(q << 16);
+        v[1] = __uint_as_float(q & 0xffff0000u);
+    }
 }
-__device__ __forceinline__ void st4v(bf16* p, const float v[4]) {
-    *reinterpret_cast<uint2*>(p) = make_uint2(dev::pack_bf16(v[0], v[1]), dev::pack_bf16(v[2], v[3]));
+template <int NZ>
+__device__ __forceinline__ void stnv(bf16* p, const float v[NZ]) {
+    if (NZ == 4) *reinterpret_cast<uint2*>(p) = make_uint2(dev::pack_bf16(v[0], v[1]), dev::pack_bf16(v[2], v[3]));
+    else *reinterpret_cast<uint32_t*>(p) = dev::pack_bf16(v[0], v[1]);
 }
 
-// vfetch on half a row: z = z0 .. z0 + 3 (z0 = 0 or 4, the same for the
-// whole wavefront); the value a z shift brings in comes from the own row
-// (the other half) or, past the brick, from the +-z neighbour leaf
-template <int OX, int OY, int OZ, class V>
-__device__ __forceinline__ void vfetch4(const ViscLevelView& L, const V* v, int ch, const VRec& R, int lx, int ly,
-                                        int z0, float out[4]) {
+// vfetch on part of a row: z = z0 .. z0 + NZ - 1 (z0 the same for the whole
+// wavefront); the value a z shift brings in comes from the own row or, past
+// the brick, from the +-z neighbour leaf
+template <int OX, int OY, int OZ, int NZ, class V>
+__device__ __forceinline__ void vfetchn(const ViscLevelView& L, const V* v, int ch, const VRec& R, int lx, int ly,
+                                        int z0, float out[NZ]) {
     const int X = lx + OX, Y = ly + OY;
     const int lf = vleaf<OX, OY, 0>(R, lx, ly);
     const bool have = lf >= 0 && lf < L.nleaf_dof;
     const V* base = v + (size_t(have ? lf : 0) * 3 + ch) * 512 + row_off(X, Y);
-    float row[4];
-    if (have) ld4v(base + z0, row);
+    float row[NZ];
+    if (have) ldnv<NZ>(base + z0, row);
     else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) row[k] = 0.f;
+        for (int k = 0; k < NZ; ++k) row[k] = 0.f;
     }
     if (OZ == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) out[k] = row[k];
+        for (int k = 0; k < NZ; ++k) out[k] = row[k];
     } else if (OZ > 0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) out[k] = row[k + 1];
-        if (z0 == 0) {
-            out[3] = have ? ld1v(base + 4) : 0.f;
+        for (int k = 0; k < NZ - 1; ++k) out[k] = row[k + 1];
+        if (z0 + NZ <= 7) {
+            out[NZ - 1] = have ? ld1v(base + z0 + NZ) : 0.f;
         } else {
             const int lz = vleaf<OX, OY, 1>(R, lx, ly);
-            out[3] = (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y)) : 0.f;
+            out[NZ - 1] =
+                (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y)) : 0.f;
         }
     } else {
 #pragma unroll
-        for (int k = 3; k > 0; --k) out[k] = row[k - 1];
-        if (z0 == 4) {
-            out[0] = have ? ld1v(base + 3) : 0.f;
+        for (int k = NZ - 1; k > 0; --k) out[k] = row[k - 1];
+        if (z0 > 0) {
+            out[0] = have ? ld1v(base + z0 - 1) : 0.f;
         } else {
             const int lz = vleaf<OX, OY, -1>(R, lx, ly);
             out[0] = (lz >= 0 && lz < L.nleaf_dof) ? ld1v(v + (size_t(lz) * 3 + ch) * 512 + row_off(X, Y) + 7) : 0.f;
@@ -371,39 +390,39 @@ __device__ __forceinline__ void vfetch4(const ViscLevelView& L, const V* v, int 
     }
 }
 
-// cfetch on half a row (same rules as cfetch)
-template <int GID, int OX, int OY, int OZ>
-__device__ __forceinline__ void cfetch4(const ViscLevelView& L, int s_own, int s_nb, int lx, int ly, int z0,
-                                        float out[4]) {
+// cfetch on part of a row (same rules as cfetch)
+template <int GID, int OX, int OY, int OZ, int NZ>
+__device__ __forceinline__ void cfetchn(const ViscLevelView& L, int s_own, int s_nb, int lx, int ly, int z0,
+                                        float out[NZ]) {
     static_assert((OX != 0) + (OY != 0) + (OZ != 0) <= 1, "single-axis coefficient offsets");
     const int X = lx + OX, Y = ly + OY;
     const bool beyond = X < 0 || X > 7 || Y < 0 || Y > 7;
     const int s = beyond ? s_nb : s_own;
     const float m = L.dflt[GID];
-    float row[4];
-    if (s >= 0) ld4v(L.pool + size_t(s) * 512 + row_off(X, Y) + z0, row);
+    float row[NZ];
+    if (s >= 0) ldnv<NZ>(L.pool + size_t(s) * 512 + row_off(X, Y) + z0, row);
     else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) row[k] = m;
+        for (int k = 0; k < NZ; ++k) row[k] = m;
     }
     if (OZ == 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) out[k] = row[k];
+        for (int k = 0; k < NZ; ++k) out[k] = row[k];
     } else if (OZ > 0) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) out[k] = row[k + 1];
-        if (z0 == 0) out[3] = s >= 0 ? L.pool[size_t(s) * 512 + row_off(X, Y) + 4] : m;
-        else out[3] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y)] : m;
+        for (int k = 0; k < NZ - 1; ++k) out[k] = row[k + 1];
+        if (z0 + NZ <= 7) out[NZ - 1] = s >= 0 ? L.pool[size_t(s) * 512 + row_off(X, Y) + z0 + NZ] : m;
+        else out[NZ - 1] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y)] : m;
     } else {
 #pragma unroll
-        for (int k = 3; k > 0; --k) out[k] = row[k - 1];
-        if (z0 == 4) out[0] = s >= 0 ? L.pool[size_t(s) * 512 + row_off(X, Y) + 3] : m;
+        for (int k = NZ - 1; k > 0; --k) out[k] = row[k - 1];
+        if (z0 > 0) out[0] = s >= 0 ? L.pool[size_t(s) * 512 + row_off(X, Y) + z0 - 1] : m;
         else out[0] = s_nb >= 0 ? L.pool[size_t(s_nb) * 512 + row_off(X, Y) + 7] : m;
     }
 }
 
-template <int MODE, int C, class V>
-__device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lane, int z0, const V* Xv, const V* Bv,
+template <int MODE, int C, int NZ, class V>
+__device__ __forceinline__ void visc_rown(const ViscLevelView& L, int e, int lane, int z0, const V* Xv, const V* Bv,
                                           V* Yv) {
     const int lx = lane >> 3, ly = lane & 7;
     const VRec R = load_vrec(L, e);
@@ -415,34 +434,34 @@ __device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lan
     using E2 = Unit<d2>;
     constexpr int kN0 = kVNeg + 3 * C, kX2 = kVCross + 4 * d2, kX1 = kVCross + 4 * d1;
 
-    float s[4];
+    float s[NZ];
     {
-        float a[4], wa[4], b[4], wb[4];
-        float f12[4], f34[4];
-        vfetch4<1, 0, 0>(L, Xv, C, R, lx, ly, z0, a);
-        cfetch4<kN0 + 0, 1, 0, 0>(L, R.own[0], R.nbs[0], lx, ly, z0, wa);
-        vfetch4<-1, 0, 0>(L, Xv, C, R, lx, ly, z0, b);
-        cfetch4<kN0 + 0, 0, 0, 0>(L, R.own[0], -1, lx, ly, z0, wb);
+        float a[NZ], wa[NZ], b[NZ], wb[NZ];
+        float f12[NZ], f34[NZ];
+        vfetchn<1, 0, 0, NZ>(L, Xv, C, R, lx, ly, z0, a);
+        cfetchn<kN0 + 0, 1, 0, 0, NZ>(L, R.own[0], R.nbs[0], lx, ly, z0, wa);
+        vfetchn<-1, 0, 0, NZ>(L, Xv, C, R, lx, ly, z0, b);
+        cfetchn<kN0 + 0, 0, 0, 0, NZ>(L, R.own[0], -1, lx, ly, z0, wb);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = a[k] * wa[k], q = b[k] * wb[k];
             f12[k] = p + q;
         }
-        vfetch4<0, 1, 0>(L, Xv, C, R, lx, ly, z0, a);
-        cfetch4<kN0 + 1, 0, 1, 0>(L, R.own[1], R.nbs[1], lx, ly, z0, wa);
-        vfetch4<0, -1, 0>(L, Xv, C, R, lx, ly, z0, b);
-        cfetch4<kN0 + 1, 0, 0, 0>(L, R.own[1], -1, lx, ly, z0, wb);
+        vfetchn<0, 1, 0, NZ>(L, Xv, C, R, lx, ly, z0, a);
+        cfetchn<kN0 + 1, 0, 1, 0, NZ>(L, R.own[1], R.nbs[1], lx, ly, z0, wa);
+        vfetchn<0, -1, 0, NZ>(L, Xv, C, R, lx, ly, z0, b);
+        cfetchn<kN0 + 1, 0, 0, 0, NZ>(L, R.own[1], -1, lx, ly, z0, wb);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = a[k] * wa[k], q = b[k] * wb[k];
             f34[k] = p + q;
         }
-        vfetch4<0, 0, 1>(L, Xv, C, R, lx, ly, z0, a);
-        cfetch4<kN0 + 2, 0, 0, 1>(L, R.own[2], R.nbs[2], lx, ly, z0, wa);
-        vfetch4<0, 0, -1>(L, Xv, C, R, lx, ly, z0, b);
-        cfetch4<kN0 + 2, 0, 0, 0>(L, R.own[2], -1, lx, ly, z0, wb);
+        vfetchn<0, 0, 1, NZ>(L, Xv, C, R, lx, ly, z0, a);
+        cfetchn<kN0 + 2, 0, 0, 1, NZ>(L, R.own[2], R.nbs[2], lx, ly, z0, wa);
+        vfetchn<0, 0, -1, NZ>(L, Xv, C, R, lx, ly, z0, b);
+        cfetchn<kN0 + 2, 0, 0, 0, NZ>(L, R.own[2], -1, lx, ly, z0, wb);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = a[k] * wa[k], q = b[k] * wb[k];
             const float f56 = p + q;
             const float t = f12[k] + f34[k];
@@ -451,27 +470,27 @@ __device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lan
     }
     {
         // normal d2: channel d1 at v, v + e1, v - e0, v - e0 + e1
-        float u[4], c[4], t01[4];
-        vfetch4<0, 0, 0>(L, Xv, d1, R, lx, ly, z0, u);
-        cfetch4<kX2 + 0, 0, 0, 0>(L, R.own[3], -1, lx, ly, z0, c);
+        float u[NZ], c[NZ], t01[NZ];
+        vfetchn<0, 0, 0, NZ>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetchn<kX2 + 0, 0, 0, 0, NZ>(L, R.own[3], -1, lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) t01[k] = u[k] * c[k];
-        vfetch4<E1::x, E1::y, E1::z>(L, Xv, d1, R, lx, ly, z0, u);
-        cfetch4<kX2 + 1, 0, 0, 0>(L, R.own[4], -1, lx, ly, z0, c);
+        for (int k = 0; k < NZ; ++k) t01[k] = u[k] * c[k];
+        vfetchn<E1::x, E1::y, E1::z, NZ>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetchn<kX2 + 1, 0, 0, 0, NZ>(L, R.own[4], -1, lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = c[k] * u[k];
             t01[k] = t01[k] + p;
         }
-        float t23[4];
-        vfetch4<-E0::x, -E0::y, -E0::z>(L, Xv, d1, R, lx, ly, z0, u);
-        cfetch4<kX2 + 2, -E0::x, -E0::y, -E0::z>(L, R.own[5], R.nbs[3], lx, ly, z0, c);
+        float t23[NZ];
+        vfetchn<-E0::x, -E0::y, -E0::z, NZ>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetchn<kX2 + 2, -E0::x, -E0::y, -E0::z, NZ>(L, R.own[5], R.nbs[3], lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) t23[k] = c[k] * u[k];
-        vfetch4<E1::x - E0::x, E1::y - E0::y, E1::z - E0::z>(L, Xv, d1, R, lx, ly, z0, u);
-        cfetch4<kX2 + 3, -E0::x, -E0::y, -E0::z>(L, R.own[6], R.nbs[4], lx, ly, z0, c);
+        for (int k = 0; k < NZ; ++k) t23[k] = c[k] * u[k];
+        vfetchn<E1::x - E0::x, E1::y - E0::y, E1::z - E0::z, NZ>(L, Xv, d1, R, lx, ly, z0, u);
+        cfetchn<kX2 + 3, -E0::x, -E0::y, -E0::z, NZ>(L, R.own[6], R.nbs[4], lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = c[k] * u[k];
             t23[k] = t23[k] + p;
             const float T2 = t01[k] + t23[k];
@@ -480,27 +499,27 @@ __device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lan
     }
     {
         // normal d1: channel d2 at v, v - e0, v + e2, v - e0 + e2
-        float u[4], c[4], t01[4];
-        vfetch4<0, 0, 0>(L, Xv, d2, R, lx, ly, z0, u);
-        cfetch4<kX1 + 0, 0, 0, 0>(L, R.own[7], -1, lx, ly, z0, c);
+        float u[NZ], c[NZ], t01[NZ];
+        vfetchn<0, 0, 0, NZ>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetchn<kX1 + 0, 0, 0, 0, NZ>(L, R.own[7], -1, lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) t01[k] = c[k] * u[k];
-        vfetch4<-E0::x, -E0::y, -E0::z>(L, Xv, d2, R, lx, ly, z0, u);
-        cfetch4<kX1 + 1, -E0::x, -E0::y, -E0::z>(L, R.own[8], R.nbs[5], lx, ly, z0, c);
+        for (int k = 0; k < NZ; ++k) t01[k] = c[k] * u[k];
+        vfetchn<-E0::x, -E0::y, -E0::z, NZ>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetchn<kX1 + 1, -E0::x, -E0::y, -E0::z, NZ>(L, R.own[8], R.nbs[5], lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = c[k] * u[k];
             t01[k] = t01[k] + p;
         }
-        float t23[4];
-        vfetch4<E2::x, E2::y, E2::z>(L, Xv, d2, R, lx, ly, z0, u);
-        cfetch4<kX1 + 2, 0, 0, 0>(L, R.own[9], -1, lx, ly, z0, c);
+        float t23[NZ];
+        vfetchn<E2::x, E2::y, E2::z, NZ>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetchn<kX1 + 2, 0, 0, 0, NZ>(L, R.own[9], -1, lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) t23[k] = c[k] * u[k];
-        vfetch4<E2::x - E0::x, E2::y - E0::y, E2::z - E0::z>(L, Xv, d2, R, lx, ly, z0, u);
-        cfetch4<kX1 + 3, -E0::x, -E0::y, -E0::z>(L, R.own[10], R.nbs[6], lx, ly, z0, c);
+        for (int k = 0; k < NZ; ++k) t23[k] = c[k] * u[k];
+        vfetchn<E2::x - E0::x, E2::y - E0::y, E2::z - E0::z, NZ>(L, Xv, d2, R, lx, ly, z0, u);
+        cfetchn<kX1 + 3, -E0::x, -E0::y, -E0::z, NZ>(L, R.own[10], R.nbs[6], lx, ly, z0, c);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             const float p = c[k] * u[k];
             t23[k] = t23[k] + p;
             const float T1 = t01[k] + t23[k];
@@ -509,14 +528,14 @@ __device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lan
     }
 
     const size_t row = (size_t(leaf) * 3 + C) * 512 + size_t(lx * 8 + ly) * 8 + size_t(z0);
-    float lhs[4], y[4];
-    ld4v(Xv + row, lhs);
+    float lhs[NZ], y[NZ];
+    ldnv<NZ>(Xv + row, lhs);
     if (MODE == kVRedGS || MODE == kVBlackGS) {
-        float b[4], inv[4];
-        ld4v(Bv + row, b);
-        cfetch4<kVInvDiag + C, 0, 0, 0>(L, R.own[12], -1, lx, ly, z0, inv);
+        float b[NZ], inv[NZ];
+        ldnv<NZ>(Bv + row, b);
+        cfetchn<kVInvDiag + C, 0, 0, 0, NZ>(L, R.own[12], -1, lx, ly, z0, inv);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NZ; ++k) {
             float t = b[k] - s[k];
             t = t * inv[k];
             t = t * 1.0f;
@@ -525,23 +544,23 @@ __device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lan
             y[k] = (red == (MODE == kVRedGS)) ? r : lhs[k];
         }
     } else {
-        float dg[4];
-        cfetch4<kVDiag + C, 0, 0, 0>(L, R.own[11], -1, lx, ly, z0, dg);
+        float dg[NZ];
+        cfetchn<kVDiag + C, 0, 0, 0, NZ>(L, R.own[11], -1, lx, ly, z0, dg);
         if (MODE == kVNormal) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) y[k] = fmaf(lhs[k], dg[k], s[k]);
+            for (int k = 0; k < NZ; ++k) y[k] = fmaf(lhs[k], dg[k], s[k]);
         } else {
-            float b[4];
-            ld4v(Bv + row, b);
+            float b[NZ];
+            ldnv<NZ>(Bv + row, b);
             if (MODE == kVResidual) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) y[k] = b[k] - fmaf(lhs[k], dg[k], s[k]);
+                for (int k = 0; k < NZ; ++k) y[k] = b[k] - fmaf(lhs[k], dg[k], s[k]);
             } else {
-                float sc[4];
-                if (MODE == kVJacobi) cfetch4<kVInvDiag + C, 0, 0, 0>(L, R.own[12], -1, lx, ly, z0, sc);
-                else cfetch4<kVSpai0 + C, 0, 0, 0>(L, R.own[13], -1, lx, ly, z0, sc);
+                float sc[NZ];
+                if (MODE == kVJacobi) cfetchn<kVInvDiag + C, 0, 0, 0, NZ>(L, R.own[12], -1, lx, ly, z0, sc);
+                else cfetchn<kVSpai0 + C, 0, 0, 0, NZ>(L, R.own[13], -1, lx, ly, z0, sc);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < NZ; ++k) {
                     const float a = fmaf(lhs[k], dg[k], s[k]);
                     float t = b[k] - a;
                     t = t * sc[k];
@@ -550,32 +569,34 @@ __device__ __forceinline__ void visc_row4(const ViscLevelView& L, int e, int lan
             }
         }
     }
-    const uint32_t rmh = (rm >> z0) & 0xfu;
-    if (!rmh) return;   // half rows without a DOF are not written
+    const uint32_t rmh = (rm >> z0) & ((1u << NZ) - 1u);
+    if (!rmh) return;   // row parts without a DOF are not written
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < NZ; ++k)
         if (!((rmh >> k) & 1u)) y[k] = 0.f;
-    st4v(Yv + row, y);
+    stnv<NZ>(Yv + row, y);
 }
 
 
-template <int MODE, class V>
-__global__ __launch_bounds__(kVThreads) void visc_apply_all_half_kernel(ViscLevelView L, const V* Xv, const V* Bv,
+template <int MODE, int NZ, class V>
+__global__ __launch_bounds__(kVThreads) void visc_apply_all_part_kernel(ViscLevelView L, const V* Xv, const V* Bv,
                                                                         V* Yv) {
+    constexpr int kParts = 8 / NZ;
     const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
-    const int e = w >> 1, z0 = (w & 1) * 4;
+    const int e = w / kParts, z0 = (w % kParts) * NZ;
     const int lane = threadIdx.x & 63;
-    if (e < L.chan_off[1]) visc_row4<MODE, 0>(L, e, lane, z0, Xv, Bv, Yv);
-    else if (e < L.chan_off[2]) visc_row4<MODE, 1>(L, e, lane, z0, Xv, Bv, Yv);
-    else if (e < L.chan_off[3]) visc_row4<MODE, 2>(L, e, lane, z0, Xv, Bv, Yv);
+    if (e < L.chan_off[1]) visc_rown<MODE, 0, NZ>(L, e, lane, z0, Xv, Bv, Yv);
+    else if (e < L.chan_off[2]) visc_rown<MODE, 1, NZ>(L, e, lane, z0, Xv, Bv, Yv);
+    else if (e < L.chan_off[3]) visc_rown<MODE, 2, NZ>(L, e, lane, z0, Xv, Bv, Yv);
 }
 
-template <int MODE, int C, class V>
-__global__ __launch_bounds__(kVThreads) void visc_apply_half_kernel(ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
+template <int MODE, int C, int NZ, class V>
+__global__ __launch_bounds__(kVThreads) void visc_apply_part_kernel(ViscLevelView L, const V* Xv, const V* Bv, V* Yv) {
+    constexpr int kParts = 8 / NZ;
     const int w = dev::logical_block() * kVWaves + (threadIdx.x >> 6);
-    const int e = w >> 1;
+    const int e = w / kParts;
     if (e >= L.chan_off[C + 1] - L.chan_off[C]) return;
-    visc_row4<MODE, C>(L, L.chan_off[C] + e, threadIdx.x & 63, (w & 1) * 4, Xv, Bv, Yv);
+    visc_rown<MODE, C, NZ>(L, L.chan_off[C] + e, threadIdx.x & 63, (w % kParts) * NZ, Xv, Bv, Yv);
 }
 
 // All three channels in one launch (modes that write y != x: the channels
@@ -1005,12 +1026,21 @@ template <int MODE, class V>
 hipError_t apply_mode(const ViscLevelView& L, int c, const V* x, const V* b, V* y, hipStream_t s) {
     const int n = L.chan_off[c + 1] - L.chan_off[c];
     if (n <= 0) return hipSuccess;
+    if (L.half == 2) {
+        const unsigned gh = unsigned((4 * n + kVWaves - 1) / kVWaves);
+        switch (c) {
+            case 0: visc_apply_part_kernel<MODE, 0, 2, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case 1: visc_apply_part_kernel<MODE, 1, 2, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            default: visc_apply_part_kernel<MODE, 2, 2, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+        }
+        return hipGetLastError();
+    }
     if (L.half) {
         const unsigned gh = unsigned((2 * n + kVWaves - 1) / kVWaves);
         switch (c) {
-            case 0: visc_apply_half_kernel<MODE, 0, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
-            case 1: visc_apply_half_kernel<MODE, 1, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
-            default: visc_apply_half_kernel<MODE, 2, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case 0: visc_apply_part_kernel<MODE, 0, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case 1: visc_apply_part_kernel<MODE, 1, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            default: visc_apply_part_kernel<MODE, 2, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
         }
         return hipGetLastError();
     }
@@ -1050,12 +1080,15 @@ hipError_t apply_all(const ViscLevelView& L, int mode, const V* x, const V* b, V
     const int n = L.chan_off[3];
     if (n <= 0) return hipSuccess;
     if (L.half) {
+        // the three channels' rows in one launch on half rows (quarter rows too only for the passes)
         const unsigned gh = unsigned((2 * n + kVWaves - 1) / kVWaves);
         switch (mode) {
-            case kVNormal: visc_apply_all_half_kernel<kVNormal, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
-            case kVResidual: visc_apply_all_half_kernel<kVResidual, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
-            case kVJacobi: visc_apply_all_half_kernel<kVJacobi, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
-            case kVSpai0Mode: visc_apply_all_half_kernel<kVSpai0Mode, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVNormal: visc_apply_all_part_kernel<kVNormal, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVResidual: visc_apply_all_part_kernel<kVResidual, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVJacobi: visc_apply_all_part_kernel<kVJacobi, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y); break;
+            case kVSpai0Mode:
+                visc_apply_all_part_kernel<kVSpai0Mode, 4, V><<<gh, kVThreads, 0, s>>>(L, x, b, y);
+                break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
