@@ -13,7 +13,9 @@ decode path against what the file itself states and what its geometry fixes:
 The Blosc and LZ4 decoders are also checked on buffers made here from their
 published formats (a greedy LZ4 encoder, the Blosc 1 container with and
 without shuffle, splits and short last blocks, the zlib codec, buffers
-stored uncompressed), and malformed inputs fail cleanly. Files written here
+stored uncompressed), and malformed inputs fail cleanly (truncations, and 300
+seeded mutations of the fixture that either fail or read back consistent
+arrays). Files written here
 from the published layout cover what the fixture does not: Vec3f grids, half
 floats, zlib and no compression, every inactive-value code of the
 active-mask compression, tiles at the root and both internal levels, several
@@ -262,6 +264,45 @@ def test_malformed_buffers_fail_cleanly(tmp_path):
     p.write_bytes(b"\0" * 64)
     with pytest.raises(U.UaamgError):
         read_vdb(str(p))
+
+
+def test_mutated_files_never_crash(tmp_path):
+    """Seeded mutations of the fixture (bit flips, truncations, random and extreme 32/64-bit words
+    anywhere, counts and offsets included): each read either raises UaamgError or returns grids
+    whose arrays agree in shape, with expanded tiles too - the reader never reads out of bounds
+    or allocates from an unchecked count."""
+    raw = FIXTURE.read_bytes()
+    rng = np.random.default_rng(2024)
+    extremes = [0, 0xFFFFFFFF, 0x7FFFFFFF, 0x80000000, 0xFFFFFFFFFFFFFFFF, 1 << 40]
+    p = tmp_path / "mut.vdb"
+    outcomes = [0, 0]
+    for i in range(300):
+        m = bytearray(raw)
+        kind = i % 4
+        if kind == 0:
+            for _ in range(int(rng.integers(1, 9))):
+                m[int(rng.integers(len(m)))] ^= 1 << int(rng.integers(8))
+        elif kind == 1:
+            del m[int(rng.integers(len(m))):]
+        elif kind == 2:
+            at = int(rng.integers(len(m) - 4))
+            m[at:at + 4] = int(rng.integers(1 << 32)).to_bytes(4, "little")
+        else:
+            at, w = int(rng.integers(len(m) - 8)), extremes[i // 4 % len(extremes)]
+            width = 8 if w > 0xFFFFFFFF or i % 8 == 3 else 4
+            m[at:at + width] = w.to_bytes(width, "little")
+        p.write_bytes(bytes(m))
+        try:
+            grids = read_vdb(str(p), expand_tiles=128, max_leaves=1 << 16)
+        except U.UaamgError:
+            outcomes[1] += 1
+            continue
+        outcomes[0] += 1
+        for g in grids:
+            if g.grid is not None:
+                n = g.grid.origins.shape[0]
+                assert g.grid.origins.shape == (n, 3) and g.grid.values.shape[:2] == (n, 512)
+    assert outcomes[0] > 0 and outcomes[1] > 0, outcomes
 
 
 # ---- files written here from the published layout: the paths the fixture leaves open
