@@ -103,17 +103,21 @@ def kernel_rooflines(lap, iterations: int, reps: int) -> list:
     out.append({"kernel": ("apply_kernel<2, true, bf16, float, bf16>" if mixed else "apply_kernel<2, true>") +
                           " (level-0 Jacobi sweep)", "bytes_per_launch": lap.algorithmic_bytes(0, mode0),
                 "ms_per_launch": ms, "launches_per_solve": 2 * W})
+    vt = "bf16" if mixed else "float"
     for l in range(1, L - 1):
         if not km[l]["block"]:
             continue
         ms = lap.time_apply(l, 7, reps=reps)
-        out.append({"kernel": f"block_visit_kernel<3, 2, {'bf16' if mixed else 'float'}> (level-{l} post-visit: "
-                              f"prolongation + 3 sweeps)", "bytes_per_launch": lap.algorithmic_bytes(l, 7),
+        name = (f"block_post_cg_kernel<{vt}> (level-{l} post-visit with the level-{L - 1} coarsest CG in front)"
+                if km[l]["post_cg"] else
+                f"block_visit_kernel<3, 2, {vt}> (level-{l} post-visit: prolongation + 3 sweeps)")
+        out.append({"kernel": name, "bytes_per_launch": lap.algorithmic_bytes(l, 7),
                     "ms_per_launch": ms, "launches_per_solve": W * 2 ** l})
+    # the standalone coarsest CG runs only when the level above does not take it into its post-visit
     ms = lap.time_apply(L - 1, 8, reps=reps)
-    out.append({"kernel": f"coarsest_cg_kernel<{'bf16' if mixed else 'float'}> (level-{L - 1} Eigen-CG restatement)",
+    out.append({"kernel": f"coarsest_cg_kernel<{vt}> (level-{L - 1} Eigen-CG restatement)",
                 "bytes_per_launch": lap.algorithmic_bytes(L - 1, 8), "ms_per_launch": ms,
-                "launches_per_solve": W * 2 ** max(L - 2, 0)})
+                "launches_per_solve": 0 if L >= 3 and km[L - 2]["post_cg"] else W * 2 ** max(L - 2, 0)})
     for k in out:
         k["ms_per_solve"] = k["ms_per_launch"] * k["launches_per_solve"]
     return out

@@ -384,8 +384,9 @@ int uaamg_poisson_coarsest_solve(uaamg_poisson* h, const float* b, float* x);
  */
 int uaamg_poisson_algorithmic_bytes(const uaamg_poisson* h, int level, int mode, double* bytes);
 /* Level-to-kernel mapping chosen when the handle was created: *block = 1 when a W-cycle visit of the
- * level runs as two block-visit launches, and the ms per visit pair of both mappings as timed then
- * (-1 when the level was not timed). No reference counterpart. */
+ * level runs as two block-visit launches (2: and the second of them also runs the coarsest solve),
+ * and the ms per visit pair of both mappings as timed then (-1 when the level was not timed).
+ * No reference counterpart. */
 int uaamg_poisson_kernel_map(const uaamg_poisson* h, int level, int32_t* block, float* ms_block, float* ms_sweep);
 int uaamg_poisson_time_apply(uaamg_poisson* h, int level, int mode, int reps, float* ms_per_launch);
 

@@ -192,3 +192,31 @@ def test_fused_and_graph_paths_bit_identical(problem, smoother):
         assert np.array_equal(z, z0), f"flags {flags}: W-cycle differs"
         assert st == st0 and it == it0, f"flags {flags}: {st}/{it} vs {st0}/{it0}"
         assert np.array_equal(x, x0), f"flags {flags}: solution differs"
+
+
+@pytest.mark.parametrize("precision", [U.PRECISION_FP32, U.PRECISION_BF16_CYCLE])
+def test_coarsest_inside_block_post_visit_bit_identical(problem, monkeypatch, precision):
+    """With block-visit kernels on every coarse level (UAAMG_BLOCK_LEAVES), the post-visit of the
+    level above the coarsest runs the coarsest CG in its own launch, each workgroup solving the
+    whole coarsest system from the same routine; UAAMG_POST_CG=0 launches the CG on its own. The
+    W-cycle, the PCG solution and its iteration count are bit-identical either way, and the
+    kernel map reports the fused form only where it runs."""
+    s, lap, orc, b = problem
+    monkeypatch.setenv("UAAMG_BLOCK_LEAVES", "1000000")
+    out = {}
+    for fused in ("1", "0"):
+        monkeypatch.setenv("UAAMG_POST_CG", fused)
+        l2 = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, precision=precision)
+        km = l2.kernel_map()
+        L = l2.nlevels
+        assert [k["post_cg"] for k in km] == [fused == "1" and L >= 3 and l == L - 2 for l in range(L)]
+        solver = U.PoissonSolver(l2)
+        z = solver.precondition(b)
+        x = l2.getZeroVectorGrid()
+        st = solver.solveMultigridPCG(x, b)
+        out[fused] = (z, x, st, solver.mIterationTaken)
+        _say(f"  precision {precision}, post_cg {fused}: levels {L}, status {st}, iterations {solver.mIterationTaken}")
+    (z1, x1, s1, i1), (z0, x0, s0, i0) = out["1"], out["0"]
+    assert np.array_equal(z1, z0)
+    assert (s1, i1) == (s0, i0) and s1 == U.SUCCESS
+    assert np.array_equal(x1, x0)

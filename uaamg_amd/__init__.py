@@ -622,12 +622,13 @@ class LaplacianWithLevel:
         return r, c, v
 
     def kernel_map(self) -> list[dict]:
-        """Per level: whether visits run as block-visit launches, and the timings behind the choice."""
+        """Per level: whether visits run as block-visit launches (post_cg: the post-visit launch also
+        runs the coarsest solve), and the timings behind the choice."""
         out = []
         for l in range(self.nlevels):
             b, mb, ms = C.c_int32(), C.c_float(), C.c_float()
             _check(lib().uaamg_poisson_kernel_map(self._h, l, C.byref(b), C.byref(mb), C.byref(ms)), "kernel_map")
-            out.append({"level": l, "block": bool(b.value), "ms_block": round(mb.value, 4),
+            out.append({"level": l, "block": bool(b.value), "post_cg": b.value == 2, "ms_block": round(mb.value, 4),
                         "ms_sweep": round(ms.value, 4)})
         return out
 

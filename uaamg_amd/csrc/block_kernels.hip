@@ -17,8 +17,17 @@
 //             (first visit, muCyclePreconditioner from a zero guess)
 //   kPreAgain (H = 4): three sweeps from the level's x, residual, restriction
 //   kPost     (H = 3): x + P x_coarse, three sweeps
+//
+// On the level above the coarsest, the post-visit can take the coarsest solve
+// in with it (block_post_cg_kernel): every workgroup runs the coarsest CG on
+// the whole coarsest system (the same routine and thread count as the
+// standalone solve, so every workgroup gets the same values the standalone
+// kernel stores), keeps the solution in LDS for its staging, and workgroup 0
+// also stores it. The coarse correction then never crosses HBM and the
+// W-cycle bottom loses a launch.
 #include <type_traits>
 
+#include "coarsest_device.h"
 #include "stencil_device.h"
 
 namespace uaamg {
@@ -98,9 +107,12 @@ __device__ __forceinline__ void sweep(float* x, const float* b, const float* dg,
         if (at[k] >= 0) x[at[k]] = out[k];
 }
 
-template <int H, int MODE, class V>
-__global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, const V* Xin, const V* Xc, const V* B,
-                                                                  V* Xout, V* Cv, float w0, float w1, float w2) {
+// The visit of one leaf. kCg (kPost only): the coarse correction comes from
+// cgx, the coarsest solution in LDS indexed by coarsest DOF (dof_of maps a
+// coarsest brick position to its DOF, -1 elsewhere), instead of Xc.
+template <int H, int MODE, class V, bool kCg>
+__device__ __forceinline__ void visit_body(const LevelView& L, const V* Xin, const V* Xc, const V* B, V* Xout, V* Cv,
+                                           float w0, float w1, float w2, const int32_t* dof_of, const float* cgx) {
     constexpr int S = Region<H>::S, N = Region<H>::N;
     extern __shared__ float lds[];
     float* b = lds;
@@ -157,17 +169,21 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
         uint64_t tm = L.mask[nv * 8 + lx];
         float tb = float(B[nv * 512 + off]);
         float tx = 0.f, tc = 0.f;
-        int pp = -1;
+        int pp = -1, di = -1;
         if (MODE != kPreZero) tx = float(Xin[nv * 512 + off]);
         if (MODE == kPost) {
             pp = par[k];
             const int p = pp >= 0 ? pp : 0, cl = p >> 3, oct = p & 7;
             const int crow = ((lx >> 1) + 4 * ((oct >> 2) & 1)) * 8 + ((ly >> 1) + 4 * ((oct >> 1) & 1));
-            tc = float(Xc[size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1)]);
+            const size_t ci = size_t(cl) * 512 + crow * 8 + 4 * (oct & 1) + (lz >> 1);
+            if constexpr (kCg) di = dof_of[ci];
+            else tc = float(Xc[ci]);
         }
         // selects, no branches: the empty asm statements (no instruction)
         // keep the compiler from sinking these loads into a branch on n
-        asm volatile("" : "+v"(tm), "+v"(tb), "+v"(tx), "+v"(tc));
+        asm volatile("" : "+v"(tm), "+v"(tb), "+v"(tx), "+v"(tc), "+v"(di));
+        // a fine DOF's parent voxel is a coarsest DOF, the only case where tc is used
+        if constexpr (kCg) tc = cgx[di >= 0 ? di : kMaxCoarsestDof];
         const bool has = n >= 0;
         const float vdg = s0 >= 0 ? tdg : L.dflt_diag;
         const float vinv = s0 >= 0 ? tinv : L.dflt_invdiag;
@@ -257,6 +273,26 @@ __global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, c
 }
 
 template <int H, int MODE, class V>
+__global__ __launch_bounds__(kBlkThreads) void block_visit_kernel(LevelView L, const V* Xin, const V* Xc, const V* B,
+                                                                  V* Xout, V* Cv, float w0, float w1, float w2) {
+    visit_body<H, MODE, V, false>(L, Xin, Xc, B, Xout, Cv, w0, w1, w2, nullptr, nullptr);
+}
+
+// The coarsest solve (right-hand side Bc, solution stored into Xc by
+// workgroup 0) followed by the kPost visit of the level above it. The CG's
+// last barrier has passed when it writes its solution into cg.sp, and the
+// visit's first barrier publishes it.
+template <class V, int kMaxDof>
+__global__ __launch_bounds__(kBlkThreads) void block_post_cg_kernel(LevelView L, CoarsestView C,
+                                                                    const int32_t* dof_of, const V* Bc, V* Xc,
+                                                                    const V* Xin, const V* B, V* Xout, float w0,
+                                                                    float w1, float w2) {
+    __shared__ CgShared<kBlkThreads> cg;
+    coarsest_cg_block<kBlkThreads, V, kMaxDof, true>(C, Bc, blockIdx.x == 0 ? Xc : nullptr, cg);
+    visit_body<3, kPost, V, true>(L, Xin, nullptr, B, Xout, nullptr, w0, w1, w2, dof_of, cg.sp);
+}
+
+template <int H, int MODE, class V>
 hipError_t launch_mode(const LevelView& L, const V* xin, const V* xc, const V* b, V* xout, V* cv, float w0, float w1,
                        float w2, hipStream_t s) {
     block_visit_kernel<H, MODE, V><<<unsigned(L.nleaf), kBlkThreads, Region<H>::lds_bytes, s>>>(L, xin, xc, b, xout,
@@ -273,6 +309,12 @@ hipError_t set_lds_limits() {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<4>::lds_bytes));
     if (e == hipSuccess)
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_visit_kernel<3, kPost, V>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_post_cg_kernel<V, kCgSmallDof>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&block_post_cg_kernel<V, kMaxCoarsestDof>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(Region<3>::lds_bytes));
     return e;
 }
@@ -292,6 +334,21 @@ hipError_t block_post_t(const LevelView& L, const V* x, const V* xc, const V* b,
     if (L.nleaf <= 0) return hipSuccess;
     if (!L.nb27) return hipErrorInvalidValue;
     return launch_mode<3, kPost, V>(L, x, xc, b, x_out, nullptr, w0, w1, w2, s);
+}
+
+template <class V>
+hipError_t block_post_cg_t(const LevelView& L, const CoarsestView& C, const int32_t* dof_of, const V* bc, V* xc,
+                           const V* x, const V* b, V* x_out, float w0, float w1, float w2, hipStream_t s) {
+    if (L.nleaf <= 0 || C.ndof <= 0) return hipErrorInvalidValue;
+    if (!L.nb27 || !dof_of || C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
+    const unsigned g = unsigned(L.nleaf);
+    const size_t lds = Region<3>::lds_bytes;
+    if (C.ndof <= kCgSmallDof)
+        block_post_cg_kernel<V, kCgSmallDof><<<g, kBlkThreads, lds, s>>>(L, C, dof_of, bc, xc, x, b, x_out, w0, w1, w2);
+    else
+        block_post_cg_kernel<V, kMaxCoarsestDof>
+            <<<g, kBlkThreads, lds, s>>>(L, C, dof_of, bc, xc, x, b, x_out, w0, w1, w2);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -320,6 +377,18 @@ hipError_t launch_block_pre(const LevelView& L, bool from_zero, const bf16* x, c
 hipError_t launch_block_post(const LevelView& L, const bf16* x, const bf16* xc, const bf16* b, bf16* x_out,
                              float w0, float w1, float w2, hipStream_t s) {
     return block_post_t<bf16>(L, x, xc, b, x_out, w0, w1, w2, s);
+}
+
+hipError_t launch_block_post_cg(const LevelView& L, const CoarsestView& C, const int32_t* dof_of, const float* bc,
+                                float* xc, const float* x, const float* b, float* x_out, float w0, float w1, float w2,
+                                hipStream_t s) {
+    return block_post_cg_t<float>(L, C, dof_of, bc, xc, x, b, x_out, w0, w1, w2, s);
+}
+
+hipError_t launch_block_post_cg(const LevelView& L, const CoarsestView& C, const int32_t* dof_of, const bf16* bc,
+                                bf16* xc, const bf16* x, const bf16* b, bf16* x_out, float w0, float w1, float w2,
+                                hipStream_t s) {
+    return block_post_cg_t<bf16>(L, C, dof_of, bc, xc, x, b, x_out, w0, w1, w2, s);
 }
 
 }  // namespace uaamg

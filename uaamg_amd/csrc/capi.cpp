@@ -230,7 +230,7 @@ int uaamg_poisson_level_triplets(const uaamg_poisson* h, int level, int64_t* cou
 int uaamg_poisson_kernel_map(const uaamg_poisson* h, int level, int32_t* block, float* ms_block, float* ms_sweep) {
     if (!h || level < 0 || level >= h->dev.nlevels()) return fail(UAAMG_EINVAL, "bad level");
     const auto& m = h->dev.block_levels();
-    if (block) *block = level < int(m.size()) ? m[size_t(level)] : 0;
+    if (block) *block = level < int(m.size()) && m[size_t(level)] ? (h->dev.post_cg_level(level) ? 2 : 1) : 0;
     if (ms_block) *ms_block = h->dev.tune_ms(level, 0);
     if (ms_sweep) *ms_sweep = h->dev.tune_ms(level, 1);
     return 0;

@@ -54,8 +54,10 @@ __device__ __forceinline__ float2 cg_sum2(float a, float b, CgShared<kThreads>& 
 // Solves the coarsest system for right-hand side B (bricks) into X (bricks,
 // DOF positions only); C.ndof <= kMaxDof (thread t owns rows t + j*kThreads,
 // so a smaller kMaxDof only drops row slots that would be empty: the results
-// do not depend on it). Every thread of the block must call it.
-template <int kThreads, class V = float, int kMaxDof = kMaxCoarsestDof>
+// do not depend on it). Every thread of the block must call it. kKeep: the
+// solution, as X would hold it, also goes to sh.sp[d] (d < ndof; the zero slot
+// stays zero), and X may be null; the caller's next barrier publishes it.
+template <int kThreads, class V = float, int kMaxDof = kMaxCoarsestDof, bool kKeep = false>
 __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgShared<kThreads>& sh) {
     static_assert(kMaxDof <= kMaxCoarsestDof && kMaxDof % kThreads == 0, "row slots");
     constexpr int kPer = kMaxDof / kThreads;
@@ -149,10 +151,19 @@ __device__ void coarsest_cg_block(const CoarsestView& C, const V* B, V* X, CgSha
             beta = abs_new / abs_old;
         }
     }
+    // every thread has passed the last barrier that precedes a read of sh.sp
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int d = tid + j * kThreads;
-        if (d < n) X[C.pos[d]] = V(x[j]);
+        if (d < n) {
+            const V xv = V(x[j]);
+            if constexpr (kKeep) {
+                sh.sp[d] = float(xv);
+                if (X) X[C.pos[d]] = xv;
+            } else {
+                X[C.pos[d]] = xv;
+            }
+        }
     }
 }
 

@@ -738,6 +738,8 @@ hipError_t finalize(int op, int n, const double* partials, PcgScalars* sc, HostM
 
 int reduction_blocks(const LevelView& L) { return int(grid_for(L.nleaf)) > 0 ? int(grid_for(L.nleaf)) : 1; }
 
+int coarsest_cg_threads() { return cg_threads(); }
+
 hipError_t launch_coarsest_cg(const CoarsestView& C, const float* b, float* x, hipStream_t s) {
     if (C.ndof <= 0) return hipSuccess;
     if (C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;

@@ -237,6 +237,18 @@ hipError_t launch_block_pre(const LevelView& L, bool from_zero, const float* x, 
 // x_out = three sweeps (w0, w1, w2) from x + P x_coarse
 hipError_t launch_block_post(const LevelView& L, const float* x, const float* xc, const float* b, float* x_out,
                              float w0, float w1, float w2, hipStream_t s);
+// The coarsest solve xc = CG(bc) (as launch_coarsest_cg stores it) and then
+// launch_block_post of the level above the coarsest with that xc, in one
+// launch. dof_of: [coarsest leaves * 512], the coarsest DOF at each brick
+// position or -1. Requires the coarsest CG at 512 threads.
+hipError_t launch_block_post_cg(const LevelView& L, const CoarsestView& C, const int32_t* dof_of, const float* bc,
+                                float* xc, const float* x, const float* b, float* x_out, float w0, float w1, float w2,
+                                hipStream_t s);
+hipError_t launch_block_post_cg(const LevelView& L, const CoarsestView& C, const int32_t* dof_of, const bf16* bc,
+                                bf16* xc, const bf16* x, const bf16* b, bf16* x_out, float w0, float w1, float w2,
+                                hipStream_t s);
+// threads of the coarsest CG workgroup (512 unless UAAMG_CG_THREADS says otherwise)
+int coarsest_cg_threads();
 
 // ---- mixed-precision W-cycle: the same passes with the level vectors stored
 // in BF16 and every operation computed in FP32. Level 0's right-hand side is

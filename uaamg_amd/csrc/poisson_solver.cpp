@@ -288,6 +288,7 @@ size_t PoissonDevice::device_bytes() const {
     add((size_t(reduction_blocks(LevelView{int32_t(n0)})) + 16) * sizeof(double));
     const auto& hc = host_.levels.back();
     add(size_t(hc.ndof) * (sizeof(int32_t) * 7 + sizeof(float) * 8) + 1024);
+    add(hc.nleaf() * kLeafSize * sizeof(int32_t));   // coarsest DOF at each brick position
     return bytes;
 }
 
@@ -478,16 +479,31 @@ void PoissonDevice::visit_pair(int l, bool block) {
     const bool weighted = opt_.smoother == kWeightedJacobi;
     auto wpre = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, i); };
     auto wpost = [&](int i) { return weighted ? 6.0f / 7.0f : scheduled_weight(kSweeps, kSweeps - 1 - i); };
+    // above the coarsest level, both mappings include the coarsest solve (the
+    // block mapping runs it inside its post-visit launch)
+    const bool cg = l + 2 == nlevels() && post_cg_;
     for (int first = 1; first >= 0; --first) {
         if (block) {
             if (mixed_) {
                 check(launch_block_pre(v, first, d.Xh, d.Bh, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "tune");
-                check(launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, wpost(0), wpost(1), wpost(2), stream_), "tune");
+                if (cg)
+                    check(launch_block_post_cg(v, cv_, coarsest_dof_of_, c.Bh, c.Xh, d.Th, d.Bh, d.Xh, wpost(0),
+                                               wpost(1), wpost(2), stream_),
+                          "tune");
+                else
+                    check(launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, wpost(0), wpost(1), wpost(2), stream_), "tune");
             } else {
                 check(launch_block_pre(v, first, d.X, d.B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "tune");
-                check(launch_block_post(v, d.T, c.X, d.B, d.X, wpost(0), wpost(1), wpost(2), stream_), "tune");
+                if (cg)
+                    check(launch_block_post_cg(v, cv_, coarsest_dof_of_, c.B, c.X, d.T, d.B, d.X, wpost(0), wpost(1),
+                                               wpost(2), stream_),
+                          "tune");
+                else
+                    check(launch_block_post(v, d.T, c.X, d.B, d.X, wpost(0), wpost(1), wpost(2), stream_), "tune");
             }
-        } else if (mixed_) {
+            continue;
+        }
+        if (mixed_) {
             if (first) {
                 check(launch_jacobi_first2(v, d.Bh, d.Th, wpre(0), wpre(1), stream_), "tune");
                 check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpre(2), stream_), "tune");
@@ -497,6 +513,7 @@ void PoissonDevice::visit_pair(int l, bool block) {
                 check(launch_jacobi(v, d.Uh, d.Bh, d.Th, wpre(2), stream_), "tune");
             }
             check(launch_residual_restrict(v, d.Uh, d.Bh, c.Bh, stream_), "tune");
+            if (cg) check(launch_coarsest_cg(cv_, c.Bh, c.Xh, stream_), "tune");
             check(launch_prolong_jacobi(v, d.Uh, c.Xh, d.Bh, d.Th, wpost(0), stream_), "tune");
             check(launch_jacobi(v, d.Th, d.Bh, d.Uh, wpost(1), stream_), "tune");
             check(launch_jacobi(v, d.Uh, d.Bh, d.Xh, wpost(2), stream_), "tune");
@@ -510,6 +527,7 @@ void PoissonDevice::visit_pair(int l, bool block) {
                 check(launch_apply(v, kModeJacobi, d.U, d.B, d.T, wpre(2), stream_), "tune");
             }
             check(launch_residual_restrict(v, d.U, d.B, c.B, stream_), "tune");
+            if (cg) check(launch_coarsest_cg(cv_, c.B, c.X, stream_), "tune");
             check(launch_prolong_jacobi(v, d.U, c.X, d.B, d.T, wpost(0), stream_), "tune");
             check(launch_apply(v, kModeJacobi, d.T, d.B, d.U, wpost(1), stream_), "tune");
             check(launch_apply(v, kModeJacobi, d.U, d.B, d.X, wpost(2), stream_), "tune");
@@ -902,6 +920,13 @@ int PoissonDevice::build_coarsest() {
     cv_.coef = static_cast<const float*>(up(coef.data(), coef.size() * sizeof(float)));
     cv_.max_iters = 10;
     cv_.tol = FLT_EPSILON;
+    // the inverse of pos, for the post-visit kernel that takes the coarsest
+    // solve in (UAAMG_POST_CG=0 keeps them apart; the results are the same)
+    std::vector<int32_t> dof_of(size_t(l.nleaf()) * kLeafSize, -1);
+    for (int64_t d = 0; d < nd; ++d) dof_of[size_t(pos[size_t(d)])] = int32_t(d);
+    coarsest_dof_of_ = static_cast<const int32_t*>(up(dof_of.data(), dof_of.size() * sizeof(int32_t)));
+    const char* env = std::getenv("UAAMG_POST_CG");
+    post_cg_ = nd > 0 && coarsest_dof_of_ && coarsest_cg_threads() == 512 && !(env && env[0] == '0');
     if (check(hipStreamSynchronize(stream_), "coarsest upload")) return -3;
     return 0;
 }
@@ -1015,6 +1040,12 @@ void PoissonDevice::precond_fused(int l, float* X, const float* B, bool skip_fir
         // prolongation + post-smoothing (the coarser visits run in between)
         DevLevel& c = lv_[l + 1];
         check(launch_block_pre(v, skip_first, X, B, d.T, c.B, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+        if (l + 1 == L - 1 && post_cg_) {
+            check(launch_block_post_cg(v, cv_, coarsest_dof_of_, c.B, c.X, d.T, B, X, wpost(0), wpost(1), wpost(2),
+                                       stream_),
+                  "coarsest + block post");
+            return;
+        }
         precond(l + 1, c.X, c.B, true, -1);
         if (l + 1 != L - 1) precond(l + 1, c.X, c.B, false, -1);
         check(launch_block_post(v, d.T, c.X, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
@@ -1102,6 +1133,12 @@ void PoissonDevice::precond_mixed(int l, TX* X, const TB* B, bool skip_first, in
     if constexpr (std::is_same_v<TX, bf16>) {
         if (l >= 1 && use_block_[size_t(l)]) {
             check(launch_block_pre(v, skip_first, X, B, d.Th, c.Bh, wpre(0), wpre(1), wpre(2), stream_), "block pre");
+            if (l + 1 == L - 1 && post_cg_) {
+                check(launch_block_post_cg(v, cv_, coarsest_dof_of_, c.Bh, c.Xh, d.Th, B, X, wpost(0), wpost(1),
+                                           wpost(2), stream_),
+                      "coarsest + block post");
+                return;
+            }
             coarse();
             check(launch_block_post(v, d.Th, c.Xh, B, X, wpost(0), wpost(1), wpost(2), stream_), "block post");
             return;
@@ -1414,6 +1451,7 @@ double PoissonDevice::algorithmic_bytes(int l, int mode) const {
     }
     double total = 0.0;
     if (mode == kModeBlockPost && l + 1 < nlevels()) total += sv * double(host_.levels[l + 1].ndof);   // coarse x
+    if (mode == kModeBlockPost && post_cg_level(l)) total += algorithmic_bytes(l + 1, kModeCoarsest);   // its solve
     for (size_t i = 0; i < h.nleaf(); ++i) {
         const double nd = mask_count(&h.mask[i * 8]);
         const int kd = h.keep[kCoefDiag][i], kx = h.keep[kCoefX][i], ky = h.keep[kCoefY][i], kz = h.keep[kCoefZ][i];
@@ -1453,6 +1491,12 @@ int PoissonDevice::time_apply(int l, int mode, int reps, float* ms) {
             DevLevel& c = lv_[l + 1];
             const float w0 = scheduled_weight(kSweeps, kSweeps - 1), w1 = scheduled_weight(kSweeps, kSweeps - 2),
                         w2 = scheduled_weight(kSweeps, kSweeps - 3);
+            if (post_cg_level(l)) {
+                if (mixed_)
+                    return launch_block_post_cg(v, cv_, coarsest_dof_of_, c.Bh, c.Xh, d.Th, d.Bh, d.Xh, w0, w1, w2,
+                                                stream_);
+                return launch_block_post_cg(v, cv_, coarsest_dof_of_, c.B, c.X, d.T, d.B, d.X, w0, w1, w2, stream_);
+            }
             if (mixed_) return launch_block_post(v, d.Th, c.Xh, d.Bh, d.Xh, w0, w1, w2, stream_);
             return launch_block_post(v, d.T, c.X, d.B, d.X, w0, w1, w2, stream_);
         }

@@ -120,6 +120,11 @@ public:
     // W-cycle visit runs as two block-visit launches, and the two timings the
     // choice came from (ms per visit pair; -1 where the level was not timed).
     const std::vector<uint8_t>& block_levels() const { return use_block_; }
+    // the block post-visit of level l also runs the coarsest solve
+    bool post_cg_level(int l) const {
+        return post_cg_ && l >= 1 && l + 2 == nlevels() && size_t(l) < use_block_.size() && use_block_[size_t(l)] &&
+               l + 1 != tail_level_;
+    }
     float tune_ms(int level, int variant) const {
         return level < int(tune_ms_[variant].size()) ? tune_ms_[variant][size_t(level)] : -1.f;
     }
@@ -194,6 +199,8 @@ private:
     PcgScalars* sc_host_ = nullptr;   // pinned mirror
     double* partials_ = nullptr;
     CoarsestView cv_{};
+    const int32_t* coarsest_dof_of_ = nullptr;   // [coarsest leaves * 512]: DOF at each brick position or -1
+    bool post_cg_ = false;                 // block post-visit of level L-2 runs the coarsest solve in its launch
     // captured PCG pieces, valid for graph_smoother_ / graph_sor_
     Graph g_init_, g_first_, g_iter_[2];
     Graph g_repl_[2];   // g_iter_ with the FP64 residual replacement (captured with refine_)
