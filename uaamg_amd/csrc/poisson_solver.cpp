@@ -100,6 +100,8 @@ void PoissonDevice::release_device(bool keep_host) {
     sc_ = nullptr;
     partials_ = nullptr;
     cv_ = CoarsestView{};
+    coarsest_dof_of_ = nullptr;
+    post_cg_ = false;
     for (void* p : spai0_bufs_) (void)hipFree(p);
     spai0_bufs_.clear();
     spai0_on_device_ = false;
