@@ -339,8 +339,8 @@ hipError_t block_post_t(const LevelView& L, const V* x, const V* xc, const V* b,
 template <class V>
 hipError_t block_post_cg_t(const LevelView& L, const CoarsestView& C, const int32_t* dof_of, const V* bc, V* xc,
                            const V* x, const V* b, V* x_out, float w0, float w1, float w2, hipStream_t s) {
-    if (L.nleaf <= 0 || C.ndof <= 0) return hipErrorInvalidValue;
-    if (!L.nb27 || !dof_of || C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
+    if (L.nleaf <= 0) return hipSuccess;   // as block_post_t: nothing reads the coarse correction
+    if (!L.nb27 || !dof_of || C.ndof <= 0 || C.ndof > kMaxCoarsestDof) return hipErrorInvalidValue;
     const unsigned g = unsigned(L.nleaf);
     const size_t lds = Region<3>::lds_bytes;
     if (C.ndof <= kCgSmallDof)
