@@ -220,3 +220,38 @@ def test_coarsest_inside_block_post_visit_bit_identical(problem, monkeypatch, pr
     assert np.array_equal(z1, z0)
     assert (s1, i1) == (s0, i0) and s1 == U.SUCCESS
     assert np.array_equal(x1, x0)
+
+
+def test_coarsest_of_more_than_2048_rows_inside_block_post_visit(hip_available, monkeypatch):
+    """The 4096-row forms of the coarsest CG (standalone and inside the block post-visit): a 72^3
+    pool scene whose three-level hierarchy ends in a 2736-row coarsest system, with block-visit
+    kernels forced on level 1. Fused and separate solves are bit-identical in both precisions, and
+    the FP32 solve matches the oracle (status, iterations within 1, 1e-5 relative L2 against it and
+    its FP64 twin)."""
+    _say("[case] Scene(0, 72, seed=0): coarsest CG of more than 2048 rows, fused and separate")
+    s = U.Scene(0, 72, seed=0)
+    orc = O.OraclePoisson(s.phi, s.weights, 1.0, s.dx)
+    b = orc.build_rhs(s.weights, s.vx, s.vy, s.vz)
+    monkeypatch.setenv("UAAMG_BLOCK_LEAVES", "1000000")
+    for precision in (U.PRECISION_FP32, U.PRECISION_BF16_CYCLE):
+        out = {}
+        for fused in ("1", "0"):
+            monkeypatch.setenv("UAAMG_POST_CG", fused)
+            lap = U.LaplacianWithLevel(s.phi, s.weights, dt=1.0, dx=s.dx, precision=precision)
+            L = lap.nlevels
+            assert L == 3 and 2048 < lap.level_info(L - 1)["ndof"] <= 4096
+            assert lap.kernel_map()[1]["post_cg"] == (fused == "1")
+            solver = U.PoissonSolver(lap)
+            z = solver.precondition(b)
+            x = lap.getZeroVectorGrid()
+            st = solver.solveMultigridPCG(x, b)
+            out[fused] = (z, x, st, solver.mIterationTaken)
+        (z1, x1, s1, i1), (z0, x0, s0, i0) = out["1"], out["0"]
+        _say(f"  precision {precision}: status {s1}, iterations {i1}")
+        assert np.array_equal(z1, z0) and np.array_equal(x1, x0) and (s1, i1) == (s0, i0)
+        assert s1 == U.SUCCESS
+        if precision == U.PRECISION_FP32:
+            st_o, it_o, xo = orc.solve_pcg(b)
+            _, _, x64 = orc.solve_pcg64(b.astype(np.float64))
+            assert st_o == 0 and abs(i1 - it_o) <= 1
+            assert _rel(x1, xo) <= 1e-5 and _rel(x1, x64) <= 1e-5
